@@ -1,0 +1,174 @@
+/*
+ * tsp_hip.h -- C ABI of the MI355X (gfx950) IISPH step: libtsp_hip.so.
+ *
+ * This is the thin layer between the C++ solver backend (SimBackend_HIP, which implements
+ * the reference's Sim::I_SimBackend) and the hand-written HIP kernels. Every entry point
+ * takes plain pointers and sizes; the context owns all device memory and one HIP stream,
+ * and keeps the particle state resident on the device between calls.
+ *
+ * Each function names the reference interface it replaces (paths relative to the
+ * reference repository). All functions return 0 (TSP_OK) or a negative tsp_status; the
+ * message of the last failure is available from tsp_last_error(). A context is not
+ * re-entrant: one caller thread at a time, like the reference backend
+ * (core/simulation.h:179, one stream per backend at GPU/backend/gpu_backend.h:174).
+ */
+#ifndef TSP_HIP_H
+#define TSP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum tsp_status {
+    TSP_OK = 0,
+    TSP_ERR_INVALID = -1,   /* bad argument, null handle, size mismatch */
+    TSP_ERR_HIP = -2,       /* a HIP runtime call failed (GPU::Check, GPU/gpu_utils.h:24-57) */
+    TSP_ERR_STATE = -3,     /* call out of order (e.g. step before pull) */
+    TSP_ERR_CAPACITY = -4,  /* a count exceeds a fixed limit (u16 neighbor counts, 2^21 cells/axis) */
+    TSP_ERR_NUMERIC = -5    /* degenerate rigid orientation (cpu_backend.cpp:46-57) */
+} tsp_status;
+
+/* Fields of Sim::SimParams (core/parameters_sim.h:25-175) that the step reads. Derived values
+   follow the reference's setters: mass = rest_density*h*h*h and kernel_radius = 2h, with
+   kernel_radius_sq = kernel_radius*kernel_radius (core/parameters_sim.cpp:1478-1483). */
+typedef struct tsp_params {
+    float time_step;
+    float particle_size;
+    float kernel_radius;
+    float kernel_radius_sq;
+    float rest_density;
+    float mass;
+    float viscosity_fluid;
+    float viscosity_boundary;
+    float gravity[3];
+    float press_gamma;
+    float jacobi_omega;
+    float jacobi_warm_start;
+    float density_dev_target;
+    float grid_cell_size_ratio;
+    uint32_t fixed_iters;
+    uint32_t min_solver_iters;
+    uint32_t max_solver_iters;
+    uint32_t termination_cond; /* Base::SolverTerminationCond: 0 FixedIters, 1 Raw, 2 Pos, 3 Abs */
+} tsp_params;
+
+/* SoA field order of sync/state_sim.h: FluidParticles :101-131, BoundaryParticles :272-285,
+   RigidParticles :382-396, RigidBodies :497-522 (orientation and inverse inertia row-major) */
+enum {
+    TSP_F_X, TSP_F_Y, TSP_F_Z, TSP_F_VX, TSP_F_VY, TSP_F_VZ,
+    TSP_F_VPX, TSP_F_VPY, TSP_F_VPZ, TSP_F_APX, TSP_F_APY, TSP_F_APZ,
+    TSP_F_SOURCE, TSP_F_DIAG, TSP_F_PRESSURE, TSP_F_DENSITY, TSP_F_COUNT
+};
+enum { TSP_B_X, TSP_B_Y, TSP_B_Z, TSP_B_VOL, TSP_B_PSI, TSP_B_COUNT };
+enum { TSP_R_XREL, TSP_R_YREL, TSP_R_ZREL, TSP_R_VOL, TSP_R_PSI, TSP_R_COUNT };
+enum {
+    TSP_RB_XCM, TSP_RB_YCM, TSP_RB_ZCM, TSP_RB_VX, TSP_RB_VY, TSP_RB_VZ,
+    TSP_RB_WX, TSP_RB_WY, TSP_RB_WZ, TSP_RB_LX, TSP_RB_LY, TSP_RB_LZ,
+    TSP_RB_A00, TSP_RB_A01, TSP_RB_A02, TSP_RB_A10, TSP_RB_A11, TSP_RB_A12,
+    TSP_RB_A20, TSP_RB_A21, TSP_RB_A22, TSP_RB_MASS, TSP_RB_MASS_INV,
+    TSP_RB_I00, TSP_RB_I01, TSP_RB_I02, TSP_RB_I10, TSP_RB_I11, TSP_RB_I12,
+    TSP_RB_I20, TSP_RB_I21, TSP_RB_I22, TSP_RB_COUNT
+};
+
+/* Host view of a Sync::UniversalSimState (sync/state_sim.h:637): caller-owned arrays.
+   tsp_pull reads every non-null array (a null fluid field starts at zero, a null id array
+   takes sequential ids like ensureParticleIdsPresent, state_sim.h:219-233); tsp_push writes
+   every non-null array. */
+typedef struct tsp_state_view {
+    uint32_t n_fluid, n_boundary, n_rigid, n_bodies;
+    uint64_t *fluid_id;    float *fluid[TSP_F_COUNT];
+    uint64_t *boundary_id; float *boundary[TSP_B_COUNT];
+    uint64_t *rigid_id;    uint32_t *rigid_body; float *rigid[TSP_R_COUNT];
+    uint32_t *body_id;     uint32_t *body_count; float *body[TSP_RB_COUNT];
+    uint64_t step;
+    double time;
+} tsp_state_view;
+
+/* flags of tsp_step_info.flags */
+enum {
+    TSP_FLAG_MAX_ITERS = 1u << 0,      /* the last solve stopped at max_solver_iters unconverged */
+    TSP_FLAG_FRAME_REBUILT = 1u << 1,  /* the cell frame was rebuilt during the call */
+    TSP_FLAG_RIGID_CONTACT = 1u << 2,  /* rigid particles have boundary/rigid neighbors: the
+                                          reference CPU would run its contact solver here */
+    TSP_FLAG_LIST_REGROWN = 1u << 3    /* a neighbor list pool was regrown and the query re-run */
+};
+
+typedef struct tsp_step_info {
+    uint64_t step;                /* simStep_ */
+    double time;                  /* simTime_ */
+    uint32_t solver_iters;        /* N_solverIters_ of the last step (the init counts as one) */
+    uint32_t flags;
+    float avg_err_raw, avg_err_pos, avg_err_abs;   /* avgRelDensityErrorF_* */
+    double err_sum_raw, err_sum_pos, err_sum_abs;  /* FP64 error sums of the last sweep */
+    uint64_t sweeps_total;        /* Jacobi sweeps over all steps since the last pull */
+    uint32_t frame_builds;
+    int32_t frame_orig[3];
+    uint32_t frame_dim[3];
+    uint64_t n_ff, n_fb, n_fr, n_rf, n_rb, n_rr; /* neighbor list entries of the last search */
+    uint64_t list_bytes;          /* bytes of the SELL pools as stored */
+    float rigid_force_torque[6];  /* body 0: force and torque of the last step */
+} tsp_step_info;
+
+typedef struct tsp_ctx tsp_ctx;
+
+/* SimBackend_GPU constructor (GPU/backend/gpu_backend.cpp:69-77): binds a device, creates the
+   context's stream. */
+int tsp_create(int device, tsp_ctx **out);
+void tsp_destroy(tsp_ctx *ctx);
+/* message of the last failed call on this context ("" if none) */
+const char *tsp_last_error(const tsp_ctx *ctx);
+/* I_SimBackend::Name (core/backend_sim.h:200) */
+const char *tsp_name(void);
+
+/* the SimParams snapshot passed to ExecuteNeighborSearch/ExecuteFullSimStep
+   (core/backend_sim.h:249-252) */
+int tsp_set_params(tsp_ctx *ctx, const tsp_params *params);
+/* I_SimBackend::SetSimDomainBounds (core/backend_sim.h:204) */
+int tsp_set_domain(tsp_ctx *ctx, const float mn[3], const float mx[3]);
+/* I_SimBackend::PullFromUniversalSimState (core/backend_sim.h:207; gpu_backend.cpp:96-240) */
+int tsp_pull(tsp_ctx *ctx, const tsp_state_view *src);
+/* I_SimBackend::PushToUniversalSimState (core/backend_sim.h:211; gpu_backend.cpp:242-344) */
+int tsp_push(tsp_ctx *ctx, tsp_state_view *dst);
+/* I_SimBackend::ExecuteNeighborSearch (core/backend_sim.h:249; gpu_backend.cpp:1227-1272):
+   re-orders every fluid field, builds the cell tables and the SELL-32 lists */
+int tsp_neighbor_search(tsp_ctx *ctx);
+/* I_SimBackend::ExecuteFullSimStep (core/backend_sim.h:252; gpu_backend.cpp:1274-1594):
+   advances n_steps IISPH steps, neighbor search included */
+int tsp_step(tsp_ctx *ctx, uint32_t n_steps);
+/* the solver status the reference keeps in SimBackend_CPU/GPU members (N_solverIters_,
+   avgRelDensityErrorF_*, simStep_) */
+int tsp_get_step_info(tsp_ctx *ctx, tsp_step_info *info);
+/* I_SimBackend::GetNumFluidParticles/GetNumBoundaryParticles/GetNumRigidParticles
+   (core/backend_sim.h:258-260); counts[3] is the rigid body count */
+int tsp_get_counts(tsp_ctx *ctx, uint32_t counts[4]);
+
+/* Neighbor lists of the last search, decoded from the SELL-32 pools (the host decoder of
+   GPU/backend/gpu_backend.cpp:42-65). relation: 0 FF, 1 FB, 2 FR, 3 RF, 4 RB, 5 RR. counts has
+   one entry per source particle; idx receives the concatenated lists in iteration order when
+   cap is large enough; *total is the number of entries. */
+int tsp_get_neighbors(tsp_ctx *ctx, int relation, uint32_t *counts, uint32_t *idx,
+                      uint64_t cap, uint64_t *total);
+/* Sorted cell ids and the permutation of the last fluid re-order (old index of each new
+   index), and the boundary order and ids (NeighborSearch::TryGetLastParticleReOrder,
+   CPU/backend/neighbor_search.h:52-54). Null pointers are skipped. */
+int tsp_get_sort(tsp_ctx *ctx, uint32_t *order_f, uint64_t *keys_f, uint32_t *order_b,
+                 uint64_t *keys_b);
+/* world positions and velocities of the rigid particles (3 floats each), as updated by the
+   last step (K_UpdateRigidParticles, GPU/backend/rigid_kernels.cu:181-219) */
+int tsp_get_rigid_world(tsp_ctx *ctx, float *xyz, float *vel);
+
+/* Per-kernel timing with HIP events on the context's stream (the per-op benchmark of
+   eval/bench/benchmarks.h:31-38). While enabled, every launch is bracketed by events. */
+int tsp_profile_enable(tsp_ctx *ctx, int on);
+/* kernel names, launch counts and summed milliseconds since enabling; returns the number of
+   kernels, fills up to cap entries. names is cap x 48 chars. */
+int tsp_profile_read(tsp_ctx *ctx, char *names, uint64_t *launches, double *ms, int cap);
+/* waits for all queued work of the context */
+int tsp_synchronize(tsp_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,223 @@
+"""ctypes binding of the oracle (oracle/_build/liboracle.so) for the tests.
+
+The oracle is the CPU restatement of the reference TBB path (oracle/iisph_oracle.h); it is the
+checker of the HIP path and is used only by tests/, the smoke check and bench.py's cpu_baseline.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from turbospharticles_amd import _capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+
+FF, FB, FR, RF, RB, RR = range(6)
+STAGE_DENSITY, STAGE_NONPRESSURE, STAGE_SOURCE_DIAG, STAGE_PRESSURE_ACC, STAGE_JACOBI, STAGE_INTEGRATION = range(1, 7)
+
+_fp = ctypes.POINTER(ctypes.c_float)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = _capi.TspParams._fields_
+
+
+class OrcState(ctypes.Structure):
+    _fields_ = [f for f in _capi.TspStateView._fields_ if f[0] not in ("step", "time")]
+
+
+class OrcInfo(ctypes.Structure):
+    _fields_ = [
+        ("solver_iters", ctypes.c_uint32),
+        ("avg_err_raw", ctypes.c_float), ("avg_err_pos", ctypes.c_float), ("avg_err_abs", ctypes.c_float),
+        ("err_sum_raw", ctypes.c_double), ("err_sum_pos", ctypes.c_double), ("err_sum_abs", ctypes.c_double),
+        ("frame_orig", ctypes.c_int32 * 3), ("frame_dim", ctypes.c_uint32 * 3),
+        ("frame_builds", ctypes.c_uint32),
+        ("n_dict_chunks", ctypes.c_uint32), ("n_chunks", ctypes.c_uint32),
+        ("n_ff", ctypes.c_uint64), ("n_fb", ctypes.c_uint64), ("n_fr", ctypes.c_uint64),
+        ("n_rf", ctypes.c_uint64), ("n_rb", ctypes.c_uint64), ("n_rr", ctypes.c_uint64),
+        ("contact_pairs", ctypes.c_uint32),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_LIB):
+            build()
+        L = ctypes.CDLL(ORACLE_LIB)
+        vp = ctypes.c_void_p
+        for p in ("orc32", "orc64"):
+            sig = {
+                "create": (vp, []),
+                "destroy": (None, [vp]),
+                "set_params": (None, [vp, ctypes.POINTER(OrcParams)]),
+                "set_domain": (None, [vp, _fp, _fp]),
+                "set_chunk_width": (None, [vp, ctypes.c_uint32]),
+                "set_rigid_sums_f64": (None, [vp, ctypes.c_int]),
+                "set_threads": (None, [vp, ctypes.c_int]),
+                "pull": (ctypes.c_int, [vp, ctypes.POINTER(OrcState)]),
+                "push": (ctypes.c_int, [vp, ctypes.POINTER(OrcState)]),
+                "neighbor_search": (ctypes.c_int, [vp]),
+                "step": (ctypes.c_int, [vp]),
+                "get_info": (None, [vp, ctypes.POINTER(OrcInfo)]),
+                "get_list": (ctypes.c_uint64, [vp, ctypes.c_int, _u32p, _u32p, _u8p, ctypes.c_uint64]),
+                "get_order": (None, [vp, _u32p, _u64p, _u32p, _u64p]),
+                "get_rigid_world": (None, [vp, _fp, _fp]),
+                "get_body_force": (None, [vp, _fp]),
+                "kernel_weight": (None, [ctypes.POINTER(OrcParams), _fp, ctypes.c_uint32, _fp, _fp, _fp]),
+                "kernels_from_lists": (ctypes.c_int, [vp, _u32p, _u32p, _u8p, _u32p, _u32p, _u32p, _u32p,
+                                                      ctypes.c_int]),
+            }
+            for name, (res, args) in sig.items():
+                fn = getattr(L, f"{p}_{name}")
+                fn.restype = res
+                fn.argtypes = args
+        L.orc_jitter3d.restype = ctypes.c_float
+        L.orc_jitter3d.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_float]
+        L.orc_spread_bits_21to63.restype = ctypes.c_uint64
+        L.orc_spread_bits_21to63.argtypes = [ctypes.c_uint32]
+        _lib = L
+    return _lib
+
+
+def to_orc_params(p):
+    o = OrcParams()
+    ctypes.memmove(ctypes.byref(o), ctypes.byref(p), ctypes.sizeof(OrcParams))
+    return o
+
+
+class Oracle:
+    """One oracle context (prefix 'orc32' for the FP32 restatement, 'orc64' for FP64)."""
+
+    def __init__(self, params, *, precision=32, chunk_width=1, threads=None, rigid_f64=True):
+        self.L = lib()
+        self.p = f"orc{precision}"
+        self.h = self.fn("create")()
+        self.params = to_orc_params(params)
+        self.fn("set_params")(self.h, ctypes.byref(self.params))
+        self.fn("set_chunk_width")(self.h, chunk_width)
+        self.fn("set_rigid_sums_f64")(self.h, 1 if rigid_f64 else 0)
+        self.fn("set_threads")(self.h, threads or min(8, os.cpu_count() or 1))
+        self.n = None
+
+    def fn(self, name):
+        return getattr(self.L, f"{self.p}_{name}")
+
+    def __del__(self):
+        try:
+            self.fn("destroy")(self.h)
+        except Exception:
+            pass
+
+    def set_domain(self, mn, mx):
+        a = (ctypes.c_float * 3)(*mn)
+        b = (ctypes.c_float * 3)(*mx)
+        self.fn("set_domain")(self.h, a, b)
+
+    def _view(self, st):
+        v = st.view()
+        o = OrcState()
+        for name, _ in OrcState._fields_:
+            setattr(o, name, getattr(v, name))
+        return o
+
+    def pull(self, st):
+        self.n = st.n
+        if st.domain is not None:
+            self.set_domain(st.domain[0], st.domain[1])
+        o = self._view(st)
+        rc = self.fn("pull")(self.h, ctypes.byref(o))
+        assert rc == 0, rc
+
+    def push(self):
+        st = _capi.HostState(*self.n)
+        o = self._view(st)
+        rc = self.fn("push")(self.h, ctypes.byref(o))
+        assert rc == 0, rc
+        return st
+
+    def neighbor_search(self):
+        assert self.fn("neighbor_search")(self.h) == 0
+
+    def step(self):
+        rc = self.fn("step")(self.h)
+        assert rc == 0, rc
+
+    def info(self):
+        i = OrcInfo()
+        self.fn("get_info")(self.h, ctypes.byref(i))
+        return i
+
+    def lists(self, rel):
+        """(counts, concatenated indices, dict flags or None) in iteration order."""
+        n = self.n[0] if rel < 3 else self.n[2]
+        counts = np.zeros(n, np.uint32)
+        tot = self.fn("get_list")(self.h, rel, _capi.u32ptr(counts), None, None, 0)
+        idx = np.zeros(max(tot, 1), np.uint32)
+        dict_flags = np.zeros(n, np.uint8) if rel == FF else None
+        self.fn("get_list")(self.h, rel, _capi.u32ptr(counts), _capi.u32ptr(idx),
+                            None if dict_flags is None else dict_flags.ctypes.data_as(_u8p), tot)
+        return counts, idx[:tot], dict_flags
+
+    def order(self):
+        nF, nB = self.n[0], self.n[1]
+        of = np.zeros(nF, np.uint32)
+        kf = np.zeros(nF, np.uint64)
+        ob = np.zeros(nB, np.uint32)
+        kb = np.zeros(nB, np.uint64)
+        self.fn("get_order")(self.h, _capi.u32ptr(of), _capi.u64ptr(kf), _capi.u32ptr(ob), _capi.u64ptr(kb))
+        return of, kf, ob, kb
+
+    def rigid_world(self):
+        n = self.n[2]
+        x = np.zeros(3 * n, np.float32)
+        v = np.zeros(3 * n, np.float32)
+        self.fn("get_rigid_world")(self.h, _capi.fptr(x), _capi.fptr(v))
+        return x.reshape(-1, 3), v.reshape(-1, 3)
+
+    def body_force(self):
+        ft = np.zeros(6 * self.n[3], np.float32)
+        self.fn("get_body_force")(self.h, _capi.fptr(ft))
+        return ft.reshape(-1, 6)
+
+    def kernels_from_lists(self, ff, fb, fr, ff_dict, stage):
+        def csr(lists):
+            off = np.zeros(len(lists) + 1, np.uint32)
+            off[1:] = np.cumsum([len(x) for x in lists])
+            flat = np.array([j for x in lists for j in x] or [0], np.uint32)
+            return off, flat
+        o1, l1 = csr(ff)
+        o2, l2 = csr(fb)
+        o3, l3 = csr(fr)
+        d = np.asarray(ff_dict, np.uint8)
+        rc = self.fn("kernels_from_lists")(self.h, _capi.u32ptr(o1), _capi.u32ptr(l1), d.ctypes.data_as(_u8p),
+                                           _capi.u32ptr(o2), _capi.u32ptr(l2), _capi.u32ptr(o3),
+                                           _capi.u32ptr(l3), stage)
+        assert rc == 0
+
+
+def kernel_weight(params, offsets, precision=32):
+    L = lib()
+    off = np.ascontiguousarray(offsets, np.float32).reshape(-1, 3)
+    n = off.shape[0]
+    w = np.zeros(n, np.float32)
+    g = np.zeros(3 * n, np.float32)
+    wu = np.zeros(n, np.float32)
+    op = to_orc_params(params)
+    getattr(L, f"orc{precision}_kernel_weight")(ctypes.byref(op), _capi.fptr(off), n, _capi.fptr(w),
+                                                _capi.fptr(g), _capi.fptr(wu))
+    return w, g.reshape(-1, 3), wu

@@ -1,0 +1,137 @@
+"""Python mirror of Sim::I_SimBackend (core/backend_sim.h:194-261) over the C ABI of libtsp_hip.so.
+
+HipBackend is what tests, the smoke check and bench.py drive; the C++ SimBackend_HIP
+(host/sim_backend_hip.h) is the same adapter for C++ callers of the reference interface.
+Every method maps to one tsp_* entry point and raises RuntimeError with tsp_last_error() on
+failure, as the reference backends throw std::runtime_error (GPU/gpu_utils.h:24-57).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi
+
+
+class HipBackend:
+    def __init__(self, device=0):
+        self.lib = _capi.hip_lib()
+        h = ctypes.c_void_p()
+        rc = self.lib.tsp_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"tsp_create failed ({rc}): no HIP device {device}")
+        self.h = h
+        self.n = None
+
+    def close(self):
+        if self.h:
+            self.lib.tsp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.tsp_last_error(self.h).decode()
+            raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+    # I_SimBackend::Name
+    def name(self):
+        return self.lib.tsp_name().decode()
+
+    def set_params(self, params):
+        self._check(self.lib.tsp_set_params(self.h, ctypes.byref(params)), "tsp_set_params")
+
+    # I_SimBackend::SetSimDomainBounds
+    def set_domain(self, mn, mx):
+        a = (ctypes.c_float * 3)(*[float(v) for v in mn])
+        b = (ctypes.c_float * 3)(*[float(v) for v in mx])
+        self._check(self.lib.tsp_set_domain(self.h, a, b), "tsp_set_domain")
+
+    # I_SimBackend::PullFromUniversalSimState
+    def pull(self, state):
+        if state.domain is not None:
+            self.set_domain(*state.domain)
+        v = state.view()
+        self._check(self.lib.tsp_pull(self.h, ctypes.byref(v)), "tsp_pull")
+        self.n = state.n
+
+    # I_SimBackend::PushToUniversalSimState
+    def push(self):
+        st = _capi.HostState(*self.n)
+        v = st.view()
+        self._check(self.lib.tsp_push(self.h, ctypes.byref(v)), "tsp_push")
+        st.step, st.time = v.step, v.time
+        return st
+
+    # I_SimBackend::ExecuteNeighborSearch
+    def neighbor_search(self):
+        self._check(self.lib.tsp_neighbor_search(self.h), "tsp_neighbor_search")
+
+    # I_SimBackend::ExecuteFullSimStep
+    def step(self, n_steps=1):
+        self._check(self.lib.tsp_step(self.h, n_steps), "tsp_step")
+
+    def info(self):
+        i = _capi.TspStepInfo()
+        self._check(self.lib.tsp_get_step_info(self.h, ctypes.byref(i)), "tsp_get_step_info")
+        return i
+
+    # I_SimBackend::GetNumFluidParticles / Boundary / Rigid (+ bodies)
+    def counts(self):
+        c = (ctypes.c_uint32 * 4)()
+        self._check(self.lib.tsp_get_counts(self.h, c), "tsp_get_counts")
+        return tuple(c)
+
+    def neighbors(self, rel):
+        """(counts, concatenated indices) of relation rel (0 FF .. 5 RR) in iteration order."""
+        n = self.n[0] if rel < 3 else self.n[2]
+        counts = np.zeros(n, np.uint32)
+        total = ctypes.c_uint64()
+        self._check(self.lib.tsp_get_neighbors(self.h, rel, _capi.u32ptr(counts), None, 0, ctypes.byref(total)),
+                    "tsp_get_neighbors")
+        if rel in (4, 5):
+            return counts, None
+        idx = np.zeros(max(total.value, 1), np.uint32)
+        self._check(self.lib.tsp_get_neighbors(self.h, rel, _capi.u32ptr(counts), _capi.u32ptr(idx), total.value,
+                                               ctypes.byref(total)), "tsp_get_neighbors")
+        return counts, idx[:total.value]
+
+    def sort_state(self):
+        nF, nB = self.n[0], self.n[1]
+        of = np.zeros(nF, np.uint32)
+        kf = np.zeros(nF, np.uint64)
+        kb = np.zeros(nB, np.uint64)
+        self._check(self.lib.tsp_get_sort(self.h, _capi.u32ptr(of), _capi.u64ptr(kf), None, _capi.u64ptr(kb)),
+                    "tsp_get_sort")
+        return of, kf, kb
+
+    def rigid_world(self):
+        n = self.n[2]
+        x = np.zeros(3 * max(n, 1), np.float32)
+        v = np.zeros(3 * max(n, 1), np.float32)
+        self._check(self.lib.tsp_get_rigid_world(self.h, _capi.fptr(x), _capi.fptr(v)), "tsp_get_rigid_world")
+        return x[:3 * n].reshape(-1, 3), v[:3 * n].reshape(-1, 3)
+
+    def profile_enable(self, on=True):
+        self._check(self.lib.tsp_profile_enable(self.h, 1 if on else 0), "tsp_profile_enable")
+
+    def profile_read(self, cap=64):
+        names = ctypes.create_string_buffer(48 * cap)
+        launches = np.zeros(cap, np.uint64)
+        ms = np.zeros(cap, np.float64)
+        n = self.lib.tsp_profile_read(self.h, names, _capi.u64ptr(launches),
+                                      ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), cap)
+        if n < 0:
+            self._check(n, "tsp_profile_read")
+        out = {}
+        for q in range(min(n, cap)):
+            nm = names.raw[48 * q:48 * (q + 1)].split(b"\0", 1)[0].decode()
+            out[nm] = (int(launches[q]), float(ms[q]))
+        return out
+
+    def synchronize(self):
+        self._check(self.lib.tsp_synchronize(self.h), "tsp_synchronize")

@@ -1,0 +1,726 @@
+// kernels_search.hip -- neighbor search of the IISPH step on gfx950.
+//
+// Re-order (DataReOrder_SolverStep), Construct and Query of the reference's compact
+// cell-linked list, written for wave64:
+//   keys      Morton cell id of every particle in the local cell frame, with the frame
+//             hysteresis test of UpdateCellFrame (CPU/backend/neighbor_search.cpp:1902-1960)
+//   sort      stable LSD radix sort of (key, index), 8-bit digits, keys clamped so that
+//             out-of-frame particles sort last (neighbor_search.cpp:403-564); per tile the
+//             stable ranks come from wave64 ballots, not from a serial scan
+//   permute   one gather of id, x, y, z, vx, vy, vz (+ pressure with warm start) per particle
+//   cells     dense {begin, end} table over the frame, written at the runs of the sorted keys
+//   count     per particle: FF/FB/FR counts over the 27-cell stencil (inclusive cutoff,
+//             distance fma(dx,dx,fma(dy,dy,dz*dz)), neighbor_search.cpp:1130-1249); per
+//             32-particle slice: pitch, base and u16/u32 format by width-32 lane reductions
+//   scan      per relation: offsets of the slices in their pool (pitch * 32 entries each)
+//   fill      writes the entries in the reference's iteration order: z-y-x stencil raster,
+//             ascending index within a cell; FF lists whose indices span more than 0xffff keep
+//             the DictDelta16x2 order of the reference's per-particle dict chunks (low group
+//             first, high group reversed, CPU/backend/neighbor_lists.h:171-197)
+#include "tsp_device.h"
+
+namespace tsp
+{
+    // ----------------------------------------------------------------------------------------
+    // keys
+
+    __device__ __forceinline__ bool insideDomain(const Frame& f, float x, float y, float z)
+    {
+        if (!f.domainSet) { return true; }
+        return x >= f.domMin[0] && x <= f.domMax[0] && y >= f.domMin[1] && y <= f.domMax[1]
+            && z >= f.domMin[2] && z <= f.domMax[2];
+    }
+
+    // cell id of one position; sets the abort flag when an in-domain particle left the inner frame
+    __device__ __forceinline__ uint32_t cellKey(const Frame& f, float x, float y, float z, bool check, Status* st)
+    {
+        const int cx = floorToInt(x * f.cellSizeInv);
+        const int cy = floorToInt(y * f.cellSizeInv);
+        const int cz = floorToInt(z * f.cellSizeInv);
+        if (check && insideDomain(f, x, y, z))
+        {
+            if (cx < f.innerLo[0] || cx > f.innerHi[0] || cy < f.innerLo[1] || cy > f.innerHi[1]
+                || cz < f.innerLo[2] || cz > f.innerHi[2])
+            {
+                atomicOr(&st->abort, 1u);
+            }
+        }
+        const int lx = cx - f.orig[0], ly = cy - f.orig[1], lz = cz - f.orig[2];
+        if (lx < 0 || lx >= static_cast<int>(f.dim[0]) || ly < 0 || ly >= static_cast<int>(f.dim[1])
+            || lz < 0 || lz >= static_cast<int>(f.dim[2]))
+        {
+            return f.emptyKey;
+        }
+        return morton(static_cast<uint32_t>(lx), static_cast<uint32_t>(ly), static_cast<uint32_t>(lz));
+    }
+
+    __global__ void __launch_bounds__(kBlock) kKeys(const float* __restrict__ x, const float* __restrict__ y,
+        const float* __restrict__ z, uint32_t n, const Frame* __restrict__ frame, Status* st, bool check,
+        uint32_t* __restrict__ key, uint32_t* __restrict__ idx)
+    {
+        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= n) { return; }
+        const Frame f = *frame;
+        key[i] = cellKey(f, x[i], y[i], z[i], check, st);
+        idx[i] = i;
+    }
+
+    // boundary keys; flags a descent so the host knows whether the order needs a sort
+    __global__ void __launch_bounds__(kBlock) kKeysSorted(const float* __restrict__ x, const float* __restrict__ y,
+        const float* __restrict__ z, uint32_t n, const Frame* __restrict__ frame, Status* st,
+        uint32_t* __restrict__ key, uint32_t* __restrict__ idx)
+    {
+        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= n) { return; }
+        const Frame f = *frame;
+        const uint32_t k = cellKey(f, x[i], y[i], z[i], false, st);
+        key[i] = k;
+        idx[i] = i;
+        if (i > 0 && cellKey(f, x[i - 1], y[i - 1], z[i - 1], false, st) > k) { atomicOr(&st->bUnsorted, 1u); }
+    }
+
+    // ----------------------------------------------------------------------------------------
+    // stable LSD radix sort, 8-bit digits, tiles of 2048 keys
+
+    constexpr uint32_t kTile = 2048;
+    constexpr uint32_t kRounds = kTile / kBlock;
+
+    __global__ void __launch_bounds__(kBlock) kRadixHist(const uint32_t* __restrict__ key, uint32_t n, uint32_t shift,
+        uint32_t nTiles, uint32_t* __restrict__ hist)
+    {
+        __shared__ uint32_t cnt[256];
+        cnt[threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t base = blockIdx.x * kTile;
+        for (uint32_t r = 0; r < kRounds; r++)
+        {
+            const uint32_t i = base + r * kBlock + threadIdx.x;
+            if (i < n) { atomicAdd(&cnt[(key[i] >> shift) & 255u], 1u); }
+        }
+        __syncthreads();
+        hist[threadIdx.x * nTiles + blockIdx.x] = cnt[threadIdx.x];
+    }
+
+    // exclusive scan of m values in one block of 1024 threads (digit-major histogram)
+    __global__ void __launch_bounds__(1024) kScanSingle(uint32_t* __restrict__ v, uint32_t m)
+    {
+        __shared__ uint32_t part[1024];
+        const uint32_t t = threadIdx.x;
+        const uint32_t per = (m + 1023u) / 1024u;
+        const uint32_t b = t * per, e = min(b + per, m);
+        uint32_t s = 0;
+        for (uint32_t i = b; i < e; i++) { s += v[i]; }
+        part[t] = s;
+        __syncthreads();
+        for (uint32_t off = 1; off < 1024; off <<= 1)
+        {
+            const uint32_t a = t >= off ? part[t - off] : 0u;
+            __syncthreads();
+            part[t] += a;
+            __syncthreads();
+        }
+        uint32_t run = part[t] - s;
+        for (uint32_t i = b; i < e; i++)
+        {
+            const uint32_t c = v[i];
+            v[i] = run;
+            run += c;
+        }
+    }
+
+    __global__ void __launch_bounds__(kBlock) kRadixScatter(const uint32_t* __restrict__ keyIn,
+        const uint32_t* __restrict__ idxIn, uint32_t* __restrict__ keyOut, uint32_t* __restrict__ idxOut, uint32_t n,
+        uint32_t shift, uint32_t nTiles, const uint32_t* __restrict__ offsets)
+    {
+        __shared__ uint32_t run[256];
+        __shared__ uint32_t waveCnt[kBlock / 64][256];
+        const uint32_t t = threadIdx.x;
+        const uint32_t wave = t >> 6, lane = t & 63u;
+        run[t] = offsets[t * nTiles + blockIdx.x];
+        for (uint32_t w = 0; w < kBlock / 64; w++) { waveCnt[w][t] = 0; }
+        __syncthreads();
+        const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+        const uint32_t base = blockIdx.x * kTile;
+        for (uint32_t r = 0; r < kRounds; r++)
+        {
+            const uint32_t i = base + r * kBlock + t;
+            const bool valid = i < n;
+            const uint32_t k = valid ? keyIn[i] : 0u;
+            const uint32_t d = (k >> shift) & 255u;
+            // lanes of this wave holding the same digit
+            uint64_t match = __ballot(valid);
+            for (uint32_t bit = 0; bit < 8; bit++)
+            {
+                const bool set = (d >> bit) & 1u;
+                const uint64_t bs = __ballot(set);
+                match &= set ? bs : ~bs;
+            }
+            const uint32_t rankInWave = __popcll(match & lanesBelow);
+            if (valid && rankInWave == 0) { waveCnt[wave][d] = __popcll(match); }
+            __syncthreads();
+            uint32_t rank = 0;
+            if (valid)
+            {
+                rank = run[d] + rankInWave;
+                for (uint32_t w = 0; w < wave; w++) { rank += waveCnt[w][d]; }
+            }
+            __syncthreads();
+            uint32_t tot = 0;
+            for (uint32_t w = 0; w < kBlock / 64; w++) { tot += waveCnt[w][t]; waveCnt[w][t] = 0; }
+            run[t] += tot;
+            __syncthreads();
+            if (valid)
+            {
+                keyOut[rank] = k;
+                idxOut[rank] = idxIn[i];
+            }
+        }
+    }
+
+    // ----------------------------------------------------------------------------------------
+    // permutation (gather into the second buffers)
+
+    // gathers up to 16 float fields and the ids by the sorted permutation
+    __global__ void __launch_bounds__(kBlock) kPermute(FieldSet fs, const uint32_t* __restrict__ idx)
+    {
+        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= fs.n) { return; }
+        const uint32_t o = idx[i];
+        for (uint32_t f = 0; f < fs.nFields; f++) { fs.dst[f][i] = fs.src[f][o]; }
+        if (fs.idSrc) { fs.idDst[i] = fs.idSrc[o]; }
+    }
+
+    // ----------------------------------------------------------------------------------------
+    // cell tables
+
+    __global__ void __launch_bounds__(kBlock) kClearCells(uint2* __restrict__ cell, const uint32_t* __restrict__ occ,
+        const uint32_t* __restrict__ nOcc)
+    {
+        const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+        if (q < *nOcc) { cell[occ[q]] = make_uint2(0u, 0u); }
+    }
+
+    __global__ void kSetU32(uint32_t* p, uint32_t v) { *p = v; }
+
+    __global__ void __launch_bounds__(kBlock) kClearAll(uint2* __restrict__ cell, uint64_t n)
+    {
+        const uint64_t q = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+        if (q < n) { cell[q] = make_uint2(0u, 0u); }
+    }
+
+    __global__ void __launch_bounds__(kBlock) kBuildCells(const uint32_t* __restrict__ key, uint32_t n,
+        const Frame* __restrict__ frame, uint2* __restrict__ cell, uint32_t* __restrict__ occ, uint32_t* nOcc)
+    {
+        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= n) { return; }
+        const Frame& f = *frame;
+        const uint32_t k = key[i];
+        if (k == f.emptyKey) { return; }
+        const uint32_t lin = linearCell(f, compact10(k), compact10(k >> 1), compact10(k >> 2));
+        if (i == 0 || key[i - 1] != k)
+        {
+            cell[lin].x = i;
+            occ[atomicAdd(nOcc, 1u)] = lin;
+        }
+        if (i + 1 == n || key[i + 1] != k) { cell[lin].y = i + 1; }
+    }
+
+    // ----------------------------------------------------------------------------------------
+    // query: one thread per fluid particle (sorted order), 27-cell stencil in z-y-x raster order
+
+    struct Stencil
+    {
+        int lo[3], hi[3];
+    };
+
+    __device__ __forceinline__ bool stencilOf(const Frame& f, uint32_t k, int reach, Stencil& s)
+    {
+        if (k == f.emptyKey) { return false; }
+        const int l[3] = { static_cast<int>(compact10(k)), static_cast<int>(compact10(k >> 1)),
+                           static_cast<int>(compact10(k >> 2)) };
+        for (int a = 0; a < 3; a++)
+        {
+            s.lo[a] = max(l[a] - reach, 0);
+            s.hi[a] = min(l[a] + reach, static_cast<int>(f.dim[a]) - 1);
+        }
+        return true;
+    }
+
+    __device__ __forceinline__ bool within(float xi, float yi, float zi, float xj, float yj, float zj, float r2)
+    {
+        const float dx = xi - xj, dy = yi - yj, dz = zi - zj;
+        return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) <= r2;
+    }
+
+    // reductions over the 32 lanes of one SELL slice (the two halves of a wave64 are separate)
+    __device__ __forceinline__ uint32_t sliceMax(uint32_t v)
+    {
+        for (int o = 16; o > 0; o >>= 1) { v = max(v, static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o, 32))); }
+        return v;
+    }
+
+    __device__ __forceinline__ uint32_t sliceMin(uint32_t v)
+    {
+        for (int o = 16; o > 0; o >>= 1) { v = min(v, static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o, 32))); }
+        return v;
+    }
+
+    __global__ void __launch_bounds__(kBlock) kQueryCount(Dev d, float r2, int reach, uint32_t nSlices)
+    {
+        const uint32_t blk = xcdBlock();
+        const uint32_t i = blk * kBlock + threadIdx.x;
+        const Frame& f = *d.frame;
+        uint32_t cnt[3] = { 0, 0, 0 };
+        uint32_t mn[3] = { kEmpty32, kEmpty32, kEmpty32 };
+        uint32_t mx[3] = { 0, 0, 0 };
+        Stencil s;
+        if (i < d.nF && stencilOf(f, d.keyF[i], reach, s))
+        {
+            const float xi = d.x[i], yi = d.y[i], zi = d.z[i];
+            for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
+            {
+                for (int cy = s.lo[1]; cy <= s.hi[1]; cy++)
+                {
+                    for (int cx = s.lo[0]; cx <= s.hi[0]; cx++)
+                    {
+                        const uint32_t lin = linearCell(f, cx, cy, cz);
+                        const uint2 rf = d.cellF[lin];
+                        for (uint32_t j = rf.x; j < rf.y; j++)
+                        {
+                            if (within(xi, yi, zi, d.x[j], d.y[j], d.z[j], r2))
+                            {
+                                cnt[0]++; mn[0] = min(mn[0], j); mx[0] = max(mx[0], j);
+                            }
+                        }
+                        const uint2 rb = d.cellB[lin];
+                        for (uint32_t j = rb.x; j < rb.y; j++)
+                        {
+                            if (within(xi, yi, zi, d.bx[j], d.by[j], d.bz[j], r2))
+                            {
+                                cnt[1]++; mn[1] = min(mn[1], j); mx[1] = max(mx[1], j);
+                            }
+                        }
+                        if (d.nR)
+                        {
+                            const uint2 rr = d.cellR[lin];
+                            for (uint32_t t = rr.x; t < rr.y; t++)
+                            {
+                                const uint32_t j = d.idxR[t];
+                                if (within(xi, yi, zi, d.rx[j], d.ry[j], d.rz[j], r2))
+                                {
+                                    cnt[2]++; mn[2] = min(mn[2], j); mx[2] = max(mx[2], j);
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        if (i < d.nF)
+        {
+            const bool wide = cnt[0] > 0 && mx[0] - mn[0] > 0xffffu;
+            d.cnt[REL_FF][i] = static_cast<uint16_t>(cnt[0] | (wide ? kWideSpanBit : 0u));
+            d.cnt[REL_FB][i] = static_cast<uint16_t>(cnt[1]);
+            d.cnt[REL_FR][i] = static_cast<uint16_t>(cnt[2]);
+            if (wide) { d.ffWideMin[i] = mn[0]; }
+            if (cnt[0] >= kWideSpanBit || cnt[1] > 0xffffu || cnt[2] > 0xffffu) { atomicOr(&d.status->overflow, 2u); }
+        }
+        // per slice: pitch, base (min index), format
+        const uint32_t slice = i / kSlice;
+        for (int r = 0; r < 3; r++)
+        {
+            const uint32_t pitch = sliceMax(cnt[r]);
+            const uint32_t smin = sliceMin(mn[r]);
+            const uint32_t smax = sliceMax(mx[r]);
+            if ((threadIdx.x & (kSlice - 1)) == 0 && slice < nSlices)
+            {
+                SliceMeta m;
+                m.pitch = static_cast<uint16_t>(pitch);
+                const bool delta = pitch == 0 || smax - smin <= 0xffffu;
+                m.format = delta ? kFmtDelta16 : kFmtAbs32;
+                m.base = (pitch == 0 || !delta) ? 0u : smin;
+                m.offset = pitch * kSlice * (delta ? 1u : 2u); // storage units; the scan turns this into an offset
+                m.pad = 0;
+                d.meta[r][slice] = m;
+            }
+        }
+    }
+
+    // offsets of the slices of the three relations; one block per relation
+    __global__ void __launch_bounds__(1024) kScanSlices(Dev d, uint32_t nSlices)
+    {
+        __shared__ uint64_t part[1024];
+        const uint32_t rel = blockIdx.x;
+        SliceMeta* meta = d.meta[rel];
+        const uint32_t t = threadIdx.x;
+        const uint32_t per = (nSlices + 1023u) / 1024u;
+        const uint32_t b = t * per, e = min(b + per, nSlices);
+        uint64_t s = 0;
+        for (uint32_t q = b; q < e; q++) { s += meta[q].offset; }
+        part[t] = s;
+        __syncthreads();
+        for (uint32_t off = 1; off < 1024; off <<= 1)
+        {
+            const uint64_t a = t >= off ? part[t - off] : 0ull;
+            __syncthreads();
+            part[t] += a;
+            __syncthreads();
+        }
+        uint64_t run = part[t] - s;
+        for (uint32_t q = b; q < e; q++)
+        {
+            const uint32_t units = meta[q].offset;
+            meta[q].offset = static_cast<uint32_t>(run);
+            run += units;
+        }
+        if (t == 1023)
+        {
+            d.status->poolUnits[rel] = part[1023];
+            if (part[1023] > d.poolCap[rel] || part[1023] > 0xffffffffull) { atomicOr(&d.status->overflow, 1u); }
+        }
+    }
+
+    // counts of the entries without padding (for reports)
+    __global__ void __launch_bounds__(kBlock) kCountEntries(Dev d)
+    {
+        __shared__ uint32_t red[3][kBlock];
+        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+        for (int r = 0; r < 3; r++)
+        {
+            red[r][threadIdx.x] = i < d.nF ? (d.cnt[r][i] & (r == REL_FF ? 0x7fffu : 0xffffu)) : 0u;
+        }
+        __syncthreads();
+        for (uint32_t o = kBlock / 2; o > 0; o >>= 1)
+        {
+            if (threadIdx.x < o) { for (int r = 0; r < 3; r++) { red[r][threadIdx.x] += red[r][threadIdx.x + o]; } }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+        {
+            for (int r = 0; r < 3; r++)
+            {
+                atomicAdd(reinterpret_cast<unsigned long long*>(&d.status->entries[r]),
+                          static_cast<unsigned long long>(red[r][0]));
+            }
+        }
+    }
+
+    __global__ void __launch_bounds__(kBlock) kQueryFill(Dev d, float r2, int reach)
+    {
+        const uint32_t blk = xcdBlock();
+        const uint32_t i = blk * kBlock + threadIdx.x;
+        if (i >= d.nF) { return; }
+        const Frame& f = *d.frame;
+        Stencil s;
+        if (!stencilOf(f, d.keyF[i], reach, s)) { return; }
+        const uint32_t lane = i & (kSlice - 1), slice = i / kSlice;
+        const SliceMeta mF = d.meta[REL_FF][slice], mB = d.meta[REL_FB][slice], mR = d.meta[REL_FR][slice];
+        const float xi = d.x[i], yi = d.y[i], zi = d.z[i];
+        const uint32_t cF = d.cnt[REL_FF][i];
+        const bool wide = (cF & kWideSpanBit) != 0u;
+        uint32_t kF = 0, kB = 0, kR = 0;
+        for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
+        {
+            for (int cy = s.lo[1]; cy <= s.hi[1]; cy++)
+            {
+                for (int cx = s.lo[0]; cx <= s.hi[0]; cx++)
+                {
+                    const uint32_t lin = linearCell(f, cx, cy, cz);
+                    if (!wide)
+                    {
+                        const uint2 rf = d.cellF[lin];
+                        for (uint32_t j = rf.x; j < rf.y; j++)
+                        {
+                            if (within(xi, yi, zi, d.x[j], d.y[j], d.z[j], r2)) { sellWrite(d.pool[REL_FF], mF, lane, kF++, j); }
+                        }
+                    }
+                    const uint2 rb = d.cellB[lin];
+                    for (uint32_t j = rb.x; j < rb.y; j++)
+                    {
+                        if (within(xi, yi, zi, d.bx[j], d.by[j], d.bz[j], r2)) { sellWrite(d.pool[REL_FB], mB, lane, kB++, j); }
+                    }
+                    if (d.nR)
+                    {
+                        const uint2 rr = d.cellR[lin];
+                        for (uint32_t t = rr.x; t < rr.y; t++)
+                        {
+                            const uint32_t j = d.idxR[t];
+                            if (within(xi, yi, zi, d.rx[j], d.ry[j], d.rz[j], r2)) { sellWrite(d.pool[REL_FR], mR, lane, kR++, j); }
+                        }
+                    }
+                }
+            }
+        }
+        if (!wide) { return; }
+
+        // DictDelta16x2 split of a per-particle dict (neighbor_lists.cpp:838-858): the neighbors
+        // within 0xffff of the smallest form the low group, the rest the high group, which is
+        // stored from the back when it fits 0xffff itself, else the order stays the candidate order
+        const uint32_t n = cF & 0x7fffu;
+        const uint32_t minRef = d.ffWideMin[i];
+        uint32_t base2 = kEmpty32, maxRef = 0;
+        for (int pass = 0; pass < 3; pass++)
+        {
+            uint32_t kLow = 0, kHigh = n;
+            for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
+            {
+                for (int cy = s.lo[1]; cy <= s.hi[1]; cy++)
+                {
+                    for (int cx = s.lo[0]; cx <= s.hi[0]; cx++)
+                    {
+                        const uint2 rf = d.cellF[linearCell(f, cx, cy, cz)];
+                        for (uint32_t j = rf.x; j < rf.y; j++)
+                        {
+                            if (!within(xi, yi, zi, d.x[j], d.y[j], d.z[j], r2)) { continue; }
+                            const bool low = j - minRef <= 0xffffu;
+                            if (pass == 0)
+                            {
+                                maxRef = max(maxRef, j);
+                                if (!low) { base2 = min(base2, j); }
+                            }
+                            else if (pass == 1)
+                            {
+                                // candidate order (DictAbs32)
+                                sellWrite(d.pool[REL_FF], mF, lane, kLow++, j);
+                            }
+                            else
+                            {
+                                if (low) { sellWrite(d.pool[REL_FF], mF, lane, kLow++, j); }
+                                else { sellWrite(d.pool[REL_FF], mF, lane, --kHigh, j); }
+                            }
+                        }
+                    }
+                }
+            }
+            if (pass == 0 && maxRef - base2 <= 0xffffu) { pass = 1; } // skip the candidate-order pass
+            else if (pass == 1) { break; }
+        }
+    }
+
+    // ----------------------------------------------------------------------------------------
+    // rigid-source lists (RF CSR, RB/RR counts), one thread per rigid particle
+
+    __global__ void __launch_bounds__(kBlock) kQueryRigid(Dev d, float r2, int reach, bool fill)
+    {
+        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= d.nR) { return; }
+        const Frame& f = *d.frame;
+        const float xi = d.rx[i], yi = d.ry[i], zi = d.rz[i];
+        const int cx = floorToInt(xi * f.cellSizeInv) - f.orig[0];
+        const int cy = floorToInt(yi * f.cellSizeInv) - f.orig[1];
+        const int cz = floorToInt(zi * f.cellSizeInv) - f.orig[2];
+        uint32_t nF = 0, nB = 0, nRR = 0;
+        uint32_t w = fill ? d.rfOff[i] : 0u;
+        const bool inside = cx >= 0 && cx < static_cast<int>(f.dim[0]) && cy >= 0 && cy < static_cast<int>(f.dim[1])
+                         && cz >= 0 && cz < static_cast<int>(f.dim[2]);
+        if (inside)
+        {
+            const int lo[3] = { max(cx - reach, 0), max(cy - reach, 0), max(cz - reach, 0) };
+            const int hi[3] = { min(cx + reach, static_cast<int>(f.dim[0]) - 1), min(cy + reach, static_cast<int>(f.dim[1]) - 1),
+                                min(cz + reach, static_cast<int>(f.dim[2]) - 1) };
+            const uint32_t body = d.rbody[i];
+            for (int z = lo[2]; z <= hi[2]; z++)
+            {
+                for (int y = lo[1]; y <= hi[1]; y++)
+                {
+                    for (int x = lo[0]; x <= hi[0]; x++)
+                    {
+                        const uint32_t lin = linearCell(f, x, y, z);
+                        const uint2 rf = d.cellF[lin];
+                        for (uint32_t j = rf.x; j < rf.y; j++)
+                        {
+                            if (within(xi, yi, zi, d.x[j], d.y[j], d.z[j], r2))
+                            {
+                                if (fill) { d.rfIdx[w++] = j; } else { nF++; }
+                            }
+                        }
+                        if (fill) { continue; }
+                        const uint2 rb = d.cellB[lin];
+                        for (uint32_t j = rb.x; j < rb.y; j++)
+                        {
+                            if (within(xi, yi, zi, d.bx[j], d.by[j], d.bz[j], r2)) { nB++; }
+                        }
+                        const uint2 rr = d.cellR[lin];
+                        for (uint32_t t = rr.x; t < rr.y; t++)
+                        {
+                            const uint32_t j = d.idxR[t];
+                            if (j == i || d.rbody[j] == body) { continue; }
+                            if (within(xi, yi, zi, d.rx[j], d.ry[j], d.rz[j], r2)) { nRR++; }
+                        }
+                    }
+                }
+            }
+        }
+        if (!fill)
+        {
+            d.rfCnt[i] = nF;
+            d.rbCnt[i] = nB;
+            d.rrCnt[i] = nRR;
+        }
+    }
+
+    // exclusive scan of the RF counts into offsets plus RB/RR totals (one block)
+    __global__ void __launch_bounds__(1024) kScanRigid(Dev d)
+    {
+        __shared__ uint64_t part[3][1024];
+        const uint32_t t = threadIdx.x, n = d.nR;
+        const uint32_t per = (n + 1023u) / 1024u;
+        const uint32_t b = t * per, e = min(b + per, n);
+        uint64_t s = 0, sb = 0, sr = 0;
+        for (uint32_t q = b; q < e; q++) { s += d.rfCnt[q]; sb += d.rbCnt[q]; sr += d.rrCnt[q]; }
+        part[0][t] = s; part[1][t] = sb; part[2][t] = sr;
+        __syncthreads();
+        for (uint32_t off = 1; off < 1024; off <<= 1)
+        {
+            uint64_t a[3];
+            for (int r = 0; r < 3; r++) { a[r] = t >= off ? part[r][t - off] : 0ull; }
+            __syncthreads();
+            for (int r = 0; r < 3; r++) { part[r][t] += a[r]; }
+            __syncthreads();
+        }
+        uint64_t run = part[0][t] - s;
+        for (uint32_t q = b; q < e; q++)
+        {
+            d.rfOff[q] = static_cast<uint32_t>(run);
+            run += d.rfCnt[q];
+        }
+        if (t == 1023)
+        {
+            d.rfOff[n] = static_cast<uint32_t>(part[0][1023]);
+            d.status->rfTotal = part[0][1023];
+            d.status->rbTotal = part[1][1023];
+            d.status->rrTotal = part[2][1023];
+            if (part[0][1023] > d.rfCap) { atomicOr(&d.status->overflow, 4u); }
+        }
+    }
+
+    // scene AABB of the positions of one particle class (ComputeSceneAABB_ParticleClass,
+    // neighbor_search.cpp:227-258); min/max are exact in any order
+    __device__ __forceinline__ uint32_t orderedBits(float v)
+    {
+        const uint32_t u = __float_as_uint(v);
+        return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    }
+
+    __global__ void __launch_bounds__(kBlock) kAabb(const float* __restrict__ x, const float* __restrict__ y,
+        const float* __restrict__ z, uint32_t n, uint32_t* __restrict__ out)
+    {
+        __shared__ uint32_t red[6][kBlock];
+        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+        uint32_t v[6] = { 0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u };
+        if (i < n)
+        {
+            const float p[3] = { x[i], y[i], z[i] };
+            for (int a = 0; a < 3; a++) { v[a] = orderedBits(p[a]); v[3 + a] = v[a]; }
+        }
+        for (int a = 0; a < 6; a++) { red[a][threadIdx.x] = v[a]; }
+        __syncthreads();
+        for (uint32_t o = kBlock / 2; o > 0; o >>= 1)
+        {
+            if (threadIdx.x < o)
+            {
+                for (int a = 0; a < 3; a++)
+                {
+                    red[a][threadIdx.x] = min(red[a][threadIdx.x], red[a][threadIdx.x + o]);
+                    red[3 + a][threadIdx.x] = max(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + o]);
+                }
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+        {
+            for (int a = 0; a < 3; a++) { atomicMin(&out[a], red[a][0]); atomicMax(&out[3 + a], red[3 + a][0]); }
+        }
+    }
+
+    // ----------------------------------------------------------------------------------------
+    // host launchers
+
+    static inline uint32_t blocksFor(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
+
+    void launchKeys(hipStream_t s, const float* x, const float* y, const float* z, uint32_t n, const Frame* f,
+                    Status* st, bool check, uint32_t* key, uint32_t* idx)
+    {
+        if (n) { kKeys<<<blocksFor(n), kBlock, 0, s>>>(x, y, z, n, f, st, check, key, idx); }
+    }
+
+    void launchKeysSorted(hipStream_t s, const float* x, const float* y, const float* z, uint32_t n, const Frame* f,
+                          Status* st, uint32_t* key, uint32_t* idx)
+    {
+        if (n) { kKeysSorted<<<blocksFor(n), kBlock, 0, s>>>(x, y, z, n, f, st, key, idx); }
+    }
+
+    // sorts (key, idx) stably by the low `bits` bits; the result ends in (key, idx)
+    void launchRadixSort(hipStream_t s, uint32_t* key, uint32_t* idx, uint32_t* key2, uint32_t* idx2, uint32_t n,
+                         uint32_t bits, uint32_t* hist)
+    {
+        if (n < 2) { return; }
+        const uint32_t nTiles = (n + kTile - 1) / kTile;
+        uint32_t* kIn = key; uint32_t* iIn = idx; uint32_t* kOut = key2; uint32_t* iOut = idx2;
+        for (uint32_t shift = 0; shift < bits; shift += 8)
+        {
+            kRadixHist<<<nTiles, kBlock, 0, s>>>(kIn, n, shift, nTiles, hist);
+            kScanSingle<<<1, 1024, 0, s>>>(hist, 256u * nTiles);
+            kRadixScatter<<<nTiles, kBlock, 0, s>>>(kIn, iIn, kOut, iOut, n, shift, nTiles, hist);
+            uint32_t* t = kIn; kIn = kOut; kOut = t;
+            t = iIn; iIn = iOut; iOut = t;
+        }
+        if (kIn != key)
+        {
+            (void)hipMemcpyAsync(key, kIn, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s);
+            (void)hipMemcpyAsync(idx, iIn, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s);
+        }
+    }
+
+    void launchPermute(hipStream_t s, const FieldSet& fs, const uint32_t* idx)
+    {
+        if (fs.n) { kPermute<<<blocksFor(fs.n), kBlock, 0, s>>>(fs, idx); }
+    }
+
+    void launchClearCells(hipStream_t s, uint2* cell, const uint32_t* occ, uint32_t* nOcc, uint32_t maxOcc)
+    {
+        if (maxOcc) { kClearCells<<<blocksFor(maxOcc), kBlock, 0, s>>>(cell, occ, nOcc); }
+        kSetU32<<<1, 1, 0, s>>>(nOcc, 0u);
+    }
+
+    void launchClearAll(hipStream_t s, uint2* cell, uint64_t n)
+    {
+        if (n) { kClearAll<<<blocksFor(n), kBlock, 0, s>>>(cell, n); }
+    }
+
+    void launchBuildCells(hipStream_t s, const uint32_t* key, uint32_t n, const Frame* f, uint2* cell, uint32_t* occ,
+                          uint32_t* nOcc)
+    {
+        if (n) { kBuildCells<<<blocksFor(n), kBlock, 0, s>>>(key, n, f, cell, occ, nOcc); }
+    }
+
+    void launchQueryCount(hipStream_t s, const Dev& d, float r2, int reach)
+    {
+        const uint32_t nSlices = (d.nF + kSlice - 1) / kSlice;
+        if (d.nF) { kQueryCount<<<blocksFor(d.nF), kBlock, 0, s>>>(d, r2, reach, nSlices); }
+    }
+
+    void launchScanSlices(hipStream_t s, const Dev& d)
+    {
+        const uint32_t nSlices = (d.nF + kSlice - 1) / kSlice;
+        kScanSlices<<<3, 1024, 0, s>>>(d, nSlices);
+        if (d.nF) { kCountEntries<<<blocksFor(d.nF), kBlock, 0, s>>>(d); }
+    }
+
+    void launchQueryFill(hipStream_t s, const Dev& d, float r2, int reach)
+    {
+        if (d.nF) { kQueryFill<<<blocksFor(d.nF), kBlock, 0, s>>>(d, r2, reach); }
+    }
+
+    void launchQueryRigid(hipStream_t s, const Dev& d, float r2, int reach, bool fill)
+    {
+        if (!d.nR) { return; }
+        kQueryRigid<<<blocksFor(d.nR), kBlock, 0, s>>>(d, r2, reach, fill);
+        if (!fill) { kScanRigid<<<1, 1024, 0, s>>>(d); }
+    }
+
+    void launchAabb(hipStream_t s, const float* x, const float* y, const float* z, uint32_t n, uint32_t* out)
+    {
+        if (n) { kAabb<<<blocksFor(n), kBlock, 0, s>>>(x, y, z, n, out); }
+    }
+}

@@ -1,0 +1,667 @@
+// kernels_sph.hip -- IISPH solver kernels of the step on gfx950.
+//
+// One thread per fluid particle in Morton order; a wave64 reads two SELL-32 slices, so the list
+// loads of a neighbor column are two 64-byte rows. FF sums use the reference CPU's dict-chunk
+// form (one scale after the sum), FB/FR sums its normal form, each in the list order the fill
+// wrote (CPU/backend/sph_solver.cpp):
+//   kDensity         Density_ChunkedSELL                            :734-890
+//   kNonPressure     NonPressureAcceleration_ChunkedSELL            :892-1149
+//   kSourceDiag      SourceTermAndDiagonalElementAndPressureJacobiInit :1225-1504
+//   kPressureAcc     PressureAcceleration_ChunkedSELL               :1507-1715
+//   kJacobi          PressureRelaxedJacobiSolver_ChunkedSELL        :1781-2049, FP64 error partials
+//   kFinalize        the error sums, averages and TerminationConditionSatisfied (:205-261, :686-732)
+//                    decided on the device, so no host round trip is needed per sweep
+//   kIntegration     Integration                                    :2051-2089
+// Rigid coupling (CPU/backend/cpu_backend.cpp:2001-2303, sph_solver.cpp:1151-1223, :1717-1778):
+// particle kinematics, reaction forces over the RF lists, FP64 per-body force/torque sums,
+// prediction and semi-implicit integration with Gram-Schmidt re-orthonormalization.
+#include "tsp_device.h"
+
+namespace tsp
+{
+    struct List
+    {
+        const uint16_t* pool;
+        SliceMeta m;
+        uint32_t lane, n;
+        __device__ __forceinline__ uint32_t operator[](uint32_t k) const { return sellRead(pool, m, lane, k); }
+    };
+
+    __device__ __forceinline__ List listOf(const Dev& d, int rel, uint32_t i)
+    {
+        List l;
+        l.pool = d.pool[rel];
+        l.m = d.meta[rel][i / kSlice];
+        l.lane = i & (kSlice - 1);
+        l.n = d.cnt[rel][i] & (rel == REL_FF ? 0x7fffu : 0xffffu);
+        return l;
+    }
+
+    __device__ __forceinline__ float dist2(float dx, float dy, float dz)
+    {
+        return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+    }
+
+    __global__ void __launch_bounds__(kBlock) kDensity(Dev d, Consts k)
+    {
+        const uint32_t i = xcdBlock() * kBlock + threadIdx.x;
+        if (i >= d.nF) { return; }
+        const float xi = d.x[i], yi = d.y[i], zi = d.z[i];
+        const List ff = listOf(d, REL_FF, i);
+        float wsum = 0.0f;
+        for (uint32_t q = 0; q < ff.n; q++)
+        {
+            const uint32_t j = ff[q];
+            wsum += kwUnscaled(dist2(xi - d.x[j], yi - d.y[j], zi - d.z[j]), k);
+        }
+        const float dF = wsum * k.weightScale;
+        float dB = 0.0f, dR = 0.0f;
+        const List fb = listOf(d, REL_FB, i);
+        for (uint32_t q = 0; q < fb.n; q++)
+        {
+            const uint32_t j = fb[q];
+            dB = __builtin_fmaf(d.bpsi[j], kwNormal(xi - d.bx[j], yi - d.by[j], zi - d.bz[j], k), dB);
+        }
+        const List fr = listOf(d, REL_FR, i);
+        for (uint32_t q = 0; q < fr.n; q++)
+        {
+            const uint32_t j = fr[q];
+            dR = __builtin_fmaf(d.rpsi[j], kwNormal(xi - d.rx[j], yi - d.ry[j], zi - d.rz[j], k), dR);
+        }
+        d.density[i] = dF + dB + dR;
+    }
+
+    __global__ void __launch_bounds__(kBlock) kNonPressure(Dev d, Consts k)
+    {
+        const uint32_t i = xcdBlock() * kBlock + threadIdx.x;
+        if (i >= d.nF) { return; }
+        const float xi = d.x[i], yi = d.y[i], zi = d.z[i], vxi = d.vx[i], vyi = d.vy[i], vzi = d.vz[i];
+        float aFx = 0.0f, aFy = 0.0f, aFz = 0.0f;
+        const List ff = listOf(d, REL_FF, i);
+        for (uint32_t q = 0; q < ff.n; q++)
+        {
+            const uint32_t j = ff[q];
+            const float dx = xi - d.x[j], dy = yi - d.y[j], dz = zi - d.z[j];
+            const float dvx = vxi - d.vx[j], dvy = vyi - d.vy[j], dvz = vzi - d.vz[j];
+            const float d2 = dist2(dx, dy, dz);
+            const float kg = kgFactor(d2, k);
+            const float dot = __builtin_fmaf(dvx, dx, __builtin_fmaf(dvy, dy, dvz * dz));
+            const float rj = vmax(d.density[j], 1e-7f);
+            const float coeff = dot / (rj * (d2 + k.hsqEps)) * kg;
+            aFx = __builtin_fmaf(coeff, dx, aFx);
+            aFy = __builtin_fmaf(coeff, dy, aFy);
+            aFz = __builtin_fmaf(coeff, dz, aFz);
+        }
+        aFx *= k.mass; aFy *= k.mass; aFz *= k.mass;
+        float aBx = 0.0f, aBy = 0.0f, aBz = 0.0f;
+        const List fb = listOf(d, REL_FB, i);
+        for (uint32_t q = 0; q < fb.n; q++)
+        {
+            const uint32_t j = fb[q];
+            const float dx = xi - d.bx[j], dy = yi - d.by[j], dz = zi - d.bz[j];
+            const float d2 = dist2(dx, dy, dz);
+            const float f = kgFactor(d2, k);
+            const float gx = f * dx, gy = f * dy, gz = f * dz;
+            const float dvx = vxi - 0.0f, dvy = vyi - 0.0f, dvz = vzi - 0.0f;
+            const float dot = __builtin_fmaf(dvx, dx, __builtin_fmaf(dvy, dy, dvz * dz));
+            const float coeff = (d.bvol[j] * dot) / (d2 + k.hsqEps);
+            aBx = __builtin_fmaf(coeff, gx, aBx);
+            aBy = __builtin_fmaf(coeff, gy, aBy);
+            aBz = __builtin_fmaf(coeff, gz, aBz);
+        }
+        float aRx = 0.0f, aRy = 0.0f, aRz = 0.0f;
+        const List fr = listOf(d, REL_FR, i);
+        for (uint32_t q = 0; q < fr.n; q++)
+        {
+            const uint32_t j = fr[q];
+            const float dx = xi - d.rx[j], dy = yi - d.ry[j], dz = zi - d.rz[j];
+            const float d2 = dist2(dx, dy, dz);
+            const float f = kgFactor(d2, k);
+            const float gx = f * dx, gy = f * dy, gz = f * dz;
+            const float dvx = vxi - d.rvx[j], dvy = vyi - d.rvy[j], dvz = vzi - d.rvz[j];
+            const float dot = __builtin_fmaf(dvx, dx, __builtin_fmaf(dvy, dy, dvz * dz));
+            const float coeff = (d.rvol[j] * dot) / (d2 + k.hsqEps);
+            aRx = __builtin_fmaf(coeff, gx, aRx);
+            aRy = __builtin_fmaf(coeff, gy, aRy);
+            aRz = __builtin_fmaf(coeff, gz, aRz);
+        }
+        const float ax = aFx * k.visF + (aBx + aRx) * k.visB + k.gx;
+        const float ay = aFy * k.visF + (aBy + aRy) * k.visB + k.gy;
+        const float az = aFz * k.visF + (aBz + aRz) * k.visB + k.gz;
+        d.vpx[i] = __builtin_fmaf(ax, k.dt, vxi);
+        d.vpy[i] = __builtin_fmaf(ay, k.dt, vyi);
+        d.vpz[i] = __builtin_fmaf(az, k.dt, vzi);
+    }
+
+    __global__ void __launch_bounds__(kBlock) kSourceDiag(Dev d, Consts k)
+    {
+        const uint32_t i = xcdBlock() * kBlock + threadIdx.x;
+        if (i >= d.nF) { return; }
+        const float xi = d.x[i], yi = d.y[i], zi = d.z[i], vxi = d.vpx[i], vyi = d.vpy[i], vzi = d.vpz[i];
+        float sF = 0.0f, cFx = 0.0f, cFy = 0.0f, cFz = 0.0f, g2 = 0.0f;
+        const List ff = listOf(d, REL_FF, i);
+        for (uint32_t q = 0; q < ff.n; q++)
+        {
+            const uint32_t j = ff[q];
+            const float dx = xi - d.x[j], dy = yi - d.y[j], dz = zi - d.z[j];
+            const float dvx = vxi - d.vpx[j], dvy = vyi - d.vpy[j], dvz = vzi - d.vpz[j];
+            const float d2 = dist2(dx, dy, dz);
+            const float kg = kgFactor(d2, k);
+            const float st = __builtin_fmaf(dvx, dx, __builtin_fmaf(dvy, dy, dvz * dz));
+            sF = __builtin_fmaf(kg, st, sF);
+            cFx = __builtin_fmaf(kg, dx, cFx);
+            cFy = __builtin_fmaf(kg, dy, cFy);
+            cFz = __builtin_fmaf(kg, dz, cFz);
+            g2 = __builtin_fmaf(kg, kg * d2, g2);
+        }
+        float sB = 0.0f, cBx = 0.0f, cBy = 0.0f, cBz = 0.0f;
+        const List fb = listOf(d, REL_FB, i);
+        for (uint32_t q = 0; q < fb.n; q++)
+        {
+            const uint32_t j = fb[q];
+            const float dx = xi - d.bx[j], dy = yi - d.by[j], dz = zi - d.bz[j];
+            const float f = kgFactor(dist2(dx, dy, dz), k);
+            const float gx = f * dx, gy = f * dy, gz = f * dz;
+            const float dvx = vxi - 0.0f, dvy = vyi - 0.0f, dvz = vzi - 0.0f;
+            const float st = __builtin_fmaf(dvx, gx, __builtin_fmaf(dvy, gy, dvz * gz));
+            const float psi = d.bpsi[j];
+            sB = __builtin_fmaf(psi, st, sB);
+            cBx = __builtin_fmaf(psi, gx, cBx);
+            cBy = __builtin_fmaf(psi, gy, cBy);
+            cBz = __builtin_fmaf(psi, gz, cBz);
+        }
+        float sR = 0.0f, cRx = 0.0f, cRy = 0.0f, cRz = 0.0f;
+        const List fr = listOf(d, REL_FR, i);
+        for (uint32_t q = 0; q < fr.n; q++)
+        {
+            const uint32_t j = fr[q];
+            const float dx = xi - d.rx[j], dy = yi - d.ry[j], dz = zi - d.rz[j];
+            const float f = kgFactor(dist2(dx, dy, dz), k);
+            const float gx = f * dx, gy = f * dy, gz = f * dz;
+            const float dvx = vxi - d.rvpx[j], dvy = vyi - d.rvpy[j], dvz = vzi - d.rvpz[j];
+            const float st = __builtin_fmaf(dvx, gx, __builtin_fmaf(dvy, gy, dvz * gz));
+            const float psi = d.rpsi[j];
+            sR = __builtin_fmaf(psi, st, sR);
+            cRx = __builtin_fmaf(psi, gx, cRx);
+            cRy = __builtin_fmaf(psi, gy, cRy);
+            cRz = __builtin_fmaf(psi, gz, cRz);
+        }
+        sF *= k.mass;
+        const float src = k.rho0 - d.density[i] - k.dt * (sF + sB + sR);
+        const float ctx = cFx * k.cFacF + (cBx + cRx) * k.cFacB;
+        const float cty = cFy * k.cFacF + (cBy + cRy) * k.cFacB;
+        const float ctz = cFz * k.cFacF + (cBz + cRz) * k.cFacB;
+        const float dA = k.mass * __builtin_fmaf(ctx, cFx, __builtin_fmaf(cty, cFy, ctz * cFz));
+        const float dB = k.mass * k.dFacB * g2;
+        const float dC = __builtin_fmaf(ctx, cBx + cRx, __builtin_fmaf(cty, cBy + cRy, ctz * (cBz + cRz)));
+        const float diag = k.dtsq * (dA + dB + dC);
+        const bool valid = __builtin_fabsf(diag) > 1e-7f;
+        const float cold = k.omega * (src / diag);
+        float init = cold;
+        if (k.warm > 0.0f)
+        {
+            const float prev = d.pressure[i];
+            init = prev > 0.0f ? k.warm * prev : cold;
+        }
+        d.source[i] = src;
+        d.diag[i] = diag;
+        d.pressure[i] = vmax(0.0f, valid ? init : 0.0f);
+    }
+
+    // TerminationConditionSatisfied, sph_solver.cpp:686-732
+    __device__ __forceinline__ bool terminated(const Status& s, const Consts& k)
+    {
+        if (s.iters < k.minIters) { return false; }
+        if (s.iters >= k.maxIters) { return true; }
+        switch (k.termCond)
+        {
+        case 0: return s.iters >= k.fixedIters;
+        case 1: return s.avgRaw <= k.densityDevTarget;
+        case 2: return s.avgPos <= k.densityDevTarget;
+        case 3: return s.avgAbs <= k.densityDevTarget;
+        default: return s.iters >= 3;
+        }
+    }
+
+    // the solver state before the first sweep: N_solverIters_ = 1, errors 0, then the
+    // increment and error decision of the first sweep (sph_solver.cpp:207-235)
+    __global__ void kSolverInit(Dev d, Consts k)
+    {
+        Status& s = *d.status;
+        s.iters = d.nF > 0 ? 1u : 0u;
+        s.avgRaw = s.avgPos = s.avgAbs = 0.0f;
+        s.sumRaw = s.sumPos = s.sumAbs = 0.0;
+        s.converged = d.nF > 0 ? 0u : 1u;
+        s.sweeps = 0;
+        s.iters++;
+        s.compute = (s.iters >= k.minIters && (k.termCond != 0 || terminated(s, k))) ? 1u : 0u;
+    }
+
+    __global__ void __launch_bounds__(kBlock) kPressureAcc(Dev d, Consts k, bool sweep)
+    {
+        if (sweep && d.status->converged) { return; }
+        const uint32_t i = xcdBlock() * kBlock + threadIdx.x;
+        if (i >= d.nF) { return; }
+        const float xi = d.x[i], yi = d.y[i], zi = d.z[i], pi = d.pressure[i];
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        const List ff = listOf(d, REL_FF, i);
+        for (uint32_t q = 0; q < ff.n; q++)
+        {
+            const uint32_t j = ff[q];
+            const float dx = xi - d.x[j], dy = yi - d.y[j], dz = zi - d.z[j];
+            const float kg = kgFactor(dist2(dx, dy, dz), k);
+            const float coeff = (pi + d.pressure[j]) * kg;
+            sx = __builtin_fmaf(coeff, dx, sx);
+            sy = __builtin_fmaf(coeff, dy, sy);
+            sz = __builtin_fmaf(coeff, dz, sz);
+        }
+        float ax = sx * k.presOuterFF, ay = sy * k.presOuterFF, az = sz * k.presOuterFF;
+        const List fb = listOf(d, REL_FB, i);
+        for (uint32_t q = 0; q < fb.n; q++)
+        {
+            const uint32_t j = fb[q];
+            const float dx = xi - d.bx[j], dy = yi - d.by[j], dz = zi - d.bz[j];
+            const float f = kgFactor(dist2(dx, dy, dz), k);
+            const float coeff = k.presOuterB * d.bpsi[j] * pi;
+            ax = __builtin_fmaf(coeff, f * dx, ax);
+            ay = __builtin_fmaf(coeff, f * dy, ay);
+            az = __builtin_fmaf(coeff, f * dz, az);
+        }
+        const List fr = listOf(d, REL_FR, i);
+        for (uint32_t q = 0; q < fr.n; q++)
+        {
+            const uint32_t j = fr[q];
+            const float dx = xi - d.rx[j], dy = yi - d.ry[j], dz = zi - d.rz[j];
+            const float f = kgFactor(dist2(dx, dy, dz), k);
+            const float coeff = k.presOuterB * d.rpsi[j] * pi;
+            ax = __builtin_fmaf(coeff, f * dx, ax);
+            ay = __builtin_fmaf(coeff, f * dy, ay);
+            az = __builtin_fmaf(coeff, f * dz, az);
+        }
+        d.apx[i] = 0.0f - ax;
+        d.apy[i] = 0.0f - ay;
+        d.apz[i] = 0.0f - az;
+    }
+
+    __device__ __forceinline__ double waveSum(double v)
+    {
+        for (int o = 32; o > 0; o >>= 1) { v += __shfl_xor(v, o, 64); }
+        return v;
+    }
+
+    __global__ void __launch_bounds__(kBlock) kJacobi(Dev d, Consts k)
+    {
+        __shared__ double red[3][kBlock / 64];
+        if (d.status->converged) { return; }
+        const bool compute = d.status->compute != 0u;
+        const uint32_t blk = xcdBlock();
+        const uint32_t i = blk * kBlock + threadIdx.x;
+        double eRaw = 0.0, ePos = 0.0, eAbs = 0.0;
+        if (i < d.nF)
+        {
+            const float xi = d.x[i], yi = d.y[i], zi = d.z[i], axi = d.apx[i], ayi = d.apy[i], azi = d.apz[i];
+            float sf = 0.0f;
+            const List ff = listOf(d, REL_FF, i);
+            for (uint32_t q = 0; q < ff.n; q++)
+            {
+                const uint32_t j = ff[q];
+                const float dx = xi - d.x[j], dy = yi - d.y[j], dz = zi - d.z[j];
+                const float kg = kgFactor(dist2(dx, dy, dz), k);
+                const float sv = __builtin_fmaf(axi - d.apx[j], dx, __builtin_fmaf(ayi - d.apy[j], dy, (azi - d.apz[j]) * dz));
+                sf = __builtin_fmaf(kg, sv, sf);
+            }
+            sf *= k.mass;
+            float sb = 0.0f;
+            const List fb = listOf(d, REL_FB, i);
+            for (uint32_t q = 0; q < fb.n; q++)
+            {
+                const uint32_t j = fb[q];
+                const float dx = xi - d.bx[j], dy = yi - d.by[j], dz = zi - d.bz[j];
+                const float f = kgFactor(dist2(dx, dy, dz), k);
+                const float sv = __builtin_fmaf(axi, f * dx, __builtin_fmaf(ayi, f * dy, azi * (f * dz)));
+                sb += d.bpsi[j] * sv;
+            }
+            const List fr = listOf(d, REL_FR, i);
+            for (uint32_t q = 0; q < fr.n; q++)
+            {
+                const uint32_t j = fr[q];
+                const float dx = xi - d.rx[j], dy = yi - d.ry[j], dz = zi - d.rz[j];
+                const float f = kgFactor(dist2(dx, dy, dz), k);
+                const float sv = __builtin_fmaf(axi, f * dx, __builtin_fmaf(ayi, f * dy, azi * (f * dz)));
+                sb += d.rpsi[j] * sv;
+            }
+            const float sys = k.dtsq * (sf + sb);
+            const float src = d.source[i], diag = d.diag[i];
+            const float resid = src - sys;
+            const float pred = sys - src;
+            const bool valid = __builtin_fabsf(diag) > 1e-7f;
+            d.pressure[i] = vmax(0.0f, valid ? __builtin_fmaf(k.omega, resid / diag, d.pressure[i]) : 0.0f);
+            if (compute && valid)
+            {
+                const double e = static_cast<double>(pred / k.rho0);
+                eRaw = e;
+                ePos = 0.0 < e ? e : 0.0;
+                eAbs = __builtin_fabs(e);
+            }
+        }
+        if (!compute) { return; }
+        eRaw = waveSum(eRaw);
+        ePos = waveSum(ePos);
+        eAbs = waveSum(eAbs);
+        const uint32_t w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63u) == 0) { red[0][w] = eRaw; red[1][w] = ePos; red[2][w] = eAbs; }
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            for (int r = 0; r < 3; r++)
+            {
+                double s = 0.0;
+                for (uint32_t q = 0; q < kBlock / 64; q++) { s += red[r][q]; }
+                d.errPartial[3 * blk + r] = s;
+            }
+        }
+    }
+
+    // sums the block partials in a fixed order, updates the averages and decides the next sweep
+    __global__ void __launch_bounds__(kBlock) kFinalize(Dev d, Consts k, uint32_t nBlocks)
+    {
+        __shared__ double red[3][kBlock];
+        Status& s = *d.status;
+        if (s.converged) { return; }
+        const bool compute = s.compute != 0u;
+        if (compute)
+        {
+            double a[3] = { 0.0, 0.0, 0.0 };
+            for (uint32_t b = threadIdx.x; b < nBlocks; b += kBlock)
+            {
+                for (int r = 0; r < 3; r++) { a[r] += d.errPartial[3 * b + r]; }
+            }
+            for (int r = 0; r < 3; r++) { red[r][threadIdx.x] = a[r]; }
+            __syncthreads();
+            for (uint32_t o = kBlock / 2; o > 0; o >>= 1)
+            {
+                if (threadIdx.x < o) { for (int r = 0; r < 3; r++) { red[r][threadIdx.x] += red[r][threadIdx.x + o]; } }
+                __syncthreads();
+            }
+        }
+        if (threadIdx.x != 0) { return; }
+        s.sweeps++;
+        if (compute)
+        {
+            s.sumRaw = red[0][0]; s.sumPos = red[1][0]; s.sumAbs = red[2][0];
+            const float n = static_cast<float>(d.nF);
+            s.avgRaw = static_cast<float>(red[0][0]) / n;
+            s.avgPos = static_cast<float>(red[1][0]) / n;
+            s.avgAbs = static_cast<float>(red[2][0]) / n;
+        }
+        if (terminated(s, k))
+        {
+            s.converged = 1u;
+            return;
+        }
+        s.iters++;
+        s.compute = (s.iters >= k.minIters && (k.termCond != 0 || terminated(s, k))) ? 1u : 0u;
+    }
+
+    __global__ void __launch_bounds__(kBlock) kIntegration(Dev d, Consts k)
+    {
+        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= d.nF) { return; }
+        const float vx = __builtin_fmaf(k.dt, d.apx[i], d.vpx[i]);
+        const float vy = __builtin_fmaf(k.dt, d.apy[i], d.vpy[i]);
+        const float vz = __builtin_fmaf(k.dt, d.apz[i], d.vpz[i]);
+        d.vx[i] = vx; d.vy[i] = vy; d.vz[i] = vz;
+        d.x[i] = __builtin_fmaf(k.dt, vx, d.x[i]);
+        d.y[i] = __builtin_fmaf(k.dt, vy, d.y[i]);
+        d.z[i] = __builtin_fmaf(k.dt, vz, d.z[i]);
+    }
+
+    // ----------------------------------------------------------------------------------------
+    // rigid bodies; the body state is field-major: rb[field * nRB + body]
+
+    enum
+    {
+        RB_XCM = 0, RB_VX = 3, RB_WX = 6, RB_LX = 9, RB_A00 = 12, RB_MASS = 21, RB_MASS_INV = 22, RB_I00 = 23
+    };
+
+    __device__ __forceinline__ float rbv(const Dev& d, int f, uint32_t b) { return d.rb[f * d.nRB + b]; }
+
+    // UpdateRigidParticles(predicted), cpu_backend.cpp:2009-2063
+    __global__ void __launch_bounds__(kBlock) kRigidUpdate(Dev d, bool predicted)
+    {
+        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= d.nR) { return; }
+        const uint32_t b = d.rbody[i];
+        const float lx = d.rxrel[i], ly = d.ryrel[i], lz = d.rzrel[i];
+        const float rx = __builtin_fmaf(rbv(d, RB_A00 + 0, b), lx, __builtin_fmaf(rbv(d, RB_A00 + 1, b), ly, rbv(d, RB_A00 + 2, b) * lz));
+        const float ry = __builtin_fmaf(rbv(d, RB_A00 + 3, b), lx, __builtin_fmaf(rbv(d, RB_A00 + 4, b), ly, rbv(d, RB_A00 + 5, b) * lz));
+        const float rz = __builtin_fmaf(rbv(d, RB_A00 + 6, b), lx, __builtin_fmaf(rbv(d, RB_A00 + 7, b), ly, rbv(d, RB_A00 + 8, b) * lz));
+        if (!predicted)
+        {
+            d.rx[i] = rbv(d, RB_XCM + 0, b) + rx;
+            d.ry[i] = rbv(d, RB_XCM + 1, b) + ry;
+            d.rz[i] = rbv(d, RB_XCM + 2, b) + rz;
+        }
+        const uint32_t nb = d.nRB;
+        const float wx = predicted ? d.rbPred[6 * nb + b] : rbv(d, RB_WX + 0, b);
+        const float wy = predicted ? d.rbPred[7 * nb + b] : rbv(d, RB_WX + 1, b);
+        const float wz = predicted ? d.rbPred[8 * nb + b] : rbv(d, RB_WX + 2, b);
+        const float vbx = predicted ? d.rbPred[0 * nb + b] : rbv(d, RB_VX + 0, b);
+        const float vby = predicted ? d.rbPred[1 * nb + b] : rbv(d, RB_VX + 1, b);
+        const float vbz = predicted ? d.rbPred[2 * nb + b] : rbv(d, RB_VX + 2, b);
+        float* ox = predicted ? d.rvpx : d.rvx;
+        float* oy = predicted ? d.rvpy : d.rvy;
+        float* oz = predicted ? d.rvpz : d.rvz;
+        ox[i] = vbx + wy * rz - wz * ry;
+        oy[i] = vby + wz * rx - wx * rz;
+        oz[i] = vbz + wx * ry - wy * rx;
+    }
+
+    // ClearRigidForcesAndAddGravity, cpu_backend.cpp:2065-2087
+    __global__ void kRigidClearGravity(Dev d, Consts k)
+    {
+        const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+        if (b >= d.nRB) { return; }
+        const uint32_t nb = d.nRB;
+        for (int f = 0; f < 6; f++) { d.rbForce[f * nb + b] = 0.0f; }
+        const float m = rbv(d, RB_MASS, b), mi = rbv(d, RB_MASS_INV, b);
+        if (m <= 0.0f || mi <= 0.0f) { return; }
+        d.rbForce[0 * nb + b] += m * k.gx;
+        d.rbForce[1 * nb + b] += m * k.gy;
+        d.rbForce[2 * nb + b] += m * k.gz;
+    }
+
+    // AccumulateRigidParticleNonPressureForces_ChunkedSELL / ...PressureForces (sph_solver.cpp:1151-1223,
+    // :1717-1778): reaction of the fluid neighbors of each rigid particle
+    __global__ void __launch_bounds__(kBlock) kRigidForces(Dev d, Consts k, bool pressure)
+    {
+        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= d.nR) { return; }
+        const float xi = d.rx[i], yi = d.ry[i], zi = d.rz[i];
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        for (uint32_t q = d.rfOff[i]; q < d.rfOff[i + 1]; q++)
+        {
+            const uint32_t j = d.rfIdx[q];
+            const float dx = d.x[j] - xi, dy = d.y[j] - yi, dz = d.z[j] - zi;
+            float gx, gy, gz;
+            kgScalar(dx, dy, dz, k, gx, gy, gz);
+            float fc;
+            if (pressure)
+            {
+                fc = k.mass * k.presOuterB * d.rpsi[i] * d.pressure[j];
+            }
+            else
+            {
+                const float dvx = d.vx[j] - d.rvx[i], dvy = d.vy[j] - d.rvy[i], dvz = d.vz[j] - d.rvz[i];
+                const float dot = __builtin_fmaf(dvx, dx, __builtin_fmaf(dvy, dy, dvz * dz));
+                const float d2 = dist2(dx, dy, dz);
+                const float coeff = (d.rvol[i] * dot) / (d2 + k.hsqEps);
+                fc = -k.mass * k.visB * coeff;
+            }
+            fx = __builtin_fmaf(fc, gx, fx);
+            fy = __builtin_fmaf(fc, gy, fy);
+            fz = __builtin_fmaf(fc, gz, fz);
+        }
+        if (pressure) { d.rfx[i] += fx; d.rfy[i] += fy; d.rfz[i] += fz; }
+        else { d.rfx[i] = fx; d.rfy[i] = fy; d.rfz[i] = fz; }
+    }
+
+    // ReduceRigidParticleForcesToBody, cpu_backend.cpp:2089-2172, with FP64 sums: one block per body
+    __global__ void __launch_bounds__(kBlock) kRigidReduce(Dev d)
+    {
+        __shared__ double red[6][kBlock];
+        const uint32_t b = blockIdx.x;
+        const uint32_t nb = d.nRB;
+        const float a[9] = { rbv(d, RB_A00 + 0, b), rbv(d, RB_A00 + 1, b), rbv(d, RB_A00 + 2, b),
+                             rbv(d, RB_A00 + 3, b), rbv(d, RB_A00 + 4, b), rbv(d, RB_A00 + 5, b),
+                             rbv(d, RB_A00 + 6, b), rbv(d, RB_A00 + 7, b), rbv(d, RB_A00 + 8, b) };
+        double acc[6] = { 0, 0, 0, 0, 0, 0 };
+        for (uint32_t q = d.byBodyOff[b] + threadIdx.x; q < d.byBodyOff[b + 1]; q += kBlock)
+        {
+            const uint32_t i = d.byBodyIdx[q];
+            const float lx = d.rxrel[i], ly = d.ryrel[i], lz = d.rzrel[i];
+            const float rx = __builtin_fmaf(a[0], lx, __builtin_fmaf(a[1], ly, a[2] * lz));
+            const float ry = __builtin_fmaf(a[3], lx, __builtin_fmaf(a[4], ly, a[5] * lz));
+            const float rz = __builtin_fmaf(a[6], lx, __builtin_fmaf(a[7], ly, a[8] * lz));
+            const double fx = d.rfx[i], fy = d.rfy[i], fz = d.rfz[i];
+            acc[0] += fx; acc[1] += fy; acc[2] += fz;
+            acc[3] += static_cast<double>(ry) * fz - static_cast<double>(rz) * fy;
+            acc[4] += static_cast<double>(rz) * fx - static_cast<double>(rx) * fz;
+            acc[5] += static_cast<double>(rx) * fy - static_cast<double>(ry) * fx;
+        }
+        for (int f = 0; f < 6; f++) { red[f][threadIdx.x] = acc[f]; }
+        __syncthreads();
+        for (uint32_t o = kBlock / 2; o > 0; o >>= 1)
+        {
+            if (threadIdx.x < o) { for (int f = 0; f < 6; f++) { red[f][threadIdx.x] += red[f][threadIdx.x + o]; } }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+        {
+            for (int f = 0; f < 6; f++) { d.rbForce[f * nb + b] += static_cast<float>(red[f][0]); }
+        }
+    }
+
+    // PredictRigidBodies, cpu_backend.cpp:2174-2235
+    __global__ void kRigidPredict(Dev d, Consts k)
+    {
+        const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+        if (b >= d.nRB) { return; }
+        const uint32_t nb = d.nRB;
+        const float mi = rbv(d, RB_MASS_INV, b);
+        float lp[3];
+        for (int a = 0; a < 3; a++)
+        {
+            d.rbPred[a * nb + b] = rbv(d, RB_VX + a, b) + k.dt * d.rbForce[a * nb + b] * mi;
+            lp[a] = rbv(d, RB_LX + a, b) + k.dt * d.rbForce[(3 + a) * nb + b];
+            d.rbPred[(3 + a) * nb + b] = lp[a];
+        }
+        float A[9], B[9], T[9], I[9];
+        for (int q = 0; q < 9; q++) { A[q] = rbv(d, RB_A00 + q, b); B[q] = rbv(d, RB_I00 + q, b); }
+        for (int r = 0; r < 3; r++)
+        {
+            for (int c = 0; c < 3; c++) { T[3 * r + c] = A[3 * r] * B[c] + A[3 * r + 1] * B[3 + c] + A[3 * r + 2] * B[6 + c]; }
+        }
+        for (int r = 0; r < 3; r++)
+        {
+            for (int c = 0; c < 3; c++) { I[3 * r + c] = T[3 * r] * A[3 * c] + T[3 * r + 1] * A[3 * c + 1] + T[3 * r + 2] * A[3 * c + 2]; }
+        }
+        for (int r = 0; r < 3; r++)
+        {
+            d.rbPred[(6 + r) * nb + b] = __builtin_fmaf(I[3 * r], lp[0], __builtin_fmaf(I[3 * r + 1], lp[1], I[3 * r + 2] * lp[2]));
+        }
+    }
+
+    __device__ __forceinline__ bool normalize3(float& x, float& y, float& z)
+    {
+        const float len = __builtin_sqrtf(__builtin_fmaf(x, x, __builtin_fmaf(y, y, z * z)));
+        if (len <= 1e-12f || !__builtin_isfinite(len)) { return false; }
+        const float inv = 1.0f / len;
+        x *= inv; y *= inv; z *= inv;
+        return true;
+    }
+
+    __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz)
+    {
+        return __builtin_fmaf(ax, bx, __builtin_fmaf(ay, by, az * bz));
+    }
+
+    // IntegrateRigidBodies, cpu_backend.cpp:2237-2303
+    __global__ void kRigidIntegrate(Dev d, Consts k)
+    {
+        const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+        if (b >= d.nRB) { return; }
+        const uint32_t nb = d.nRB;
+        float* rb = d.rb;
+        for (int a = 0; a < 3; a++)
+        {
+            rb[(RB_VX + a) * nb + b] = d.rbPred[a * nb + b];
+            rb[(RB_LX + a) * nb + b] = d.rbPred[(3 + a) * nb + b];
+        }
+        for (int a = 0; a < 3; a++) { rb[(RB_XCM + a) * nb + b] += k.dt * rb[(RB_VX + a) * nb + b]; }
+        const float wx = d.rbPred[6 * nb + b], wy = d.rbPred[7 * nb + b], wz = d.rbPred[8 * nb + b];
+        rb[(RB_WX + 0) * nb + b] = wx;
+        rb[(RB_WX + 1) * nb + b] = wy;
+        rb[(RB_WX + 2) * nb + b] = wz;
+        float A[9];
+        for (int q = 0; q < 9; q++) { A[q] = rb[(RB_A00 + q) * nb + b]; }
+        const float dt = k.dt;
+        float N[9];
+        N[0] = A[0] + dt * (-wz * A[3] + wy * A[6]);
+        N[1] = A[1] + dt * (-wz * A[4] + wy * A[7]);
+        N[2] = A[2] + dt * (-wz * A[5] + wy * A[8]);
+        N[3] = A[3] + dt * (wz * A[0] - wx * A[6]);
+        N[4] = A[4] + dt * (wz * A[1] - wx * A[7]);
+        N[5] = A[5] + dt * (wz * A[2] - wx * A[8]);
+        N[6] = A[6] + dt * (-wy * A[0] + wx * A[3]);
+        N[7] = A[7] + dt * (-wy * A[1] + wx * A[4]);
+        N[8] = A[8] + dt * (-wy * A[2] + wx * A[5]);
+        float b1x = N[0], b1y = N[3], b1z = N[6];
+        float b2x = N[1], b2y = N[4], b2z = N[7];
+        float b3x = N[2], b3y = N[5], b3z = N[8];
+        bool ok = normalize3(b1x, b1y, b1z);
+        const float d12 = dot3(b1x, b1y, b1z, b2x, b2y, b2z);
+        b2x -= d12 * b1x; b2y -= d12 * b1y; b2z -= d12 * b1z;
+        ok = ok && normalize3(b2x, b2y, b2z);
+        const float d13 = dot3(b1x, b1y, b1z, b3x, b3y, b3z);
+        const float d23 = dot3(b2x, b2y, b2z, b3x, b3y, b3z);
+        b3x -= d13 * b1x + d23 * b2x;
+        b3y -= d13 * b1y + d23 * b2y;
+        b3z -= d13 * b1z + d23 * b2z;
+        ok = ok && normalize3(b3x, b3y, b3z);
+        if (!ok) { atomicOr(&d.status->rigidDegenerate, 1u); return; }
+        const float out[9] = { b1x, b2x, b3x, b1y, b2y, b3y, b1z, b2z, b3z };
+        for (int q = 0; q < 9; q++) { rb[(RB_A00 + q) * nb + b] = out[q]; }
+    }
+
+    // ----------------------------------------------------------------------------------------
+    // host launchers
+
+    static inline uint32_t blocksOf(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
+
+    void launchDensity(hipStream_t s, const Dev& d, const Consts& k) { if (d.nF) { kDensity<<<blocksOf(d.nF), kBlock, 0, s>>>(d, k); } }
+    void launchNonPressure(hipStream_t s, const Dev& d, const Consts& k) { if (d.nF) { kNonPressure<<<blocksOf(d.nF), kBlock, 0, s>>>(d, k); } }
+    void launchSourceDiag(hipStream_t s, const Dev& d, const Consts& k)
+    {
+        if (d.nF) { kSourceDiag<<<blocksOf(d.nF), kBlock, 0, s>>>(d, k); }
+        kSolverInit<<<1, 1, 0, s>>>(d, k);
+    }
+    void launchPressureAcc(hipStream_t s, const Dev& d, const Consts& k, bool sweep)
+    {
+        if (d.nF) { kPressureAcc<<<blocksOf(d.nF), kBlock, 0, s>>>(d, k, sweep); }
+    }
+    void launchJacobi(hipStream_t s, const Dev& d, const Consts& k)
+    {
+        if (!d.nF) { return; }
+        kJacobi<<<blocksOf(d.nF), kBlock, 0, s>>>(d, k);
+        kFinalize<<<1, kBlock, 0, s>>>(d, k, blocksOf(d.nF));
+    }
+    void launchIntegration(hipStream_t s, const Dev& d, const Consts& k) { if (d.nF) { kIntegration<<<blocksOf(d.nF), kBlock, 0, s>>>(d, k); } }
+
+    void launchRigidUpdate(hipStream_t s, const Dev& d, bool predicted) { if (d.nR) { kRigidUpdate<<<blocksOf(d.nR), kBlock, 0, s>>>(d, predicted); } }
+    void launchRigidClearGravity(hipStream_t s, const Dev& d, const Consts& k) { if (d.nRB) { kRigidClearGravity<<<blocksOf(d.nRB), kBlock, 0, s>>>(d, k); } }
+    void launchRigidForces(hipStream_t s, const Dev& d, const Consts& k, bool pressure) { if (d.nR) { kRigidForces<<<blocksOf(d.nR), kBlock, 0, s>>>(d, k, pressure); } }
+    void launchRigidReduce(hipStream_t s, const Dev& d) { if (d.nRB) { kRigidReduce<<<d.nRB, kBlock, 0, s>>>(d); } }
+    void launchRigidPredict(hipStream_t s, const Dev& d, const Consts& k) { if (d.nRB) { kRigidPredict<<<blocksOf(d.nRB), kBlock, 0, s>>>(d, k); } }
+    void launchRigidIntegrate(hipStream_t s, const Dev& d, const Consts& k) { if (d.nRB) { kRigidIntegrate<<<blocksOf(d.nRB), kBlock, 0, s>>>(d, k); } }
+}

@@ -1,0 +1,259 @@
+// tsp_device.h -- device data layout and shared helpers of the gfx950 IISPH step.
+//
+// Layout in HBM (all SoA, one array per component, indexed in the current Morton order):
+//   fluid      x y z | vx vy vz | vpx vpy vpz | apx apy apz | source diag pressure density  (f32)
+//              id (u64); x..vz and id are double-buffered so the re-order is a gather + swap
+//   boundary   x y z vol psi (f32), id (u64): sorted by cell id once, static afterwards
+//   rigid      xrel yrel zrel vol psi, body (u32), id (u64); world x y z, v, predicted v,
+//              reaction force (f32), per-body SoA state (f32) with FP64 force/torque sums
+//   cell grid  dense over the cell frame, one uint2 {begin, end} per cell and particle class
+//   SELL-32    per slice of 32 consecutive fluid particles and relation (FF, FB, FR): one
+//              SliceMeta {offset in u16 units, base, pitch, format}; entry k of lane l at
+//              offset + k*32 + l, as u16 (j - base) or u32 j
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tsp
+{
+    constexpr int kBlock = 256;          // threads per block: 4 waves of 64
+    constexpr uint32_t kSlice = 32;      // SELL slice height (a wave covers two slices)
+    constexpr uint32_t kEmpty32 = 0xffffffffu;
+    constexpr uint32_t kFmtDelta16 = 0;
+    constexpr uint32_t kFmtAbs32 = 1;
+    constexpr uint32_t kWideSpanBit = 0x8000u; // FF count flag: neighbors span more than 0xffff
+
+    enum Rel { REL_FF = 0, REL_FB = 1, REL_FR = 2 };
+
+    struct SliceMeta
+    {
+        uint32_t offset; // in u16 units from the start of the pool
+        uint32_t base;
+        uint16_t pitch;
+        uint16_t format;
+        uint32_t pad;
+    };
+
+    // solver status shared by the kernels of one step and read back by the host
+    struct Status
+    {
+        // neighbor search
+        uint32_t abort;          // a particle left the inner cell frame: the step must restart
+        uint32_t overflow;       // a SELL pool is too small
+        uint32_t bUnsorted;      // boundary ids are not non-decreasing under the frame
+        uint32_t nOccF, nOccR;   // occupied cells written by the last table build
+        uint32_t pad0;
+        uint64_t poolUnits[3];   // u16 units used by FF, FB, FR
+        uint64_t entries[3];     // entries (without padding) of FF, FB, FR
+        uint64_t rfTotal, rbTotal, rrTotal;
+        // pressure solve (SimBackend_CPU members N_solverIters_, avgRelDensityErrorF_*)
+        uint32_t iters;
+        uint32_t compute;        // the current sweep reduces errors
+        uint32_t converged;
+        uint32_t sweeps;
+        float avgRaw, avgPos, avgAbs;
+        uint32_t pad1;
+        double sumRaw, sumPos, sumAbs;
+        // rigid
+        uint32_t rigidDegenerate;
+        uint32_t pad2;
+    };
+
+    struct Frame
+    {
+        int32_t orig[3];
+        uint32_t dim[3];
+        float cellSizeInv;
+        uint32_t emptyKey;   // 1 << (3 * axisBits): sorts behind every in-frame id
+        int32_t innerLo[3], innerHi[3];
+        int32_t domainSet;
+        float domMin[3], domMax[3];
+    };
+
+    struct Consts
+    {
+        float dt, hinv, hsq, alpha, rho0, rho0SqInv, r2, mass;
+        float ksize, ktwo, kinSlope, kinOff, koutScale;
+        float weightScale;     // mass * alpha * hinv^3 (dict FF density)
+        float hsqEps;          // 0.01 h^2
+        float visF, visB;      // 10 nu
+        float gx, gy, gz;
+        float cFacF, cFacB, dFacB, omega, warm;
+        float presDis, presOuterB, presOuterFF;
+        float dtsq;
+        float densityDevTarget;
+        uint32_t minIters, maxIters, fixedIters, termCond;
+        uint32_t nF;
+    };
+
+    // fields gathered by one permutation launch
+    struct FieldSet
+    {
+        const float* src[16];
+        float* dst[16];
+        const uint64_t* idSrc;
+        uint64_t* idDst;
+        uint32_t nFields;
+        uint32_t n;
+    };
+
+    // device pointers of one context (passed by value to every kernel)
+    struct Dev
+    {
+        uint32_t nF, nB, nR, nRB;
+        // fluid
+        float *x, *y, *z, *vx, *vy, *vz, *vpx, *vpy, *vpz, *apx, *apy, *apz;
+        float *source, *diag, *pressure, *density;
+        uint64_t *id;
+        // boundary
+        float *bx, *by, *bz, *bvol, *bpsi;
+        // rigid
+        float *rxrel, *ryrel, *rzrel, *rvol, *rpsi;
+        uint32_t *rbody;
+        float *rx, *ry, *rz, *rvx, *rvy, *rvz, *rvpx, *rvpy, *rvpz, *rfx, *rfy, *rfz;
+        float *rb;               // nRB x 32 state (TSP_RB_* field order, field-major)
+        float *rbPred;           // nRB x 9: v, L, omega predicted
+        float *rbForce;          // nRB x 6: force, torque (float, like the reference members)
+        uint32_t *byBodyOff, *byBodyIdx;
+        // neighbor search
+        uint32_t *keyF, *keyF2, *idxF, *idxF2;
+        uint32_t *keyR, *keyR2, *idxR, *idxR2;
+        uint2 *cellF, *cellB, *cellR;
+        uint32_t *occF, *occR;
+        // SELL-32
+        uint16_t *cnt[3];
+        SliceMeta *meta[3];
+        uint16_t *pool[3];
+        uint64_t poolCap[3];     // u16 units
+        uint32_t *ffWideMin;     // per particle: min FF index when the wide-span bit is set
+        // rigid-source CSR (RF), counts of RB and RR
+        uint32_t *rfCnt, *rfOff, *rfIdx;
+        uint32_t *rbCnt, *rrCnt;
+        uint64_t rfCap;
+        // reductions
+        double *errPartial;      // 3 per block of the Jacobi kernel
+        Status *status;
+        Frame *frame;
+    };
+
+    // --- FP32 helpers matching the reference's operation order (no contraction) ---
+
+    __device__ __forceinline__ float vmax(float a, float b) { return a > b ? a : b; }
+
+    // FloorToInt, CPU/backend/neighbor_search.cpp:157-162
+    __device__ __forceinline__ int floorToInt(float v)
+    {
+        const int i = static_cast<int>(v);
+        return (v < static_cast<float>(i)) ? i - 1 : i;
+    }
+
+    __device__ __forceinline__ uint32_t spread10(uint32_t v)
+    {
+        uint32_t x = v & 0x3ffu;
+        x = (x | (x << 16)) & 0x030000ffu;
+        x = (x | (x << 8)) & 0x0300f00fu;
+        x = (x | (x << 4)) & 0x030c30c3u;
+        x = (x | (x << 2)) & 0x09249249u;
+        return x;
+    }
+
+    __device__ __forceinline__ uint32_t compact10(uint32_t x)
+    {
+        x &= 0x09249249u;
+        x = (x | (x >> 2)) & 0x030c30c3u;
+        x = (x | (x >> 4)) & 0x0300f00fu;
+        x = (x | (x >> 8)) & 0x030000ffu;
+        x = (x | (x >> 16)) & 0x3ffu;
+        return x;
+    }
+
+    // Morton id of local cell coordinates (x bits at 0,3,6..., y at 1,4,..., z at 2,5,...),
+    // MapToCellId_Morton, neighbor_search.cpp:2032-2055, for frames of up to 1024 cells per axis
+    __device__ __forceinline__ uint32_t morton(uint32_t lx, uint32_t ly, uint32_t lz)
+    {
+        return spread10(lx) | (spread10(ly) << 1) | (spread10(lz) << 2);
+    }
+
+    __device__ __forceinline__ uint32_t linearCell(const Frame& f, int lx, int ly, int lz)
+    {
+        return static_cast<uint32_t>(lx) + f.dim[0] * (static_cast<uint32_t>(ly) + f.dim[1] * static_cast<uint32_t>(lz));
+    }
+
+    // KernelWeight_CubicSpline_SIMD, CPU/simd.h:279-309
+    __device__ __forceinline__ float kwNormal(float dx, float dy, float dz, const Consts& k)
+    {
+        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        const float dist = __builtin_sqrtf(d2);
+        const float q = dist * k.hinv;
+        const float t1 = vmax(1.0f - q, 0.0f);
+        const float t2 = vmax(2.0f - q, 0.0f);
+        const float t1_2 = t1 * t1, t2_2 = t2 * t2;
+        const float t1_3 = t1_2 * t1, t2_3 = t2_2 * t2;
+        const float w = k.alpha * __builtin_fmaf(-4.0f, t1_3, t2_3);
+        return d2 < k.r2 ? w : 0.0f;
+    }
+
+    // KernelWeightUnscaled_CubicSpline_SIMD, CPU/simd.h:392-404
+    __device__ __forceinline__ float kwUnscaled(float d2, const Consts& k)
+    {
+        const float dist = __builtin_sqrtf(d2);
+        const float t1 = vmax(k.ksize - dist, 0.0f);
+        const float t2 = vmax(k.ktwo - dist, 0.0f);
+        return __builtin_fmaf(-4.0f, t1 * t1 * t1, t2 * t2 * t2);
+    }
+
+    // KernelGradientFactorNoMinDist_CubicSpline_SIMD, CPU/simd.h:374-388
+    __device__ __forceinline__ float kgFactor(float d2, const Consts& k)
+    {
+        const float dist = __builtin_sqrtf(d2);
+        const float inner = __builtin_fmaf(k.kinSlope, dist, k.kinOff);
+        const float t2 = vmax(k.ktwo - dist, 0.0f);
+        const float outer = (t2 * t2) * (k.koutScale / dist);
+        return dist < k.ksize ? inner : outer;
+    }
+
+    // KernelGradient_CubicSpline_Scalar, CPU/simd.h:311-347 (rigid reaction forces)
+    __device__ __forceinline__ void kgScalar(float dx, float dy, float dz, const Consts& k, float& gx, float& gy, float& gz)
+    {
+        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        if (d2 >= k.r2) { gx = gy = gz = 0.0f; return; }
+        const float dist = __builtin_sqrtf(d2);
+        const float q = dist * k.hinv;
+        float f;
+        if (q < 1.0f)
+        {
+            f = k.alpha * k.hinv * k.hinv * __builtin_fmaf(9.0f, q, -12.0f);
+        }
+        else
+        {
+            const float t2v = 2.0f - q;
+            const float t2 = t2v < 0.0f ? 0.0f : t2v;
+            f = k.alpha * (-3.0f * t2 * t2) * (k.hinv / dist);
+        }
+        gx = f * dx; gy = f * dy; gz = f * dz;
+    }
+
+    // entry k of lane `lane` of a SELL-32 slice
+    __device__ __forceinline__ uint32_t sellRead(const uint16_t* pool, const SliceMeta& m, uint32_t lane, uint32_t k)
+    {
+        if (m.format == kFmtDelta16) { return m.base + static_cast<uint32_t>(pool[m.offset + k * kSlice + lane]); }
+        return reinterpret_cast<const uint32_t*>(pool + m.offset)[k * kSlice + lane];
+    }
+
+    __device__ __forceinline__ void sellWrite(uint16_t* pool, const SliceMeta& m, uint32_t lane, uint32_t k, uint32_t j)
+    {
+        if (m.format == kFmtDelta16) { pool[m.offset + k * kSlice + lane] = static_cast<uint16_t>(j - m.base); }
+        else { reinterpret_cast<uint32_t*>(pool + m.offset)[k * kSlice + lane] = j; }
+    }
+
+    // blocks are dispatched round-robin over the 8 XCDs; this maps them so that each XCD walks
+    // one contiguous range of particles, which keeps neighbor gathers inside its own L2
+    __device__ __forceinline__ uint32_t xcdBlock()
+    {
+        const uint32_t nb = gridDim.x, b = blockIdx.x;
+        const uint32_t xcd = b & 7u, k = b >> 3;
+        const uint32_t per = nb >> 3, rem = nb & 7u;
+        return xcd < rem ? xcd * (per + 1u) + k : rem * (per + 1u) + (xcd - rem) * per + k;
+    }
+}
