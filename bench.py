@@ -1,0 +1,146 @@
+#!/usr/bin/env python3
+"""IISPH particle-steps/s of the gfx950 step on one MI355X.
+
+Workload: the rigid-body coupling dam break -- 1,000,000 fluid particles (100^3 lattice at
+h = 0.004, seeded +-1% jitter) in a 1.0 x 0.5 x 0.5 tank with 320,564 wall particles, and a
+hollow 0.1 m cube of density 500 (6,364 rigid particles) released at rest at height 0.3 with a
+seeded random rotation. dt 1e-3, omega 0.5, solve to 0.1% average compression error with 2..100
+solver iterations. A step is one full IISPH step, neighbor search included; the state stays on
+the device, inputs are resident before the timed region starts.
+
+Prints one JSON line: value = N_F * steps / wall time of the timed steps; roofline of the
+dominant kernel from HIP events recorded on the context's stream during the timed steps; the
+CPU baseline is the oracle (CPU restatement of the reference TBB path) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "IISPH particle-steps/s at 1M; batched-vs-plain throughput; p99 scene latency"
+HBM_PEAK_GBS = 8000.0
+H = 0.004
+N_SIDE = 100
+
+
+def algorithmic_bytes(info, n_f):
+    """Compulsory bytes per fluid particle of each kernel (each SoA array counted once per
+    pass; neighbor gathers hit in cache). L: the list bytes per particle at 2 B per entry, the
+    three u16 counts and the slice metadata."""
+    entries = (info.n_ff + info.n_fb + info.n_fr) / max(n_f, 1)
+    L = 2.0 * entries + 6.0 + 48.0 / 32.0
+    return {
+        "density": 12 + L + 4,
+        "nonpressure": 12 + 12 + 4 + L + 12,
+        "source_diag": 12 + 12 + 4 + L + 12,
+        "pressure_acc": 12 + 4 + L + 12,
+        "pressure_acc_final": 12 + 4 + L + 12,
+        "jacobi": 12 + 12 + 4 + 4 + 8 + L,
+        "integration": 60.0,
+        "_L": L,
+    }
+
+
+def cpu_baseline(state, params, steps, threads):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    o = O.Oracle(params, chunk_width=1, threads=threads)
+    o.pull(state)
+    t0 = time.perf_counter()
+    sweeps = 0
+    for _ in range(steps):
+        o.step()
+        sweeps += max(o.info().solver_iters - 1, 0)
+    dt = time.perf_counter() - t0
+    return state.n[0] * steps / dt, dt, sweeps / steps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--n-side", type=int, default=N_SIDE)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs one scene on one GPU; --gpus {args.gpus} is not supported "
+              "(independent scenes would be spread over GPUs by HBM footprint).", file=sys.stderr)
+        sys.exit(2)
+
+    from turbospharticles_amd import scenes
+    from turbospharticles_amd.backend import HipBackend
+
+    h = H * N_SIDE / args.n_side
+    spec = scenes.rigid_cube_spec(args.n_side, h, side=0.1, density=500.0, center=(0.7, 0.3, 0.25),
+                                  rotation_seed=7)
+    state = scenes.build_scene(spec)
+    params = scenes.make_params(h, termination_cond=scenes.TERM_POS)
+    n_f = state.n[0]
+
+    gpu = HipBackend(0)
+    gpu.set_params(params)
+    gpu.pull(state)
+    gpu.step(args.warmup)
+    gpu.synchronize()
+    sweeps0 = gpu.info().sweeps_total
+
+    gpu.profile_enable(True)
+    gpu.synchronize()
+    t0 = time.perf_counter()
+    gpu.step(args.steps)
+    gpu.synchronize()
+    t1 = time.perf_counter()
+    prof = gpu.profile_read()
+    gpu.profile_enable(False)
+    info = gpu.info()
+    wall = t1 - t0
+    value = n_f * args.steps / wall
+    k_bar = (info.sweeps_total - sweeps0) / args.steps
+
+    ab = algorithmic_bytes(info, n_f)
+    candidates = {k: v for k, v in prof.items() if k in ab}
+    dominant = max(candidates, key=lambda k: candidates[k][1]) if candidates else None
+    roof = None
+    if dominant:
+        launches, ms = candidates[dominant]
+        avg_s = ms / 1e3 / launches
+        bytes_launch = ab[dominant] * n_f
+        achieved = bytes_launch / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": int(bytes_launch), "avg_launch_us": round(avg_s * 1e6, 2)}
+    b_step = 846.0 + 214.0 * k_bar
+    step_gbs = n_f * b_step * args.steps / wall / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        v, dt, kc = cpu_baseline(state, params, args.cpu_steps, args.cpu_threads)
+        cpu = {"value": round(v, 1), "unit": "particle-steps/s", "cores": args.cpu_threads, "kind": "port",
+               "sample": f"oracle (C restatement of the TBB path, OpenMP) on the first {args.cpu_steps} steps "
+                         f"of the same workload from its initial state: {dt:.1f} s, mean {kc:.1f} Jacobi sweeps/step"}
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"rigid cube drop, {n_f} fluid ({args.n_side}^3, h={h:g}) + {state.n[1]} boundary "
+                               f"+ {state.n[2]} rigid, dt 1e-3, 0.1% avg compression error, 2..100 iterations",
+                   "mean_jacobi_sweeps": round(k_bar, 2),
+                   "step_algorithmic_GBps": round(step_gbs, 1),
+                   "kernel_ms": {k: round(v[1], 3) for k, v in prof.items()}},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -110,3 +110,41 @@ def test_ten_steps_match_oracle(scene64k):
     assert np.array_equal(sg.fluid_id, so.fluid_id)
     for f in ("x", "y", "z", "vx", "vy", "vz", "density", "pressure"):
         assert rel_err(sg.fluid[f], so.fluid[f]) <= 1e-5, f
+
+
+def _cube_scene():
+    # 16^3 fluid block in a 0.4 x 0.3 x 0.3 tank; a 0.1 cube resting on the fluid surface
+    spec = scenes.rigid_cube_spec(16, H, side=0.1, density=500.0, center=(0.1, 0.2, 0.1),
+                                  rotation_seed=3, tank=(0.4, 0.3, 0.3))
+    return scenes.build_scene(spec)
+
+
+def test_rigid_coupling_matches_oracle():
+    print("case: rigid cube on a 16^3 dam break, 5 steps", flush=True)
+    st = _cube_scene()
+    assert st.n[2] > 0 and st.n[3] == 1
+    p = _params()
+    g = _gpu(st, p)
+    o = _oracle(st, p)
+    for s in range(5):
+        g.step(1)
+        o.step()
+        assert g.info().solver_iters == o.info().solver_iters, s
+    sg, so = g.push(), o.push()
+    assert np.array_equal(sg.fluid_id, so.fluid_id)
+    # fluid-rigid lists exist and match
+    cg, ig = g.neighbors(O.FR)
+    co, io, _ = o.lists(O.FR)
+    assert co.sum() > 0
+    assert np.array_equal(cg, co) and np.array_equal(ig, io)
+    cg, ig = g.neighbors(O.RF)
+    co, io, _ = o.lists(O.RF)
+    assert np.array_equal(cg, co) and np.array_equal(ig, io)
+    for f in ("x", "y", "z", "vx", "vy", "vz", "density", "pressure"):
+        assert rel_err(sg.fluid[f], so.fluid[f]) <= 1e-5, f
+    for f in ("xcm", "ycm", "zcm", "vx", "vy", "vz", "wx", "wy", "wz", "a00", "a11", "a22", "a01"):
+        assert rel_err(sg.body[f], so.body[f]) <= 1e-5, f
+    xg, vg = g.rigid_world()
+    xo, vo = o.rigid_world()
+    assert rel_err(xg, xo) <= 1e-6
+    assert rel_err(vg, vo) <= 1e-5

@@ -652,9 +652,11 @@ namespace tsp
     }
     void launchJacobi(hipStream_t s, const Dev& d, const Consts& k)
     {
-        if (!d.nF) { return; }
-        kJacobi<<<blocksOf(d.nF), kBlock, 0, s>>>(d, k);
-        kFinalize<<<1, kBlock, 0, s>>>(d, k, blocksOf(d.nF));
+        if (d.nF) { kJacobi<<<blocksOf(d.nF), kBlock, 0, s>>>(d, k); }
+    }
+    void launchFinalize(hipStream_t s, const Dev& d, const Consts& k)
+    {
+        if (d.nF) { kFinalize<<<1, kBlock, 0, s>>>(d, k, blocksOf(d.nF)); }
     }
     void launchIntegration(hipStream_t s, const Dev& d, const Consts& k) { if (d.nF) { kIntegration<<<blocksOf(d.nF), kBlock, 0, s>>>(d, k); } }
 

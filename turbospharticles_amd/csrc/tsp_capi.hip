@@ -41,6 +41,7 @@ namespace tsp
     void launchSourceDiag(hipStream_t, const Dev&, const Consts&);
     void launchPressureAcc(hipStream_t, const Dev&, const Consts&, bool);
     void launchJacobi(hipStream_t, const Dev&, const Consts&);
+    void launchFinalize(hipStream_t, const Dev&, const Consts&);
     void launchIntegration(hipStream_t, const Dev&, const Consts&);
     void launchRigidUpdate(hipStream_t, const Dev&, bool);
     void launchRigidClearGravity(hipStream_t, const Dev&, const Consts&);
@@ -544,8 +545,12 @@ struct tsp_ctx
                     Scope sc(prof, stream, "pressure_acc");
                     launchPressureAcc(stream, d, k, true);
                 }
-                Scope sc(prof, stream, "jacobi");
-                launchJacobi(stream, d, k);
+                {
+                    Scope sc(prof, stream, "jacobi");
+                    launchJacobi(stream, d, k);
+                }
+                Scope sc(prof, stream, "finalize");
+                launchFinalize(stream, d, k);
             }
             queued += n;
             readStatus();
