@@ -7,20 +7,103 @@
 //   sort      stable LSD radix sort of (key, index), 8-bit digits, keys clamped so that
 //             out-of-frame particles sort last (neighbor_search.cpp:403-564); per tile the
 //             stable ranks come from wave64 ballots, not from a serial scan
-//   permute   one gather of id, x, y, z, vx, vy, vz (+ pressure with warm start) per particle
-//   cells     dense {begin, end} table over the frame, written at the runs of the sorted keys
+//   permute   one gather of the packed particle records per particle
+//   cells     dense {begin, end} table over the frame, written at the runs of the sorted keys;
+//             the previous step's runs are cleared first, so no global counter is needed
 //   count     per particle: FF/FB/FR counts over the 27-cell stencil (inclusive cutoff,
 //             distance fma(dx,dx,fma(dy,dy,dz*dz)), neighbor_search.cpp:1130-1249); per
 //             32-particle slice: pitch, base and u16/u32 format by width-32 lane reductions
-//   scan      per relation: offsets of the slices in their pool (pitch * 32 entries each)
+//   scan      multi-block exclusive scans (slice offsets, radix digit offsets, RF offsets)
 //   fill      writes the entries in the reference's iteration order: z-y-x stencil raster,
-//             ascending index within a cell; FF lists whose indices span more than 0xffff keep
-//             the DictDelta16x2 order of the reference's per-particle dict chunks (low group
-//             first, high group reversed, CPU/backend/neighbor_lists.h:171-197)
+//             ascending index within a cell; FF entries are staged in LDS and written as whole
+//             slice rows. FF lists whose indices span more than 0xffff keep the DictDelta16x2
+//             order of the reference's per-particle dict chunks (low group first, high group
+//             reversed, CPU/backend/neighbor_lists.h:171-197)
 #include "tsp_device.h"
 
 namespace tsp
 {
+    // ----------------------------------------------------------------------------------------
+    // multi-block exclusive scan of u32 values (tiles of 2048 = 256 threads x 8)
+
+    constexpr uint32_t kScanTile = 2048;
+
+    __device__ __forceinline__ uint32_t blockExclusiveScan(uint32_t v, uint32_t& total)
+    {
+        __shared__ uint32_t s[kBlock];
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (uint32_t off = 1; off < kBlock; off <<= 1)
+        {
+            const uint32_t a = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+            __syncthreads();
+            s[threadIdx.x] += a;
+            __syncthreads();
+        }
+        total = s[kBlock - 1];
+        const uint32_t r = s[threadIdx.x] - v;
+        __syncthreads();
+        return r;
+    }
+
+    __global__ void __launch_bounds__(kBlock) kScanSums(const uint32_t* __restrict__ v, uint32_t m, uint32_t* __restrict__ sums)
+    {
+        const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 8u;
+        uint32_t s = 0;
+        for (uint32_t q = 0; q < 8; q++) { if (base + q < m) { s += v[base + q]; } }
+        uint32_t total;
+        blockExclusiveScan(s, total);
+        if (threadIdx.x == 0) { sums[blockIdx.x] = total; }
+    }
+
+    // one block: exclusive scan of the tile sums (any count)
+    __global__ void __launch_bounds__(kBlock) kScanTop(uint32_t* __restrict__ sums, uint32_t nb)
+    {
+        uint32_t carry = 0;
+        for (uint32_t b0 = 0; b0 < nb; b0 += kBlock)
+        {
+            const uint32_t b = b0 + threadIdx.x;
+            const uint32_t v = b < nb ? sums[b] : 0u;
+            uint32_t total;
+            const uint32_t r = blockExclusiveScan(v, total);
+            if (b < nb) { sums[b] = carry + r; }
+            carry += total;
+        }
+    }
+
+    __global__ void __launch_bounds__(kBlock) kScanApply(uint32_t* __restrict__ v, uint32_t m, const uint32_t* __restrict__ sums)
+    {
+        const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 8u;
+        uint32_t loc[8];
+        uint32_t s = 0;
+        for (uint32_t q = 0; q < 8; q++) { loc[q] = base + q < m ? v[base + q] : 0u; s += loc[q]; }
+        uint32_t total;
+        uint32_t run = sums[blockIdx.x] + blockExclusiveScan(s, total);
+        for (uint32_t q = 0; q < 8; q++)
+        {
+            if (base + q < m) { v[base + q] = run; }
+            run += loc[q];
+        }
+    }
+
+    // exclusive scan of m values in place; tmp holds ceil(m / 2048) block sums
+    void launchScan(hipStream_t s, uint32_t* v, uint32_t m, uint32_t* tmp)
+    {
+        if (m == 0) { return; }
+        const uint32_t nb = (m + kScanTile - 1) / kScanTile;
+        kScanSums<<<nb, kBlock, 0, s>>>(v, m, tmp);
+        kScanTop<<<1, kBlock, 0, s>>>(tmp, nb);
+        kScanApply<<<nb, kBlock, 0, s>>>(v, m, tmp);
+    }
+
+    // clears the per-search words of the status block (everything before the solver state)
+    __global__ void kResetSearch(Status* st)
+    {
+        constexpr uint32_t words = static_cast<uint32_t>(offsetof(Status, iters) / sizeof(uint32_t));
+        static_assert(words <= 64, "the reset covers one wave");
+        if (threadIdx.x < words) { reinterpret_cast<uint32_t*>(st)[threadIdx.x] = 0u; }
+    }
+
     // ----------------------------------------------------------------------------------------
     // keys
 
@@ -54,29 +137,33 @@ namespace tsp
         return morton(static_cast<uint32_t>(lx), static_cast<uint32_t>(ly), static_cast<uint32_t>(lz));
     }
 
-    __global__ void __launch_bounds__(kBlock) kKeys(const float* __restrict__ x, const float* __restrict__ y,
-        const float* __restrict__ z, uint32_t n, const Frame* __restrict__ frame, Status* st, bool check,
-        uint32_t* __restrict__ key, uint32_t* __restrict__ idx)
+    __global__ void __launch_bounds__(kBlock) kKeys(const float4* __restrict__ p, uint32_t n,
+        const Frame* __restrict__ frame, Status* st, bool check, uint32_t* __restrict__ key, uint32_t* __restrict__ idx)
     {
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) { return; }
         const Frame f = *frame;
-        key[i] = cellKey(f, x[i], y[i], z[i], check, st);
+        const float4 q = p[i];
+        key[i] = cellKey(f, q.x, q.y, q.z, check, st);
         idx[i] = i;
     }
 
     // boundary keys; flags a descent so the host knows whether the order needs a sort
-    __global__ void __launch_bounds__(kBlock) kKeysSorted(const float* __restrict__ x, const float* __restrict__ y,
-        const float* __restrict__ z, uint32_t n, const Frame* __restrict__ frame, Status* st,
-        uint32_t* __restrict__ key, uint32_t* __restrict__ idx)
+    __global__ void __launch_bounds__(kBlock) kKeysSorted(const float4* __restrict__ p, uint32_t n,
+        const Frame* __restrict__ frame, Status* st, uint32_t* __restrict__ key, uint32_t* __restrict__ idx)
     {
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) { return; }
         const Frame f = *frame;
-        const uint32_t k = cellKey(f, x[i], y[i], z[i], false, st);
+        const float4 q = p[i];
+        const uint32_t k = cellKey(f, q.x, q.y, q.z, false, st);
         key[i] = k;
         idx[i] = i;
-        if (i > 0 && cellKey(f, x[i - 1], y[i - 1], z[i - 1], false, st) > k) { atomicOr(&st->bUnsorted, 1u); }
+        if (i > 0)
+        {
+            const float4 r = p[i - 1];
+            if (cellKey(f, r.x, r.y, r.z, false, st) > k) { atomicOr(&st->bUnsorted, 1u); }
+        }
     }
 
     // ----------------------------------------------------------------------------------------
@@ -99,33 +186,6 @@ namespace tsp
         }
         __syncthreads();
         hist[threadIdx.x * nTiles + blockIdx.x] = cnt[threadIdx.x];
-    }
-
-    // exclusive scan of m values in one block of 1024 threads (digit-major histogram)
-    __global__ void __launch_bounds__(1024) kScanSingle(uint32_t* __restrict__ v, uint32_t m)
-    {
-        __shared__ uint32_t part[1024];
-        const uint32_t t = threadIdx.x;
-        const uint32_t per = (m + 1023u) / 1024u;
-        const uint32_t b = t * per, e = min(b + per, m);
-        uint32_t s = 0;
-        for (uint32_t i = b; i < e; i++) { s += v[i]; }
-        part[t] = s;
-        __syncthreads();
-        for (uint32_t off = 1; off < 1024; off <<= 1)
-        {
-            const uint32_t a = t >= off ? part[t - off] : 0u;
-            __syncthreads();
-            part[t] += a;
-            __syncthreads();
-        }
-        uint32_t run = part[t] - s;
-        for (uint32_t i = b; i < e; i++)
-        {
-            const uint32_t c = v[i];
-            v[i] = run;
-            run += c;
-        }
     }
 
     __global__ void __launch_bounds__(kBlock) kRadixScatter(const uint32_t* __restrict__ keyIn,
@@ -180,27 +240,35 @@ namespace tsp
     // ----------------------------------------------------------------------------------------
     // permutation (gather into the second buffers)
 
-    // gathers up to 16 float fields and the ids by the sorted permutation
-    __global__ void __launch_bounds__(kBlock) kPermute(FieldSet fs, const uint32_t* __restrict__ idx)
+    __global__ void __launch_bounds__(kBlock) kPermute(Gather g, const uint32_t* __restrict__ idx)
     {
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-        if (i >= fs.n) { return; }
+        if (i >= g.n) { return; }
         const uint32_t o = idx[i];
-        for (uint32_t f = 0; f < fs.nFields; f++) { fs.dst[f][i] = fs.src[f][o]; }
-        if (fs.idSrc) { fs.idDst[i] = fs.idSrc[o]; }
+        for (uint32_t f = 0; f < g.n4; f++) { g.d4[f][i] = g.s4[f][o]; }
+        for (uint32_t f = 0; f < g.n1; f++) { g.d1[f][i] = g.s1[f][o]; }
+        if (g.idS) { g.idD[i] = g.idS[o]; }
     }
 
     // ----------------------------------------------------------------------------------------
     // cell tables
 
-    __global__ void __launch_bounds__(kBlock) kClearCells(uint2* __restrict__ cell, const uint32_t* __restrict__ occ,
-        const uint32_t* __restrict__ nOcc)
+    __device__ __forceinline__ uint32_t linearOfKey(const Frame& f, uint32_t k)
     {
-        const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-        if (q < *nOcc) { cell[occ[q]] = make_uint2(0u, 0u); }
+        return linearCell(f, compact10(k), compact10(k >> 1), compact10(k >> 2));
     }
 
-    __global__ void kSetU32(uint32_t* p, uint32_t v) { *p = v; }
+    // clears the cells of the runs of the previous sorted keys
+    __global__ void __launch_bounds__(kBlock) kClearCells(const uint32_t* __restrict__ key, uint32_t n,
+        const Frame* __restrict__ frame, uint2* __restrict__ cell)
+    {
+        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= n) { return; }
+        const Frame& f = *frame;
+        const uint32_t k = key[i];
+        if (k == f.emptyKey) { return; }
+        if (i == 0 || key[i - 1] != k) { cell[linearOfKey(f, k)] = make_uint2(0u, 0u); }
+    }
 
     __global__ void __launch_bounds__(kBlock) kClearAll(uint2* __restrict__ cell, uint64_t n)
     {
@@ -209,20 +277,19 @@ namespace tsp
     }
 
     __global__ void __launch_bounds__(kBlock) kBuildCells(const uint32_t* __restrict__ key, uint32_t n,
-        const Frame* __restrict__ frame, uint2* __restrict__ cell, uint32_t* __restrict__ occ, uint32_t* nOcc)
+        const Frame* __restrict__ frame, uint2* __restrict__ cell)
     {
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) { return; }
         const Frame& f = *frame;
         const uint32_t k = key[i];
         if (k == f.emptyKey) { return; }
-        const uint32_t lin = linearCell(f, compact10(k), compact10(k >> 1), compact10(k >> 2));
-        if (i == 0 || key[i - 1] != k)
-        {
-            cell[lin].x = i;
-            occ[atomicAdd(nOcc, 1u)] = lin;
-        }
-        if (i + 1 == n || key[i + 1] != k) { cell[lin].y = i + 1; }
+        const bool first = i == 0 || key[i - 1] != k;
+        const bool last = i + 1 == n || key[i + 1] != k;
+        if (!first && !last) { return; }
+        const uint32_t lin = linearOfKey(f, k);
+        if (first) { cell[lin].x = i; }
+        if (last) { cell[lin].y = i + 1; }
     }
 
     // ----------------------------------------------------------------------------------------
@@ -246,9 +313,9 @@ namespace tsp
         return true;
     }
 
-    __device__ __forceinline__ bool within(float xi, float yi, float zi, float xj, float yj, float zj, float r2)
+    __device__ __forceinline__ bool within(float xi, float yi, float zi, const float4& pj, float r2)
     {
-        const float dx = xi - xj, dy = yi - yj, dz = zi - zj;
+        const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
         return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) <= r2;
     }
 
@@ -265,6 +332,12 @@ namespace tsp
         return v;
     }
 
+    __device__ __forceinline__ uint32_t sliceSum(uint32_t v)
+    {
+        for (int o = 16; o > 0; o >>= 1) { v += static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o, 32)); }
+        return v;
+    }
+
     __global__ void __launch_bounds__(kBlock) kQueryCount(Dev d, float r2, int reach, uint32_t nSlices)
     {
         const uint32_t blk = xcdBlock();
@@ -276,7 +349,7 @@ namespace tsp
         Stencil s;
         if (i < d.nF && stencilOf(f, d.keyF[i], reach, s))
         {
-            const float xi = d.x[i], yi = d.y[i], zi = d.z[i];
+            const float4 pi = d.pos[i];
             for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
             {
                 for (int cy = s.lo[1]; cy <= s.hi[1]; cy++)
@@ -287,7 +360,7 @@ namespace tsp
                         const uint2 rf = d.cellF[lin];
                         for (uint32_t j = rf.x; j < rf.y; j++)
                         {
-                            if (within(xi, yi, zi, d.x[j], d.y[j], d.z[j], r2))
+                            if (within(pi.x, pi.y, pi.z, d.pos[j], r2))
                             {
                                 cnt[0]++; mn[0] = min(mn[0], j); mx[0] = max(mx[0], j);
                             }
@@ -295,7 +368,7 @@ namespace tsp
                         const uint2 rb = d.cellB[lin];
                         for (uint32_t j = rb.x; j < rb.y; j++)
                         {
-                            if (within(xi, yi, zi, d.bx[j], d.by[j], d.bz[j], r2))
+                            if (within(pi.x, pi.y, pi.z, d.bpos[j], r2))
                             {
                                 cnt[1]++; mn[1] = min(mn[1], j); mx[1] = max(mx[1], j);
                             }
@@ -306,7 +379,7 @@ namespace tsp
                             for (uint32_t t = rr.x; t < rr.y; t++)
                             {
                                 const uint32_t j = d.idxR[t];
-                                if (within(xi, yi, zi, d.rx[j], d.ry[j], d.rz[j], r2))
+                                if (within(pi.x, pi.y, pi.z, d.rpos[j], r2))
                                 {
                                     cnt[2]++; mn[2] = min(mn[2], j); mx[2] = max(mx[2], j);
                                 }
@@ -325,13 +398,14 @@ namespace tsp
             if (wide) { d.ffWideMin[i] = mn[0]; }
             if (cnt[0] >= kWideSpanBit || cnt[1] > 0xffffu || cnt[2] > 0xffffu) { atomicOr(&d.status->overflow, 2u); }
         }
-        // per slice: pitch, base (min index), format
+        // per slice: pitch, base (min index), format, entries
         const uint32_t slice = i / kSlice;
         for (int r = 0; r < 3; r++)
         {
             const uint32_t pitch = sliceMax(cnt[r]);
             const uint32_t smin = sliceMin(mn[r]);
             const uint32_t smax = sliceMax(mx[r]);
+            const uint32_t sum = sliceSum(cnt[r]);
             if ((threadIdx.x & (kSlice - 1)) == 0 && slice < nSlices)
             {
                 SliceMeta m;
@@ -339,161 +413,162 @@ namespace tsp
                 const bool delta = pitch == 0 || smax - smin <= 0xffffu;
                 m.format = delta ? kFmtDelta16 : kFmtAbs32;
                 m.base = (pitch == 0 || !delta) ? 0u : smin;
-                m.offset = pitch * kSlice * (delta ? 1u : 2u); // storage units; the scan turns this into an offset
-                m.pad = 0;
+                m.offset = pitch * kSlice * (delta ? 1u : 2u);
+                m.entries = sum;
                 d.meta[r][slice] = m;
             }
         }
     }
 
-    // offsets of the slices of the three relations; one block per relation
-    __global__ void __launch_bounds__(1024) kScanSlices(Dev d, uint32_t nSlices)
+    // storage units of every slice into a scan buffer of nSlices + 1 entries per relation
+    __global__ void __launch_bounds__(kBlock) kSliceUnits(Dev d, uint32_t nSlices, uint32_t* __restrict__ units)
     {
-        __shared__ uint64_t part[1024];
-        const uint32_t rel = blockIdx.x;
-        SliceMeta* meta = d.meta[rel];
-        const uint32_t t = threadIdx.x;
-        const uint32_t per = (nSlices + 1023u) / 1024u;
-        const uint32_t b = t * per, e = min(b + per, nSlices);
-        uint64_t s = 0;
-        for (uint32_t q = b; q < e; q++) { s += meta[q].offset; }
-        part[t] = s;
-        __syncthreads();
-        for (uint32_t off = 1; off < 1024; off <<= 1)
-        {
-            const uint64_t a = t >= off ? part[t - off] : 0ull;
-            __syncthreads();
-            part[t] += a;
-            __syncthreads();
-        }
-        uint64_t run = part[t] - s;
-        for (uint32_t q = b; q < e; q++)
-        {
-            const uint32_t units = meta[q].offset;
-            meta[q].offset = static_cast<uint32_t>(run);
-            run += units;
-        }
-        if (t == 1023)
-        {
-            d.status->poolUnits[rel] = part[1023];
-            if (part[1023] > d.poolCap[rel] || part[1023] > 0xffffffffull) { atomicOr(&d.status->overflow, 1u); }
-        }
+        const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+        const uint32_t rel = blockIdx.y;
+        if (q > nSlices) { return; }
+        units[rel * (nSlices + 1) + q] = q < nSlices ? d.meta[rel][q].offset : 0u;
     }
 
-    // counts of the entries without padding (for reports)
-    __global__ void __launch_bounds__(kBlock) kCountEntries(Dev d)
+    // writes the scanned offsets back, sums the entries and checks the pool capacity
+    __global__ void __launch_bounds__(kBlock) kSliceApply(Dev d, uint32_t nSlices, const uint32_t* __restrict__ offs)
     {
-        __shared__ uint32_t red[3][kBlock];
-        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-        for (int r = 0; r < 3; r++)
+        __shared__ uint32_t red[kBlock];
+        const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+        const uint32_t rel = blockIdx.y;
+        uint32_t e = 0;
+        if (q < nSlices)
         {
-            red[r][threadIdx.x] = i < d.nF ? (d.cnt[r][i] & (r == REL_FF ? 0x7fffu : 0xffffu)) : 0u;
+            SliceMeta& m = d.meta[rel][q];
+            e = m.entries;
+            m.offset = offs[rel * (nSlices + 1) + q];
         }
+        red[threadIdx.x] = e;
         __syncthreads();
         for (uint32_t o = kBlock / 2; o > 0; o >>= 1)
         {
-            if (threadIdx.x < o) { for (int r = 0; r < 3; r++) { red[r][threadIdx.x] += red[r][threadIdx.x + o]; } }
+            if (threadIdx.x < o) { red[threadIdx.x] += red[threadIdx.x + o]; }
             __syncthreads();
         }
         if (threadIdx.x == 0)
         {
-            for (int r = 0; r < 3; r++)
-            {
-                atomicAdd(reinterpret_cast<unsigned long long*>(&d.status->entries[r]),
-                          static_cast<unsigned long long>(red[r][0]));
-            }
+            atomicAdd(reinterpret_cast<unsigned long long*>(&d.status->entries[rel]), static_cast<unsigned long long>(red[0]));
+        }
+        if (q == 0)
+        {
+            const uint64_t total = offs[rel * (nSlices + 1) + nSlices];
+            d.status->poolUnits[rel] = total;
+            if (total > d.poolCap[rel]) { atomicOr(&d.status->overflow, 1u); }
         }
     }
 
+    constexpr uint32_t kStage = 48; // FF entries per thread staged in LDS by the fill
+
     __global__ void __launch_bounds__(kBlock) kQueryFill(Dev d, float r2, int reach)
     {
+        __shared__ uint32_t stage[kStage][kBlock];
         const uint32_t blk = xcdBlock();
-        const uint32_t i = blk * kBlock + threadIdx.x;
-        if (i >= d.nF) { return; }
+        const uint32_t t = threadIdx.x;
+        const uint32_t i = blk * kBlock + t;
         const Frame& f = *d.frame;
-        Stencil s;
-        if (!stencilOf(f, d.keyF[i], reach, s)) { return; }
         const uint32_t lane = i & (kSlice - 1), slice = i / kSlice;
-        const SliceMeta mF = d.meta[REL_FF][slice], mB = d.meta[REL_FB][slice], mR = d.meta[REL_FR][slice];
-        const float xi = d.x[i], yi = d.y[i], zi = d.z[i];
-        const uint32_t cF = d.cnt[REL_FF][i];
+        Stencil s;
+        const bool active = i < d.nF && stencilOf(f, d.keyF[i], reach, s);
+        const uint32_t cF = i < d.nF ? d.cnt[REL_FF][i] : 0u;
         const bool wide = (cF & kWideSpanBit) != 0u;
-        uint32_t kF = 0, kB = 0, kR = 0;
-        for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
+        SliceMeta mF{}, mB{}, mR{};
+        if (i < d.nF) { mF = d.meta[REL_FF][slice]; mB = d.meta[REL_FB][slice]; mR = d.meta[REL_FR][slice]; }
+        if (active)
         {
-            for (int cy = s.lo[1]; cy <= s.hi[1]; cy++)
-            {
-                for (int cx = s.lo[0]; cx <= s.hi[0]; cx++)
-                {
-                    const uint32_t lin = linearCell(f, cx, cy, cz);
-                    if (!wide)
-                    {
-                        const uint2 rf = d.cellF[lin];
-                        for (uint32_t j = rf.x; j < rf.y; j++)
-                        {
-                            if (within(xi, yi, zi, d.x[j], d.y[j], d.z[j], r2)) { sellWrite(d.pool[REL_FF], mF, lane, kF++, j); }
-                        }
-                    }
-                    const uint2 rb = d.cellB[lin];
-                    for (uint32_t j = rb.x; j < rb.y; j++)
-                    {
-                        if (within(xi, yi, zi, d.bx[j], d.by[j], d.bz[j], r2)) { sellWrite(d.pool[REL_FB], mB, lane, kB++, j); }
-                    }
-                    if (d.nR)
-                    {
-                        const uint2 rr = d.cellR[lin];
-                        for (uint32_t t = rr.x; t < rr.y; t++)
-                        {
-                            const uint32_t j = d.idxR[t];
-                            if (within(xi, yi, zi, d.rx[j], d.ry[j], d.rz[j], r2)) { sellWrite(d.pool[REL_FR], mR, lane, kR++, j); }
-                        }
-                    }
-                }
-            }
-        }
-        if (!wide) { return; }
-
-        // DictDelta16x2 split of a per-particle dict (neighbor_lists.cpp:838-858): the neighbors
-        // within 0xffff of the smallest form the low group, the rest the high group, which is
-        // stored from the back when it fits 0xffff itself, else the order stays the candidate order
-        const uint32_t n = cF & 0x7fffu;
-        const uint32_t minRef = d.ffWideMin[i];
-        uint32_t base2 = kEmpty32, maxRef = 0;
-        for (int pass = 0; pass < 3; pass++)
-        {
-            uint32_t kLow = 0, kHigh = n;
+            const float4 pi = d.pos[i];
+            uint32_t kF = 0, kB = 0, kR = 0;
             for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
             {
                 for (int cy = s.lo[1]; cy <= s.hi[1]; cy++)
                 {
                     for (int cx = s.lo[0]; cx <= s.hi[0]; cx++)
                     {
-                        const uint2 rf = d.cellF[linearCell(f, cx, cy, cz)];
-                        for (uint32_t j = rf.x; j < rf.y; j++)
+                        const uint32_t lin = linearCell(f, cx, cy, cz);
+                        if (!wide)
                         {
-                            if (!within(xi, yi, zi, d.x[j], d.y[j], d.z[j], r2)) { continue; }
-                            const bool low = j - minRef <= 0xffffu;
-                            if (pass == 0)
+                            const uint2 rf = d.cellF[lin];
+                            for (uint32_t j = rf.x; j < rf.y; j++)
                             {
-                                maxRef = max(maxRef, j);
-                                if (!low) { base2 = min(base2, j); }
+                                if (within(pi.x, pi.y, pi.z, d.pos[j], r2))
+                                {
+                                    if (kF < kStage) { stage[kF][t] = j; } else { sellWrite(d.pool[REL_FF], mF, lane, kF, j); }
+                                    kF++;
+                                }
                             }
-                            else if (pass == 1)
+                        }
+                        const uint2 rb = d.cellB[lin];
+                        for (uint32_t j = rb.x; j < rb.y; j++)
+                        {
+                            if (within(pi.x, pi.y, pi.z, d.bpos[j], r2)) { sellWrite(d.pool[REL_FB], mB, lane, kB++, j); }
+                        }
+                        if (d.nR)
+                        {
+                            const uint2 rr = d.cellR[lin];
+                            for (uint32_t q = rr.x; q < rr.y; q++)
                             {
-                                // candidate order (DictAbs32)
-                                sellWrite(d.pool[REL_FF], mF, lane, kLow++, j);
-                            }
-                            else
-                            {
-                                if (low) { sellWrite(d.pool[REL_FF], mF, lane, kLow++, j); }
-                                else { sellWrite(d.pool[REL_FF], mF, lane, --kHigh, j); }
+                                const uint32_t j = d.idxR[q];
+                                if (within(pi.x, pi.y, pi.z, d.rpos[j], r2)) { sellWrite(d.pool[REL_FR], mR, lane, kR++, j); }
                             }
                         }
                     }
                 }
             }
-            if (pass == 0 && maxRef - base2 <= 0xffffu) { pass = 1; } // skip the candidate-order pass
-            else if (pass == 1) { break; }
+            if (wide)
+            {
+                // DictDelta16x2 split of a per-particle dict (neighbor_lists.cpp:838-858): the neighbors
+                // within 0xffff of the smallest form the low group, the rest the high group, which is
+                // stored from the back when it fits 0xffff itself, else the candidate order stays
+                const uint32_t n = cF & 0x7fffu;
+                const uint32_t minRef = d.ffWideMin[i];
+                uint32_t base2 = kEmpty32, maxRef = 0;
+                for (int pass = 0; pass < 3; pass++)
+                {
+                    uint32_t kLow = 0, kHigh = n;
+                    for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
+                    {
+                        for (int cy = s.lo[1]; cy <= s.hi[1]; cy++)
+                        {
+                            for (int cx = s.lo[0]; cx <= s.hi[0]; cx++)
+                            {
+                                const uint2 rf = d.cellF[linearCell(f, cx, cy, cz)];
+                                for (uint32_t j = rf.x; j < rf.y; j++)
+                                {
+                                    if (!within(pi.x, pi.y, pi.z, d.pos[j], r2)) { continue; }
+                                    const bool low = j - minRef <= 0xffffu;
+                                    if (pass == 0)
+                                    {
+                                        maxRef = max(maxRef, j);
+                                        if (!low) { base2 = min(base2, j); }
+                                    }
+                                    else if (pass == 1)
+                                    {
+                                        sellWrite(d.pool[REL_FF], mF, lane, kLow++, j);
+                                    }
+                                    else
+                                    {
+                                        if (low) { sellWrite(d.pool[REL_FF], mF, lane, kLow++, j); }
+                                        else { sellWrite(d.pool[REL_FF], mF, lane, --kHigh, j); }
+                                    }
+                                }
+                            }
+                        }
+                    }
+                    if (pass == 0 && maxRef - base2 <= 0xffffu) { pass = 1; } // skip the candidate-order pass
+                    else if (pass == 1) { break; }
+                }
+            }
+        }
+        // the staged FF entries leave as whole slice rows: column k of the 32 lanes per store
+        __syncthreads();
+        const uint32_t own = (active && !wide) ? (cF & 0x7fffu) : 0u;
+        const uint32_t rows = min(sliceMax(own), kStage);
+        for (uint32_t k = 0; k < rows; k++)
+        {
+            if (k < own) { sellWrite(d.pool[REL_FF], mF, lane, k, stage[k][t]); }
         }
     }
 
@@ -505,10 +580,10 @@ namespace tsp
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= d.nR) { return; }
         const Frame& f = *d.frame;
-        const float xi = d.rx[i], yi = d.ry[i], zi = d.rz[i];
-        const int cx = floorToInt(xi * f.cellSizeInv) - f.orig[0];
-        const int cy = floorToInt(yi * f.cellSizeInv) - f.orig[1];
-        const int cz = floorToInt(zi * f.cellSizeInv) - f.orig[2];
+        const float4 pi = d.rpos[i];
+        const int cx = floorToInt(pi.x * f.cellSizeInv) - f.orig[0];
+        const int cy = floorToInt(pi.y * f.cellSizeInv) - f.orig[1];
+        const int cz = floorToInt(pi.z * f.cellSizeInv) - f.orig[2];
         uint32_t nF = 0, nB = 0, nRR = 0;
         uint32_t w = fill ? d.rfOff[i] : 0u;
         const bool inside = cx >= 0 && cx < static_cast<int>(f.dim[0]) && cy >= 0 && cy < static_cast<int>(f.dim[1])
@@ -529,7 +604,7 @@ namespace tsp
                         const uint2 rf = d.cellF[lin];
                         for (uint32_t j = rf.x; j < rf.y; j++)
                         {
-                            if (within(xi, yi, zi, d.x[j], d.y[j], d.z[j], r2))
+                            if (within(pi.x, pi.y, pi.z, d.pos[j], r2))
                             {
                                 if (fill) { d.rfIdx[w++] = j; } else { nF++; }
                             }
@@ -538,14 +613,14 @@ namespace tsp
                         const uint2 rb = d.cellB[lin];
                         for (uint32_t j = rb.x; j < rb.y; j++)
                         {
-                            if (within(xi, yi, zi, d.bx[j], d.by[j], d.bz[j], r2)) { nB++; }
+                            if (within(pi.x, pi.y, pi.z, d.bpos[j], r2)) { nB++; }
                         }
                         const uint2 rr = d.cellR[lin];
                         for (uint32_t t = rr.x; t < rr.y; t++)
                         {
                             const uint32_t j = d.idxR[t];
                             if (j == i || d.rbody[j] == body) { continue; }
-                            if (within(xi, yi, zi, d.rx[j], d.ry[j], d.rz[j], r2)) { nRR++; }
+                            if (within(pi.x, pi.y, pi.z, d.rpos[j], r2)) { nRR++; }
                         }
                     }
                 }
@@ -553,44 +628,36 @@ namespace tsp
         }
         if (!fill)
         {
+            if (i == 0) { d.rfOff[d.nR] = 0u; }
             d.rfCnt[i] = nF;
+            d.rfOff[i] = nF;
             d.rbCnt[i] = nB;
             d.rrCnt[i] = nRR;
         }
     }
 
-    // exclusive scan of the RF counts into offsets plus RB/RR totals (one block)
-    __global__ void __launch_bounds__(1024) kScanRigid(Dev d)
+    // totals of the rigid-source relations and the RF capacity check (after the RF offset scan)
+    __global__ void __launch_bounds__(kBlock) kRigidTotals(Dev d)
     {
-        __shared__ uint64_t part[3][1024];
-        const uint32_t t = threadIdx.x, n = d.nR;
-        const uint32_t per = (n + 1023u) / 1024u;
-        const uint32_t b = t * per, e = min(b + per, n);
-        uint64_t s = 0, sb = 0, sr = 0;
-        for (uint32_t q = b; q < e; q++) { s += d.rfCnt[q]; sb += d.rbCnt[q]; sr += d.rrCnt[q]; }
-        part[0][t] = s; part[1][t] = sb; part[2][t] = sr;
+        __shared__ uint32_t red[2][kBlock];
+        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+        red[0][threadIdx.x] = i < d.nR ? d.rbCnt[i] : 0u;
+        red[1][threadIdx.x] = i < d.nR ? d.rrCnt[i] : 0u;
         __syncthreads();
-        for (uint32_t off = 1; off < 1024; off <<= 1)
+        for (uint32_t o = kBlock / 2; o > 0; o >>= 1)
         {
-            uint64_t a[3];
-            for (int r = 0; r < 3; r++) { a[r] = t >= off ? part[r][t - off] : 0ull; }
-            __syncthreads();
-            for (int r = 0; r < 3; r++) { part[r][t] += a[r]; }
+            if (threadIdx.x < o) { red[0][threadIdx.x] += red[0][threadIdx.x + o]; red[1][threadIdx.x] += red[1][threadIdx.x + o]; }
             __syncthreads();
         }
-        uint64_t run = part[0][t] - s;
-        for (uint32_t q = b; q < e; q++)
+        if (threadIdx.x == 0)
         {
-            d.rfOff[q] = static_cast<uint32_t>(run);
-            run += d.rfCnt[q];
+            atomicAdd(reinterpret_cast<unsigned long long*>(&d.status->rbTotal), static_cast<unsigned long long>(red[0][0]));
+            atomicAdd(reinterpret_cast<unsigned long long*>(&d.status->rrTotal), static_cast<unsigned long long>(red[1][0]));
         }
-        if (t == 1023)
+        if (i == 0)
         {
-            d.rfOff[n] = static_cast<uint32_t>(part[0][1023]);
-            d.status->rfTotal = part[0][1023];
-            d.status->rbTotal = part[1][1023];
-            d.status->rrTotal = part[2][1023];
-            if (part[0][1023] > d.rfCap) { atomicOr(&d.status->overflow, 4u); }
+            d.status->rfTotal = d.rfOff[d.nR];
+            if (d.rfOff[d.nR] > d.rfCap) { atomicOr(&d.status->overflow, 4u); }
         }
     }
 
@@ -602,16 +669,16 @@ namespace tsp
         return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
     }
 
-    __global__ void __launch_bounds__(kBlock) kAabb(const float* __restrict__ x, const float* __restrict__ y,
-        const float* __restrict__ z, uint32_t n, uint32_t* __restrict__ out)
+    __global__ void __launch_bounds__(kBlock) kAabb(const float4* __restrict__ p, uint32_t n, uint32_t* __restrict__ out)
     {
         __shared__ uint32_t red[6][kBlock];
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         uint32_t v[6] = { 0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u };
         if (i < n)
         {
-            const float p[3] = { x[i], y[i], z[i] };
-            for (int a = 0; a < 3; a++) { v[a] = orderedBits(p[a]); v[3 + a] = v[a]; }
+            const float4 q = p[i];
+            const float c[3] = { q.x, q.y, q.z };
+            for (int a = 0; a < 3; a++) { v[a] = orderedBits(c[a]); v[3 + a] = v[a]; }
         }
         for (int a = 0; a < 6; a++) { red[a][threadIdx.x] = v[a]; }
         __syncthreads();
@@ -638,21 +705,23 @@ namespace tsp
 
     static inline uint32_t blocksFor(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
 
-    void launchKeys(hipStream_t s, const float* x, const float* y, const float* z, uint32_t n, const Frame* f,
-                    Status* st, bool check, uint32_t* key, uint32_t* idx)
+    void launchResetSearch(hipStream_t s, Status* st) { kResetSearch<<<1, 64, 0, s>>>(st); }
+
+    void launchKeys(hipStream_t s, const float4* p, uint32_t n, const Frame* f, Status* st, bool check, uint32_t* key,
+                    uint32_t* idx)
     {
-        if (n) { kKeys<<<blocksFor(n), kBlock, 0, s>>>(x, y, z, n, f, st, check, key, idx); }
+        if (n) { kKeys<<<blocksFor(n), kBlock, 0, s>>>(p, n, f, st, check, key, idx); }
     }
 
-    void launchKeysSorted(hipStream_t s, const float* x, const float* y, const float* z, uint32_t n, const Frame* f,
-                          Status* st, uint32_t* key, uint32_t* idx)
+    void launchKeysSorted(hipStream_t s, const float4* p, uint32_t n, const Frame* f, Status* st, uint32_t* key,
+                          uint32_t* idx)
     {
-        if (n) { kKeysSorted<<<blocksFor(n), kBlock, 0, s>>>(x, y, z, n, f, st, key, idx); }
+        if (n) { kKeysSorted<<<blocksFor(n), kBlock, 0, s>>>(p, n, f, st, key, idx); }
     }
 
     // sorts (key, idx) stably by the low `bits` bits; the result ends in (key, idx)
     void launchRadixSort(hipStream_t s, uint32_t* key, uint32_t* idx, uint32_t* key2, uint32_t* idx2, uint32_t n,
-                         uint32_t bits, uint32_t* hist)
+                         uint32_t bits, uint32_t* hist, uint32_t* tmp)
     {
         if (n < 2) { return; }
         const uint32_t nTiles = (n + kTile - 1) / kTile;
@@ -660,7 +729,7 @@ namespace tsp
         for (uint32_t shift = 0; shift < bits; shift += 8)
         {
             kRadixHist<<<nTiles, kBlock, 0, s>>>(kIn, n, shift, nTiles, hist);
-            kScanSingle<<<1, 1024, 0, s>>>(hist, 256u * nTiles);
+            launchScan(s, hist, 256u * nTiles, tmp);
             kRadixScatter<<<nTiles, kBlock, 0, s>>>(kIn, iIn, kOut, iOut, n, shift, nTiles, hist);
             uint32_t* t = kIn; kIn = kOut; kOut = t;
             t = iIn; iIn = iOut; iOut = t;
@@ -672,15 +741,14 @@ namespace tsp
         }
     }
 
-    void launchPermute(hipStream_t s, const FieldSet& fs, const uint32_t* idx)
+    void launchPermute(hipStream_t s, const Gather& g, const uint32_t* idx)
     {
-        if (fs.n) { kPermute<<<blocksFor(fs.n), kBlock, 0, s>>>(fs, idx); }
+        if (g.n) { kPermute<<<blocksFor(g.n), kBlock, 0, s>>>(g, idx); }
     }
 
-    void launchClearCells(hipStream_t s, uint2* cell, const uint32_t* occ, uint32_t* nOcc, uint32_t maxOcc)
+    void launchClearCells(hipStream_t s, const uint32_t* key, uint32_t n, const Frame* f, uint2* cell)
     {
-        if (maxOcc) { kClearCells<<<blocksFor(maxOcc), kBlock, 0, s>>>(cell, occ, nOcc); }
-        kSetU32<<<1, 1, 0, s>>>(nOcc, 0u);
+        if (n) { kClearCells<<<blocksFor(n), kBlock, 0, s>>>(key, n, f, cell); }
     }
 
     void launchClearAll(hipStream_t s, uint2* cell, uint64_t n)
@@ -688,10 +756,9 @@ namespace tsp
         if (n) { kClearAll<<<blocksFor(n), kBlock, 0, s>>>(cell, n); }
     }
 
-    void launchBuildCells(hipStream_t s, const uint32_t* key, uint32_t n, const Frame* f, uint2* cell, uint32_t* occ,
-                          uint32_t* nOcc)
+    void launchBuildCells(hipStream_t s, const uint32_t* key, uint32_t n, const Frame* f, uint2* cell)
     {
-        if (n) { kBuildCells<<<blocksFor(n), kBlock, 0, s>>>(key, n, f, cell, occ, nOcc); }
+        if (n) { kBuildCells<<<blocksFor(n), kBlock, 0, s>>>(key, n, f, cell); }
     }
 
     void launchQueryCount(hipStream_t s, const Dev& d, float r2, int reach)
@@ -700,11 +767,14 @@ namespace tsp
         if (d.nF) { kQueryCount<<<blocksFor(d.nF), kBlock, 0, s>>>(d, r2, reach, nSlices); }
     }
 
-    void launchScanSlices(hipStream_t s, const Dev& d)
+    // units is 3 x (nSlices + 1) u32
+    void launchScanSlices(hipStream_t s, const Dev& d, uint32_t* units, uint32_t* tmp)
     {
         const uint32_t nSlices = (d.nF + kSlice - 1) / kSlice;
-        kScanSlices<<<3, 1024, 0, s>>>(d, nSlices);
-        if (d.nF) { kCountEntries<<<blocksFor(d.nF), kBlock, 0, s>>>(d); }
+        const dim3 g(blocksFor(nSlices + 1ull), 3);
+        kSliceUnits<<<g, kBlock, 0, s>>>(d, nSlices, units);
+        for (uint32_t r = 0; r < 3; r++) { launchScan(s, units + r * (nSlices + 1), nSlices + 1, tmp); }
+        kSliceApply<<<g, kBlock, 0, s>>>(d, nSlices, units);
     }
 
     void launchQueryFill(hipStream_t s, const Dev& d, float r2, int reach)
@@ -712,15 +782,19 @@ namespace tsp
         if (d.nF) { kQueryFill<<<blocksFor(d.nF), kBlock, 0, s>>>(d, r2, reach); }
     }
 
-    void launchQueryRigid(hipStream_t s, const Dev& d, float r2, int reach, bool fill)
+    void launchQueryRigid(hipStream_t s, const Dev& d, float r2, int reach, bool fill, uint32_t* tmp)
     {
         if (!d.nR) { return; }
         kQueryRigid<<<blocksFor(d.nR), kBlock, 0, s>>>(d, r2, reach, fill);
-        if (!fill) { kScanRigid<<<1, 1024, 0, s>>>(d); }
+        if (!fill)
+        {
+            launchScan(s, d.rfOff, d.nR + 1, tmp);
+            kRigidTotals<<<blocksFor(d.nR), kBlock, 0, s>>>(d);
+        }
     }
 
-    void launchAabb(hipStream_t s, const float* x, const float* y, const float* z, uint32_t n, uint32_t* out)
+    void launchAabb(hipStream_t s, const float4* p, uint32_t n, uint32_t* out)
     {
-        if (n) { kAabb<<<blocksFor(n), kBlock, 0, s>>>(x, y, z, n, out); }
+        if (n) { kAabb<<<blocksFor(n), kBlock, 0, s>>>(p, n, out); }
     }
 }

@@ -46,47 +46,50 @@ namespace tsp
     {
         const uint32_t i = xcdBlock() * kBlock + threadIdx.x;
         if (i >= d.nF) { return; }
-        const float xi = d.x[i], yi = d.y[i], zi = d.z[i];
+        const float4 pi = d.pos[i];
+        const float xi = pi.x, yi = pi.y, zi = pi.z;
         const List ff = listOf(d, REL_FF, i);
         float wsum = 0.0f;
         for (uint32_t q = 0; q < ff.n; q++)
         {
-            const uint32_t j = ff[q];
-            wsum += kwUnscaled(dist2(xi - d.x[j], yi - d.y[j], zi - d.z[j]), k);
+            const float4 pj = d.pos[ff[q]];
+            wsum += kwUnscaled(dist2(xi - pj.x, yi - pj.y, zi - pj.z), k);
         }
         const float dF = wsum * k.weightScale;
         float dB = 0.0f, dR = 0.0f;
         const List fb = listOf(d, REL_FB, i);
         for (uint32_t q = 0; q < fb.n; q++)
         {
-            const uint32_t j = fb[q];
-            dB = __builtin_fmaf(d.bpsi[j], kwNormal(xi - d.bx[j], yi - d.by[j], zi - d.bz[j], k), dB);
+            const float4 bj = d.bpos[fb[q]];
+            dB = __builtin_fmaf(bj.w, kwNormal(xi - bj.x, yi - bj.y, zi - bj.z, k), dB);
         }
         const List fr = listOf(d, REL_FR, i);
         for (uint32_t q = 0; q < fr.n; q++)
         {
-            const uint32_t j = fr[q];
-            dR = __builtin_fmaf(d.rpsi[j], kwNormal(xi - d.rx[j], yi - d.ry[j], zi - d.rz[j], k), dR);
+            const float4 rj = d.rpos[fr[q]];
+            dR = __builtin_fmaf(rj.w, kwNormal(xi - rj.x, yi - rj.y, zi - rj.z, k), dR);
         }
-        d.density[i] = dF + dB + dR;
+        d.vel[i].w = dF + dB + dR;
     }
 
     __global__ void __launch_bounds__(kBlock) kNonPressure(Dev d, Consts k)
     {
         const uint32_t i = xcdBlock() * kBlock + threadIdx.x;
         if (i >= d.nF) { return; }
-        const float xi = d.x[i], yi = d.y[i], zi = d.z[i], vxi = d.vx[i], vyi = d.vy[i], vzi = d.vz[i];
+        const float4 pi = d.pos[i], vi = d.vel[i];
+        const float xi = pi.x, yi = pi.y, zi = pi.z, vxi = vi.x, vyi = vi.y, vzi = vi.z;
         float aFx = 0.0f, aFy = 0.0f, aFz = 0.0f;
         const List ff = listOf(d, REL_FF, i);
         for (uint32_t q = 0; q < ff.n; q++)
         {
             const uint32_t j = ff[q];
-            const float dx = xi - d.x[j], dy = yi - d.y[j], dz = zi - d.z[j];
-            const float dvx = vxi - d.vx[j], dvy = vyi - d.vy[j], dvz = vzi - d.vz[j];
+            const float4 pj = d.pos[j], vj = d.vel[j];
+            const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
+            const float dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
             const float d2 = dist2(dx, dy, dz);
             const float kg = kgFactor(d2, k);
             const float dot = __builtin_fmaf(dvx, dx, __builtin_fmaf(dvy, dy, dvz * dz));
-            const float rj = vmax(d.density[j], 1e-7f);
+            const float rj = vmax(vj.w, 1e-7f);
             const float coeff = dot / (rj * (d2 + k.hsqEps)) * kg;
             aFx = __builtin_fmaf(coeff, dx, aFx);
             aFy = __builtin_fmaf(coeff, dy, aFy);
@@ -98,7 +101,8 @@ namespace tsp
         for (uint32_t q = 0; q < fb.n; q++)
         {
             const uint32_t j = fb[q];
-            const float dx = xi - d.bx[j], dy = yi - d.by[j], dz = zi - d.bz[j];
+            const float4 bj = d.bpos[j];
+            const float dx = xi - bj.x, dy = yi - bj.y, dz = zi - bj.z;
             const float d2 = dist2(dx, dy, dz);
             const float f = kgFactor(d2, k);
             const float gx = f * dx, gy = f * dy, gz = f * dz;
@@ -114,13 +118,14 @@ namespace tsp
         for (uint32_t q = 0; q < fr.n; q++)
         {
             const uint32_t j = fr[q];
-            const float dx = xi - d.rx[j], dy = yi - d.ry[j], dz = zi - d.rz[j];
+            const float4 rj = d.rpos[j], uj = d.rvel[j];
+            const float dx = xi - rj.x, dy = yi - rj.y, dz = zi - rj.z;
             const float d2 = dist2(dx, dy, dz);
             const float f = kgFactor(d2, k);
             const float gx = f * dx, gy = f * dy, gz = f * dz;
-            const float dvx = vxi - d.rvx[j], dvy = vyi - d.rvy[j], dvz = vzi - d.rvz[j];
+            const float dvx = vxi - uj.x, dvy = vyi - uj.y, dvz = vzi - uj.z;
             const float dot = __builtin_fmaf(dvx, dx, __builtin_fmaf(dvy, dy, dvz * dz));
-            const float coeff = (d.rvol[j] * dot) / (d2 + k.hsqEps);
+            const float coeff = (uj.w * dot) / (d2 + k.hsqEps);
             aRx = __builtin_fmaf(coeff, gx, aRx);
             aRy = __builtin_fmaf(coeff, gy, aRy);
             aRz = __builtin_fmaf(coeff, gz, aRz);
@@ -128,23 +133,23 @@ namespace tsp
         const float ax = aFx * k.visF + (aBx + aRx) * k.visB + k.gx;
         const float ay = aFy * k.visF + (aBy + aRy) * k.visB + k.gy;
         const float az = aFz * k.visF + (aBz + aRz) * k.visB + k.gz;
-        d.vpx[i] = __builtin_fmaf(ax, k.dt, vxi);
-        d.vpy[i] = __builtin_fmaf(ay, k.dt, vyi);
-        d.vpz[i] = __builtin_fmaf(az, k.dt, vzi);
+        d.vp[i] = make_float4(__builtin_fmaf(ax, k.dt, vxi), __builtin_fmaf(ay, k.dt, vyi), __builtin_fmaf(az, k.dt, vzi), 0.0f);
     }
 
     __global__ void __launch_bounds__(kBlock) kSourceDiag(Dev d, Consts k)
     {
         const uint32_t i = xcdBlock() * kBlock + threadIdx.x;
         if (i >= d.nF) { return; }
-        const float xi = d.x[i], yi = d.y[i], zi = d.z[i], vxi = d.vpx[i], vyi = d.vpy[i], vzi = d.vpz[i];
+        const float4 pi = d.pos[i], vi = d.vp[i];
+        const float xi = pi.x, yi = pi.y, zi = pi.z, vxi = vi.x, vyi = vi.y, vzi = vi.z;
         float sF = 0.0f, cFx = 0.0f, cFy = 0.0f, cFz = 0.0f, g2 = 0.0f;
         const List ff = listOf(d, REL_FF, i);
         for (uint32_t q = 0; q < ff.n; q++)
         {
             const uint32_t j = ff[q];
-            const float dx = xi - d.x[j], dy = yi - d.y[j], dz = zi - d.z[j];
-            const float dvx = vxi - d.vpx[j], dvy = vyi - d.vpy[j], dvz = vzi - d.vpz[j];
+            const float4 pj = d.pos[j], vj = d.vp[j];
+            const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
+            const float dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
             const float d2 = dist2(dx, dy, dz);
             const float kg = kgFactor(d2, k);
             const float st = __builtin_fmaf(dvx, dx, __builtin_fmaf(dvy, dy, dvz * dz));
@@ -158,13 +163,13 @@ namespace tsp
         const List fb = listOf(d, REL_FB, i);
         for (uint32_t q = 0; q < fb.n; q++)
         {
-            const uint32_t j = fb[q];
-            const float dx = xi - d.bx[j], dy = yi - d.by[j], dz = zi - d.bz[j];
+            const float4 bj = d.bpos[fb[q]];
+            const float dx = xi - bj.x, dy = yi - bj.y, dz = zi - bj.z;
             const float f = kgFactor(dist2(dx, dy, dz), k);
             const float gx = f * dx, gy = f * dy, gz = f * dz;
             const float dvx = vxi - 0.0f, dvy = vyi - 0.0f, dvz = vzi - 0.0f;
             const float st = __builtin_fmaf(dvx, gx, __builtin_fmaf(dvy, gy, dvz * gz));
-            const float psi = d.bpsi[j];
+            const float psi = bj.w;
             sB = __builtin_fmaf(psi, st, sB);
             cBx = __builtin_fmaf(psi, gx, cBx);
             cBy = __builtin_fmaf(psi, gy, cBy);
@@ -175,19 +180,20 @@ namespace tsp
         for (uint32_t q = 0; q < fr.n; q++)
         {
             const uint32_t j = fr[q];
-            const float dx = xi - d.rx[j], dy = yi - d.ry[j], dz = zi - d.rz[j];
+            const float4 rj = d.rpos[j], uj = d.rvp[j];
+            const float dx = xi - rj.x, dy = yi - rj.y, dz = zi - rj.z;
             const float f = kgFactor(dist2(dx, dy, dz), k);
             const float gx = f * dx, gy = f * dy, gz = f * dz;
-            const float dvx = vxi - d.rvpx[j], dvy = vyi - d.rvpy[j], dvz = vzi - d.rvpz[j];
+            const float dvx = vxi - uj.x, dvy = vyi - uj.y, dvz = vzi - uj.z;
             const float st = __builtin_fmaf(dvx, gx, __builtin_fmaf(dvy, gy, dvz * gz));
-            const float psi = d.rpsi[j];
+            const float psi = rj.w;
             sR = __builtin_fmaf(psi, st, sR);
             cRx = __builtin_fmaf(psi, gx, cRx);
             cRy = __builtin_fmaf(psi, gy, cRy);
             cRz = __builtin_fmaf(psi, gz, cRz);
         }
         sF *= k.mass;
-        const float src = k.rho0 - d.density[i] - k.dt * (sF + sB + sR);
+        const float src = k.rho0 - d.vel[i].w - k.dt * (sF + sB + sR);
         const float ctx = cFx * k.cFacF + (cBx + cRx) * k.cFacB;
         const float cty = cFy * k.cFacF + (cBy + cRy) * k.cFacB;
         const float ctz = cFz * k.cFacF + (cBz + cRz) * k.cFacB;
@@ -200,12 +206,12 @@ namespace tsp
         float init = cold;
         if (k.warm > 0.0f)
         {
-            const float prev = d.pressure[i];
+            const float prev = pi.w;
             init = prev > 0.0f ? k.warm * prev : cold;
         }
         d.source[i] = src;
         d.diag[i] = diag;
-        d.pressure[i] = vmax(0.0f, valid ? init : 0.0f);
+        d.pos[i].w = vmax(0.0f, valid ? init : 0.0f);
     }
 
     // TerminationConditionSatisfied, sph_solver.cpp:686-732
@@ -242,15 +248,16 @@ namespace tsp
         if (sweep && d.status->converged) { return; }
         const uint32_t i = xcdBlock() * kBlock + threadIdx.x;
         if (i >= d.nF) { return; }
-        const float xi = d.x[i], yi = d.y[i], zi = d.z[i], pi = d.pressure[i];
+        const float4 p4 = d.pos[i];
+        const float xi = p4.x, yi = p4.y, zi = p4.z, pi = p4.w;
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         const List ff = listOf(d, REL_FF, i);
         for (uint32_t q = 0; q < ff.n; q++)
         {
-            const uint32_t j = ff[q];
-            const float dx = xi - d.x[j], dy = yi - d.y[j], dz = zi - d.z[j];
+            const float4 pj = d.pos[ff[q]];
+            const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
             const float kg = kgFactor(dist2(dx, dy, dz), k);
-            const float coeff = (pi + d.pressure[j]) * kg;
+            const float coeff = (pi + pj.w) * kg;
             sx = __builtin_fmaf(coeff, dx, sx);
             sy = __builtin_fmaf(coeff, dy, sy);
             sz = __builtin_fmaf(coeff, dz, sz);
@@ -259,10 +266,10 @@ namespace tsp
         const List fb = listOf(d, REL_FB, i);
         for (uint32_t q = 0; q < fb.n; q++)
         {
-            const uint32_t j = fb[q];
-            const float dx = xi - d.bx[j], dy = yi - d.by[j], dz = zi - d.bz[j];
+            const float4 bj = d.bpos[fb[q]];
+            const float dx = xi - bj.x, dy = yi - bj.y, dz = zi - bj.z;
             const float f = kgFactor(dist2(dx, dy, dz), k);
-            const float coeff = k.presOuterB * d.bpsi[j] * pi;
+            const float coeff = k.presOuterB * bj.w * pi;
             ax = __builtin_fmaf(coeff, f * dx, ax);
             ay = __builtin_fmaf(coeff, f * dy, ay);
             az = __builtin_fmaf(coeff, f * dz, az);
@@ -270,17 +277,15 @@ namespace tsp
         const List fr = listOf(d, REL_FR, i);
         for (uint32_t q = 0; q < fr.n; q++)
         {
-            const uint32_t j = fr[q];
-            const float dx = xi - d.rx[j], dy = yi - d.ry[j], dz = zi - d.rz[j];
+            const float4 rj = d.rpos[fr[q]];
+            const float dx = xi - rj.x, dy = yi - rj.y, dz = zi - rj.z;
             const float f = kgFactor(dist2(dx, dy, dz), k);
-            const float coeff = k.presOuterB * d.rpsi[j] * pi;
+            const float coeff = k.presOuterB * rj.w * pi;
             ax = __builtin_fmaf(coeff, f * dx, ax);
             ay = __builtin_fmaf(coeff, f * dy, ay);
             az = __builtin_fmaf(coeff, f * dz, az);
         }
-        d.apx[i] = 0.0f - ax;
-        d.apy[i] = 0.0f - ay;
-        d.apz[i] = 0.0f - az;
+        d.ap[i] = make_float4(0.0f - ax, 0.0f - ay, 0.0f - az, 0.0f);
     }
 
     __device__ __forceinline__ double waveSum(double v)
@@ -299,15 +304,17 @@ namespace tsp
         double eRaw = 0.0, ePos = 0.0, eAbs = 0.0;
         if (i < d.nF)
         {
-            const float xi = d.x[i], yi = d.y[i], zi = d.z[i], axi = d.apx[i], ayi = d.apy[i], azi = d.apz[i];
+            const float4 pi = d.pos[i], ai = d.ap[i];
+            const float xi = pi.x, yi = pi.y, zi = pi.z, axi = ai.x, ayi = ai.y, azi = ai.z;
             float sf = 0.0f;
             const List ff = listOf(d, REL_FF, i);
             for (uint32_t q = 0; q < ff.n; q++)
             {
                 const uint32_t j = ff[q];
-                const float dx = xi - d.x[j], dy = yi - d.y[j], dz = zi - d.z[j];
+                const float4 pj = d.pos[j], aj = d.ap[j];
+                const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
                 const float kg = kgFactor(dist2(dx, dy, dz), k);
-                const float sv = __builtin_fmaf(axi - d.apx[j], dx, __builtin_fmaf(ayi - d.apy[j], dy, (azi - d.apz[j]) * dz));
+                const float sv = __builtin_fmaf(axi - aj.x, dx, __builtin_fmaf(ayi - aj.y, dy, (azi - aj.z) * dz));
                 sf = __builtin_fmaf(kg, sv, sf);
             }
             sf *= k.mass;
@@ -315,27 +322,27 @@ namespace tsp
             const List fb = listOf(d, REL_FB, i);
             for (uint32_t q = 0; q < fb.n; q++)
             {
-                const uint32_t j = fb[q];
-                const float dx = xi - d.bx[j], dy = yi - d.by[j], dz = zi - d.bz[j];
+                const float4 bj = d.bpos[fb[q]];
+                const float dx = xi - bj.x, dy = yi - bj.y, dz = zi - bj.z;
                 const float f = kgFactor(dist2(dx, dy, dz), k);
                 const float sv = __builtin_fmaf(axi, f * dx, __builtin_fmaf(ayi, f * dy, azi * (f * dz)));
-                sb += d.bpsi[j] * sv;
+                sb += bj.w * sv;
             }
             const List fr = listOf(d, REL_FR, i);
             for (uint32_t q = 0; q < fr.n; q++)
             {
-                const uint32_t j = fr[q];
-                const float dx = xi - d.rx[j], dy = yi - d.ry[j], dz = zi - d.rz[j];
+                const float4 rj = d.rpos[fr[q]];
+                const float dx = xi - rj.x, dy = yi - rj.y, dz = zi - rj.z;
                 const float f = kgFactor(dist2(dx, dy, dz), k);
                 const float sv = __builtin_fmaf(axi, f * dx, __builtin_fmaf(ayi, f * dy, azi * (f * dz)));
-                sb += d.rpsi[j] * sv;
+                sb += rj.w * sv;
             }
             const float sys = k.dtsq * (sf + sb);
             const float src = d.source[i], diag = d.diag[i];
             const float resid = src - sys;
             const float pred = sys - src;
             const bool valid = __builtin_fabsf(diag) > 1e-7f;
-            d.pressure[i] = vmax(0.0f, valid ? __builtin_fmaf(k.omega, resid / diag, d.pressure[i]) : 0.0f);
+            d.pos[i].w = vmax(0.0f, valid ? __builtin_fmaf(k.omega, resid / diag, pi.w) : 0.0f);
             if (compute && valid)
             {
                 const double e = static_cast<double>(pred / k.rho0);
@@ -407,13 +414,12 @@ namespace tsp
     {
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= d.nF) { return; }
-        const float vx = __builtin_fmaf(k.dt, d.apx[i], d.vpx[i]);
-        const float vy = __builtin_fmaf(k.dt, d.apy[i], d.vpy[i]);
-        const float vz = __builtin_fmaf(k.dt, d.apz[i], d.vpz[i]);
-        d.vx[i] = vx; d.vy[i] = vy; d.vz[i] = vz;
-        d.x[i] = __builtin_fmaf(k.dt, vx, d.x[i]);
-        d.y[i] = __builtin_fmaf(k.dt, vy, d.y[i]);
-        d.z[i] = __builtin_fmaf(k.dt, vz, d.z[i]);
+        const float4 a = d.ap[i], v = d.vp[i], p = d.pos[i];
+        const float vx = __builtin_fmaf(k.dt, a.x, v.x);
+        const float vy = __builtin_fmaf(k.dt, a.y, v.y);
+        const float vz = __builtin_fmaf(k.dt, a.z, v.z);
+        d.vel[i] = make_float4(vx, vy, vz, d.vel[i].w);
+        d.pos[i] = make_float4(__builtin_fmaf(k.dt, vx, p.x), __builtin_fmaf(k.dt, vy, p.y), __builtin_fmaf(k.dt, vz, p.z), p.w);
     }
 
     // ----------------------------------------------------------------------------------------
@@ -438,9 +444,8 @@ namespace tsp
         const float rz = __builtin_fmaf(rbv(d, RB_A00 + 6, b), lx, __builtin_fmaf(rbv(d, RB_A00 + 7, b), ly, rbv(d, RB_A00 + 8, b) * lz));
         if (!predicted)
         {
-            d.rx[i] = rbv(d, RB_XCM + 0, b) + rx;
-            d.ry[i] = rbv(d, RB_XCM + 1, b) + ry;
-            d.rz[i] = rbv(d, RB_XCM + 2, b) + rz;
+            const float psi = d.rpos[i].w;
+            d.rpos[i] = make_float4(rbv(d, RB_XCM + 0, b) + rx, rbv(d, RB_XCM + 1, b) + ry, rbv(d, RB_XCM + 2, b) + rz, psi);
         }
         const uint32_t nb = d.nRB;
         const float wx = predicted ? d.rbPred[6 * nb + b] : rbv(d, RB_WX + 0, b);
@@ -449,12 +454,8 @@ namespace tsp
         const float vbx = predicted ? d.rbPred[0 * nb + b] : rbv(d, RB_VX + 0, b);
         const float vby = predicted ? d.rbPred[1 * nb + b] : rbv(d, RB_VX + 1, b);
         const float vbz = predicted ? d.rbPred[2 * nb + b] : rbv(d, RB_VX + 2, b);
-        float* ox = predicted ? d.rvpx : d.rvx;
-        float* oy = predicted ? d.rvpy : d.rvy;
-        float* oz = predicted ? d.rvpz : d.rvz;
-        ox[i] = vbx + wy * rz - wz * ry;
-        oy[i] = vby + wz * rx - wx * rz;
-        oz[i] = vbz + wx * ry - wy * rx;
+        float4* out = predicted ? d.rvp : d.rvel;
+        out[i] = make_float4(vbx + wy * rz - wz * ry, vby + wz * rx - wx * rz, vbz + wx * ry - wy * rx, out[i].w);
     }
 
     // ClearRigidForcesAndAddGravity, cpu_backend.cpp:2065-2087
@@ -477,25 +478,28 @@ namespace tsp
     {
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= d.nR) { return; }
-        const float xi = d.rx[i], yi = d.ry[i], zi = d.rz[i];
+        const float4 ri = d.rpos[i], ui = d.rvel[i];
+        const float xi = ri.x, yi = ri.y, zi = ri.z;
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
         for (uint32_t q = d.rfOff[i]; q < d.rfOff[i + 1]; q++)
         {
             const uint32_t j = d.rfIdx[q];
-            const float dx = d.x[j] - xi, dy = d.y[j] - yi, dz = d.z[j] - zi;
+            const float4 pj = d.pos[j];
+            const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
             float gx, gy, gz;
             kgScalar(dx, dy, dz, k, gx, gy, gz);
             float fc;
             if (pressure)
             {
-                fc = k.mass * k.presOuterB * d.rpsi[i] * d.pressure[j];
+                fc = k.mass * k.presOuterB * ri.w * pj.w;
             }
             else
             {
-                const float dvx = d.vx[j] - d.rvx[i], dvy = d.vy[j] - d.rvy[i], dvz = d.vz[j] - d.rvz[i];
+                const float4 vj = d.vel[j];
+                const float dvx = vj.x - ui.x, dvy = vj.y - ui.y, dvz = vj.z - ui.z;
                 const float dot = __builtin_fmaf(dvx, dx, __builtin_fmaf(dvy, dy, dvz * dz));
                 const float d2 = dist2(dx, dy, dz);
-                const float coeff = (d.rvol[i] * dot) / (d2 + k.hsqEps);
+                const float coeff = (ui.w * dot) / (d2 + k.hsqEps);
                 fc = -k.mass * k.visB * coeff;
             }
             fx = __builtin_fmaf(fc, gx, fx);

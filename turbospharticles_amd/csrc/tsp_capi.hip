@@ -22,20 +22,19 @@
 
 namespace tsp
 {
-    void launchKeys(hipStream_t, const float*, const float*, const float*, uint32_t, const Frame*, Status*, bool,
-                    uint32_t*, uint32_t*);
-    void launchKeysSorted(hipStream_t, const float*, const float*, const float*, uint32_t, const Frame*, Status*,
-                          uint32_t*, uint32_t*);
-    void launchRadixSort(hipStream_t, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t*);
-    void launchPermute(hipStream_t, const FieldSet&, const uint32_t*);
-    void launchClearCells(hipStream_t, uint2*, const uint32_t*, uint32_t*, uint32_t);
+    void launchResetSearch(hipStream_t, Status*);
+    void launchKeys(hipStream_t, const float4*, uint32_t, const Frame*, Status*, bool, uint32_t*, uint32_t*);
+    void launchKeysSorted(hipStream_t, const float4*, uint32_t, const Frame*, Status*, uint32_t*, uint32_t*);
+    void launchRadixSort(hipStream_t, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*);
+    void launchPermute(hipStream_t, const Gather&, const uint32_t*);
+    void launchClearCells(hipStream_t, const uint32_t*, uint32_t, const Frame*, uint2*);
     void launchClearAll(hipStream_t, uint2*, uint64_t);
-    void launchBuildCells(hipStream_t, const uint32_t*, uint32_t, const Frame*, uint2*, uint32_t*, uint32_t*);
+    void launchBuildCells(hipStream_t, const uint32_t*, uint32_t, const Frame*, uint2*);
     void launchQueryCount(hipStream_t, const Dev&, float, int);
-    void launchScanSlices(hipStream_t, const Dev&);
+    void launchScanSlices(hipStream_t, const Dev&, uint32_t*, uint32_t*);
     void launchQueryFill(hipStream_t, const Dev&, float, int);
-    void launchQueryRigid(hipStream_t, const Dev&, float, int, bool);
-    void launchAabb(hipStream_t, const float*, const float*, const float*, uint32_t, uint32_t*);
+    void launchQueryRigid(hipStream_t, const Dev&, float, int, bool, uint32_t*);
+    void launchAabb(hipStream_t, const float4*, uint32_t, uint32_t*);
     void launchDensity(hipStream_t, const Dev&, const Consts&);
     void launchNonPressure(hipStream_t, const Dev&, const Consts&);
     void launchSourceDiag(hipStream_t, const Dev&, const Consts&);
@@ -142,25 +141,25 @@ struct tsp_ctx
     bool pulled = false;
 
     Dev d{};
-    float* F[TSP_F_COUNT] = {};
-    float* F2[TSP_F_COUNT] = {};
-    uint64_t *idF = nullptr, *idF2 = nullptr;
-    float* B[TSP_B_COUNT] = {};
-    float* B2[TSP_B_COUNT] = {};
+    // second buffers of the re-order (swapped with the ones in d after each gather)
+    float4 *pos2 = nullptr, *vel2 = nullptr, *vp2 = nullptr, *ap2 = nullptr;
+    float *source2 = nullptr, *diag2 = nullptr;
+    uint64_t* idF2 = nullptr;
+    float4* bpos2 = nullptr;
+    float* bvol2 = nullptr;
     uint64_t *idB = nullptr, *idB2 = nullptr;
     uint64_t* idR = nullptr;
-    uint32_t *idRB = nullptr, *countRB = nullptr; // host copies (static)
     std::vector<uint32_t> hostIdRB, hostCountRB;
     uint32_t* radixHist = nullptr;
+    uint32_t* sliceUnits = nullptr; // 3 x (slices + 1) scan buffer
     uint32_t* aabb = nullptr;
-    uint32_t* occB = nullptr;
-    uint32_t* nOccB = nullptr;
     std::vector<void*> allocs;      // particle-count sized
     std::vector<void*> frameAllocs; // frame sized (cell grids)
     std::vector<void*> poolAllocs;
 
     Frame frame{};
     bool frameValid = false;
+    bool cellsClean = false;        // fluid and rigid cell tables hold no runs (frame just built)
     uint32_t keyBits = 0;
     uint32_t reach = 1;
     float cellSize = 0.0f;
@@ -215,16 +214,6 @@ struct tsp_ctx
         sync();
         prof.resolve();
         last = *hostStatus;
-    }
-
-    void refreshFluidPointers()
-    {
-        float** dst[TSP_F_COUNT] = { &d.x, &d.y, &d.z, &d.vx, &d.vy, &d.vz, &d.vpx, &d.vpy, &d.vpz,
-                                     &d.apx, &d.apy, &d.apz, &d.source, &d.diag, &d.pressure, &d.density };
-        for (int f = 0; f < TSP_F_COUNT; f++) { *dst[f] = F[f]; }
-        d.id = idF;
-        d.bx = B[TSP_B_X]; d.by = B[TSP_B_Y]; d.bz = B[TSP_B_Z];
-        d.bvol = B[TSP_B_VOL]; d.bpsi = B[TSP_B_PSI];
     }
 
     // --- constants of the kernels (solver_constants.h:20-36, simd.h:359-370, hoisted kernel constants) ---
@@ -298,9 +287,9 @@ struct tsp_ctx
         // scene AABB of F, B and R on the device (order-free min/max)
         const uint32_t init[6] = { 0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u };
         hipCheck(hipMemcpyAsync(aabb, init, sizeof(init), hipMemcpyHostToDevice, stream), "aabb init");
-        launchAabb(stream, d.x, d.y, d.z, d.nF, aabb);
-        launchAabb(stream, d.bx, d.by, d.bz, d.nB, aabb);
-        launchAabb(stream, d.rx, d.ry, d.rz, d.nR, aabb);
+        launchAabb(stream, d.pos, d.nF, aabb);
+        launchAabb(stream, d.bpos, d.nB, aabb);
+        launchAabb(stream, d.rpos, d.nR, aabb);
         uint32_t bits[6];
         hipCheck(hipMemcpyAsync(bits, aabb, sizeof(bits), hipMemcpyDeviceToHost, stream), "aabb D2H");
         sync();
@@ -363,34 +352,30 @@ struct tsp_ctx
         launchClearAll(stream, d.cellF, nCells);
         launchClearAll(stream, d.cellB, nCells);
         launchClearAll(stream, d.cellR, nCells);
-        const uint32_t zero2[2] = { 0, 0 };
-        hipCheck(hipMemcpyAsync(&d.status->nOccF, zero2, sizeof(zero2), hipMemcpyHostToDevice, stream), "occ reset");
+        cellsClean = true;
 
         // boundary: re-sorted only if its ids are not already non-decreasing (neighbor_search.cpp:1755-1811)
         if (d.nB)
         {
-            const uint32_t z = 0;
-            hipCheck(hipMemcpyAsync(&d.status->bUnsorted, &z, 4, hipMemcpyHostToDevice, stream), "flag reset");
-            launchKeysSorted(stream, d.bx, d.by, d.bz, d.nB, d.frame, d.status, d.keyF, d.idxF);
+            launchResetSearch(stream, d.status);
+            launchKeysSorted(stream, d.bpos, d.nB, d.frame, d.status, d.keyF, d.idxF);
             uint32_t unsorted = 0;
             hipCheck(hipMemcpyAsync(&unsorted, &d.status->bUnsorted, 4, hipMemcpyDeviceToHost, stream), "flag D2H");
             sync();
             if (unsorted)
             {
-                launchRadixSort(stream, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nB, keyBits, radixHist);
-                FieldSet fs{};
-                for (int k = 0; k < TSP_B_COUNT; k++) { fs.src[k] = B[k]; fs.dst[k] = B2[k]; }
-                fs.nFields = TSP_B_COUNT;
-                fs.idSrc = idB;
-                fs.idDst = idB2;
-                fs.n = d.nB;
-                launchPermute(stream, fs, d.idxF);
-                for (int k = 0; k < TSP_B_COUNT; k++) { std::swap(B[k], B2[k]); }
+                launchRadixSort(stream, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nB, keyBits, radixHist, d.scanTmp);
+                Gather g{};
+                g.s4[0] = d.bpos; g.d4[0] = bpos2; g.n4 = 1;
+                g.s1[0] = d.bvol; g.d1[0] = bvol2; g.n1 = 1;
+                g.idS = idB; g.idD = idB2;
+                g.n = d.nB;
+                launchPermute(stream, g, d.idxF);
+                std::swap(d.bpos, bpos2);
+                std::swap(d.bvol, bvol2);
                 std::swap(idB, idB2);
-                refreshFluidPointers();
             }
-            hipCheck(hipMemcpyAsync(nOccB, zero2, 4, hipMemcpyHostToDevice, stream), "occ reset");
-            launchBuildCells(stream, d.keyF, d.nB, d.frame, d.cellB, occB, nOccB);
+            launchBuildCells(stream, d.keyF, d.nB, d.frame, d.cellB);
             sync();
         }
         frameValid = true;
@@ -406,68 +391,74 @@ struct tsp_ctx
         {
             {
                 Scope sc(prof, stream, "reset");
-                Status zero{};
-                // per-search fields only: keep the occupancy counters and solver state
-                hipCheck(hipMemcpyAsync(d.status, &zero, offsetof(Status, nOccF), hipMemcpyHostToDevice, stream), "status reset");
-                hipCheck(hipMemcpyAsync(&d.status->poolUnits[0], &zero.poolUnits[0],
-                                        offsetof(Status, iters) - offsetof(Status, poolUnits), hipMemcpyHostToDevice, stream),
-                         "status reset");
+                launchResetSearch(stream, d.status);
+                // the runs of the previous sorted keys are the only non-empty cells
+                if (!cellsClean)
+                {
+                    launchClearCells(stream, d.keyF, d.nF, d.frame, d.cellF);
+                    launchClearCells(stream, d.keyR, d.nR, d.frame, d.cellR);
+                }
             }
             if (d.nR)
             {
                 Scope sc(prof, stream, "keys_sort_rigid");
-                launchKeys(stream, d.rx, d.ry, d.rz, d.nR, d.frame, d.status, true, d.keyR, d.idxR);
-                launchRadixSort(stream, d.keyR, d.idxR, d.keyR2, d.idxR2, d.nR, keyBits, radixHist);
+                launchKeys(stream, d.rpos, d.nR, d.frame, d.status, true, d.keyR, d.idxR);
+                launchRadixSort(stream, d.keyR, d.idxR, d.keyR2, d.idxR2, d.nR, keyBits, radixHist, d.scanTmp);
             }
-            FieldSet fs{};
             {
                 Scope sc(prof, stream, "keys_sort");
-                launchKeys(stream, d.x, d.y, d.z, d.nF, d.frame, d.status, true, d.keyF, d.idxF);
-                launchRadixSort(stream, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nF, keyBits, radixHist);
+                launchKeys(stream, d.pos, d.nF, d.frame, d.status, true, d.keyF, d.idxF);
+                launchRadixSort(stream, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nF, keyBits, radixHist, d.scanTmp);
             }
-            {
-                Scope sc(prof, stream, "permute");
-                const bool warm = params.jacobi_warm_start > 0.0f;
-                for (int f = 0; f < TSP_F_COUNT; f++)
-                {
-                    const bool keep = f <= TSP_F_VZ || (f == TSP_F_PRESSURE && warm);
-                    if (allFields || keep) { fs.src[fs.nFields] = F[f]; fs.dst[fs.nFields] = F2[f]; fs.nFields++; }
-                }
-                fs.idSrc = idF;
-                fs.idDst = idF2;
-                fs.n = d.nF;
-                launchPermute(stream, fs, d.idxF);
-            }
+            // the step re-orders positions, velocities (with density) and pressure (pos.w, the
+            // warm start); the standalone search re-orders every field
             const auto swapFields = [&]()
             {
-                const bool warm = params.jacobi_warm_start > 0.0f;
-                for (int f = 0; f < TSP_F_COUNT; f++)
+                std::swap(d.pos, pos2);
+                std::swap(d.vel, vel2);
+                std::swap(d.id, idF2);
+                if (allFields)
                 {
-                    const bool keep = f <= TSP_F_VZ || (f == TSP_F_PRESSURE && warm);
-                    if (allFields || keep) { std::swap(F[f], F2[f]); }
+                    std::swap(d.vp, vp2);
+                    std::swap(d.ap, ap2);
+                    std::swap(d.source, source2);
+                    std::swap(d.diag, diag2);
                 }
-                std::swap(idF, idF2);
-                refreshFluidPointers();
             };
+            {
+                Scope sc(prof, stream, "permute");
+                Gather g{};
+                g.s4[0] = d.pos; g.d4[0] = pos2;
+                g.s4[1] = d.vel; g.d4[1] = vel2;
+                g.n4 = 2;
+                if (allFields)
+                {
+                    g.s4[2] = d.vp; g.d4[2] = vp2;
+                    g.s4[3] = d.ap; g.d4[3] = ap2;
+                    g.n4 = 4;
+                    g.s1[0] = d.source; g.d1[0] = source2;
+                    g.s1[1] = d.diag; g.d1[1] = diag2;
+                    g.n1 = 2;
+                }
+                g.idS = d.id; g.idD = idF2;
+                g.n = d.nF;
+                launchPermute(stream, g, d.idxF);
+            }
             swapFields();
             {
                 Scope sc(prof, stream, "cells");
-                launchClearCells(stream, d.cellF, d.occF, &d.status->nOccF, d.nF);
-                launchBuildCells(stream, d.keyF, d.nF, d.frame, d.cellF, d.occF, &d.status->nOccF);
-                if (d.nR)
-                {
-                    launchClearCells(stream, d.cellR, d.occR, &d.status->nOccR, d.nR);
-                    launchBuildCells(stream, d.keyR, d.nR, d.frame, d.cellR, d.occR, &d.status->nOccR);
-                }
+                launchBuildCells(stream, d.keyF, d.nF, d.frame, d.cellF);
+                launchBuildCells(stream, d.keyR, d.nR, d.frame, d.cellR);
+                cellsClean = false;
             }
             {
                 Scope sc(prof, stream, "query_count");
                 launchQueryCount(stream, d, params.kernel_radius_sq, static_cast<int>(reach));
-                launchQueryRigid(stream, d, params.kernel_radius_sq, static_cast<int>(reach), false);
+                launchQueryRigid(stream, d, params.kernel_radius_sq, static_cast<int>(reach), false, d.scanTmp);
             }
             {
                 Scope sc(prof, stream, "scan");
-                launchScanSlices(stream, d);
+                launchScanSlices(stream, d, sliceUnits, d.scanTmp);
             }
             readStatus();
             if (last.abort)
@@ -494,7 +485,7 @@ struct tsp_ctx
             {
                 Scope sc(prof, stream, "query_fill");
                 launchQueryFill(stream, d, params.kernel_radius_sq, static_cast<int>(reach));
-                if (d.nR) { launchQueryRigid(stream, d, params.kernel_radius_sq, static_cast<int>(reach), true); }
+                launchQueryRigid(stream, d, params.kernel_radius_sq, static_cast<int>(reach), true, d.scanTmp);
             }
             if (last.rbTotal + last.rrTotal > 0) { flags |= TSP_FLAG_RIGID_CONTACT; }
             (void)k;
@@ -679,6 +670,16 @@ int tsp_set_domain(tsp_ctx* ctx, const float mn[3], const float mx[3])
         if (!mn || !mx) { throw Error{ TSP_ERR_INVALID, "null domain" }; }
         for (int a = 0; a < 3; a++) { ctx->domMin[a] = mn[a]; ctx->domMax[a] = mx[a]; }
         ctx->domainSet = true;
+        // the hysteresis test reads the domain of every step (IsInsideSimDomain); the frame
+        // extent itself changes only at the next rebuild
+        if (ctx->frameValid)
+        {
+            Frame& f = ctx->frame;
+            f.domainSet = (mx[0] > mn[0] && mx[1] > mn[1] && mx[2] > mn[2]) ? 1 : 0;
+            for (int a = 0; a < 3; a++) { f.domMin[a] = mn[a]; f.domMax[a] = mx[a]; }
+            hipCheck(hipMemcpyAsync(ctx->d.frame, &f, sizeof(Frame), hipMemcpyHostToDevice, ctx->stream), "frame H2D");
+            ctx->sync();
+        }
         return TSP_OK;
     });
 }
@@ -699,25 +700,46 @@ int tsp_pull(tsp_ctx* ctx, const tsp_state_view* s)
         ctx->sync();
         ctx->releaseAll();
         ctx->d = Dev{};
-        std::memset(ctx->F, 0, sizeof(ctx->F));
-        std::memset(ctx->F2, 0, sizeof(ctx->F2));
         Dev& d = ctx->d;
         d.nF = s->n_fluid; d.nB = s->n_boundary; d.nR = s->n_rigid; d.nRB = s->n_bodies;
         auto& A = ctx->allocs;
         hipStream_t st = ctx->stream;
         const auto h2d = [&](void* dst, const void* src, size_t bytes) {
             if (bytes) { hipCheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), "H2D"); }
+            ctx->sync();
         };
-        // fluid
-        for (int f = 0; f < TSP_F_COUNT; f++)
+        const auto field = [](const float* src, uint32_t i, float dflt) { return src ? src[i] : dflt; };
+        // fluid: the SoA fields of the view packed into the float4 records
         {
-            ctx->F[f] = ctx->alloc<float>(d.nF, A);
-            ctx->F2[f] = ctx->alloc<float>(d.nF, A);
-            if (s->fluid[f]) { h2d(ctx->F[f], s->fluid[f], sizeof(float) * d.nF); }
-            else { hipCheck(hipMemsetAsync(ctx->F[f], 0, sizeof(float) * std::max<uint32_t>(d.nF, 1), st), "memset"); }
+            const uint32_t n = d.nF;
+            std::vector<float4> q[4];
+            for (auto& v : q) { v.resize(n); }
+            std::vector<float> src1(n), diag1(n);
+            float* const* f = s->fluid;
+            for (uint32_t i = 0; i < n; i++)
+            {
+                q[0][i] = make_float4(field(f[TSP_F_X], i, 0), field(f[TSP_F_Y], i, 0), field(f[TSP_F_Z], i, 0), field(f[TSP_F_PRESSURE], i, 0));
+                q[1][i] = make_float4(field(f[TSP_F_VX], i, 0), field(f[TSP_F_VY], i, 0), field(f[TSP_F_VZ], i, 0), field(f[TSP_F_DENSITY], i, 0));
+                q[2][i] = make_float4(field(f[TSP_F_VPX], i, 0), field(f[TSP_F_VPY], i, 0), field(f[TSP_F_VPZ], i, 0), 0.0f);
+                q[3][i] = make_float4(field(f[TSP_F_APX], i, 0), field(f[TSP_F_APY], i, 0), field(f[TSP_F_APZ], i, 0), 0.0f);
+                src1[i] = field(f[TSP_F_SOURCE], i, 0);
+                diag1[i] = field(f[TSP_F_DIAG], i, 0);
+            }
+            float4** dst[4] = { &d.pos, &d.vel, &d.vp, &d.ap };
+            float4** dst2[4] = { &ctx->pos2, &ctx->vel2, &ctx->vp2, &ctx->ap2 };
+            for (int r = 0; r < 4; r++)
+            {
+                *dst[r] = ctx->alloc<float4>(n, A);
+                *dst2[r] = ctx->alloc<float4>(n, A);
+                h2d(*dst[r], q[r].data(), sizeof(float4) * n);
+            }
+            d.source = ctx->alloc<float>(n, A);
+            d.diag = ctx->alloc<float>(n, A);
+            ctx->source2 = ctx->alloc<float>(n, A);
+            ctx->diag2 = ctx->alloc<float>(n, A);
+            h2d(d.source, src1.data(), sizeof(float) * n);
+            h2d(d.diag, diag1.data(), sizeof(float) * n);
         }
-        ctx->idF = ctx->alloc<uint64_t>(d.nF, A);
-        ctx->idF2 = ctx->alloc<uint64_t>(d.nF, A);
         std::vector<uint64_t> seq;
         const auto ids = [&](const uint64_t* src, uint32_t n) -> const uint64_t* {
             if (src) { return src; }
@@ -725,47 +747,78 @@ int tsp_pull(tsp_ctx* ctx, const tsp_state_view* s)
             for (uint32_t i = 0; i < n; i++) { seq[i] = i; }
             return seq.data();
         };
-        h2d(ctx->idF, ids(s->fluid_id, d.nF), sizeof(uint64_t) * d.nF);
-        ctx->sync();
+        d.id = ctx->alloc<uint64_t>(d.nF, A);
+        ctx->idF2 = ctx->alloc<uint64_t>(d.nF, A);
+        h2d(d.id, ids(s->fluid_id, d.nF), sizeof(uint64_t) * d.nF);
         // boundary (ensureBoundaryCoefficientsPresent defaults: psi = massB, V = psi / rho0)
-        for (int f = 0; f < TSP_B_COUNT; f++)
         {
-            ctx->B[f] = ctx->alloc<float>(d.nB, A);
-            ctx->B2[f] = ctx->alloc<float>(d.nB, A);
-        }
-        for (int f = 0; f < TSP_B_COUNT; f++)
-        {
-            if (s->boundary[f]) { h2d(ctx->B[f], s->boundary[f], sizeof(float) * d.nB); continue; }
+            const uint32_t n = d.nB;
             const float h = ctx->params.particle_size, rho0 = ctx->params.rest_density;
             const float massB = rho0 * h * h * h;
-            std::vector<float> v(d.nB, f == TSP_B_PSI ? massB : (f == TSP_B_VOL ? (rho0 > 0.0f ? massB / rho0 : 0.0f) : 0.0f));
-            h2d(ctx->B[f], v.data(), sizeof(float) * d.nB);
-            ctx->sync();
+            const float volB = rho0 > 0.0f ? massB / rho0 : 0.0f;
+            std::vector<float4> q(n);
+            std::vector<float> vol(n);
+            float* const* f = s->boundary;
+            for (uint32_t i = 0; i < n; i++)
+            {
+                q[i] = make_float4(field(f[TSP_B_X], i, 0), field(f[TSP_B_Y], i, 0), field(f[TSP_B_Z], i, 0), field(f[TSP_B_PSI], i, massB));
+                vol[i] = field(f[TSP_B_VOL], i, volB);
+            }
+            d.bpos = ctx->alloc<float4>(n, A);
+            ctx->bpos2 = ctx->alloc<float4>(n, A);
+            d.bvol = ctx->alloc<float>(n, A);
+            ctx->bvol2 = ctx->alloc<float>(n, A);
+            h2d(d.bpos, q.data(), sizeof(float4) * n);
+            h2d(d.bvol, vol.data(), sizeof(float) * n);
         }
         ctx->idB = ctx->alloc<uint64_t>(d.nB, A);
         ctx->idB2 = ctx->alloc<uint64_t>(d.nB, A);
         h2d(ctx->idB, ids(s->boundary_id, d.nB), sizeof(uint64_t) * d.nB);
-        ctx->sync();
-        // rigid particles
-        float** rs[TSP_R_COUNT] = { &d.rxrel, &d.ryrel, &d.rzrel, &d.rvol, &d.rpsi };
-        for (int f = 0; f < TSP_R_COUNT; f++)
+        // rigid particles: body-frame offsets; world records are written from the body state below
         {
-            *rs[f] = ctx->alloc<float>(d.nR, A);
-            if (s->rigid[f]) { h2d(*rs[f], s->rigid[f], sizeof(float) * d.nR); }
+            const uint32_t n = d.nR;
+            float* const* f = s->rigid;
+            std::vector<float> rel[3];
+            std::vector<float4> wp(n), wv(n), wvp(n);
+            for (int a2 = 0; a2 < 3; a2++) { rel[a2].resize(n); }
+            for (uint32_t i = 0; i < n; i++)
+            {
+                rel[0][i] = field(f[TSP_R_XREL], i, 0);
+                rel[1][i] = field(f[TSP_R_YREL], i, 0);
+                rel[2][i] = field(f[TSP_R_ZREL], i, 0);
+                wp[i] = make_float4(0.0f, 0.0f, 0.0f, field(f[TSP_R_PSI], i, 0));
+                wv[i] = make_float4(0.0f, 0.0f, 0.0f, field(f[TSP_R_VOL], i, 0));
+                wvp[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+            float** rr[3] = { &d.rxrel, &d.ryrel, &d.rzrel };
+            for (int a2 = 0; a2 < 3; a2++)
+            {
+                *rr[a2] = ctx->alloc<float>(n, A);
+                h2d(*rr[a2], rel[a2].data(), sizeof(float) * n);
+            }
+            d.rpos = ctx->alloc<float4>(n, A);
+            d.rvel = ctx->alloc<float4>(n, A);
+            d.rvp = ctx->alloc<float4>(n, A);
+            h2d(d.rpos, wp.data(), sizeof(float4) * n);
+            h2d(d.rvel, wv.data(), sizeof(float4) * n);
+            h2d(d.rvp, wvp.data(), sizeof(float4) * n);
         }
         d.rbody = ctx->alloc<uint32_t>(d.nR, A);
         h2d(d.rbody, s->rigid_body, sizeof(uint32_t) * d.nR);
         ctx->idR = ctx->alloc<uint64_t>(d.nR, A);
         h2d(ctx->idR, ids(s->rigid_id, d.nR), sizeof(uint64_t) * d.nR);
-        ctx->sync();
-        float** rw[12] = { &d.rx, &d.ry, &d.rz, &d.rvx, &d.rvy, &d.rvz, &d.rvpx, &d.rvpy, &d.rvpz, &d.rfx, &d.rfy, &d.rfz };
-        for (int q = 0; q < 12; q++) { *rw[q] = ctx->alloc<float>(d.nR, A); }
+        d.rfx = ctx->alloc<float>(d.nR, A);
+        d.rfy = ctx->alloc<float>(d.nR, A);
+        d.rfz = ctx->alloc<float>(d.nR, A);
         // bodies
         d.rb = ctx->alloc<float>(static_cast<size_t>(TSP_RB_COUNT) * d.nRB, A);
         d.rbPred = ctx->alloc<float>(9ull * std::max<uint32_t>(d.nRB, 1), A);
         d.rbForce = ctx->alloc<float>(6ull * std::max<uint32_t>(d.nRB, 1), A);
-        hipCheck(hipMemsetAsync(d.rbForce, 0, sizeof(float) * 6 * std::max<uint32_t>(d.nRB, 1), st), "memset");
-        hipCheck(hipMemsetAsync(d.rbPred, 0, sizeof(float) * 9 * std::max<uint32_t>(d.nRB, 1), st), "memset");
+        {
+            std::vector<float> zeros(9ull * std::max<uint32_t>(d.nRB, 1), 0.0f);
+            h2d(d.rbForce, zeros.data(), sizeof(float) * 6 * std::max<uint32_t>(d.nRB, 1));
+            h2d(d.rbPred, zeros.data(), sizeof(float) * 9 * std::max<uint32_t>(d.nRB, 1));
+        }
         for (int f = 0; f < TSP_RB_COUNT; f++)
         {
             if (s->body[f]) { h2d(d.rb + static_cast<size_t>(f) * d.nRB, s->body[f], sizeof(float) * d.nRB); }
@@ -773,7 +826,6 @@ int tsp_pull(tsp_ctx* ctx, const tsp_state_view* s)
             {
                 std::vector<float> v(d.nRB, (f == TSP_RB_A00 || f == TSP_RB_A11 || f == TSP_RB_A22) ? 1.0f : 0.0f);
                 h2d(d.rb + static_cast<size_t>(f) * d.nRB, v.data(), sizeof(float) * d.nRB);
-                ctx->sync();
             }
         }
         ctx->hostIdRB.assign(d.nRB, 0);
@@ -794,7 +846,6 @@ int tsp_pull(tsp_ctx* ctx, const tsp_state_view* s)
             d.byBodyIdx = ctx->alloc<uint32_t>(d.nR, A);
             h2d(d.byBodyOff, off.data(), sizeof(uint32_t) * (d.nRB + 1));
             h2d(d.byBodyIdx, idx.data(), sizeof(uint32_t) * d.nR);
-            ctx->sync();
         }
         // neighbor search buffers
         const uint32_t nSort = std::max(d.nF, std::max(d.nB, d.nR));
@@ -806,17 +857,20 @@ int tsp_pull(tsp_ctx* ctx, const tsp_state_view* s)
         d.keyR2 = ctx->alloc<uint32_t>(d.nR, A);
         d.idxR = ctx->alloc<uint32_t>(d.nR, A);
         d.idxR2 = ctx->alloc<uint32_t>(d.nR, A);
-        d.occF = ctx->alloc<uint32_t>(d.nF, A);
-        d.occR = ctx->alloc<uint32_t>(d.nR, A);
-        ctx->occB = ctx->alloc<uint32_t>(d.nB, A);
-        ctx->nOccB = ctx->alloc<uint32_t>(1, A);
-        ctx->radixHist = ctx->alloc<uint32_t>(256ull * ((nSort + 2047) / 2048 + 1), A);
+        const uint64_t nTiles = (nSort + 2047ull) / 2048ull;
+        ctx->radixHist = ctx->alloc<uint32_t>(256ull * nTiles, A);
         ctx->aabb = ctx->alloc<uint32_t>(6, A);
         const uint32_t nSlices = (d.nF + kSlice - 1) / kSlice;
         for (int r = 0; r < 3; r++)
         {
             d.cnt[r] = ctx->alloc<uint16_t>(d.nF, A);
             d.meta[r] = ctx->alloc<SliceMeta>(nSlices, A);
+        }
+        ctx->sliceUnits = ctx->alloc<uint32_t>(3ull * (nSlices + 1), A);
+        {
+            // block sums of the scans: radix digit offsets, slice units, RF offsets
+            const uint64_t m = std::max<uint64_t>(256ull * nTiles, std::max<uint64_t>(nSlices + 1ull, d.nR + 1ull));
+            d.scanTmp = ctx->alloc<uint32_t>((m + 2047ull) / 2048ull + 1ull, A);
         }
         d.ffWideMin = ctx->alloc<uint32_t>(d.nF, A);
         d.rfCnt = ctx->alloc<uint32_t>(d.nR, A);
@@ -826,11 +880,13 @@ int tsp_pull(tsp_ctx* ctx, const tsp_state_view* s)
         d.errPartial = ctx->alloc<double>(3ull * ((d.nF + kBlock - 1) / kBlock + 1), A);
         d.status = ctx->alloc<Status>(1, A);
         d.frame = ctx->alloc<Frame>(1, A);
-        hipCheck(hipMemsetAsync(d.status, 0, sizeof(Status), st), "memset");
+        {
+            const Status zero{};
+            h2d(d.status, &zero, sizeof(Status));
+        }
         // initial pool sizes: about 40 FF, 16 FB and 8 FR entries per particle, grown on demand
         const uint64_t cap[3] = { 48ull * d.nF + 1024, 16ull * d.nF + 1024, 8ull * d.nF + 1024 };
         ctx->allocPools(cap, 64ull * d.nR + 1024);
-        ctx->refreshFluidPointers();
         ctx->sync();
         launchRigidUpdate(st, d, false);
         ctx->sync();
@@ -860,12 +916,64 @@ int tsp_push(tsp_ctx* ctx, tsp_state_view* s)
         const auto d2h = [&](void* dst, const void* src, size_t bytes) {
             if (dst && bytes) { hipCheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st), "D2H"); }
         };
-        for (int f = 0; f < TSP_F_COUNT; f++) { d2h(s->fluid[f], ctx->F[f], sizeof(float) * d.nF); }
-        d2h(s->fluid_id, ctx->idF, sizeof(uint64_t) * d.nF);
-        for (int f = 0; f < TSP_B_COUNT; f++) { d2h(s->boundary[f], ctx->B[f], sizeof(float) * d.nB); }
+        // fluid records unpacked into the SoA fields of the view
+        {
+            const uint32_t n = d.nF;
+            std::vector<float4> q[4];
+            const float4* srcs[4] = { d.pos, d.vel, d.vp, d.ap };
+            for (int r = 0; r < 4; r++)
+            {
+                q[r].resize(n);
+                d2h(q[r].data(), srcs[r], sizeof(float4) * n);
+            }
+            ctx->sync();
+            float* const* f = s->fluid;
+            const int comp[4][4] = { { TSP_F_X, TSP_F_Y, TSP_F_Z, TSP_F_PRESSURE }, { TSP_F_VX, TSP_F_VY, TSP_F_VZ, TSP_F_DENSITY },
+                                     { TSP_F_VPX, TSP_F_VPY, TSP_F_VPZ, -1 }, { TSP_F_APX, TSP_F_APY, TSP_F_APZ, -1 } };
+            for (int r = 0; r < 4; r++)
+            {
+                for (int c = 0; c < 4; c++)
+                {
+                    float* out = comp[r][c] >= 0 ? f[comp[r][c]] : nullptr;
+                    if (!out) { continue; }
+                    for (uint32_t i = 0; i < n; i++) { out[i] = reinterpret_cast<const float*>(&q[r][i])[c]; }
+                }
+            }
+            d2h(f[TSP_F_SOURCE], d.source, sizeof(float) * n);
+            d2h(f[TSP_F_DIAG], d.diag, sizeof(float) * n);
+        }
+        d2h(s->fluid_id, d.id, sizeof(uint64_t) * d.nF);
+        {
+            const uint32_t n = d.nB;
+            std::vector<float4> q(n);
+            d2h(q.data(), d.bpos, sizeof(float4) * n);
+            ctx->sync();
+            float* const* f = s->boundary;
+            const int comp[4] = { TSP_B_X, TSP_B_Y, TSP_B_Z, TSP_B_PSI };
+            for (int c = 0; c < 4; c++)
+            {
+                if (!f[comp[c]]) { continue; }
+                for (uint32_t i = 0; i < n; i++) { f[comp[c]][i] = reinterpret_cast<const float*>(&q[i])[c]; }
+            }
+            d2h(f[TSP_B_VOL], d.bvol, sizeof(float) * n);
+        }
         d2h(s->boundary_id, ctx->idB, sizeof(uint64_t) * d.nB);
-        const float* rs[TSP_R_COUNT] = { d.rxrel, d.ryrel, d.rzrel, d.rvol, d.rpsi };
-        for (int f = 0; f < TSP_R_COUNT; f++) { d2h(s->rigid[f], rs[f], sizeof(float) * d.nR); }
+        {
+            const uint32_t n = d.nR;
+            float* const* f = s->rigid;
+            d2h(f[TSP_R_XREL], d.rxrel, sizeof(float) * n);
+            d2h(f[TSP_R_YREL], d.ryrel, sizeof(float) * n);
+            d2h(f[TSP_R_ZREL], d.rzrel, sizeof(float) * n);
+            std::vector<float4> wp(n), wv(n);
+            d2h(wp.data(), d.rpos, sizeof(float4) * n);
+            d2h(wv.data(), d.rvel, sizeof(float4) * n);
+            ctx->sync();
+            for (uint32_t i = 0; i < n; i++)
+            {
+                if (f[TSP_R_PSI]) { f[TSP_R_PSI][i] = wp[i].w; }
+                if (f[TSP_R_VOL]) { f[TSP_R_VOL][i] = wv[i].w; }
+            }
+        }
         d2h(s->rigid_body, d.rbody, sizeof(uint32_t) * d.nR);
         d2h(s->rigid_id, ctx->idR, sizeof(uint64_t) * d.nR);
         for (int f = 0; f < TSP_RB_COUNT; f++) { d2h(s->body[f], d.rb + static_cast<size_t>(f) * d.nRB, sizeof(float) * d.nRB); }
@@ -1031,7 +1139,7 @@ int tsp_get_sort(tsp_ctx* ctx, uint32_t* orderF, uint64_t* keysF, uint32_t* orde
         {
             // the boundary order relative to the pulled order, from the ids (stable sequence ids on pull)
             std::vector<uint32_t> kb(d.nB), ib(d.nB);
-            launchKeys(ctx->stream, d.bx, d.by, d.bz, d.nB, d.frame, d.status, false, d.keyF2, d.idxF2);
+            launchKeys(ctx->stream, d.bpos, d.nB, d.frame, d.status, false, d.keyF2, d.idxF2);
             ctx->sync();
             if (keysB) { widen(keysB, d.keyF2, d.nB); }
             if (orderB)
@@ -1050,16 +1158,14 @@ int tsp_get_rigid_world(tsp_ctx* ctx, float* xyz, float* vel)
     return guarded(ctx, [&]() {
         const Dev& d = ctx->d;
         ctx->sync();
-        std::vector<float> a(d.nR), b(d.nR), c(d.nR);
-        const float* src[2][3] = { { d.rx, d.ry, d.rz }, { d.rvx, d.rvy, d.rvz } };
+        std::vector<float4> q(d.nR);
+        const float4* src[2] = { d.rpos, d.rvel };
         float* dst[2] = { xyz, vel };
-        for (int q = 0; q < 2; q++)
+        for (int r = 0; r < 2; r++)
         {
-            if (!dst[q] || !d.nR) { continue; }
-            hipCheck(hipMemcpy(a.data(), src[q][0], 4ull * d.nR, hipMemcpyDeviceToHost), "D2H");
-            hipCheck(hipMemcpy(b.data(), src[q][1], 4ull * d.nR, hipMemcpyDeviceToHost), "D2H");
-            hipCheck(hipMemcpy(c.data(), src[q][2], 4ull * d.nR, hipMemcpyDeviceToHost), "D2H");
-            for (uint32_t i = 0; i < d.nR; i++) { dst[q][3 * i] = a[i]; dst[q][3 * i + 1] = b[i]; dst[q][3 * i + 2] = c[i]; }
+            if (!dst[r] || !d.nR) { continue; }
+            hipCheck(hipMemcpy(q.data(), src[r], sizeof(float4) * d.nR, hipMemcpyDeviceToHost), "D2H");
+            for (uint32_t i = 0; i < d.nR; i++) { dst[r][3 * i] = q[i].x; dst[r][3 * i + 1] = q[i].y; dst[r][3 * i + 2] = q[i].z; }
         }
         return TSP_OK;
     });

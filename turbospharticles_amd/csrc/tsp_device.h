@@ -1,11 +1,15 @@
 // tsp_device.h -- device data layout and shared helpers of the gfx950 IISPH step.
 //
-// Layout in HBM (all SoA, one array per component, indexed in the current Morton order):
-//   fluid      x y z | vx vy vz | vpx vpy vpz | apx apy apz | source diag pressure density  (f32)
-//              id (u64); x..vz and id are double-buffered so the re-order is a gather + swap
-//   boundary   x y z vol psi (f32), id (u64): sorted by cell id once, static afterwards
-//   rigid      xrel yrel zrel vol psi, body (u32), id (u64); world x y z, v, predicted v,
-//              reaction force (f32), per-body SoA state (f32) with FP64 force/torque sums
+// Layout in HBM (indexed in the current Morton order; one neighbor of a sum costs one or two
+// 16-byte gathers):
+//   fluid      pos (x, y, z, pressure) | vel (vx, vy, vz, density) | vp (v*, 0) | ap (a^p, 0)
+//              as float4; source, diag (f32); id (u64). Every array has a second buffer, so the
+//              re-order is one gather + pointer swap
+//   boundary   bpos (x, y, z, psi) float4, V (f32), id (u64): sorted by cell id when the frame
+//              is built, static afterwards
+//   rigid      world (x, y, z, psi), velocity (v, V), predicted velocity (v*, 0) as float4;
+//              body-frame offsets, body (u32), id (u64), reaction force (f32); per-body state
+//              field-major (f32) with FP64 force/torque sums
 //   cell grid  dense over the cell frame, one uint2 {begin, end} per cell and particle class
 //   SELL-32    per slice of 32 consecutive fluid particles and relation (FF, FB, FR): one
 //              SliceMeta {offset in u16 units, base, pitch, format}; entry k of lane l at
@@ -28,11 +32,11 @@ namespace tsp
 
     struct SliceMeta
     {
-        uint32_t offset; // in u16 units from the start of the pool
+        uint32_t offset;  // in u16 units from the start of the pool (storage units before the scan)
         uint32_t base;
         uint16_t pitch;
         uint16_t format;
-        uint32_t pad;
+        uint32_t entries; // list entries of the slice without padding
     };
 
     // solver status shared by the kernels of one step and read back by the host
@@ -42,8 +46,7 @@ namespace tsp
         uint32_t abort;          // a particle left the inner cell frame: the step must restart
         uint32_t overflow;       // a SELL pool is too small
         uint32_t bUnsorted;      // boundary ids are not non-decreasing under the frame
-        uint32_t nOccF, nOccR;   // occupied cells written by the last table build
-        uint32_t pad0;
+        uint32_t pad0[3];
         uint64_t poolUnits[3];   // u16 units used by FF, FB, FR
         uint64_t entries[3];     // entries (without padding) of FF, FB, FR
         uint64_t rfTotal, rbTotal, rrTotal;
@@ -87,14 +90,17 @@ namespace tsp
         uint32_t nF;
     };
 
-    // fields gathered by one permutation launch
-    struct FieldSet
+    // records gathered by one permutation launch
+    struct Gather
     {
-        const float* src[16];
-        float* dst[16];
-        const uint64_t* idSrc;
-        uint64_t* idDst;
-        uint32_t nFields;
+        const float4* s4[4];
+        float4* d4[4];
+        uint32_t n4;
+        const float* s1[4];
+        float* d1[4];
+        uint32_t n1;
+        const uint64_t* idS;
+        uint64_t* idD;
         uint32_t n;
     };
 
@@ -102,16 +108,19 @@ namespace tsp
     struct Dev
     {
         uint32_t nF, nB, nR, nRB;
-        // fluid
-        float *x, *y, *z, *vx, *vy, *vz, *vpx, *vpy, *vpz, *apx, *apy, *apz;
-        float *source, *diag, *pressure, *density;
+        // fluid, packed so that one neighbor costs one 16-byte gather per quantity:
+        //   pos (x, y, z, pressure), vel (vx, vy, vz, density), vp (v*, 0), ap (a^p, 0)
+        float4 *pos, *vel, *vp, *ap;
+        float *source, *diag;
         uint64_t *id;
-        // boundary
-        float *bx, *by, *bz, *bvol, *bpsi;
-        // rigid
-        float *rxrel, *ryrel, *rzrel, *rvol, *rpsi;
+        // boundary: (x, y, z, psi) and V
+        float4 *bpos;
+        float *bvol;
+        // rigid: world (x, y, z, psi), velocity (v, V), predicted velocity (v*, 0); body-frame offsets
+        float4 *rpos, *rvel, *rvp;
+        float *rxrel, *ryrel, *rzrel;
         uint32_t *rbody;
-        float *rx, *ry, *rz, *rvx, *rvy, *rvz, *rvpx, *rvpy, *rvpz, *rfx, *rfy, *rfz;
+        float *rfx, *rfy, *rfz;
         float *rb;               // nRB x 32 state (TSP_RB_* field order, field-major)
         float *rbPred;           // nRB x 9: v, L, omega predicted
         float *rbForce;          // nRB x 6: force, torque (float, like the reference members)
@@ -120,7 +129,6 @@ namespace tsp
         uint32_t *keyF, *keyF2, *idxF, *idxF2;
         uint32_t *keyR, *keyR2, *idxR, *idxR2;
         uint2 *cellF, *cellB, *cellR;
-        uint32_t *occF, *occR;
         // SELL-32
         uint16_t *cnt[3];
         SliceMeta *meta[3];
@@ -131,8 +139,9 @@ namespace tsp
         uint32_t *rfCnt, *rfOff, *rfIdx;
         uint32_t *rbCnt, *rrCnt;
         uint64_t rfCap;
-        // reductions
+        // reductions and scans
         double *errPartial;      // 3 per block of the Jacobi kernel
+        uint32_t *scanTmp;       // block sums of the multi-block scans
         Status *status;
         Frame *frame;
     };
