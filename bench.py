@@ -59,6 +59,18 @@ def cpu_baseline(state, params, steps, threads):
     return state.n[0] * steps / dt, dt, sweeps / steps
 
 
+def copy_probe():
+    """Three plain device-to-device copies of 1 GiB: the counters' reading for a kernel that
+    reads and writes exactly 1 GiB per launch."""
+    import torch
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": src.numel() * 4, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,7 +80,12 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n-side", type=int, default=N_SIDE)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy a 1 GiB device buffer 3 times (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py runs one scene on one GPU; --gpus {args.gpus} is not supported "
               "(independent scenes would be spread over GPUs by HBM footprint).", file=sys.stderr)

@@ -115,6 +115,9 @@ enum { ORC_FF, ORC_FB, ORC_FR, ORC_RF, ORC_RB, ORC_RR };
     void P##_set_chunk_width(P##_ctx *c, uint32_t w);                                    \
     void P##_set_rigid_sums_f64(P##_ctx *c, int on);                                     \
     void P##_set_threads(P##_ctx *c, int n);                                             \
+    /* the cubic-spline alpha the kernels take (0: h^-3/(4 pi) like the solver); tests     \
+       of the single kernels pass the reference tests' alpha */                          \
+    void P##_set_kernel_alpha(P##_ctx *c, float alpha);                                  \
     int P##_pull(P##_ctx *c, const orc_state *s);                                        \
     int P##_push(P##_ctx *c, orc_state *s);                                              \
     int P##_neighbor_search(P##_ctx *c);                                                 \

@@ -69,6 +69,7 @@ def lib():
                 "set_chunk_width": (None, [vp, ctypes.c_uint32]),
                 "set_rigid_sums_f64": (None, [vp, ctypes.c_int]),
                 "set_threads": (None, [vp, ctypes.c_int]),
+                "set_kernel_alpha": (None, [vp, ctypes.c_float]),
                 "pull": (ctypes.c_int, [vp, ctypes.POINTER(OrcState)]),
                 "push": (ctypes.c_int, [vp, ctypes.POINTER(OrcState)]),
                 "neighbor_search": (ctypes.c_int, [vp]),
@@ -103,7 +104,7 @@ def to_orc_params(p):
 class Oracle:
     """One oracle context (prefix 'orc32' for the FP32 restatement, 'orc64' for FP64)."""
 
-    def __init__(self, params, *, precision=32, chunk_width=1, threads=None, rigid_f64=True):
+    def __init__(self, params, *, precision=32, chunk_width=1, threads=None, rigid_f64=True, alpha=0.0):
         self.L = lib()
         self.p = f"orc{precision}"
         self.h = self.fn("create")()
@@ -112,6 +113,7 @@ class Oracle:
         self.fn("set_chunk_width")(self.h, chunk_width)
         self.fn("set_rigid_sums_f64")(self.h, 1 if rigid_f64 else 0)
         self.fn("set_threads")(self.h, threads or min(8, os.cpu_count() or 1))
+        self.fn("set_kernel_alpha")(self.h, alpha)
         self.n = None
 
     def fn(self, name):

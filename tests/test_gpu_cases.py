@@ -1,0 +1,146 @@
+"""More parity cases of the HIP path against the oracle, and invariants checked on the GPU alone.
+
+- Ragged particle counts (not multiples of the 32-particle slice or the 256-thread block) and a
+  single particle.
+- A 48^3 dam break (110,592 fluid particles): FF lists whose indices span more than 0xffff take
+  the DictDelta16x2 order of the reference (low group, then the high group reversed); lists and
+  one step must match the oracle bit for bit.
+- Warm start, and the four termination conditions, over several steps.
+- A rigid body in free fall follows the semi-implicit Euler closed form; a spinning body keeps an
+  orthonormal rotation.
+Tolerances as in test_gpu_parity.py: ids, lists and one step bit-exact, several steps 1e-5
+relative with equal iteration counts; closed forms to 1e-6 + 1e-5 of the displacement.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from turbospharticles_amd import scenes
+from turbospharticles_amd.backend import HipBackend
+
+import oracle_lib as O
+from helpers import FIELDS_STEP, rel_err
+from test_oracle_step import _single_body, free_fall_expected
+
+pytestmark = pytest.mark.gpu
+
+H = 0.01
+
+
+def _pair(st, p):
+    g = HipBackend(0)
+    g.set_params(p)
+    g.pull(st)
+    o = O.Oracle(p, chunk_width=1, threads=8)
+    o.pull(st)
+    return g, o
+
+
+@pytest.mark.parametrize("counts", [(17, 13, 11), (1, 1, 1), (33, 2, 5), (64, 4, 1)])
+def test_ragged_counts_one_step_bit_exact(counts):
+    print(f"case: ragged fluid lattice {counts}", flush=True)
+    st = scenes.build_scene(scenes.dam_break_spec(0, H, counts=counts, tank=(0.4, 0.3, 0.3)))
+    p = scenes.make_params(H, termination_cond=scenes.TERM_POS)
+    g, o = _pair(st, p)
+    g.step(1)
+    o.step()
+    sg, so = g.push(), o.push()
+    assert g.info().solver_iters == o.info().solver_iters
+    assert np.array_equal(sg.fluid_id, so.fluid_id)
+    for f in FIELDS_STEP:
+        assert np.array_equal(sg.fluid[f], so.fluid[f]), f
+
+
+def test_wide_index_span_lists_and_step_bit_exact():
+    print("case: dam break 48^3 (wide-span FF lists)", flush=True)
+    st = scenes.build_scene(scenes.dam_break_spec(48, H))
+    p = scenes.make_params(H, termination_cond=scenes.TERM_POS)
+    g, o = _pair(st, p)
+    g.neighbor_search()
+    o.neighbor_search()
+    cg, ig = g.neighbors(O.FF)
+    co, io, _ = o.lists(O.FF)
+    assert np.array_equal(cg, co) and np.array_equal(ig, io)
+    off = np.concatenate([[0], np.cumsum(co.astype(np.int64))])
+    spans = np.array([io[off[i]:off[i + 1]].max() - io[off[i]:off[i + 1]].min() for i in range(len(co)) if co[i]])
+    assert (spans > 0xffff).sum() > 0, "the scene must hold wide-span lists"
+    g.step(1)
+    o.step()
+    sg, so = g.push(), o.push()
+    assert g.info().solver_iters == o.info().solver_iters
+    for f in FIELDS_STEP:
+        assert np.array_equal(sg.fluid[f], so.fluid[f]), f
+
+
+@pytest.mark.parametrize("cond,warm", [(scenes.TERM_RAW, 0.0), (scenes.TERM_ABS, 0.0), (scenes.TERM_FIXED, 0.0),
+                                       (scenes.TERM_POS, 0.5)])
+def test_termination_conditions_and_warm_start_five_steps(cond, warm):
+    print(f"case: 20^3 dam break, termination {cond}, warm start {warm}, 5 steps", flush=True)
+    st = scenes.build_scene(scenes.dam_break_spec(20, H, tank=(0.4, 0.3, 0.3)))
+    p = scenes.make_params(H, termination_cond=cond, jacobi_warm_start=warm, max_solver_iters=40)
+    g, o = _pair(st, p)
+    for s in range(5):
+        g.step(1)
+        o.step()
+        assert g.info().solver_iters == o.info().solver_iters, s
+    sg, so = g.push(), o.push()
+    assert np.array_equal(sg.fluid_id, so.fluid_id)
+    for f in ("x", "y", "z", "vx", "vy", "vz", "density", "pressure"):
+        assert rel_err(sg.fluid[f], so.fluid[f]) <= 1e-5, f
+
+
+def test_rigid_free_fall_closed_form_on_gpu():
+    print("case: rigid body free fall, 40 steps", flush=True)
+    p = scenes.make_params(H)
+    g = HipBackend(0)
+    g.set_params(p)
+    g.pull(_single_body((0.5, 0.5, 0.5), vel=(0.1, 0.0, -0.2)))
+    n = 40
+    g.step(n)
+    b = g.push().body
+    dt = float(np.float32(1e-3))
+    for ax, (x0, v0, gr) in enumerate(((0.5, 0.1, 0.0), (0.5, 0.0, -9.81), (0.5, -0.2, 0.0))):
+        exp = free_fall_expected(x0, v0, gr, dt, n)
+        got = float(b[("xcm", "ycm", "zcm")[ax]][0])
+        assert abs(got - exp) <= 1e-6 + 1e-5 * abs(exp - x0), ax
+
+
+def test_spinning_body_orthonormal_on_gpu():
+    print("case: spinning rigid body, 200 steps", flush=True)
+    p = scenes.make_params(H, gravity=(0, 0, 0))
+    g = HipBackend(0)
+    g.set_params(p)
+    g.pull(_single_body((0.5, 0.5, 0.5), omega=(3.0, -2.0, 5.0)))
+    g.step(200)
+    b = g.push().body
+    A = np.array([[b[f"a{r}{c}"][0] for c in range(3)] for r in range(3)], np.float64)
+    assert np.abs(A @ A.T - np.eye(3)).max() <= 1e-6
+    assert (b["wx"][0], b["wy"][0], b["wz"][0]) == pytest.approx((3.0, -2.0, 5.0), rel=1e-5)
+
+
+def test_cpp_backend_matches_the_python_binding(tmp_path):
+    # the C++ SimBackend_HIP driver and the Python binding run the same scene through the same
+    # C ABI: three steps of a 12^3 dam break must agree bit for bit
+    print("case: C++ SimBackend_HIP round trip, 12^3, 3 steps", flush=True)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "turbospharticles_amd", "lib", "sim_backend_roundtrip")
+    out = tmp_path / "state.bin"
+    r = subprocess.run([exe, str(out), "12", "3"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    st = scenes.build_scene(scenes.dam_break_spec(12, H, tank=(0.4, 0.3, 0.3)))
+    p = scenes.make_params(H, termination_cond=scenes.TERM_POS)
+    g = HipBackend(0)
+    g.set_params(p)
+    g.pull(st)
+    g.step(3)
+    sg = g.push()
+    n = st.n[0]
+    raw = np.fromfile(out, np.uint8)
+    ids = raw[:8 * n].view(np.uint64)
+    fields = raw[8 * n:].view(np.float32).reshape(16, n)
+    assert np.array_equal(ids, sg.fluid_id)
+    from turbospharticles_amd import _capi
+    for k, f in enumerate(_capi.F_FIELDS):
+        assert np.array_equal(fields[k], sg.fluid[f]), f

@@ -2,6 +2,7 @@
 
   lib/libtsp_hip.so   hipcc --offload-arch=gfx950: csrc/*.hip (kernels + C ABI)
   lib/libtsp_host.so  g++: host/*.cpp (scene builder, SimBackend_HIP), linked to libtsp_hip.so
+  lib/sim_backend_roundtrip  g++: the C++ driver of SimBackend_HIP that the tests run
 
 Floating point: -ffp-contract=off everywhere (the reference's ISO C++ build contracts nothing;
 every fused multiply-add is an explicit fmaf), correctly rounded f32 division and square root,
@@ -62,9 +63,22 @@ def build_host(verbose=False):
     return out
 
 
+def build_driver(verbose=False):
+    out = os.path.join(LIB, "sim_backend_roundtrip")
+    src = os.path.join(ROOT, "tests", "cpp", "sim_backend_roundtrip.cpp")
+    cmd = ["g++", "-O2", "-std=c++20", "-Wall", "-I" + INC, "-o", out + ".tmp", src,
+           "-L" + LIB, "-ltsp_host", "-ltsp_hip", "-Wl,-rpath,$ORIGIN"]
+    r = _run(cmd)
+    os.replace(out + ".tmp", out)
+    if verbose and r.stderr:
+        print(r.stderr)
+    return out
+
+
 def build_all(verbose=False):
     build_hip(verbose)
     build_host(verbose)
+    build_driver(verbose)
 
 
 if __name__ == "__main__":

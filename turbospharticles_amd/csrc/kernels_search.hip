@@ -461,11 +461,11 @@ namespace tsp
         }
     }
 
-    constexpr uint32_t kStage = 48; // FF entries per thread staged in LDS by the fill
+    constexpr uint32_t kStage = 40; // FF entries per thread staged in LDS by the fill (u16 slices)
 
     __global__ void __launch_bounds__(kBlock) kQueryFill(Dev d, float r2, int reach)
     {
-        __shared__ uint32_t stage[kStage][kBlock];
+        __shared__ uint16_t stage[kStage][kBlock];
         const uint32_t blk = xcdBlock();
         const uint32_t t = threadIdx.x;
         const uint32_t i = blk * kBlock + t;
@@ -477,6 +477,7 @@ namespace tsp
         const bool wide = (cF & kWideSpanBit) != 0u;
         SliceMeta mF{}, mB{}, mR{};
         if (i < d.nF) { mF = d.meta[REL_FF][slice]; mB = d.meta[REL_FB][slice]; mR = d.meta[REL_FR][slice]; }
+        const bool staged = !wide && mF.format == kFmtDelta16;
         if (active)
         {
             const float4 pi = d.pos[i];
@@ -495,7 +496,8 @@ namespace tsp
                             {
                                 if (within(pi.x, pi.y, pi.z, d.pos[j], r2))
                                 {
-                                    if (kF < kStage) { stage[kF][t] = j; } else { sellWrite(d.pool[REL_FF], mF, lane, kF, j); }
+                                    if (staged && kF < kStage) { stage[kF][t] = static_cast<uint16_t>(j - mF.base); }
+                                    else { sellWrite(d.pool[REL_FF], mF, lane, kF, j); }
                                     kF++;
                                 }
                             }
@@ -564,11 +566,12 @@ namespace tsp
         }
         // the staged FF entries leave as whole slice rows: column k of the 32 lanes per store
         __syncthreads();
-        const uint32_t own = (active && !wide) ? (cF & 0x7fffu) : 0u;
-        const uint32_t rows = min(sliceMax(own), kStage);
+        const uint32_t own = (active && staged) ? min(cF & 0x7fffu, kStage) : 0u;
+        const uint32_t rows = sliceMax(own);
+        uint16_t* row = d.pool[REL_FF] + mF.offset + lane;
         for (uint32_t k = 0; k < rows; k++)
         {
-            if (k < own) { sellWrite(d.pool[REL_FF], mF, lane, k, stage[k][t]); }
+            if (k < own) { row[k * kSlice] = stage[k][t]; }
         }
     }
 

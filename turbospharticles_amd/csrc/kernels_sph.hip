@@ -9,8 +9,9 @@
 //   kSourceDiag      SourceTermAndDiagonalElementAndPressureJacobiInit :1225-1504
 //   kPressureAcc     PressureAcceleration_ChunkedSELL               :1507-1715
 //   kJacobi          PressureRelaxedJacobiSolver_ChunkedSELL        :1781-2049, FP64 error partials
-//   kFinalize        the error sums, averages and TerminationConditionSatisfied (:205-261, :686-732)
-//                    decided on the device, so no host round trip is needed per sweep
+//   finalizeSweep    the error sums, averages and TerminationConditionSatisfied (:205-261, :686-732)
+//                    decided on the device by block 0 of the next sweep's kPressureAcc (kFinalize
+//                    after the last sweep of a group), so no host round trip is needed per sweep
 //   kIntegration     Integration                                    :2051-2089
 // Rigid coupling (CPU/backend/cpu_backend.cpp:2001-2303, sph_solver.cpp:1151-1223, :1717-1778):
 // particle kinematics, reaction forces over the RF lists, FP64 per-body force/torque sums,
@@ -37,6 +38,37 @@ namespace tsp
         return l;
     }
 
+    // visits the entries of one list in storage order; the index loads of four entries are issued
+    // together so that their gathers overlap, the sums still run entry by entry
+    template <typename Fn>
+    __device__ __forceinline__ void visit(const List& l, Fn&& fn)
+    {
+        uint32_t q = 0;
+        if (l.m.format == kFmtDelta16)
+        {
+            const uint16_t* p = l.pool + l.m.offset + l.lane;
+            const uint32_t base = l.m.base;
+            for (; q + 4 <= l.n; q += 4)
+            {
+                const uint32_t j0 = p[(q + 0) * kSlice], j1 = p[(q + 1) * kSlice];
+                const uint32_t j2 = p[(q + 2) * kSlice], j3 = p[(q + 3) * kSlice];
+                fn(base + j0); fn(base + j1); fn(base + j2); fn(base + j3);
+            }
+            for (; q < l.n; q++) { fn(base + p[q * kSlice]); }
+        }
+        else
+        {
+            const uint32_t* p = reinterpret_cast<const uint32_t*>(l.pool + l.m.offset) + l.lane;
+            for (; q + 4 <= l.n; q += 4)
+            {
+                const uint32_t j0 = p[(q + 0) * kSlice], j1 = p[(q + 1) * kSlice];
+                const uint32_t j2 = p[(q + 2) * kSlice], j3 = p[(q + 3) * kSlice];
+                fn(j0); fn(j1); fn(j2); fn(j3);
+            }
+            for (; q < l.n; q++) { fn(p[q * kSlice]); }
+        }
+    }
+
     __device__ __forceinline__ float dist2(float dx, float dy, float dz)
     {
         return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
@@ -50,25 +82,25 @@ namespace tsp
         const float xi = pi.x, yi = pi.y, zi = pi.z;
         const List ff = listOf(d, REL_FF, i);
         float wsum = 0.0f;
-        for (uint32_t q = 0; q < ff.n; q++)
+        visit(ff, [&](uint32_t j)
         {
-            const float4 pj = d.pos[ff[q]];
+            const float4 pj = d.pos[j];
             wsum += kwUnscaled(dist2(xi - pj.x, yi - pj.y, zi - pj.z), k);
-        }
+        });
         const float dF = wsum * k.weightScale;
         float dB = 0.0f, dR = 0.0f;
         const List fb = listOf(d, REL_FB, i);
-        for (uint32_t q = 0; q < fb.n; q++)
+        visit(fb, [&](uint32_t j)
         {
-            const float4 bj = d.bpos[fb[q]];
+            const float4 bj = d.bpos[j];
             dB = __builtin_fmaf(bj.w, kwNormal(xi - bj.x, yi - bj.y, zi - bj.z, k), dB);
-        }
+        });
         const List fr = listOf(d, REL_FR, i);
-        for (uint32_t q = 0; q < fr.n; q++)
+        visit(fr, [&](uint32_t j)
         {
-            const float4 rj = d.rpos[fr[q]];
+            const float4 rj = d.rpos[j];
             dR = __builtin_fmaf(rj.w, kwNormal(xi - rj.x, yi - rj.y, zi - rj.z, k), dR);
-        }
+        });
         d.vel[i].w = dF + dB + dR;
     }
 
@@ -80,9 +112,8 @@ namespace tsp
         const float xi = pi.x, yi = pi.y, zi = pi.z, vxi = vi.x, vyi = vi.y, vzi = vi.z;
         float aFx = 0.0f, aFy = 0.0f, aFz = 0.0f;
         const List ff = listOf(d, REL_FF, i);
-        for (uint32_t q = 0; q < ff.n; q++)
+        visit(ff, [&](uint32_t j)
         {
-            const uint32_t j = ff[q];
             const float4 pj = d.pos[j], vj = d.vel[j];
             const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
             const float dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
@@ -94,13 +125,12 @@ namespace tsp
             aFx = __builtin_fmaf(coeff, dx, aFx);
             aFy = __builtin_fmaf(coeff, dy, aFy);
             aFz = __builtin_fmaf(coeff, dz, aFz);
-        }
+        });
         aFx *= k.mass; aFy *= k.mass; aFz *= k.mass;
         float aBx = 0.0f, aBy = 0.0f, aBz = 0.0f;
         const List fb = listOf(d, REL_FB, i);
-        for (uint32_t q = 0; q < fb.n; q++)
+        visit(fb, [&](uint32_t j)
         {
-            const uint32_t j = fb[q];
             const float4 bj = d.bpos[j];
             const float dx = xi - bj.x, dy = yi - bj.y, dz = zi - bj.z;
             const float d2 = dist2(dx, dy, dz);
@@ -112,12 +142,11 @@ namespace tsp
             aBx = __builtin_fmaf(coeff, gx, aBx);
             aBy = __builtin_fmaf(coeff, gy, aBy);
             aBz = __builtin_fmaf(coeff, gz, aBz);
-        }
+        });
         float aRx = 0.0f, aRy = 0.0f, aRz = 0.0f;
         const List fr = listOf(d, REL_FR, i);
-        for (uint32_t q = 0; q < fr.n; q++)
+        visit(fr, [&](uint32_t j)
         {
-            const uint32_t j = fr[q];
             const float4 rj = d.rpos[j], uj = d.rvel[j];
             const float dx = xi - rj.x, dy = yi - rj.y, dz = zi - rj.z;
             const float d2 = dist2(dx, dy, dz);
@@ -129,7 +158,7 @@ namespace tsp
             aRx = __builtin_fmaf(coeff, gx, aRx);
             aRy = __builtin_fmaf(coeff, gy, aRy);
             aRz = __builtin_fmaf(coeff, gz, aRz);
-        }
+        });
         const float ax = aFx * k.visF + (aBx + aRx) * k.visB + k.gx;
         const float ay = aFy * k.visF + (aBy + aRy) * k.visB + k.gy;
         const float az = aFz * k.visF + (aBz + aRz) * k.visB + k.gz;
@@ -144,9 +173,8 @@ namespace tsp
         const float xi = pi.x, yi = pi.y, zi = pi.z, vxi = vi.x, vyi = vi.y, vzi = vi.z;
         float sF = 0.0f, cFx = 0.0f, cFy = 0.0f, cFz = 0.0f, g2 = 0.0f;
         const List ff = listOf(d, REL_FF, i);
-        for (uint32_t q = 0; q < ff.n; q++)
+        visit(ff, [&](uint32_t j)
         {
-            const uint32_t j = ff[q];
             const float4 pj = d.pos[j], vj = d.vp[j];
             const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
             const float dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
@@ -158,12 +186,12 @@ namespace tsp
             cFy = __builtin_fmaf(kg, dy, cFy);
             cFz = __builtin_fmaf(kg, dz, cFz);
             g2 = __builtin_fmaf(kg, kg * d2, g2);
-        }
+        });
         float sB = 0.0f, cBx = 0.0f, cBy = 0.0f, cBz = 0.0f;
         const List fb = listOf(d, REL_FB, i);
-        for (uint32_t q = 0; q < fb.n; q++)
+        visit(fb, [&](uint32_t j)
         {
-            const float4 bj = d.bpos[fb[q]];
+            const float4 bj = d.bpos[j];
             const float dx = xi - bj.x, dy = yi - bj.y, dz = zi - bj.z;
             const float f = kgFactor(dist2(dx, dy, dz), k);
             const float gx = f * dx, gy = f * dy, gz = f * dz;
@@ -174,12 +202,11 @@ namespace tsp
             cBx = __builtin_fmaf(psi, gx, cBx);
             cBy = __builtin_fmaf(psi, gy, cBy);
             cBz = __builtin_fmaf(psi, gz, cBz);
-        }
+        });
         float sR = 0.0f, cRx = 0.0f, cRy = 0.0f, cRz = 0.0f;
         const List fr = listOf(d, REL_FR, i);
-        for (uint32_t q = 0; q < fr.n; q++)
+        visit(fr, [&](uint32_t j)
         {
-            const uint32_t j = fr[q];
             const float4 rj = d.rpos[j], uj = d.rvp[j];
             const float dx = xi - rj.x, dy = yi - rj.y, dz = zi - rj.z;
             const float f = kgFactor(dist2(dx, dy, dz), k);
@@ -191,7 +218,7 @@ namespace tsp
             cRx = __builtin_fmaf(psi, gx, cRx);
             cRy = __builtin_fmaf(psi, gy, cRy);
             cRz = __builtin_fmaf(psi, gz, cRz);
-        }
+        });
         sF *= k.mass;
         const float src = k.rho0 - d.vel[i].w - k.dt * (sF + sB + sR);
         const float ctx = cFx * k.cFacF + (cBx + cRx) * k.cFacB;
@@ -239,52 +266,120 @@ namespace tsp
         s.sumRaw = s.sumPos = s.sumAbs = 0.0;
         s.converged = d.nF > 0 ? 0u : 1u;
         s.sweeps = 0;
+        s.pending = 0;
         s.iters++;
         s.compute = (s.iters >= k.minIters && (k.termCond != 0 || terminated(s, k))) ? 1u : 0u;
     }
 
-    __global__ void __launch_bounds__(kBlock) kPressureAcc(Dev d, Consts k, bool sweep)
+    // the bookkeeping after one Jacobi sweep (sph_solver.cpp:207-261): the block partials summed in
+    // a fixed order, the averages, TerminationConditionSatisfied and the error decision of the next
+    // sweep. Run by one whole block.
+    __device__ void finalizeSweep(const Dev& d, const Consts& k, uint32_t nBlocks)
     {
-        if (sweep && d.status->converged) { return; }
+        __shared__ double red[3][kBlock];
+        __shared__ uint32_t flags[2];
+        Status& s = *d.status;
+        // one read of the flags for the whole block, so every wave takes the same path
+        if (threadIdx.x == 0) { flags[0] = s.pending; flags[1] = s.compute; }
+        __syncthreads();
+        if (!flags[0]) { return; }
+        const bool compute = flags[1] != 0u;
+        if (compute)
+        {
+            for (int r = 0; r < 3; r++)
+            {
+                const double* src = d.errPartial + static_cast<size_t>(r) * nBlocks;
+                double a = 0.0;
+                uint32_t b = threadIdx.x;
+                for (; b + 3 * kBlock < nBlocks; b += 4 * kBlock)
+                {
+                    const double v0 = src[b], v1 = src[b + kBlock], v2 = src[b + 2 * kBlock], v3 = src[b + 3 * kBlock];
+                    a += v0; a += v1; a += v2; a += v3;
+                }
+                for (; b < nBlocks; b += kBlock) { a += src[b]; }
+                red[r][threadIdx.x] = a;
+            }
+            __syncthreads();
+            for (uint32_t o = kBlock / 2; o > 0; o >>= 1)
+            {
+                if (threadIdx.x < o) { for (int r = 0; r < 3; r++) { red[r][threadIdx.x] += red[r][threadIdx.x + o]; } }
+                __syncthreads();
+            }
+        }
+        if (threadIdx.x == 0)
+        {
+            s.pending = 0;
+            s.sweeps++;
+            if (compute)
+            {
+                s.sumRaw = red[0][0]; s.sumPos = red[1][0]; s.sumAbs = red[2][0];
+                const float n = static_cast<float>(d.nF);
+                s.avgRaw = static_cast<float>(red[0][0]) / n;
+                s.avgPos = static_cast<float>(red[1][0]) / n;
+                s.avgAbs = static_cast<float>(red[2][0]) / n;
+            }
+            if (terminated(s, k))
+            {
+                s.converged = 1u;
+            }
+            else
+            {
+                s.iters++;
+                s.compute = (s.iters >= k.minIters && (k.termCond != 0 || terminated(s, k))) ? 1u : 0u;
+            }
+        }
+        __syncthreads();
+    }
+
+    // a sweep's pressure accelerations; block 0 of a sweep launch first finishes the bookkeeping of
+    // the previous Jacobi sweep, which the next Jacobi launch reads. Blocks that see the solve
+    // converged skip their particles: the final launch (sweep = false) recomputes every a^p.
+    __global__ void __launch_bounds__(kBlock) kPressureAcc(Dev d, Consts k, bool sweep, uint32_t nBlocks)
+    {
+        if (sweep)
+        {
+            if (blockIdx.x == 0) { finalizeSweep(d, k, nBlocks); }
+            if (__hip_atomic_load(&d.status->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { return; }
+        }
         const uint32_t i = xcdBlock() * kBlock + threadIdx.x;
         if (i >= d.nF) { return; }
         const float4 p4 = d.pos[i];
         const float xi = p4.x, yi = p4.y, zi = p4.z, pi = p4.w;
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         const List ff = listOf(d, REL_FF, i);
-        for (uint32_t q = 0; q < ff.n; q++)
+        visit(ff, [&](uint32_t j)
         {
-            const float4 pj = d.pos[ff[q]];
+            const float4 pj = d.pos[j];
             const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
             const float kg = kgFactor(dist2(dx, dy, dz), k);
             const float coeff = (pi + pj.w) * kg;
             sx = __builtin_fmaf(coeff, dx, sx);
             sy = __builtin_fmaf(coeff, dy, sy);
             sz = __builtin_fmaf(coeff, dz, sz);
-        }
+        });
         float ax = sx * k.presOuterFF, ay = sy * k.presOuterFF, az = sz * k.presOuterFF;
         const List fb = listOf(d, REL_FB, i);
-        for (uint32_t q = 0; q < fb.n; q++)
+        visit(fb, [&](uint32_t j)
         {
-            const float4 bj = d.bpos[fb[q]];
+            const float4 bj = d.bpos[j];
             const float dx = xi - bj.x, dy = yi - bj.y, dz = zi - bj.z;
             const float f = kgFactor(dist2(dx, dy, dz), k);
             const float coeff = k.presOuterB * bj.w * pi;
             ax = __builtin_fmaf(coeff, f * dx, ax);
             ay = __builtin_fmaf(coeff, f * dy, ay);
             az = __builtin_fmaf(coeff, f * dz, az);
-        }
+        });
         const List fr = listOf(d, REL_FR, i);
-        for (uint32_t q = 0; q < fr.n; q++)
+        visit(fr, [&](uint32_t j)
         {
-            const float4 rj = d.rpos[fr[q]];
+            const float4 rj = d.rpos[j];
             const float dx = xi - rj.x, dy = yi - rj.y, dz = zi - rj.z;
             const float f = kgFactor(dist2(dx, dy, dz), k);
             const float coeff = k.presOuterB * rj.w * pi;
             ax = __builtin_fmaf(coeff, f * dx, ax);
             ay = __builtin_fmaf(coeff, f * dy, ay);
             az = __builtin_fmaf(coeff, f * dz, az);
-        }
+        });
         d.ap[i] = make_float4(0.0f - ax, 0.0f - ay, 0.0f - az, 0.0f);
     }
 
@@ -299,6 +394,7 @@ namespace tsp
         __shared__ double red[3][kBlock / 64];
         if (d.status->converged) { return; }
         const bool compute = d.status->compute != 0u;
+        if (blockIdx.x == 0 && threadIdx.x == 0) { d.status->pending = 1u; }
         const uint32_t blk = xcdBlock();
         const uint32_t i = blk * kBlock + threadIdx.x;
         double eRaw = 0.0, ePos = 0.0, eAbs = 0.0;
@@ -308,35 +404,34 @@ namespace tsp
             const float xi = pi.x, yi = pi.y, zi = pi.z, axi = ai.x, ayi = ai.y, azi = ai.z;
             float sf = 0.0f;
             const List ff = listOf(d, REL_FF, i);
-            for (uint32_t q = 0; q < ff.n; q++)
+            visit(ff, [&](uint32_t j)
             {
-                const uint32_t j = ff[q];
                 const float4 pj = d.pos[j], aj = d.ap[j];
                 const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
                 const float kg = kgFactor(dist2(dx, dy, dz), k);
                 const float sv = __builtin_fmaf(axi - aj.x, dx, __builtin_fmaf(ayi - aj.y, dy, (azi - aj.z) * dz));
                 sf = __builtin_fmaf(kg, sv, sf);
-            }
+            });
             sf *= k.mass;
             float sb = 0.0f;
             const List fb = listOf(d, REL_FB, i);
-            for (uint32_t q = 0; q < fb.n; q++)
+            visit(fb, [&](uint32_t j)
             {
-                const float4 bj = d.bpos[fb[q]];
+                const float4 bj = d.bpos[j];
                 const float dx = xi - bj.x, dy = yi - bj.y, dz = zi - bj.z;
                 const float f = kgFactor(dist2(dx, dy, dz), k);
                 const float sv = __builtin_fmaf(axi, f * dx, __builtin_fmaf(ayi, f * dy, azi * (f * dz)));
                 sb += bj.w * sv;
-            }
+            });
             const List fr = listOf(d, REL_FR, i);
-            for (uint32_t q = 0; q < fr.n; q++)
+            visit(fr, [&](uint32_t j)
             {
-                const float4 rj = d.rpos[fr[q]];
+                const float4 rj = d.rpos[j];
                 const float dx = xi - rj.x, dy = yi - rj.y, dz = zi - rj.z;
                 const float f = kgFactor(dist2(dx, dy, dz), k);
                 const float sv = __builtin_fmaf(axi, f * dx, __builtin_fmaf(ayi, f * dy, azi * (f * dz)));
                 sb += rj.w * sv;
-            }
+            });
             const float sys = k.dtsq * (sf + sb);
             const float src = d.source[i], diag = d.diag[i];
             const float resid = src - sys;
@@ -364,50 +459,15 @@ namespace tsp
             {
                 double s = 0.0;
                 for (uint32_t q = 0; q < kBlock / 64; q++) { s += red[r][q]; }
-                d.errPartial[3 * blk + r] = s;
+                d.errPartial[static_cast<size_t>(r) * gridDim.x + blk] = s;
             }
         }
     }
 
-    // sums the block partials in a fixed order, updates the averages and decides the next sweep
+    // the bookkeeping of the last sweep of a group, before the host reads the status
     __global__ void __launch_bounds__(kBlock) kFinalize(Dev d, Consts k, uint32_t nBlocks)
     {
-        __shared__ double red[3][kBlock];
-        Status& s = *d.status;
-        if (s.converged) { return; }
-        const bool compute = s.compute != 0u;
-        if (compute)
-        {
-            double a[3] = { 0.0, 0.0, 0.0 };
-            for (uint32_t b = threadIdx.x; b < nBlocks; b += kBlock)
-            {
-                for (int r = 0; r < 3; r++) { a[r] += d.errPartial[3 * b + r]; }
-            }
-            for (int r = 0; r < 3; r++) { red[r][threadIdx.x] = a[r]; }
-            __syncthreads();
-            for (uint32_t o = kBlock / 2; o > 0; o >>= 1)
-            {
-                if (threadIdx.x < o) { for (int r = 0; r < 3; r++) { red[r][threadIdx.x] += red[r][threadIdx.x + o]; } }
-                __syncthreads();
-            }
-        }
-        if (threadIdx.x != 0) { return; }
-        s.sweeps++;
-        if (compute)
-        {
-            s.sumRaw = red[0][0]; s.sumPos = red[1][0]; s.sumAbs = red[2][0];
-            const float n = static_cast<float>(d.nF);
-            s.avgRaw = static_cast<float>(red[0][0]) / n;
-            s.avgPos = static_cast<float>(red[1][0]) / n;
-            s.avgAbs = static_cast<float>(red[2][0]) / n;
-        }
-        if (terminated(s, k))
-        {
-            s.converged = 1u;
-            return;
-        }
-        s.iters++;
-        s.compute = (s.iters >= k.minIters && (k.termCond != 0 || terminated(s, k))) ? 1u : 0u;
+        finalizeSweep(d, k, nBlocks);
     }
 
     __global__ void __launch_bounds__(kBlock) kIntegration(Dev d, Consts k)
@@ -652,7 +712,7 @@ namespace tsp
     }
     void launchPressureAcc(hipStream_t s, const Dev& d, const Consts& k, bool sweep)
     {
-        if (d.nF) { kPressureAcc<<<blocksOf(d.nF), kBlock, 0, s>>>(d, k, sweep); }
+        if (d.nF) { kPressureAcc<<<blocksOf(d.nF), kBlock, 0, s>>>(d, k, sweep, blocksOf(d.nF)); }
     }
     void launchJacobi(hipStream_t s, const Dev& d, const Consts& k)
     {

@@ -536,10 +536,10 @@ struct tsp_ctx
                     Scope sc(prof, stream, "pressure_acc");
                     launchPressureAcc(stream, d, k, true);
                 }
-                {
-                    Scope sc(prof, stream, "jacobi");
-                    launchJacobi(stream, d, k);
-                }
+                Scope sc(prof, stream, "jacobi");
+                launchJacobi(stream, d, k);
+            }
+            {
                 Scope sc(prof, stream, "finalize");
                 launchFinalize(stream, d, k);
             }
@@ -877,7 +877,7 @@ int tsp_pull(tsp_ctx* ctx, const tsp_state_view* s)
         d.rbCnt = ctx->alloc<uint32_t>(d.nR, A);
         d.rrCnt = ctx->alloc<uint32_t>(d.nR, A);
         d.rfOff = ctx->alloc<uint32_t>(d.nR + 1ull, A);
-        d.errPartial = ctx->alloc<double>(3ull * ((d.nF + kBlock - 1) / kBlock + 1), A);
+        d.errPartial = ctx->alloc<double>(3ull * ((d.nF + kBlock - 1) / kBlock), A);
         d.status = ctx->alloc<Status>(1, A);
         d.frame = ctx->alloc<Frame>(1, A);
         {

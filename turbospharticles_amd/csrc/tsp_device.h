@@ -56,7 +56,7 @@ namespace tsp
         uint32_t converged;
         uint32_t sweeps;
         float avgRaw, avgPos, avgAbs;
-        uint32_t pad1;
+        uint32_t pending;        // a Jacobi sweep ran whose errors are not yet summed
         double sumRaw, sumPos, sumAbs;
         // rigid
         uint32_t rigidDegenerate;
@@ -140,7 +140,7 @@ namespace tsp
         uint32_t *rbCnt, *rrCnt;
         uint64_t rfCap;
         // reductions and scans
-        double *errPartial;      // 3 per block of the Jacobi kernel
+        double *errPartial;      // Jacobi error partials, [3][blocks of the fluid grid]
         uint32_t *scanTmp;       // block sums of the multi-block scans
         Status *status;
         Frame *frame;
