@@ -45,6 +45,16 @@ def algorithmic_bytes(info, n_f):
     }
 
 
+def measured_traffic(kernel, n_side):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary of this workload (profiles/<kernel>_1M_cube_traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_1M_cube_traffic.json")
+    if n_side != N_SIDE or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return int(json.load(f)["hbm_bytes_per_launch"])
+
+
 def cpu_baseline(state, params, steps, threads):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
@@ -131,7 +141,8 @@ def main():
         bytes_launch = ab[dominant] * n_f
         achieved = bytes_launch / avg_s / 1e9
         roof = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(dominant, args.n_side),
                 "bytes_per_launch": int(bytes_launch), "avg_launch_us": round(avg_s * 1e6, 2)}
     b_step = 846.0 + 214.0 * k_bar
     step_gbs = n_f * b_step * args.steps / wall / 1e9

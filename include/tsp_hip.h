@@ -1,8 +1,9 @@
 /*
  * tsp_hip.h -- C ABI of the MI355X (gfx950) IISPH step: libtsp_hip.so.
  *
- * This is the thin layer between the C++ solver backend (SimBackend_HIP, which implements
- * the reference's Sim::I_SimBackend) and the hand-written HIP kernels. Every entry point
+ * This is the thin layer between the C++ solver backend (SimBackend_HIP in
+ * turbospharticles_amd/host/sim_backend_hip.h, with the methods of the reference's
+ * Sim::I_SimBackend) and the hand-written HIP kernels. Every entry point
  * takes plain pointers and sizes; the context owns all device memory and one HIP stream,
  * and keeps the particle state resident on the device between calls.
  *
@@ -26,7 +27,8 @@ typedef enum tsp_status {
     TSP_ERR_INVALID = -1,   /* bad argument, null handle, size mismatch */
     TSP_ERR_HIP = -2,       /* a HIP runtime call failed (GPU::Check, GPU/gpu_utils.h:24-57) */
     TSP_ERR_STATE = -3,     /* call out of order (e.g. step before pull) */
-    TSP_ERR_CAPACITY = -4,  /* a count exceeds a fixed limit (u16 neighbor counts, 2^21 cells/axis) */
+    TSP_ERR_CAPACITY = -4,  /* a count exceeds a fixed limit (u16 neighbor counts, 1024 cells per
+                               frame axis for the 32-bit Morton ids) */
     TSP_ERR_NUMERIC = -5    /* degenerate rigid orientation (cpu_backend.cpp:46-57) */
 } tsp_status;
 
