@@ -19,6 +19,9 @@
 //             slice rows. FF lists whose indices span more than 0xffff keep the DictDelta16x2
 //             order of the reference's per-particle dict chunks (low group first, high group
 //             reversed, CPU/backend/neighbor_lists.h:171-197)
+//   halo      per 256-particle block: the distinct FF neighbors (LDS hash set, bitonic sort) and
+//             the block's FF entries rewritten as slots of that list, so the solver kernels
+//             stage the neighbor records in LDS once per block
 #include "tsp_device.h"
 
 namespace tsp
@@ -576,6 +579,116 @@ namespace tsp
     }
 
     // ----------------------------------------------------------------------------------------
+    // FF halo of each 256-particle block: the distinct neighbors of the block's particles,
+    // ascending, and the block's FF entries rewritten as slots of that list. One block per
+    // particle block; a halo larger than kHaloMax leaves the block's entries as particle indices.
+
+    constexpr uint32_t kHashSlots = 2048;   // > 2 (kHaloMax + kBlock): probes always end
+    constexpr uint32_t kSortKeys = 1024;    // power of two >= kHaloMax
+
+    __device__ __forceinline__ uint32_t haloHash(uint32_t j) { return (j * 2654435761u) >> (32 - 11); }
+
+    // slot of j in the hash table (j must be present)
+    __device__ __forceinline__ uint32_t haloFind(const uint32_t* table, uint32_t j)
+    {
+        uint32_t h = haloHash(j);
+        for (uint32_t probe = 0; probe < kHashSlots && table[h] != j; probe++) { h = (h + 1) & (kHashSlots - 1); }
+        return h;
+    }
+
+    __global__ void __launch_bounds__(kBlock) kHaloBuild(Dev d)
+    {
+        __shared__ uint32_t table[kHashSlots];
+        __shared__ uint16_t rank[kHashSlots];
+        __shared__ uint32_t keys[kSortKeys];
+        __shared__ uint32_t unique, filled;
+        const uint32_t blk = blockIdx.x;
+        const uint32_t t = threadIdx.x;
+        const uint32_t i = blk * kBlock + t;
+        for (uint32_t q = t; q < kHashSlots; q += kBlock) { table[q] = kEmpty32; }
+        if (t == 0) { unique = 0; filled = 0; }
+        __syncthreads();
+        uint32_t n = 0;
+        SliceMeta m{};
+        if (i < d.nF)
+        {
+            n = d.cnt[REL_FF][i] & 0x7fffu;
+            m = d.meta[REL_FF][i / kSlice];
+        }
+        const uint32_t lane = i & (kSlice - 1);
+        // insert: open addressing, a plain read before the compare-and-swap (most entries are
+        // already present); a thread stops once the halo is known to be too large, so at most
+        // kHaloMax + kBlock values enter the kHashSlots slots and a probe always ends
+        for (uint32_t k = 0; k < n; k++)
+        {
+            if (*static_cast<volatile uint32_t*>(&unique) > kHaloMax) { break; }
+            const uint32_t j = sellRead(d.pool[REL_FF], m, lane, k);
+            uint32_t h = haloHash(j);
+            for (uint32_t probe = 0; probe < kHashSlots; probe++)
+            {
+                uint32_t v = *static_cast<volatile uint32_t*>(&table[h]);
+                if (v == kEmpty32)
+                {
+                    v = atomicCAS(&table[h], kEmpty32, j);
+                    if (v == kEmpty32) { atomicAdd(&unique, 1u); break; }
+                }
+                if (v == j) { break; }
+                h = (h + 1) & (kHashSlots - 1);
+            }
+        }
+        __syncthreads();
+        static_assert(kHaloMax + kBlock < kHashSlots && kHaloMax <= kSortKeys, "halo table sizes");
+        const uint32_t count = unique;
+        if (count > kHaloMax)
+        {
+            if (t == 0) { d.haloCount[blk] = kNoHalo; }
+            return;
+        }
+        // compact, pad, sort ascending (bitonic over kSortKeys keys)
+        for (uint32_t q = t; q < kSortKeys; q += kBlock) { keys[q] = kEmpty32; }
+        __syncthreads();
+        for (uint32_t q = t; q < kHashSlots; q += kBlock)
+        {
+            const uint32_t v = table[q];
+            if (v != kEmpty32) { keys[atomicAdd(&filled, 1u)] = v; }
+        }
+        __syncthreads();
+        for (uint32_t size = 2; size <= kSortKeys; size <<= 1)
+        {
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
+            {
+                for (uint32_t q = t; q < kSortKeys; q += kBlock)
+                {
+                    const uint32_t r = q ^ stride;
+                    if (r > q)
+                    {
+                        const uint32_t a = keys[q], b = keys[r];
+                        const bool up = (q & size) == 0;
+                        if ((a > b) == up) { keys[q] = b; keys[r] = a; }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        uint32_t* out = d.haloIdx + static_cast<size_t>(blk) * kHaloMax;
+        for (uint32_t q = t; q < count; q += kBlock)
+        {
+            out[q] = keys[q];
+            rank[haloFind(table, keys[q])] = static_cast<uint16_t>(q);
+        }
+        if (t == 0) { d.haloCount[blk] = count; }
+        __syncthreads();
+        // entries become their rank in the sorted halo, in their storage format
+        for (uint32_t k = 0; k < n; k++)
+        {
+            const uint32_t j = sellRead(d.pool[REL_FF], m, lane, k);
+            const uint32_t slot = rank[haloFind(table, j)];
+            if (m.format == kFmtDelta16) { d.pool[REL_FF][m.offset + k * kSlice + lane] = static_cast<uint16_t>(slot); }
+            else { reinterpret_cast<uint32_t*>(d.pool[REL_FF] + m.offset)[k * kSlice + lane] = slot; }
+        }
+    }
+
+    // ----------------------------------------------------------------------------------------
     // rigid-source lists (RF CSR, RB/RR counts), one thread per rigid particle
 
     __global__ void __launch_bounds__(kBlock) kQueryRigid(Dev d, float r2, int reach, bool fill)
@@ -783,6 +896,11 @@ namespace tsp
     void launchQueryFill(hipStream_t s, const Dev& d, float r2, int reach)
     {
         if (d.nF) { kQueryFill<<<blocksFor(d.nF), kBlock, 0, s>>>(d, r2, reach); }
+    }
+
+    void launchHaloBuild(hipStream_t s, const Dev& d)
+    {
+        if (d.nF) { kHaloBuild<<<blocksFor(d.nF), kBlock, 0, s>>>(d); }
     }
 
     void launchQueryRigid(hipStream_t s, const Dev& d, float r2, int reach, bool fill, uint32_t* tmp)

@@ -38,35 +38,66 @@ namespace tsp
         return l;
     }
 
-    // visits the entries of one list in storage order; the index loads of four entries are issued
-    // together so that their gathers overlap, the sums still run entry by entry
-    template <typename Fn>
-    __device__ __forceinline__ void visit(const List& l, Fn&& fn)
+    // visits the entries of one list in storage order, as fn(j, e): j the neighbor (or its halo
+    // slot when kSlots: the stored value without the slice base) and e the entry's position in
+    // u16 units of the pool, which also indexes the per-entry gradient factors. The index loads of
+    // four entries are issued together so that their gathers overlap; the sums still run entry
+    // by entry.
+    template <bool kSlots, typename Fn>
+    __device__ __forceinline__ void visitEntries(const List& l, Fn&& fn)
     {
         uint32_t q = 0;
         if (l.m.format == kFmtDelta16)
         {
-            const uint16_t* p = l.pool + l.m.offset + l.lane;
-            const uint32_t base = l.m.base;
+            const uint32_t e0 = l.m.offset + l.lane;
+            const uint16_t* p = l.pool + e0;
+            const uint32_t base = kSlots ? 0u : l.m.base;
             for (; q + 4 <= l.n; q += 4)
             {
                 const uint32_t j0 = p[(q + 0) * kSlice], j1 = p[(q + 1) * kSlice];
                 const uint32_t j2 = p[(q + 2) * kSlice], j3 = p[(q + 3) * kSlice];
-                fn(base + j0); fn(base + j1); fn(base + j2); fn(base + j3);
+                fn(base + j0, e0 + (q + 0) * kSlice); fn(base + j1, e0 + (q + 1) * kSlice);
+                fn(base + j2, e0 + (q + 2) * kSlice); fn(base + j3, e0 + (q + 3) * kSlice);
             }
-            for (; q < l.n; q++) { fn(base + p[q * kSlice]); }
+            for (; q < l.n; q++) { fn(base + p[q * kSlice], e0 + q * kSlice); }
         }
         else
         {
+            const uint32_t e0 = l.m.offset + 2u * l.lane;
             const uint32_t* p = reinterpret_cast<const uint32_t*>(l.pool + l.m.offset) + l.lane;
             for (; q + 4 <= l.n; q += 4)
             {
                 const uint32_t j0 = p[(q + 0) * kSlice], j1 = p[(q + 1) * kSlice];
                 const uint32_t j2 = p[(q + 2) * kSlice], j3 = p[(q + 3) * kSlice];
-                fn(j0); fn(j1); fn(j2); fn(j3);
+                fn(j0, e0 + 2u * (q + 0) * kSlice); fn(j1, e0 + 2u * (q + 1) * kSlice);
+                fn(j2, e0 + 2u * (q + 2) * kSlice); fn(j3, e0 + 2u * (q + 3) * kSlice);
             }
-            for (; q < l.n; q++) { fn(p[q * kSlice]); }
+            for (; q < l.n; q++) { fn(p[q * kSlice], e0 + 2u * q * kSlice); }
         }
+    }
+
+    template <typename Fn>
+    __device__ __forceinline__ void visit(const List& l, Fn&& fn) { visitEntries<false>(l, fn); }
+
+    // FF neighbors of a particle: from the block's LDS halo when it has one, else from HBM
+    template <typename FromHalo, typename FromGlobal>
+    __device__ __forceinline__ void visitFF(const List& l, bool halo, FromHalo&& fh, FromGlobal&& fg)
+    {
+        if (halo) { visitEntries<true>(l, fh); }
+        else { visitEntries<false>(l, fg); }
+    }
+
+    // copies the FF halo records of a block into LDS (load(slot, particle index)); every thread
+    // of the block calls it. A count above kHaloMax (kNoHalo) means the block has no halo.
+    template <typename Load>
+    __device__ __forceinline__ void stageHalo(const Dev& d, uint32_t blk, uint32_t hc, Load&& load)
+    {
+        if (hc <= kHaloMax)
+        {
+            const uint32_t* idx = d.haloIdx + static_cast<size_t>(blk) * kHaloMax;
+            for (uint32_t q = threadIdx.x; q < hc; q += kBlock) { load(q, idx[q]); }
+        }
+        __syncthreads();
     }
 
     __device__ __forceinline__ float dist2(float dx, float dy, float dz)
@@ -76,27 +107,28 @@ namespace tsp
 
     __global__ void __launch_bounds__(kBlock) kDensity(Dev d, Consts k)
     {
-        const uint32_t i = xcdBlock() * kBlock + threadIdx.x;
+        __shared__ float4 hp[kHaloMax];
+        const uint32_t blk = xcdBlock();
+        const uint32_t i = blk * kBlock + threadIdx.x;
+        const uint32_t hc = d.haloCount[blk];
+        stageHalo(d, blk, hc, [&](uint32_t q, uint32_t j) { hp[q] = d.pos[j]; });
         if (i >= d.nF) { return; }
         const float4 pi = d.pos[i];
         const float xi = pi.x, yi = pi.y, zi = pi.z;
         const List ff = listOf(d, REL_FF, i);
         float wsum = 0.0f;
-        visit(ff, [&](uint32_t j)
-        {
-            const float4 pj = d.pos[j];
-            wsum += kwUnscaled(dist2(xi - pj.x, yi - pj.y, zi - pj.z), k);
-        });
+        const auto ffTerm = [&](const float4& pj) { wsum += kwUnscaled(dist2(xi - pj.x, yi - pj.y, zi - pj.z), k); };
+        visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(hp[q]); }, [&](uint32_t j, uint32_t) { ffTerm(d.pos[j]); });
         const float dF = wsum * k.weightScale;
         float dB = 0.0f, dR = 0.0f;
         const List fb = listOf(d, REL_FB, i);
-        visit(fb, [&](uint32_t j)
+        visit(fb, [&](uint32_t j, uint32_t)
         {
             const float4 bj = d.bpos[j];
             dB = __builtin_fmaf(bj.w, kwNormal(xi - bj.x, yi - bj.y, zi - bj.z, k), dB);
         });
         const List fr = listOf(d, REL_FR, i);
-        visit(fr, [&](uint32_t j)
+        visit(fr, [&](uint32_t j, uint32_t)
         {
             const float4 rj = d.rpos[j];
             dR = __builtin_fmaf(rj.w, kwNormal(xi - rj.x, yi - rj.y, zi - rj.z, k), dR);
@@ -106,15 +138,18 @@ namespace tsp
 
     __global__ void __launch_bounds__(kBlock) kNonPressure(Dev d, Consts k)
     {
-        const uint32_t i = xcdBlock() * kBlock + threadIdx.x;
+        __shared__ float4 hp[kHaloMax], hv[kHaloMax];
+        const uint32_t blk = xcdBlock();
+        const uint32_t i = blk * kBlock + threadIdx.x;
+        const uint32_t hc = d.haloCount[blk];
+        stageHalo(d, blk, hc, [&](uint32_t q, uint32_t j) { hp[q] = d.pos[j]; hv[q] = d.vel[j]; });
         if (i >= d.nF) { return; }
         const float4 pi = d.pos[i], vi = d.vel[i];
         const float xi = pi.x, yi = pi.y, zi = pi.z, vxi = vi.x, vyi = vi.y, vzi = vi.z;
         float aFx = 0.0f, aFy = 0.0f, aFz = 0.0f;
         const List ff = listOf(d, REL_FF, i);
-        visit(ff, [&](uint32_t j)
+        const auto ffTerm = [&](const float4& pj, const float4& vj)
         {
-            const float4 pj = d.pos[j], vj = d.vel[j];
             const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
             const float dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
             const float d2 = dist2(dx, dy, dz);
@@ -125,11 +160,13 @@ namespace tsp
             aFx = __builtin_fmaf(coeff, dx, aFx);
             aFy = __builtin_fmaf(coeff, dy, aFy);
             aFz = __builtin_fmaf(coeff, dz, aFz);
-        });
+        };
+        visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(hp[q], hv[q]); },
+                [&](uint32_t j, uint32_t) { ffTerm(d.pos[j], d.vel[j]); });
         aFx *= k.mass; aFy *= k.mass; aFz *= k.mass;
         float aBx = 0.0f, aBy = 0.0f, aBz = 0.0f;
         const List fb = listOf(d, REL_FB, i);
-        visit(fb, [&](uint32_t j)
+        visit(fb, [&](uint32_t j, uint32_t e)
         {
             const float4 bj = d.bpos[j];
             const float dx = xi - bj.x, dy = yi - bj.y, dz = zi - bj.z;
@@ -145,7 +182,7 @@ namespace tsp
         });
         float aRx = 0.0f, aRy = 0.0f, aRz = 0.0f;
         const List fr = listOf(d, REL_FR, i);
-        visit(fr, [&](uint32_t j)
+        visit(fr, [&](uint32_t j, uint32_t e)
         {
             const float4 rj = d.rpos[j], uj = d.rvel[j];
             const float dx = xi - rj.x, dy = yi - rj.y, dz = zi - rj.z;
@@ -167,15 +204,18 @@ namespace tsp
 
     __global__ void __launch_bounds__(kBlock) kSourceDiag(Dev d, Consts k)
     {
-        const uint32_t i = xcdBlock() * kBlock + threadIdx.x;
+        __shared__ float4 hp[kHaloMax], hv[kHaloMax];
+        const uint32_t blk = xcdBlock();
+        const uint32_t i = blk * kBlock + threadIdx.x;
+        const uint32_t hc = d.haloCount[blk];
+        stageHalo(d, blk, hc, [&](uint32_t q, uint32_t j) { hp[q] = d.pos[j]; hv[q] = d.vp[j]; });
         if (i >= d.nF) { return; }
         const float4 pi = d.pos[i], vi = d.vp[i];
         const float xi = pi.x, yi = pi.y, zi = pi.z, vxi = vi.x, vyi = vi.y, vzi = vi.z;
         float sF = 0.0f, cFx = 0.0f, cFy = 0.0f, cFz = 0.0f, g2 = 0.0f;
         const List ff = listOf(d, REL_FF, i);
-        visit(ff, [&](uint32_t j)
+        const auto ffTerm = [&](const float4& pj, const float4& vj)
         {
-            const float4 pj = d.pos[j], vj = d.vp[j];
             const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
             const float dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
             const float d2 = dist2(dx, dy, dz);
@@ -186,10 +226,12 @@ namespace tsp
             cFy = __builtin_fmaf(kg, dy, cFy);
             cFz = __builtin_fmaf(kg, dz, cFz);
             g2 = __builtin_fmaf(kg, kg * d2, g2);
-        });
+        };
+        visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(hp[q], hv[q]); },
+                [&](uint32_t j, uint32_t) { ffTerm(d.pos[j], d.vp[j]); });
         float sB = 0.0f, cBx = 0.0f, cBy = 0.0f, cBz = 0.0f;
         const List fb = listOf(d, REL_FB, i);
-        visit(fb, [&](uint32_t j)
+        visit(fb, [&](uint32_t j, uint32_t)
         {
             const float4 bj = d.bpos[j];
             const float dx = xi - bj.x, dy = yi - bj.y, dz = zi - bj.z;
@@ -205,7 +247,7 @@ namespace tsp
         });
         float sR = 0.0f, cRx = 0.0f, cRy = 0.0f, cRz = 0.0f;
         const List fr = listOf(d, REL_FR, i);
-        visit(fr, [&](uint32_t j)
+        visit(fr, [&](uint32_t j, uint32_t)
         {
             const float4 rj = d.rpos[j], uj = d.rvp[j];
             const float dx = xi - rj.x, dy = yi - rj.y, dz = zi - rj.z;
@@ -336,30 +378,38 @@ namespace tsp
     // converged skip their particles: the final launch (sweep = false) recomputes every a^p.
     __global__ void __launch_bounds__(kBlock) kPressureAcc(Dev d, Consts k, bool sweep, uint32_t nBlocks)
     {
+        __shared__ float4 hp[kHaloMax];
+        __shared__ uint32_t skip;
         if (sweep)
         {
             if (blockIdx.x == 0) { finalizeSweep(d, k, nBlocks); }
-            if (__hip_atomic_load(&d.status->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { return; }
+            // one read per block, so the whole block leaves together
+            if (threadIdx.x == 0) { skip = __hip_atomic_load(&d.status->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+            __syncthreads();
+            if (skip) { return; }
         }
-        const uint32_t i = xcdBlock() * kBlock + threadIdx.x;
+        const uint32_t blk = xcdBlock();
+        const uint32_t i = blk * kBlock + threadIdx.x;
+        const uint32_t hc = d.haloCount[blk];
+        stageHalo(d, blk, hc, [&](uint32_t q, uint32_t j) { hp[q] = d.pos[j]; });
         if (i >= d.nF) { return; }
         const float4 p4 = d.pos[i];
         const float xi = p4.x, yi = p4.y, zi = p4.z, pi = p4.w;
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         const List ff = listOf(d, REL_FF, i);
-        visit(ff, [&](uint32_t j)
+        const auto ffTerm = [&](const float4& pj)
         {
-            const float4 pj = d.pos[j];
             const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
             const float kg = kgFactor(dist2(dx, dy, dz), k);
             const float coeff = (pi + pj.w) * kg;
             sx = __builtin_fmaf(coeff, dx, sx);
             sy = __builtin_fmaf(coeff, dy, sy);
             sz = __builtin_fmaf(coeff, dz, sz);
-        });
+        };
+        visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(hp[q]); }, [&](uint32_t j, uint32_t) { ffTerm(d.pos[j]); });
         float ax = sx * k.presOuterFF, ay = sy * k.presOuterFF, az = sz * k.presOuterFF;
         const List fb = listOf(d, REL_FB, i);
-        visit(fb, [&](uint32_t j)
+        visit(fb, [&](uint32_t j, uint32_t)
         {
             const float4 bj = d.bpos[j];
             const float dx = xi - bj.x, dy = yi - bj.y, dz = zi - bj.z;
@@ -370,7 +420,7 @@ namespace tsp
             az = __builtin_fmaf(coeff, f * dz, az);
         });
         const List fr = listOf(d, REL_FR, i);
-        visit(fr, [&](uint32_t j)
+        visit(fr, [&](uint32_t j, uint32_t)
         {
             const float4 rj = d.rpos[j];
             const float dx = xi - rj.x, dy = yi - rj.y, dz = zi - rj.z;
@@ -392,11 +442,16 @@ namespace tsp
     __global__ void __launch_bounds__(kBlock) kJacobi(Dev d, Consts k)
     {
         __shared__ double red[3][kBlock / 64];
+        __shared__ float4 hp[kHaloMax], ha[kHaloMax];
+        // the flags were written by an earlier launch: every thread reads the same values
         if (d.status->converged) { return; }
         const bool compute = d.status->compute != 0u;
         if (blockIdx.x == 0 && threadIdx.x == 0) { d.status->pending = 1u; }
         const uint32_t blk = xcdBlock();
         const uint32_t i = blk * kBlock + threadIdx.x;
+        const uint32_t hc = d.haloCount[blk];
+        // (positions only: the pressures this launch writes into pos.w are not read here)
+        stageHalo(d, blk, hc, [&](uint32_t q, uint32_t j) { hp[q] = d.pos[j]; ha[q] = d.ap[j]; });
         double eRaw = 0.0, ePos = 0.0, eAbs = 0.0;
         if (i < d.nF)
         {
@@ -404,18 +459,19 @@ namespace tsp
             const float xi = pi.x, yi = pi.y, zi = pi.z, axi = ai.x, ayi = ai.y, azi = ai.z;
             float sf = 0.0f;
             const List ff = listOf(d, REL_FF, i);
-            visit(ff, [&](uint32_t j)
+            const auto ffTerm = [&](const float4& pj, const float4& aj)
             {
-                const float4 pj = d.pos[j], aj = d.ap[j];
                 const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
                 const float kg = kgFactor(dist2(dx, dy, dz), k);
                 const float sv = __builtin_fmaf(axi - aj.x, dx, __builtin_fmaf(ayi - aj.y, dy, (azi - aj.z) * dz));
                 sf = __builtin_fmaf(kg, sv, sf);
-            });
+            };
+            visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(hp[q], ha[q]); },
+                    [&](uint32_t j, uint32_t) { ffTerm(d.pos[j], d.ap[j]); });
             sf *= k.mass;
             float sb = 0.0f;
             const List fb = listOf(d, REL_FB, i);
-            visit(fb, [&](uint32_t j)
+            visit(fb, [&](uint32_t j, uint32_t)
             {
                 const float4 bj = d.bpos[j];
                 const float dx = xi - bj.x, dy = yi - bj.y, dz = zi - bj.z;
@@ -424,7 +480,7 @@ namespace tsp
                 sb += bj.w * sv;
             });
             const List fr = listOf(d, REL_FR, i);
-            visit(fr, [&](uint32_t j)
+            visit(fr, [&](uint32_t j, uint32_t)
             {
                 const float4 rj = d.rpos[j];
                 const float dx = xi - rj.x, dy = yi - rj.y, dz = zi - rj.z;

@@ -34,6 +34,7 @@ namespace tsp
     void launchScanSlices(hipStream_t, const Dev&, uint32_t*, uint32_t*);
     void launchQueryFill(hipStream_t, const Dev&, float, int);
     void launchQueryRigid(hipStream_t, const Dev&, float, int, bool, uint32_t*);
+    void launchHaloBuild(hipStream_t, const Dev&);
     void launchAabb(hipStream_t, const float4*, uint32_t, uint32_t*);
     void launchDensity(hipStream_t, const Dev&, const Consts&);
     void launchNonPressure(hipStream_t, const Dev&, const Consts&);
@@ -487,6 +488,10 @@ struct tsp_ctx
                 launchQueryFill(stream, d, params.kernel_radius_sq, static_cast<int>(reach));
                 launchQueryRigid(stream, d, params.kernel_radius_sq, static_cast<int>(reach), true, d.scanTmp);
             }
+            {
+                Scope sc(prof, stream, "halo");
+                launchHaloBuild(stream, d);
+            }
             if (last.rbTotal + last.rrTotal > 0) { flags |= TSP_FLAG_RIGID_CONTACT; }
             (void)k;
             return;
@@ -873,6 +878,13 @@ int tsp_pull(tsp_ctx* ctx, const tsp_state_view* s)
             d.scanTmp = ctx->alloc<uint32_t>((m + 2047ull) / 2048ull + 1ull, A);
         }
         d.ffWideMin = ctx->alloc<uint32_t>(d.nF, A);
+        {
+            const uint64_t nBlocks = (d.nF + kBlock - 1) / kBlock;
+            d.haloIdx = ctx->alloc<uint32_t>(nBlocks * kHaloMax, A);
+            d.haloCount = ctx->alloc<uint32_t>(nBlocks, A);
+            // no block has a halo until the first search builds one
+            hipCheck(hipMemsetAsync(d.haloCount, 0xff, sizeof(uint32_t) * std::max<uint64_t>(nBlocks, 1), st), "memset");
+        }
         d.rfCnt = ctx->alloc<uint32_t>(d.nR, A);
         d.rbCnt = ctx->alloc<uint32_t>(d.nR, A);
         d.rrCnt = ctx->alloc<uint32_t>(d.nR, A);
@@ -1105,17 +1117,30 @@ int tsp_get_neighbors(tsp_ctx* ctx, int rel, uint32_t* counts, uint32_t* idx, ui
         const uint64_t units = ctx->last.poolUnits[rel];
         std::vector<uint16_t> pool(units);
         if (units) { hipCheck(hipMemcpy(pool.data(), d.pool[rel], sizeof(uint16_t) * units, hipMemcpyDeviceToHost), "D2H"); }
+        // FF entries of blocks with a halo are halo slots
+        const uint32_t nBlocks = (n + kBlock - 1) / kBlock;
+        std::vector<uint32_t> haloCount, haloIdx;
+        if (rel == REL_FF && n)
+        {
+            haloCount.resize(nBlocks);
+            haloIdx.resize(static_cast<size_t>(nBlocks) * kHaloMax);
+            hipCheck(hipMemcpy(haloCount.data(), d.haloCount, sizeof(uint32_t) * nBlocks, hipMemcpyDeviceToHost), "D2H");
+            hipCheck(hipMemcpy(haloIdx.data(), d.haloIdx, sizeof(uint32_t) * haloIdx.size(), hipMemcpyDeviceToHost), "D2H");
+        }
         // host decoder of the SELL-32 storage (ReadCompressedSell32Neighbor, gpu_backend.cpp:42-65)
         uint64_t w = 0;
         for (uint32_t i = 0; i < n; i++)
         {
             const SliceMeta& m = meta[i / kSlice];
             const uint32_t lane = i % kSlice;
+            const uint32_t blk = i / kBlock;
+            const bool slots = rel == REL_FF && haloCount[blk] <= kHaloMax;
             for (uint32_t k = 0; k < (cnt[i] & mask); k++)
             {
                 uint32_t j;
-                if (m.format == kFmtDelta16) { j = m.base + pool[m.offset + k * kSlice + lane]; }
+                if (m.format == kFmtDelta16) { j = (slots ? 0u : m.base) + pool[m.offset + k * kSlice + lane]; }
                 else { std::memcpy(&j, &pool[m.offset + 2ull * (k * kSlice + lane)], 4); }
+                if (slots) { j = haloIdx[static_cast<size_t>(blk) * kHaloMax + j]; }
                 idx[w++] = j;
             }
         }

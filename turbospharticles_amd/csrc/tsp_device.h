@@ -13,7 +13,9 @@
 //   cell grid  dense over the cell frame, one uint2 {begin, end} per cell and particle class
 //   SELL-32    per slice of 32 consecutive fluid particles and relation (FF, FB, FR): one
 //              SliceMeta {offset in u16 units, base, pitch, format}; entry k of lane l at
-//              offset + k*32 + l, as u16 (j - base) or u32 j
+//              offset + k*32 + l, as u16 (j - base) or u32 j. FF entries of a block with a halo
+//              hold the slot of j in the block's halo instead (same u16/u32 storage, no base)
+//   halo       per 256-particle block: up to 1000 distinct FF neighbor indices, ascending
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,6 +29,10 @@ namespace tsp
     constexpr uint32_t kFmtDelta16 = 0;
     constexpr uint32_t kFmtAbs32 = 1;
     constexpr uint32_t kWideSpanBit = 0x8000u; // FF count flag: neighbors span more than 0xffff
+    // FF neighbor halo of a 256-particle block: the distinct FF neighbors of its particles,
+    // ascending; the FF entries of the block then hold halo slots instead of particle indices
+    constexpr uint32_t kHaloMax = 1000;         // 2 x 16 B x 1000 records + reductions: 5 blocks per CU
+    constexpr uint32_t kNoHalo = 0xffffffffu;   // halo count of a block that keeps particle indices
 
     enum Rel { REL_FF = 0, REL_FB = 1, REL_FR = 2 };
 
@@ -135,6 +141,8 @@ namespace tsp
         uint16_t *pool[3];
         uint64_t poolCap[3];     // u16 units
         uint32_t *ffWideMin;     // per particle: min FF index when the wide-span bit is set
+        uint32_t *haloIdx;       // per block: kHaloMax particle indices, ascending
+        uint32_t *haloCount;     // per block: halo size, or kNoHalo
         // rigid-source CSR (RF), counts of RB and RR
         uint32_t *rfCnt, *rfOff, *rfIdx;
         uint32_t *rbCnt, *rrCnt;
