@@ -170,6 +170,41 @@ int tsp_profile_read(tsp_ctx *ctx, char *names, uint64_t *launches, double *ms, 
 /* waits for all queued work of the context */
 int tsp_synchronize(tsp_ctx *ctx);
 
+/*
+ * Several independent scenes in one context. The reference runs one scene per backend
+ * (core/simulation.h:179, one SimBackend per Simulation); these calls let one context hold many
+ * scenes and advance all of them with one launch per kernel. Each scene keeps its own parameters,
+ * domain, cell frame, solver state and counters; its results are bit-identical to the same scene
+ * run alone in a context of its own. The single-scene calls above act on the first scene (tsp_pull
+ * replaces every scene with one; tsp_set_params and tsp_set_domain apply to every scene).
+ * Scene ids are assigned by the context and stay valid until the scene is removed.
+ */
+/* one relayout: removes n_remove scenes, then appends n_add new ones (params[q], *views[q],
+   domains[6q..6q+5] = min xyz, max xyz, or domains null for no domain); out_ids receives the
+   new ids. Resident scenes keep their device state. */
+int tsp_batch_update(tsp_ctx *ctx, uint32_t n_remove, const uint32_t *remove_ids, uint32_t n_add,
+                     const tsp_params *params, const tsp_state_view *const *views,
+                     const float *domains, uint32_t *out_ids);
+/* tsp_batch_update with one scene to add (domain: 6 floats, or null) */
+int tsp_batch_add_scene(tsp_ctx *ctx, const tsp_params *params, const tsp_state_view *state,
+                        const float domain[6], uint32_t *scene_id);
+/* tsp_batch_update with one scene to remove */
+int tsp_batch_remove_scene(tsp_ctx *ctx, uint32_t scene_id);
+/* ids of the resident scenes in arena order (up to cap), *n = their number */
+int tsp_batch_scene_ids(tsp_ctx *ctx, uint32_t *ids, uint32_t cap, uint32_t *n);
+/* tsp_get_counts of one scene */
+int tsp_batch_scene_counts(tsp_ctx *ctx, uint32_t scene_id, uint32_t counts[4]);
+/* tsp_push of one scene */
+int tsp_batch_push_scene(tsp_ctx *ctx, uint32_t scene_id, tsp_state_view *dst);
+/* tsp_get_step_info of one scene (the neighbor-list totals count every scene) */
+int tsp_batch_scene_info(tsp_ctx *ctx, uint32_t scene_id, tsp_step_info *info);
+/* advances every scene n_steps IISPH steps (ExecuteFullSimStep of each scene's backend) */
+int tsp_batch_step(tsp_ctx *ctx, uint32_t n_steps);
+/* device bytes a scene of these counts holds while resident (an upper estimate at the initial
+   list pool sizes), for admission against an HBM budget */
+uint64_t tsp_batch_footprint(uint32_t n_fluid, uint32_t n_boundary, uint32_t n_rigid,
+                             uint32_t n_bodies);
+
 #ifdef __cplusplus
 }
 #endif

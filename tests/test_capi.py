@@ -110,3 +110,20 @@ def test_cpp_backend_reports_a_missing_device(tmp_path):
     r = subprocess.run([exe, str(tmp_path / "out.bin"), "4", "1"], capture_output=True, text=True)
     assert r.returncode == 3
     assert "no usable gfx950 device" in r.stderr
+
+
+def test_batch_footprint_grows_with_every_population():
+    lib = _capi.hip_lib()
+    base = lib.tsp_batch_footprint(1000, 1000, 100, 1)
+    assert base > 0
+    for n in ((2000, 1000, 100, 1), (1000, 2000, 100, 1), (1000, 1000, 200, 1), (1000, 1000, 100, 2)):
+        assert lib.tsp_batch_footprint(*n) > base, n
+    # the fluid segment is rounded to the 256-particle block
+    assert lib.tsp_batch_footprint(1, 0, 0, 0) == lib.tsp_batch_footprint(256, 0, 0, 0)
+
+
+def test_batch_calls_reject_a_null_context():
+    lib = _capi.hip_lib()
+    assert lib.tsp_batch_step(None, 1) == -1
+    assert lib.tsp_batch_remove_scene(None, 1) == -1
+    assert lib.tsp_batch_update(None, 0, None, 0, None, None, None, None) == -1

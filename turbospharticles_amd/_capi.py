@@ -132,6 +132,17 @@ def _declare_hip(lib):
         "tsp_profile_enable": (i, [vp, i]),
         "tsp_profile_read": (i, [vp, ctypes.c_char_p, _u64p, ctypes.POINTER(ctypes.c_double), i]),
         "tsp_synchronize": (i, [vp]),
+        "tsp_batch_update": (i, [vp, ctypes.c_uint32, _u32p, ctypes.c_uint32, ctypes.POINTER(TspParams),
+                                 ctypes.POINTER(ctypes.POINTER(TspStateView)), _fp, _u32p]),
+        "tsp_batch_add_scene": (i, [vp, ctypes.POINTER(TspParams), ctypes.POINTER(TspStateView), _fp, _u32p]),
+        "tsp_batch_remove_scene": (i, [vp, ctypes.c_uint32]),
+        "tsp_batch_scene_ids": (i, [vp, _u32p, ctypes.c_uint32, _u32p]),
+        "tsp_batch_scene_counts": (i, [vp, ctypes.c_uint32, _u32p]),
+        "tsp_batch_push_scene": (i, [vp, ctypes.c_uint32, ctypes.POINTER(TspStateView)]),
+        "tsp_batch_scene_info": (i, [vp, ctypes.c_uint32, ctypes.POINTER(TspStepInfo)]),
+        "tsp_batch_step": (i, [vp, ctypes.c_uint32]),
+        "tsp_batch_footprint": (ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                  ctypes.c_uint32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -135,3 +135,108 @@ class HipBackend:
 
     def synchronize(self):
         self._check(self.lib.tsp_synchronize(self.h), "tsp_synchronize")
+
+
+class HipBatch:
+    """Many independent scenes in one context, advanced together (tsp_batch_* of libtsp_hip.so).
+
+    Each scene keeps its own parameters, domain, frame and solver state; a scene's results are
+    bit-identical to the same scene run alone in a HipBackend. Scenes are named by the ids the
+    context assigns when they are added.
+    """
+
+    def __init__(self, device=0):
+        self.lib = _capi.hip_lib()
+        h = ctypes.c_void_p()
+        rc = self.lib.tsp_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"tsp_create failed ({rc}): no HIP device {device}")
+        self.h = h
+        self.counts = {}
+
+    def close(self):
+        if self.h:
+            self.lib.tsp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.tsp_last_error(self.h).decode()
+            raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+    def update(self, add=(), remove=()):
+        """Removes the scenes `remove` (ids), then appends `add`, a sequence of (params, state)
+        pairs (state.domain, when set, is the scene's domain); returns the new ids."""
+        add = list(add)
+        rm = np.asarray(list(remove), np.uint32)
+        n = len(add)
+        params = (_capi.TspParams * max(n, 1))()
+        views = [st.view() for _, st in add]
+        vptrs = (ctypes.POINTER(_capi.TspStateView) * max(n, 1))()
+        dom = np.zeros(6 * max(n, 1), np.float32)
+        any_dom = False
+        for q, (p, st) in enumerate(add):
+            params[q] = p
+            vptrs[q] = ctypes.pointer(views[q])
+            if st.domain is not None:
+                any_dom = True
+                dom[6 * q:6 * q + 3] = st.domain[0]
+                dom[6 * q + 3:6 * q + 6] = st.domain[1]
+        if any_dom and not all(st.domain is not None for _, st in add):
+            raise ValueError("either every added scene has a domain or none has")
+        ids = np.zeros(max(n, 1), np.uint32)
+        self._check(self.lib.tsp_batch_update(self.h, len(rm), _capi.u32ptr(rm) if len(rm) else None, n, params,
+                                              vptrs, _capi.fptr(dom) if any_dom else None, _capi.u32ptr(ids)),
+                    "tsp_batch_update")
+        for r in rm:
+            self.counts.pop(int(r), None)
+        out = [int(v) for v in ids[:n]]
+        for sid, (_, st) in zip(out, add):
+            self.counts[sid] = st.n
+        return out
+
+    def add(self, params, state):
+        return self.update(add=[(params, state)])[0]
+
+    def remove(self, scene_id):
+        self.update(remove=[scene_id])
+
+    def ids(self):
+        n = ctypes.c_uint32()
+        self._check(self.lib.tsp_batch_scene_ids(self.h, None, 0, ctypes.byref(n)), "tsp_batch_scene_ids")
+        buf = np.zeros(max(n.value, 1), np.uint32)
+        self._check(self.lib.tsp_batch_scene_ids(self.h, _capi.u32ptr(buf), n.value, ctypes.byref(n)),
+                    "tsp_batch_scene_ids")
+        return [int(v) for v in buf[:n.value]]
+
+    def step(self, n_steps=1):
+        self._check(self.lib.tsp_batch_step(self.h, n_steps), "tsp_batch_step")
+
+    def push(self, scene_id):
+        st = _capi.HostState(*self.counts[scene_id])
+        v = st.view()
+        self._check(self.lib.tsp_batch_push_scene(self.h, scene_id, ctypes.byref(v)), "tsp_batch_push_scene")
+        st.step, st.time = v.step, v.time
+        return st
+
+    def info(self, scene_id):
+        i = _capi.TspStepInfo()
+        self._check(self.lib.tsp_batch_scene_info(self.h, scene_id, ctypes.byref(i)), "tsp_batch_scene_info")
+        return i
+
+    def synchronize(self):
+        self._check(self.lib.tsp_synchronize(self.h), "tsp_synchronize")
+
+    def profile_enable(self, on=True):
+        self._check(self.lib.tsp_profile_enable(self.h, 1 if on else 0), "tsp_profile_enable")
+
+    @staticmethod
+    def footprint(n):
+        """device bytes a scene with counts n = (fluid, boundary, rigid, bodies) holds while resident"""
+        return int(_capi.hip_lib().tsp_batch_footprint(*[int(v) for v in n]))

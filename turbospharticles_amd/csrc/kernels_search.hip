@@ -99,12 +99,13 @@ namespace tsp
         kScanApply<<<nb, kBlock, 0, s>>>(v, m, tmp);
     }
 
-    // clears the per-search words of the status block (everything before the solver state)
-    __global__ void kResetSearch(Status* st)
+    // clears the search status block and the scenes' frame-abort flags
+    __global__ void __launch_bounds__(kBlock) kResetSearch(Dev d)
     {
-        constexpr uint32_t words = static_cast<uint32_t>(offsetof(Status, iters) / sizeof(uint32_t));
-        static_assert(words <= 64, "the reset covers one wave");
-        if (threadIdx.x < words) { reinterpret_cast<uint32_t*>(st)[threadIdx.x] = 0u; }
+        constexpr uint32_t words = static_cast<uint32_t>(sizeof(Status) / sizeof(uint32_t));
+        static_assert(words <= kBlock, "the reset covers one block");
+        if (threadIdx.x < words) { reinterpret_cast<uint32_t*>(d.status)[threadIdx.x] = 0u; }
+        for (uint32_t s = threadIdx.x; s < d.nScenes; s += kBlock) { d.solve[s].abort = 0u; }
     }
 
     // ----------------------------------------------------------------------------------------
@@ -117,8 +118,9 @@ namespace tsp
             && z >= f.domMin[2] && z <= f.domMax[2];
     }
 
-    // cell id of one position; sets the abort flag when an in-domain particle left the inner frame
-    __device__ __forceinline__ uint32_t cellKey(const Frame& f, float x, float y, float z, bool check, Status* st)
+    // cell id of one position in its scene's key range; sets the abort flags when an in-domain
+    // particle left the inner frame
+    __device__ __forceinline__ uint32_t cellKey(const Frame& f, float x, float y, float z, bool check, Status* st, Solve* so)
     {
         const int cx = floorToInt(x * f.cellSizeInv);
         const int cy = floorToInt(y * f.cellSizeInv);
@@ -129,6 +131,7 @@ namespace tsp
                 || cz < f.innerLo[2] || cz > f.innerHi[2])
             {
                 atomicOr(&st->abort, 1u);
+                atomicOr(&so->abort, 1u);
             }
         }
         const int lx = cx - f.orig[0], ly = cy - f.orig[1], lz = cz - f.orig[2];
@@ -137,36 +140,47 @@ namespace tsp
         {
             return f.emptyKey;
         }
-        return morton(static_cast<uint32_t>(lx), static_cast<uint32_t>(ly), static_cast<uint32_t>(lz));
+        return f.keyBase + morton(static_cast<uint32_t>(lx), static_cast<uint32_t>(ly), static_cast<uint32_t>(lz));
     }
 
-    __global__ void __launch_bounds__(kBlock) kKeys(const float4* __restrict__ p, uint32_t n,
-        const Frame* __restrict__ frame, Status* st, bool check, uint32_t* __restrict__ key, uint32_t* __restrict__ idx)
+    enum Pop { POP_F = 0, POP_B = 1, POP_R = 2 };
+
+    __device__ __forceinline__ uint32_t sceneOfParticle(const Dev& d, int pop, uint32_t i)
+    {
+        if (pop == POP_F) { return d.blockScene[i / kBlock]; }
+        if (pop == POP_B) { return d.sceneB[i]; }
+        return d.bodyScene[d.rbody[i]];
+    }
+
+    // key of particle i of a population; the padding slots of a fluid segment take the dead key
+    __device__ __forceinline__ uint32_t particleKey(const Dev& d, int pop, const float4* p, uint32_t i, bool check)
+    {
+        const uint32_t s = sceneOfParticle(d, pop, i);
+        const Frame& f = d.frames[s];
+        if (pop == POP_F && i - d.scenes[s].offF >= d.scenes[s].nF) { return f.deadKey; }
+        const float4 q = p[i];
+        return cellKey(f, q.x, q.y, q.z, check, d.status, &d.solve[s]);
+    }
+
+    __global__ void __launch_bounds__(kBlock) kKeys(Dev d, int pop, const float4* __restrict__ p, uint32_t n, bool check,
+        uint32_t* __restrict__ key, uint32_t* __restrict__ idx)
     {
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) { return; }
-        const Frame f = *frame;
-        const float4 q = p[i];
-        key[i] = cellKey(f, q.x, q.y, q.z, check, st);
+        key[i] = particleKey(d, pop, p, i, check);
         idx[i] = i;
     }
 
     // boundary keys; flags a descent so the host knows whether the order needs a sort
-    __global__ void __launch_bounds__(kBlock) kKeysSorted(const float4* __restrict__ p, uint32_t n,
-        const Frame* __restrict__ frame, Status* st, uint32_t* __restrict__ key, uint32_t* __restrict__ idx)
+    __global__ void __launch_bounds__(kBlock) kKeysSorted(Dev d, const float4* __restrict__ p, uint32_t n,
+        uint32_t* __restrict__ key, uint32_t* __restrict__ idx)
     {
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) { return; }
-        const Frame f = *frame;
-        const float4 q = p[i];
-        const uint32_t k = cellKey(f, q.x, q.y, q.z, false, st);
+        const uint32_t k = particleKey(d, POP_B, p, i, false);
         key[i] = k;
         idx[i] = i;
-        if (i > 0)
-        {
-            const float4 r = p[i - 1];
-            if (cellKey(f, r.x, r.y, r.z, false, st) > k) { atomicOr(&st->bUnsorted, 1u); }
-        }
+        if (i > 0 && particleKey(d, POP_B, p, i - 1, false) > k) { atomicOr(&d.status->bUnsorted, 1u); }
     }
 
     // ----------------------------------------------------------------------------------------
@@ -256,20 +270,22 @@ namespace tsp
     // ----------------------------------------------------------------------------------------
     // cell tables
 
+    // cell-table index of an in-frame key (local Morton id = key - keyBase)
     __device__ __forceinline__ uint32_t linearOfKey(const Frame& f, uint32_t k)
     {
-        return linearCell(f, compact10(k), compact10(k >> 1), compact10(k >> 2));
+        const uint32_t m = k - f.keyBase;
+        return linearCell(f, compact10(m), compact10(m >> 1), compact10(m >> 2));
     }
 
     // clears the cells of the runs of the previous sorted keys
-    __global__ void __launch_bounds__(kBlock) kClearCells(const uint32_t* __restrict__ key, uint32_t n,
-        const Frame* __restrict__ frame, uint2* __restrict__ cell)
+    __global__ void __launch_bounds__(kBlock) kClearCells(Dev d, const uint32_t* __restrict__ key, uint32_t n,
+        uint2* __restrict__ cell)
     {
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) { return; }
-        const Frame& f = *frame;
         const uint32_t k = key[i];
-        if (k == f.emptyKey) { return; }
+        const Frame& f = d.frames[sceneOfKey(d, k)];
+        if (k >= f.emptyKey) { return; }
         if (i == 0 || key[i - 1] != k) { cell[linearOfKey(f, k)] = make_uint2(0u, 0u); }
     }
 
@@ -279,14 +295,14 @@ namespace tsp
         if (q < n) { cell[q] = make_uint2(0u, 0u); }
     }
 
-    __global__ void __launch_bounds__(kBlock) kBuildCells(const uint32_t* __restrict__ key, uint32_t n,
-        const Frame* __restrict__ frame, uint2* __restrict__ cell)
+    __global__ void __launch_bounds__(kBlock) kBuildCells(Dev d, const uint32_t* __restrict__ key, uint32_t n,
+        uint2* __restrict__ cell)
     {
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) { return; }
-        const Frame& f = *frame;
         const uint32_t k = key[i];
-        if (k == f.emptyKey) { return; }
+        const Frame& f = d.frames[sceneOfKey(d, k)];
+        if (k >= f.emptyKey) { return; }
         const bool first = i == 0 || key[i - 1] != k;
         const bool last = i + 1 == n || key[i + 1] != k;
         if (!first && !last) { return; }
@@ -303,15 +319,16 @@ namespace tsp
         int lo[3], hi[3];
     };
 
-    __device__ __forceinline__ bool stencilOf(const Frame& f, uint32_t k, int reach, Stencil& s)
+    __device__ __forceinline__ bool stencilOf(const Frame& f, uint32_t k, Stencil& s)
     {
-        if (k == f.emptyKey) { return false; }
-        const int l[3] = { static_cast<int>(compact10(k)), static_cast<int>(compact10(k >> 1)),
-                           static_cast<int>(compact10(k >> 2)) };
+        if (k >= f.emptyKey) { return false; }
+        const uint32_t m = k - f.keyBase;
+        const int l[3] = { static_cast<int>(compact10(m)), static_cast<int>(compact10(m >> 1)),
+                           static_cast<int>(compact10(m >> 2)) };
         for (int a = 0; a < 3; a++)
         {
-            s.lo[a] = max(l[a] - reach, 0);
-            s.hi[a] = min(l[a] + reach, static_cast<int>(f.dim[a]) - 1);
+            s.lo[a] = max(l[a] - f.reach, 0);
+            s.hi[a] = min(l[a] + f.reach, static_cast<int>(f.dim[a]) - 1);
         }
         return true;
     }
@@ -341,16 +358,18 @@ namespace tsp
         return v;
     }
 
-    __global__ void __launch_bounds__(kBlock) kQueryCount(Dev d, float r2, int reach, uint32_t nSlices)
+    __global__ void __launch_bounds__(kBlock) kQueryCount(Dev d, uint32_t nSlices)
     {
         const uint32_t blk = xcdBlock();
         const uint32_t i = blk * kBlock + threadIdx.x;
-        const Frame& f = *d.frame;
+        const uint32_t sc = d.blockScene[blk];
+        const Frame& f = d.frames[sc];
+        const float r2 = d.consts[sc].r2;
         uint32_t cnt[3] = { 0, 0, 0 };
         uint32_t mn[3] = { kEmpty32, kEmpty32, kEmpty32 };
         uint32_t mx[3] = { 0, 0, 0 };
         Stencil s;
-        if (i < d.nF && stencilOf(f, d.keyF[i], reach, s))
+        if (i < d.nF && stencilOf(f, d.keyF[i], s))
         {
             const float4 pi = d.pos[i];
             for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
@@ -466,16 +485,18 @@ namespace tsp
 
     constexpr uint32_t kStage = 40; // FF entries per thread staged in LDS by the fill (u16 slices)
 
-    __global__ void __launch_bounds__(kBlock) kQueryFill(Dev d, float r2, int reach)
+    __global__ void __launch_bounds__(kBlock) kQueryFill(Dev d)
     {
         __shared__ uint16_t stage[kStage][kBlock];
         const uint32_t blk = xcdBlock();
         const uint32_t t = threadIdx.x;
         const uint32_t i = blk * kBlock + t;
-        const Frame& f = *d.frame;
+        const uint32_t sc = d.blockScene[blk];
+        const Frame& f = d.frames[sc];
+        const float r2 = d.consts[sc].r2;
         const uint32_t lane = i & (kSlice - 1), slice = i / kSlice;
         Stencil s;
-        const bool active = i < d.nF && stencilOf(f, d.keyF[i], reach, s);
+        const bool active = i < d.nF && stencilOf(f, d.keyF[i], s);
         const uint32_t cF = i < d.nF ? d.cnt[REL_FF][i] : 0u;
         const bool wide = (cF & kWideSpanBit) != 0u;
         SliceMeta mF{}, mB{}, mR{};
@@ -691,11 +712,14 @@ namespace tsp
     // ----------------------------------------------------------------------------------------
     // rigid-source lists (RF CSR, RB/RR counts), one thread per rigid particle
 
-    __global__ void __launch_bounds__(kBlock) kQueryRigid(Dev d, float r2, int reach, bool fill)
+    __global__ void __launch_bounds__(kBlock) kQueryRigid(Dev d, bool fill)
     {
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= d.nR) { return; }
-        const Frame& f = *d.frame;
+        const uint32_t sc = d.bodyScene[d.rbody[i]];
+        const Frame& f = d.frames[sc];
+        const float r2 = d.consts[sc].r2;
+        const int reach = f.reach;
         const float4 pi = d.rpos[i];
         const int cx = floorToInt(pi.x * f.cellSizeInv) - f.orig[0];
         const int cy = floorToInt(pi.y * f.cellSizeInv) - f.orig[1];
@@ -785,34 +809,48 @@ namespace tsp
         return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
     }
 
-    __global__ void __launch_bounds__(kBlock) kAabb(const float4* __restrict__ p, uint32_t n, uint32_t* __restrict__ out)
+    // per-scene bounds of one population: out[6 s + a] holds the ordered bits of the min x, y, z
+    // and then the max; a wave whose valid lanes share one scene reduces before its atomics
+    __global__ void __launch_bounds__(kBlock) kAabb(Dev d, int pop, const float4* __restrict__ p, uint32_t n,
+        uint32_t* __restrict__ out)
     {
-        __shared__ uint32_t red[6][kBlock];
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+        bool valid = i < n;
+        uint32_t sc = 0;
+        if (valid)
+        {
+            sc = sceneOfParticle(d, pop, i);
+            if (pop == POP_F && i - d.scenes[sc].offF >= d.scenes[sc].nF) { valid = false; }
+        }
         uint32_t v[6] = { 0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u };
-        if (i < n)
+        if (valid)
         {
             const float4 q = p[i];
             const float c[3] = { q.x, q.y, q.z };
             for (int a = 0; a < 3; a++) { v[a] = orderedBits(c[a]); v[3 + a] = v[a]; }
         }
-        for (int a = 0; a < 6; a++) { red[a][threadIdx.x] = v[a]; }
-        __syncthreads();
-        for (uint32_t o = kBlock / 2; o > 0; o >>= 1)
+        const uint64_t vm = __ballot(valid);
+        if (vm == 0) { return; }
+        const int first = __ffsll(static_cast<unsigned long long>(vm)) - 1;
+        const uint32_t s0 = static_cast<uint32_t>(__shfl(static_cast<int>(sc), first, 64));
+        if (__ballot(valid && sc != s0) == 0)
         {
-            if (threadIdx.x < o)
+            for (int o = 32; o > 0; o >>= 1)
             {
                 for (int a = 0; a < 3; a++)
                 {
-                    red[a][threadIdx.x] = min(red[a][threadIdx.x], red[a][threadIdx.x + o]);
-                    red[3 + a][threadIdx.x] = max(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + o]);
+                    v[a] = min(v[a], static_cast<uint32_t>(__shfl_xor(static_cast<int>(v[a]), o, 64)));
+                    v[3 + a] = max(v[3 + a], static_cast<uint32_t>(__shfl_xor(static_cast<int>(v[3 + a]), o, 64)));
                 }
             }
-            __syncthreads();
+            if ((threadIdx.x & 63u) == 0)
+            {
+                for (int a = 0; a < 3; a++) { atomicMin(&out[6 * s0 + a], v[a]); atomicMax(&out[6 * s0 + 3 + a], v[3 + a]); }
+            }
         }
-        if (threadIdx.x == 0)
+        else if (valid)
         {
-            for (int a = 0; a < 3; a++) { atomicMin(&out[a], red[a][0]); atomicMax(&out[3 + a], red[3 + a][0]); }
+            for (int a = 0; a < 3; a++) { atomicMin(&out[6 * sc + a], v[a]); atomicMax(&out[6 * sc + 3 + a], v[3 + a]); }
         }
     }
 
@@ -821,18 +859,17 @@ namespace tsp
 
     static inline uint32_t blocksFor(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
 
-    void launchResetSearch(hipStream_t s, Status* st) { kResetSearch<<<1, 64, 0, s>>>(st); }
+    void launchResetSearch(hipStream_t s, const Dev& d) { kResetSearch<<<1, kBlock, 0, s>>>(d); }
 
-    void launchKeys(hipStream_t s, const float4* p, uint32_t n, const Frame* f, Status* st, bool check, uint32_t* key,
+    void launchKeys(hipStream_t s, const Dev& d, int pop, const float4* p, uint32_t n, bool check, uint32_t* key,
                     uint32_t* idx)
     {
-        if (n) { kKeys<<<blocksFor(n), kBlock, 0, s>>>(p, n, f, st, check, key, idx); }
+        if (n) { kKeys<<<blocksFor(n), kBlock, 0, s>>>(d, pop, p, n, check, key, idx); }
     }
 
-    void launchKeysSorted(hipStream_t s, const float4* p, uint32_t n, const Frame* f, Status* st, uint32_t* key,
-                          uint32_t* idx)
+    void launchKeysSorted(hipStream_t s, const Dev& d, const float4* p, uint32_t n, uint32_t* key, uint32_t* idx)
     {
-        if (n) { kKeysSorted<<<blocksFor(n), kBlock, 0, s>>>(p, n, f, st, key, idx); }
+        if (n) { kKeysSorted<<<blocksFor(n), kBlock, 0, s>>>(d, p, n, key, idx); }
     }
 
     // sorts (key, idx) stably by the low `bits` bits; the result ends in (key, idx)
@@ -862,9 +899,9 @@ namespace tsp
         if (g.n) { kPermute<<<blocksFor(g.n), kBlock, 0, s>>>(g, idx); }
     }
 
-    void launchClearCells(hipStream_t s, const uint32_t* key, uint32_t n, const Frame* f, uint2* cell)
+    void launchClearCells(hipStream_t s, const Dev& d, const uint32_t* key, uint32_t n, uint2* cell)
     {
-        if (n) { kClearCells<<<blocksFor(n), kBlock, 0, s>>>(key, n, f, cell); }
+        if (n) { kClearCells<<<blocksFor(n), kBlock, 0, s>>>(d, key, n, cell); }
     }
 
     void launchClearAll(hipStream_t s, uint2* cell, uint64_t n)
@@ -872,15 +909,15 @@ namespace tsp
         if (n) { kClearAll<<<blocksFor(n), kBlock, 0, s>>>(cell, n); }
     }
 
-    void launchBuildCells(hipStream_t s, const uint32_t* key, uint32_t n, const Frame* f, uint2* cell)
+    void launchBuildCells(hipStream_t s, const Dev& d, const uint32_t* key, uint32_t n, uint2* cell)
     {
-        if (n) { kBuildCells<<<blocksFor(n), kBlock, 0, s>>>(key, n, f, cell); }
+        if (n) { kBuildCells<<<blocksFor(n), kBlock, 0, s>>>(d, key, n, cell); }
     }
 
-    void launchQueryCount(hipStream_t s, const Dev& d, float r2, int reach)
+    void launchQueryCount(hipStream_t s, const Dev& d)
     {
         const uint32_t nSlices = (d.nF + kSlice - 1) / kSlice;
-        if (d.nF) { kQueryCount<<<blocksFor(d.nF), kBlock, 0, s>>>(d, r2, reach, nSlices); }
+        if (d.nF) { kQueryCount<<<blocksFor(d.nF), kBlock, 0, s>>>(d, nSlices); }
     }
 
     // units is 3 x (nSlices + 1) u32
@@ -893,9 +930,9 @@ namespace tsp
         kSliceApply<<<g, kBlock, 0, s>>>(d, nSlices, units);
     }
 
-    void launchQueryFill(hipStream_t s, const Dev& d, float r2, int reach)
+    void launchQueryFill(hipStream_t s, const Dev& d)
     {
-        if (d.nF) { kQueryFill<<<blocksFor(d.nF), kBlock, 0, s>>>(d, r2, reach); }
+        if (d.nF) { kQueryFill<<<blocksFor(d.nF), kBlock, 0, s>>>(d); }
     }
 
     void launchHaloBuild(hipStream_t s, const Dev& d)
@@ -903,10 +940,10 @@ namespace tsp
         if (d.nF) { kHaloBuild<<<blocksFor(d.nF), kBlock, 0, s>>>(d); }
     }
 
-    void launchQueryRigid(hipStream_t s, const Dev& d, float r2, int reach, bool fill, uint32_t* tmp)
+    void launchQueryRigid(hipStream_t s, const Dev& d, bool fill, uint32_t* tmp)
     {
         if (!d.nR) { return; }
-        kQueryRigid<<<blocksFor(d.nR), kBlock, 0, s>>>(d, r2, reach, fill);
+        kQueryRigid<<<blocksFor(d.nR), kBlock, 0, s>>>(d, fill);
         if (!fill)
         {
             launchScan(s, d.rfOff, d.nR + 1, tmp);
@@ -914,8 +951,8 @@ namespace tsp
         }
     }
 
-    void launchAabb(hipStream_t s, const float4* p, uint32_t n, uint32_t* out)
+    void launchAabb(hipStream_t s, const Dev& d, int pop, const float4* p, uint32_t n, uint32_t* out)
     {
-        if (n) { kAabb<<<blocksFor(n), kBlock, 0, s>>>(p, n, out); }
+        if (n) { kAabb<<<blocksFor(n), kBlock, 0, s>>>(d, pop, p, n, out); }
     }
 }

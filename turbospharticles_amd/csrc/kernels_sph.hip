@@ -105,14 +105,16 @@ namespace tsp
         return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
     }
 
-    __global__ void __launch_bounds__(kBlock) kDensity(Dev d, Consts k)
+    __global__ void __launch_bounds__(kBlock) kDensity(Dev d)
     {
         __shared__ float4 hp[kHaloMax];
         const uint32_t blk = xcdBlock();
         const uint32_t i = blk * kBlock + threadIdx.x;
+        const uint32_t sc = d.blockScene[blk];
+        const Consts& k = d.consts[sc];
         const uint32_t hc = d.haloCount[blk];
         stageHalo(d, blk, hc, [&](uint32_t q, uint32_t j) { hp[q] = d.pos[j]; });
-        if (i >= d.nF) { return; }
+        if (i - d.scenes[sc].offF >= d.scenes[sc].nF) { return; }
         const float4 pi = d.pos[i];
         const float xi = pi.x, yi = pi.y, zi = pi.z;
         const List ff = listOf(d, REL_FF, i);
@@ -136,14 +138,16 @@ namespace tsp
         d.vel[i].w = dF + dB + dR;
     }
 
-    __global__ void __launch_bounds__(kBlock) kNonPressure(Dev d, Consts k)
+    __global__ void __launch_bounds__(kBlock) kNonPressure(Dev d)
     {
         __shared__ float4 hp[kHaloMax], hv[kHaloMax];
         const uint32_t blk = xcdBlock();
         const uint32_t i = blk * kBlock + threadIdx.x;
+        const uint32_t sc = d.blockScene[blk];
+        const Consts& k = d.consts[sc];
         const uint32_t hc = d.haloCount[blk];
         stageHalo(d, blk, hc, [&](uint32_t q, uint32_t j) { hp[q] = d.pos[j]; hv[q] = d.vel[j]; });
-        if (i >= d.nF) { return; }
+        if (i - d.scenes[sc].offF >= d.scenes[sc].nF) { return; }
         const float4 pi = d.pos[i], vi = d.vel[i];
         const float xi = pi.x, yi = pi.y, zi = pi.z, vxi = vi.x, vyi = vi.y, vzi = vi.z;
         float aFx = 0.0f, aFy = 0.0f, aFz = 0.0f;
@@ -202,14 +206,16 @@ namespace tsp
         d.vp[i] = make_float4(__builtin_fmaf(ax, k.dt, vxi), __builtin_fmaf(ay, k.dt, vyi), __builtin_fmaf(az, k.dt, vzi), 0.0f);
     }
 
-    __global__ void __launch_bounds__(kBlock) kSourceDiag(Dev d, Consts k)
+    __global__ void __launch_bounds__(kBlock) kSourceDiag(Dev d)
     {
         __shared__ float4 hp[kHaloMax], hv[kHaloMax];
         const uint32_t blk = xcdBlock();
         const uint32_t i = blk * kBlock + threadIdx.x;
+        const uint32_t sc = d.blockScene[blk];
+        const Consts& k = d.consts[sc];
         const uint32_t hc = d.haloCount[blk];
         stageHalo(d, blk, hc, [&](uint32_t q, uint32_t j) { hp[q] = d.pos[j]; hv[q] = d.vp[j]; });
-        if (i >= d.nF) { return; }
+        if (i - d.scenes[sc].offF >= d.scenes[sc].nF) { return; }
         const float4 pi = d.pos[i], vi = d.vp[i];
         const float xi = pi.x, yi = pi.y, zi = pi.z, vxi = vi.x, vyi = vi.y, vzi = vi.z;
         float sF = 0.0f, cFx = 0.0f, cFy = 0.0f, cFz = 0.0f, g2 = 0.0f;
@@ -284,7 +290,7 @@ namespace tsp
     }
 
     // TerminationConditionSatisfied, sph_solver.cpp:686-732
-    __device__ __forceinline__ bool terminated(const Status& s, const Consts& k)
+    __device__ __forceinline__ bool terminated(const Solve& s, const Consts& k)
     {
         if (s.iters < k.minIters) { return false; }
         if (s.iters >= k.maxIters) { return true; }
@@ -300,28 +306,37 @@ namespace tsp
 
     // the solver state before the first sweep: N_solverIters_ = 1, errors 0, then the
     // increment and error decision of the first sweep (sph_solver.cpp:207-235)
-    __global__ void kSolverInit(Dev d, Consts k)
+    __global__ void __launch_bounds__(kBlock) kSolverInit(Dev d)
     {
-        Status& s = *d.status;
-        s.iters = d.nF > 0 ? 1u : 0u;
-        s.avgRaw = s.avgPos = s.avgAbs = 0.0f;
-        s.sumRaw = s.sumPos = s.sumAbs = 0.0;
-        s.converged = d.nF > 0 ? 0u : 1u;
-        s.sweeps = 0;
-        s.pending = 0;
-        s.iters++;
-        s.compute = (s.iters >= k.minIters && (k.termCond != 0 || terminated(s, k))) ? 1u : 0u;
+        for (uint32_t sc = threadIdx.x; sc < d.nScenes; sc += kBlock)
+        {
+            Solve& s = d.solve[sc];
+            const Consts& k = d.consts[sc];
+            const bool any = d.scenes[sc].nF > 0;
+            s.iters = any ? 1u : 0u;
+            s.avgRaw = s.avgPos = s.avgAbs = 0.0f;
+            s.sumRaw = s.sumPos = s.sumAbs = 0.0;
+            s.converged = any ? 0u : 1u;
+            s.sweeps = 0;
+            s.pending = 0;
+            s.iters++;
+            s.compute = (s.iters >= k.minIters && (k.termCond != 0 || terminated(s, k))) ? 1u : 0u;
+        }
     }
 
-    // the bookkeeping after one Jacobi sweep (sph_solver.cpp:207-261): the block partials summed in
-    // a fixed order, the averages, TerminationConditionSatisfied and the error decision of the next
-    // sweep. Run by one whole block.
-    __device__ void finalizeSweep(const Dev& d, const Consts& k, uint32_t nBlocks)
+    // the bookkeeping of one scene after a Jacobi sweep (sph_solver.cpp:207-261): the scene's block
+    // partials summed in a fixed order, the averages, TerminationConditionSatisfied and the error
+    // decision of the next sweep. Run by one whole block; nTot is the grid size of the sweep.
+    __device__ void finalizeSweep(const Dev& d, uint32_t sc, uint32_t nTot)
     {
         __shared__ double red[3][kBlock];
         __shared__ uint32_t flags[2];
-        Status& s = *d.status;
-        // one read of the flags for the whole block, so every wave takes the same path
+        Solve& s = d.solve[sc];
+        const Consts& k = d.consts[sc];
+        const uint32_t b0 = d.scenes[sc].blk0, nBlocks = d.scenes[sc].nBlk;
+        // one read of the flags for the whole block, so every wave takes the same path (the first
+        // barrier lets a previous call's readers finish before thread 0 writes them again)
+        __syncthreads();
         if (threadIdx.x == 0) { flags[0] = s.pending; flags[1] = s.compute; }
         __syncthreads();
         if (!flags[0]) { return; }
@@ -330,7 +345,7 @@ namespace tsp
         {
             for (int r = 0; r < 3; r++)
             {
-                const double* src = d.errPartial + static_cast<size_t>(r) * nBlocks;
+                const double* src = d.errPartial + static_cast<size_t>(r) * nTot + b0;
                 double a = 0.0;
                 uint32_t b = threadIdx.x;
                 for (; b + 3 * kBlock < nBlocks; b += 4 * kBlock)
@@ -355,7 +370,7 @@ namespace tsp
             if (compute)
             {
                 s.sumRaw = red[0][0]; s.sumPos = red[1][0]; s.sumAbs = red[2][0];
-                const float n = static_cast<float>(d.nF);
+                const float n = static_cast<float>(d.scenes[sc].nF);
                 s.avgRaw = static_cast<float>(red[0][0]) / n;
                 s.avgPos = static_cast<float>(red[1][0]) / n;
                 s.avgAbs = static_cast<float>(red[2][0]) / n;
@@ -376,23 +391,26 @@ namespace tsp
     // a sweep's pressure accelerations; block 0 of a sweep launch first finishes the bookkeeping of
     // the previous Jacobi sweep, which the next Jacobi launch reads. Blocks that see the solve
     // converged skip their particles: the final launch (sweep = false) recomputes every a^p.
-    __global__ void __launch_bounds__(kBlock) kPressureAcc(Dev d, Consts k, bool sweep, uint32_t nBlocks)
+    __global__ void __launch_bounds__(kBlock) kPressureAcc(Dev d, bool sweep)
     {
         __shared__ float4 hp[kHaloMax];
         __shared__ uint32_t skip;
+        const uint32_t blk = xcdBlock();
+        const uint32_t i = blk * kBlock + threadIdx.x;
+        const uint32_t sc = d.blockScene[blk];
+        const Consts& k = d.consts[sc];
         if (sweep)
         {
-            if (blockIdx.x == 0) { finalizeSweep(d, k, nBlocks); }
+            // block b of the launch finishes the previous sweep of scenes b, b + grid, ...
+            for (uint32_t q = blockIdx.x; q < d.nScenes; q += gridDim.x) { finalizeSweep(d, q, gridDim.x); }
             // one read per block, so the whole block leaves together
-            if (threadIdx.x == 0) { skip = __hip_atomic_load(&d.status->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+            if (threadIdx.x == 0) { skip = __hip_atomic_load(&d.solve[sc].converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
             __syncthreads();
             if (skip) { return; }
         }
-        const uint32_t blk = xcdBlock();
-        const uint32_t i = blk * kBlock + threadIdx.x;
         const uint32_t hc = d.haloCount[blk];
         stageHalo(d, blk, hc, [&](uint32_t q, uint32_t j) { hp[q] = d.pos[j]; });
-        if (i >= d.nF) { return; }
+        if (i - d.scenes[sc].offF >= d.scenes[sc].nF) { return; }
         const float4 p4 = d.pos[i];
         const float xi = p4.x, yi = p4.y, zi = p4.z, pi = p4.w;
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
@@ -439,21 +457,24 @@ namespace tsp
         return v;
     }
 
-    __global__ void __launch_bounds__(kBlock) kJacobi(Dev d, Consts k)
+    __global__ void __launch_bounds__(kBlock) kJacobi(Dev d)
     {
         __shared__ double red[3][kBlock / 64];
         __shared__ float4 hp[kHaloMax], ha[kHaloMax];
-        // the flags were written by an earlier launch: every thread reads the same values
-        if (d.status->converged) { return; }
-        const bool compute = d.status->compute != 0u;
-        if (blockIdx.x == 0 && threadIdx.x == 0) { d.status->pending = 1u; }
         const uint32_t blk = xcdBlock();
         const uint32_t i = blk * kBlock + threadIdx.x;
+        const uint32_t sc = d.blockScene[blk];
+        const Consts& k = d.consts[sc];
+        const Scene& S = d.scenes[sc];
+        // the scene's flags were written by an earlier launch: every thread reads the same values
+        if (d.solve[sc].converged) { return; }
+        const bool compute = d.solve[sc].compute != 0u;
+        if (blk == S.blk0 && threadIdx.x == 0) { d.solve[sc].pending = 1u; }
         const uint32_t hc = d.haloCount[blk];
         // (positions only: the pressures this launch writes into pos.w are not read here)
         stageHalo(d, blk, hc, [&](uint32_t q, uint32_t j) { hp[q] = d.pos[j]; ha[q] = d.ap[j]; });
         double eRaw = 0.0, ePos = 0.0, eAbs = 0.0;
-        if (i < d.nF)
+        if (i - S.offF < S.nF)
         {
             const float4 pi = d.pos[i], ai = d.ap[i];
             const float xi = pi.x, yi = pi.y, zi = pi.z, axi = ai.x, ayi = ai.y, azi = ai.z;
@@ -520,16 +541,19 @@ namespace tsp
         }
     }
 
-    // the bookkeeping of the last sweep of a group, before the host reads the status
-    __global__ void __launch_bounds__(kBlock) kFinalize(Dev d, Consts k, uint32_t nBlocks)
+    // the bookkeeping of the last sweep of a group, one block per scene, before the host reads
+    // the solver states; nTot is the grid size of the sweeps
+    __global__ void __launch_bounds__(kBlock) kFinalize(Dev d, uint32_t nTot)
     {
-        finalizeSweep(d, k, nBlocks);
+        finalizeSweep(d, blockIdx.x, nTot);
     }
 
-    __global__ void __launch_bounds__(kBlock) kIntegration(Dev d, Consts k)
+    __global__ void __launch_bounds__(kBlock) kIntegration(Dev d)
     {
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-        if (i >= d.nF) { return; }
+        const uint32_t sc = d.blockScene[blockIdx.x];
+        const Consts& k = d.consts[sc];
+        if (i - d.scenes[sc].offF >= d.scenes[sc].nF) { return; }
         const float4 a = d.ap[i], v = d.vp[i], p = d.pos[i];
         const float vx = __builtin_fmaf(k.dt, a.x, v.x);
         const float vy = __builtin_fmaf(k.dt, a.y, v.y);
@@ -575,10 +599,11 @@ namespace tsp
     }
 
     // ClearRigidForcesAndAddGravity, cpu_backend.cpp:2065-2087
-    __global__ void kRigidClearGravity(Dev d, Consts k)
+    __global__ void kRigidClearGravity(Dev d)
     {
         const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
         if (b >= d.nRB) { return; }
+        const Consts& k = d.consts[d.bodyScene[b]];
         const uint32_t nb = d.nRB;
         for (int f = 0; f < 6; f++) { d.rbForce[f * nb + b] = 0.0f; }
         const float m = rbv(d, RB_MASS, b), mi = rbv(d, RB_MASS_INV, b);
@@ -590,10 +615,11 @@ namespace tsp
 
     // AccumulateRigidParticleNonPressureForces_ChunkedSELL / ...PressureForces (sph_solver.cpp:1151-1223,
     // :1717-1778): reaction of the fluid neighbors of each rigid particle
-    __global__ void __launch_bounds__(kBlock) kRigidForces(Dev d, Consts k, bool pressure)
+    __global__ void __launch_bounds__(kBlock) kRigidForces(Dev d, bool pressure)
     {
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= d.nR) { return; }
+        const Consts& k = d.consts[d.bodyScene[d.rbody[i]]];
         const float4 ri = d.rpos[i], ui = d.rvel[i];
         const float xi = ri.x, yi = ri.y, zi = ri.z;
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
@@ -663,10 +689,11 @@ namespace tsp
     }
 
     // PredictRigidBodies, cpu_backend.cpp:2174-2235
-    __global__ void kRigidPredict(Dev d, Consts k)
+    __global__ void kRigidPredict(Dev d)
     {
         const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
         if (b >= d.nRB) { return; }
+        const Consts& k = d.consts[d.bodyScene[b]];
         const uint32_t nb = d.nRB;
         const float mi = rbv(d, RB_MASS_INV, b);
         float lp[3];
@@ -707,10 +734,11 @@ namespace tsp
     }
 
     // IntegrateRigidBodies, cpu_backend.cpp:2237-2303
-    __global__ void kRigidIntegrate(Dev d, Consts k)
+    __global__ void kRigidIntegrate(Dev d)
     {
         const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
         if (b >= d.nRB) { return; }
+        const Consts& k = d.consts[d.bodyScene[b]];
         const uint32_t nb = d.nRB;
         float* rb = d.rb;
         for (int a = 0; a < 3; a++)
@@ -749,7 +777,7 @@ namespace tsp
         b3y -= d13 * b1y + d23 * b2y;
         b3z -= d13 * b1z + d23 * b2z;
         ok = ok && normalize3(b3x, b3y, b3z);
-        if (!ok) { atomicOr(&d.status->rigidDegenerate, 1u); return; }
+        if (!ok) { atomicOr(&d.solve[d.bodyScene[b]].rigidDegenerate, 1u); return; }
         const float out[9] = { b1x, b2x, b3x, b1y, b2y, b3y, b1z, b2z, b3z };
         for (int q = 0; q < 9; q++) { rb[(RB_A00 + q) * nb + b] = out[q]; }
     }
@@ -759,31 +787,31 @@ namespace tsp
 
     static inline uint32_t blocksOf(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
 
-    void launchDensity(hipStream_t s, const Dev& d, const Consts& k) { if (d.nF) { kDensity<<<blocksOf(d.nF), kBlock, 0, s>>>(d, k); } }
-    void launchNonPressure(hipStream_t s, const Dev& d, const Consts& k) { if (d.nF) { kNonPressure<<<blocksOf(d.nF), kBlock, 0, s>>>(d, k); } }
-    void launchSourceDiag(hipStream_t s, const Dev& d, const Consts& k)
+    void launchDensity(hipStream_t s, const Dev& d) { if (d.nF) { kDensity<<<blocksOf(d.nF), kBlock, 0, s>>>(d); } }
+    void launchNonPressure(hipStream_t s, const Dev& d) { if (d.nF) { kNonPressure<<<blocksOf(d.nF), kBlock, 0, s>>>(d); } }
+    void launchSourceDiag(hipStream_t s, const Dev& d)
     {
-        if (d.nF) { kSourceDiag<<<blocksOf(d.nF), kBlock, 0, s>>>(d, k); }
-        kSolverInit<<<1, 1, 0, s>>>(d, k);
+        if (d.nF) { kSourceDiag<<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
+        if (d.nScenes) { kSolverInit<<<1, kBlock, 0, s>>>(d); }
     }
-    void launchPressureAcc(hipStream_t s, const Dev& d, const Consts& k, bool sweep)
+    void launchPressureAcc(hipStream_t s, const Dev& d, bool sweep)
     {
-        if (d.nF) { kPressureAcc<<<blocksOf(d.nF), kBlock, 0, s>>>(d, k, sweep, blocksOf(d.nF)); }
+        if (d.nF) { kPressureAcc<<<blocksOf(d.nF), kBlock, 0, s>>>(d, sweep); }
     }
-    void launchJacobi(hipStream_t s, const Dev& d, const Consts& k)
+    void launchJacobi(hipStream_t s, const Dev& d)
     {
-        if (d.nF) { kJacobi<<<blocksOf(d.nF), kBlock, 0, s>>>(d, k); }
+        if (d.nF) { kJacobi<<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
     }
-    void launchFinalize(hipStream_t s, const Dev& d, const Consts& k)
+    void launchFinalize(hipStream_t s, const Dev& d)
     {
-        if (d.nF) { kFinalize<<<1, kBlock, 0, s>>>(d, k, blocksOf(d.nF)); }
+        if (d.nF && d.nScenes) { kFinalize<<<d.nScenes, kBlock, 0, s>>>(d, blocksOf(d.nF)); }
     }
-    void launchIntegration(hipStream_t s, const Dev& d, const Consts& k) { if (d.nF) { kIntegration<<<blocksOf(d.nF), kBlock, 0, s>>>(d, k); } }
+    void launchIntegration(hipStream_t s, const Dev& d) { if (d.nF) { kIntegration<<<blocksOf(d.nF), kBlock, 0, s>>>(d); } }
 
     void launchRigidUpdate(hipStream_t s, const Dev& d, bool predicted) { if (d.nR) { kRigidUpdate<<<blocksOf(d.nR), kBlock, 0, s>>>(d, predicted); } }
-    void launchRigidClearGravity(hipStream_t s, const Dev& d, const Consts& k) { if (d.nRB) { kRigidClearGravity<<<blocksOf(d.nRB), kBlock, 0, s>>>(d, k); } }
-    void launchRigidForces(hipStream_t s, const Dev& d, const Consts& k, bool pressure) { if (d.nR) { kRigidForces<<<blocksOf(d.nR), kBlock, 0, s>>>(d, k, pressure); } }
+    void launchRigidClearGravity(hipStream_t s, const Dev& d) { if (d.nRB) { kRigidClearGravity<<<blocksOf(d.nRB), kBlock, 0, s>>>(d); } }
+    void launchRigidForces(hipStream_t s, const Dev& d, bool pressure) { if (d.nR) { kRigidForces<<<blocksOf(d.nR), kBlock, 0, s>>>(d, pressure); } }
     void launchRigidReduce(hipStream_t s, const Dev& d) { if (d.nRB) { kRigidReduce<<<d.nRB, kBlock, 0, s>>>(d); } }
-    void launchRigidPredict(hipStream_t s, const Dev& d, const Consts& k) { if (d.nRB) { kRigidPredict<<<blocksOf(d.nRB), kBlock, 0, s>>>(d, k); } }
-    void launchRigidIntegrate(hipStream_t s, const Dev& d, const Consts& k) { if (d.nRB) { kRigidIntegrate<<<blocksOf(d.nRB), kBlock, 0, s>>>(d, k); } }
+    void launchRigidPredict(hipStream_t s, const Dev& d) { if (d.nRB) { kRigidPredict<<<blocksOf(d.nRB), kBlock, 0, s>>>(d); } }
+    void launchRigidIntegrate(hipStream_t s, const Dev& d) { if (d.nRB) { kRigidIntegrate<<<blocksOf(d.nRB), kBlock, 0, s>>>(d); } }
 }

@@ -1,15 +1,19 @@
 // tsp_capi.hip -- host orchestration of the gfx950 IISPH step and its C ABI (include/tsp_hip.h).
 //
-// One context = one device, one HIP stream, all particle state resident in HBM. The step
-// sequence follows SimBackend_CPU::Step_IISPH (CPU/backend/sph_solver.cpp:34-319); the
-// differences from the reference's CUDA host (GPU/backend/gpu_backend.cpp:1274-1594) are that
-// list sizes and solver convergence live on the device: the host reads one small status block
-// after the neighbor count (pool growth and frame hysteresis) and after groups of Jacobi sweeps,
-// and sweeps queued past convergence exit at once.
+// One context = one device, one HIP stream, and any number of independent scenes whose state
+// stays resident in HBM. The scenes share every array: each owns a segment of the fluid,
+// boundary, rigid and body arrays (fluid segments aligned to the 256-thread block), a range of the
+// 32-bit sort-key space and a range of the cell tables, so one launch of each kernel advances all
+// scenes. A context with one scene is the plain backend.
+//
+// The step sequence follows SimBackend_CPU::Step_IISPH (CPU/backend/sph_solver.cpp:34-319). The
+// differences from the reference's CUDA host (GPU/backend/gpu_backend.cpp:1274-1594) are that list
+// sizes and solver convergence live on the device: the host reads one small status block after the
+// neighbor count (pool growth and frame hysteresis) and the scenes' solver states after groups of
+// Jacobi sweeps; sweeps queued past a scene's convergence exit at once for that scene.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -22,39 +26,41 @@
 
 namespace tsp
 {
-    void launchResetSearch(hipStream_t, Status*);
-    void launchKeys(hipStream_t, const float4*, uint32_t, const Frame*, Status*, bool, uint32_t*, uint32_t*);
-    void launchKeysSorted(hipStream_t, const float4*, uint32_t, const Frame*, Status*, uint32_t*, uint32_t*);
+    void launchResetSearch(hipStream_t, const Dev&);
+    void launchKeys(hipStream_t, const Dev&, int, const float4*, uint32_t, bool, uint32_t*, uint32_t*);
+    void launchKeysSorted(hipStream_t, const Dev&, const float4*, uint32_t, uint32_t*, uint32_t*);
     void launchRadixSort(hipStream_t, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*);
     void launchPermute(hipStream_t, const Gather&, const uint32_t*);
-    void launchClearCells(hipStream_t, const uint32_t*, uint32_t, const Frame*, uint2*);
+    void launchClearCells(hipStream_t, const Dev&, const uint32_t*, uint32_t, uint2*);
     void launchClearAll(hipStream_t, uint2*, uint64_t);
-    void launchBuildCells(hipStream_t, const uint32_t*, uint32_t, const Frame*, uint2*);
-    void launchQueryCount(hipStream_t, const Dev&, float, int);
+    void launchBuildCells(hipStream_t, const Dev&, const uint32_t*, uint32_t, uint2*);
+    void launchQueryCount(hipStream_t, const Dev&);
     void launchScanSlices(hipStream_t, const Dev&, uint32_t*, uint32_t*);
-    void launchQueryFill(hipStream_t, const Dev&, float, int);
-    void launchQueryRigid(hipStream_t, const Dev&, float, int, bool, uint32_t*);
+    void launchQueryFill(hipStream_t, const Dev&);
+    void launchQueryRigid(hipStream_t, const Dev&, bool, uint32_t*);
     void launchHaloBuild(hipStream_t, const Dev&);
-    void launchAabb(hipStream_t, const float4*, uint32_t, uint32_t*);
-    void launchDensity(hipStream_t, const Dev&, const Consts&);
-    void launchNonPressure(hipStream_t, const Dev&, const Consts&);
-    void launchSourceDiag(hipStream_t, const Dev&, const Consts&);
-    void launchPressureAcc(hipStream_t, const Dev&, const Consts&, bool);
-    void launchJacobi(hipStream_t, const Dev&, const Consts&);
-    void launchFinalize(hipStream_t, const Dev&, const Consts&);
-    void launchIntegration(hipStream_t, const Dev&, const Consts&);
+    void launchAabb(hipStream_t, const Dev&, int, const float4*, uint32_t, uint32_t*);
+    void launchDensity(hipStream_t, const Dev&);
+    void launchNonPressure(hipStream_t, const Dev&);
+    void launchSourceDiag(hipStream_t, const Dev&);
+    void launchPressureAcc(hipStream_t, const Dev&, bool);
+    void launchJacobi(hipStream_t, const Dev&);
+    void launchFinalize(hipStream_t, const Dev&);
+    void launchIntegration(hipStream_t, const Dev&);
     void launchRigidUpdate(hipStream_t, const Dev&, bool);
-    void launchRigidClearGravity(hipStream_t, const Dev&, const Consts&);
-    void launchRigidForces(hipStream_t, const Dev&, const Consts&, bool);
+    void launchRigidClearGravity(hipStream_t, const Dev&);
+    void launchRigidForces(hipStream_t, const Dev&, bool);
     void launchRigidReduce(hipStream_t, const Dev&);
-    void launchRigidPredict(hipStream_t, const Dev&, const Consts&);
-    void launchRigidIntegrate(hipStream_t, const Dev&, const Consts&);
+    void launchRigidPredict(hipStream_t, const Dev&);
+    void launchRigidIntegrate(hipStream_t, const Dev&);
 }
 
 using namespace tsp;
 
 namespace
 {
+    constexpr int kPopF = 0, kPopB = 1, kPopR = 2;
+
     struct Error
     {
         int code;
@@ -65,6 +71,8 @@ namespace
     {
         if (e != hipSuccess) { throw Error{ TSP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e) }; }
     }
+
+    uint32_t roundUp(uint32_t n, uint32_t m) { return (n + m - 1) / m * m; }
 
     // events around each named launch group while profiling is enabled
     struct Profiler
@@ -128,99 +136,10 @@ namespace
             p.pending.push_back({ name, a, b });
         }
     };
-}
 
-struct tsp_ctx
-{
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    tsp_params params{};
-    bool paramsSet = false;
-    float domMin[3] = { 0, 0, 0 }, domMax[3] = { 0, 0, 0 };
-    bool domainSet = false;
-    bool pulled = false;
-
-    Dev d{};
-    // second buffers of the re-order (swapped with the ones in d after each gather)
-    float4 *pos2 = nullptr, *vel2 = nullptr, *vp2 = nullptr, *ap2 = nullptr;
-    float *source2 = nullptr, *diag2 = nullptr;
-    uint64_t* idF2 = nullptr;
-    float4* bpos2 = nullptr;
-    float* bvol2 = nullptr;
-    uint64_t *idB = nullptr, *idB2 = nullptr;
-    uint64_t* idR = nullptr;
-    std::vector<uint32_t> hostIdRB, hostCountRB;
-    uint32_t* radixHist = nullptr;
-    uint32_t* sliceUnits = nullptr; // 3 x (slices + 1) scan buffer
-    uint32_t* aabb = nullptr;
-    std::vector<void*> allocs;      // particle-count sized
-    std::vector<void*> frameAllocs; // frame sized (cell grids)
-    std::vector<void*> poolAllocs;
-
-    Frame frame{};
-    bool frameValid = false;
-    bool cellsClean = false;        // fluid and rigid cell tables hold no runs (frame just built)
-    uint32_t keyBits = 0;
-    uint32_t reach = 1;
-    float cellSize = 0.0f;
-    uint32_t frameBuilds = 0;
-    bool bSorted = false;
-
-    Status* hostStatus = nullptr; // pinned
-    uint64_t step = 0;
-    double time = 0.0;
-    uint64_t sweepsTotal = 0;
-    uint32_t lastSweeps = 0;
-    uint32_t flags = 0;
-    Status last{};
-
-    Profiler prof;
-
-    ~tsp_ctx()
+    // constants of the kernels (solver_constants.h:20-36, simd.h:359-370, hoisted kernel constants)
+    Consts makeConsts(const tsp_params& p, uint32_t nF)
     {
-        releaseAll();
-        if (hostStatus) { (void)hipHostFree(hostStatus); }
-        if (stream) { (void)hipStreamDestroy(stream); }
-    }
-
-    template <typename T>
-    T* alloc(size_t n, std::vector<void*>& list)
-    {
-        void* p = nullptr;
-        hipCheck(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)), "hipMalloc");
-        list.push_back(p);
-        return static_cast<T*>(p);
-    }
-
-    static void releaseList(std::vector<void*>& list)
-    {
-        for (void* p : list) { (void)hipFree(p); }
-        list.clear();
-    }
-
-    void releaseAll()
-    {
-        releaseList(poolAllocs);
-        releaseList(frameAllocs);
-        releaseList(allocs);
-    }
-
-    void sync() { hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
-
-    void readStatus()
-    {
-        hipCheck(hipGetLastError(), "kernel launch");
-        hipCheck(hipMemcpyAsync(hostStatus, d.status, sizeof(Status), hipMemcpyDeviceToHost, stream), "status D2H");
-        sync();
-        prof.resolve();
-        last = *hostStatus;
-    }
-
-    // --- constants of the kernels (solver_constants.h:20-36, simd.h:359-370, hoisted kernel constants) ---
-    Consts consts() const
-    {
-        const tsp_params& p = params;
         Consts k{};
         const double h = static_cast<double>(p.particle_size);
         const double hinv = 1.0 / h;
@@ -257,8 +176,165 @@ struct tsp_ctx
         k.maxIters = p.max_solver_iters;
         k.fixedIters = p.fixed_iters;
         k.termCond = p.termination_cond;
-        k.nF = d.nF;
+        k.nF = nF;
         return k;
+    }
+
+    void validateParams(const tsp_params& p)
+    {
+        if (!(p.particle_size > 0.0f) || !(p.kernel_radius > 0.0f) || !(p.kernel_radius_sq > 0.0f)
+            || !(p.rest_density > 0.0f) || !(p.time_step > 0.0f) || !(p.grid_cell_size_ratio > 0.0f)
+            || p.max_solver_iters < 1)
+        {
+            throw Error{ TSP_ERR_INVALID, "invalid solver parameters" };
+        }
+    }
+
+    // one scene as the host sees it
+    struct SceneHost
+    {
+        uint32_t id = 0;
+        tsp_params params{};
+        float domMin[3] = { 0, 0, 0 }, domMax[3] = { 0, 0, 0 };
+        bool domainSet = false;
+        uint32_t nF = 0, nB = 0, nR = 0, nRB = 0;
+        // place in the arena
+        uint32_t offF = 0, segF = 0, blk0 = 0, nBlk = 0, offB = 0, offR = 0, offRB = 0;
+        // static rigid topology (scene-local body of each rigid particle) and body records
+        std::vector<uint32_t> rbodyLocal, bodyId, bodyCount;
+        // cell frame
+        Frame frame{};
+        bool frameValid = false;
+        uint32_t frameBuilds = 0;
+        // solver bookkeeping
+        uint64_t step = 0;
+        double time = 0.0;
+        uint64_t sweepsTotal = 0;
+        uint32_t lastSweeps = 0;
+        uint32_t flags = 0;
+        Solve last{};
+    };
+
+    // where a scene of a new layout comes from: a resident scene, or a host state
+    struct Source
+    {
+        int keep = -1;
+        const tsp_state_view* view = nullptr;
+        SceneHost host;
+    };
+}
+
+struct tsp_ctx
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    tsp_params defaults{};
+    bool paramsSet = false;
+    float domMin[3] = { 0, 0, 0 }, domMax[3] = { 0, 0, 0 };
+    bool domainSet = false;
+    uint32_t nextId = 1;
+
+    std::vector<SceneHost> scenes;
+    Dev d{};
+    // second buffers of the re-order (swapped with the ones in d after each gather)
+    float4 *pos2 = nullptr, *vel2 = nullptr, *vp2 = nullptr, *ap2 = nullptr;
+    float *source2 = nullptr, *diag2 = nullptr;
+    uint64_t* idF2 = nullptr;
+    float4* bpos2 = nullptr;
+    float* bvol2 = nullptr;
+    uint64_t *idB = nullptr, *idB2 = nullptr;
+    uint64_t* idR = nullptr;
+    uint32_t* radixHist = nullptr;
+    uint32_t* sliceUnits = nullptr; // 3 x (slices + 1) scan buffer
+    uint32_t* aabb = nullptr;       // 6 per scene
+    std::vector<void*> arena;       // particle and body arrays (replaced by a relayout)
+    std::vector<void*> work;        // search and scene-table buffers sized by the layout
+    std::vector<void*> frameAllocs; // cell tables
+    std::vector<void*> poolAllocs;  // SELL-32 pools and the RF list
+    uint64_t cellCap = 0;
+
+    bool layoutDirty = true;        // key bases and cell offsets must be assigned again
+    bool cellsClean = false;        // fluid and rigid cell tables hold no runs
+    uint32_t keyBits = 1;
+
+    Status* hostStatus = nullptr;   // pinned
+    Solve* hostSolve = nullptr;     // pinned, solveCap entries
+    uint32_t solveCap = 0;
+    Status last{};
+
+    Profiler prof;
+
+    ~tsp_ctx()
+    {
+        releaseList(poolAllocs);
+        releaseList(frameAllocs);
+        releaseList(work);
+        releaseList(arena);
+        if (hostStatus) { (void)hipHostFree(hostStatus); }
+        if (hostSolve) { (void)hipHostFree(hostSolve); }
+        if (stream) { (void)hipStreamDestroy(stream); }
+    }
+
+    template <typename T>
+    T* alloc(size_t n, std::vector<void*>& list)
+    {
+        void* p = nullptr;
+        hipCheck(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)), "hipMalloc");
+        list.push_back(p);
+        return static_cast<T*>(p);
+    }
+
+    static void releaseList(std::vector<void*>& list)
+    {
+        for (void* p : list) { (void)hipFree(p); }
+        list.clear();
+    }
+
+    void sync() { hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+
+    void h2d(void* dst, const void* src, size_t bytes)
+    {
+        if (bytes) { hipCheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream), "H2D"); sync(); }
+    }
+
+    void d2d(void* dst, const void* src, size_t bytes)
+    {
+        if (bytes) { hipCheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream), "D2D"); }
+    }
+
+    void zero(void* dst, size_t bytes)
+    {
+        if (bytes) { hipCheck(hipMemsetAsync(dst, 0, bytes, stream), "hipMemsetAsync"); }
+    }
+
+    uint32_t nScenes() const { return static_cast<uint32_t>(scenes.size()); }
+
+    int indexOf(uint32_t id) const
+    {
+        for (uint32_t s = 0; s < nScenes(); s++) { if (scenes[s].id == id) { return static_cast<int>(s); } }
+        return -1;
+    }
+
+    // the search status and every scene's solver state
+    void readStatus()
+    {
+        hipCheck(hipGetLastError(), "kernel launch");
+        hipCheck(hipMemcpyAsync(hostStatus, d.status, sizeof(Status), hipMemcpyDeviceToHost, stream), "status D2H");
+        if (nScenes())
+        {
+            hipCheck(hipMemcpyAsync(hostSolve, d.solve, sizeof(Solve) * nScenes(), hipMemcpyDeviceToHost, stream), "solve D2H");
+        }
+        sync();
+        prof.resolve();
+        last = *hostStatus;
+    }
+
+    void uploadConsts()
+    {
+        std::vector<Consts> k(nScenes());
+        for (uint32_t s = 0; s < nScenes(); s++) { k[s] = makeConsts(scenes[s].params, scenes[s].nF); }
+        h2d(d.consts, k.data(), sizeof(Consts) * k.size());
     }
 
     // --- pools of the SELL-32 lists ---
@@ -274,92 +350,486 @@ struct tsp_ctx
         d.rfIdx = alloc<uint32_t>(rfCap, poolAllocs);
     }
 
-    // --- cell frame: UpdateCellFrame, CPU/backend/neighbor_search.cpp:1902-2006 ---
-    void buildFrame()
+    // --- the arena: the particles and bodies of every scene, laid out anew. Resident scenes
+    //     move on the device; new scenes are packed from their host state ---
+    void relayout(std::vector<Source>& src)
     {
-        const tsp_params& p = params;
-        const float cs = p.kernel_radius * p.grid_cell_size_ratio;
-        const float span = p.kernel_radius / cs;
-        const float sm = std::max(0.0f, span - 1e-6f);
-        reach = std::max<uint32_t>(1u, static_cast<uint32_t>(std::ceil(sm)));
-        cellSize = cs;
-        const float inv = 1.0f / cs;
-
-        // scene AABB of F, B and R on the device (order-free min/max)
-        const uint32_t init[6] = { 0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u };
-        hipCheck(hipMemcpyAsync(aabb, init, sizeof(init), hipMemcpyHostToDevice, stream), "aabb init");
-        launchAabb(stream, d.pos, d.nF, aabb);
-        launchAabb(stream, d.bpos, d.nB, aabb);
-        launchAabb(stream, d.rpos, d.nR, aabb);
-        uint32_t bits[6];
-        hipCheck(hipMemcpyAsync(bits, aabb, sizeof(bits), hipMemcpyDeviceToHost, stream), "aabb D2H");
         sync();
-        const auto toFloat = [](uint32_t u)
+        std::vector<SceneHost> ns(src.size());
+        uint64_t oF = 0, oB = 0, oR = 0, oRB = 0;
+        for (size_t q = 0; q < src.size(); q++)
         {
-            const uint32_t v = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
-            float f;
-            std::memcpy(&f, &v, 4);
-            return f;
-        };
-        float mn[3], mx[3];
-        for (int a = 0; a < 3; a++) { mn[a] = toFloat(bits[a]); mx[a] = toFloat(bits[3 + a]); }
-        const bool domOk = domainSet && domMax[0] > domMin[0] && domMax[1] > domMin[1] && domMax[2] > domMin[2];
-        if (domOk)
-        {
-            for (int a = 0; a < 3; a++)
-            {
-                mn[a] = std::clamp(mn[a], domMin[a], domMax[a]);
-                mx[a] = std::clamp(mx[a], domMin[a], domMax[a]);
-            }
+            SceneHost& h = ns[q];
+            h = src[q].keep >= 0 ? scenes[static_cast<size_t>(src[q].keep)] : src[q].host;
+            h.offF = static_cast<uint32_t>(oF);
+            h.segF = roundUp(h.nF, kBlock);
+            h.blk0 = static_cast<uint32_t>(oF / kBlock);
+            h.nBlk = h.segF / kBlock;
+            h.offB = static_cast<uint32_t>(oB);
+            h.offR = static_cast<uint32_t>(oR);
+            h.offRB = static_cast<uint32_t>(oRB);
+            oF += h.segF; oB += h.nB; oR += h.nR; oRB += h.nRB;
         }
-        const auto floorToIntHost = [](float v)
+        if (oF > 0x7fffffffull || oB > 0x7fffffffull || oR > 0x7fffffffull || oRB > 0x7fffffffull)
         {
-            const int i = static_cast<int>(v);
-            return (v < static_cast<float>(i)) ? i - 1 : i;
-        };
-        const int64_t margin = static_cast<int64_t>(reach) + 8;
-        Frame f{};
-        uint32_t dimMax = 0;
-        for (int a = 0; a < 3; a++)
-        {
-            const int64_t o = static_cast<int64_t>(floorToIntHost(mn[a] * inv)) - margin;
-            const int64_t e = static_cast<int64_t>(floorToIntHost(mx[a] * inv)) + margin;
-            if (o > e || e - o + 1 > 1024)
-            {
-                throw Error{ TSP_ERR_CAPACITY, "cell frame exceeds 1024 cells per axis (32-bit Morton keys)" };
-            }
-            f.orig[a] = static_cast<int32_t>(o);
-            f.dim[a] = static_cast<uint32_t>(e - o + 1);
-            dimMax = std::max(dimMax, f.dim[a]);
-            f.innerLo[a] = f.orig[a] + static_cast<int32_t>(reach);
-            f.innerHi[a] = f.orig[a] + static_cast<int32_t>(f.dim[a]) - 1 - static_cast<int32_t>(reach);
+            throw Error{ TSP_ERR_CAPACITY, "particle count exceeds the local particle limit" };
         }
-        uint32_t axisBits = 1;
-        if (dimMax >= 2) { axisBits = 0; for (uint32_t v = dimMax - 1; v; v >>= 1) { axisBits++; } }
-        keyBits = 3u * axisBits + 1u;
-        f.emptyKey = 1u << (3u * axisBits);
-        f.cellSizeInv = inv;
-        f.domainSet = domOk ? 1 : 0;
-        for (int a = 0; a < 3; a++) { f.domMin[a] = domMin[a]; f.domMax[a] = domMax[a]; }
-        frame = f;
-        hipCheck(hipMemcpyAsync(d.frame, &frame, sizeof(Frame), hipMemcpyHostToDevice, stream), "frame H2D");
+        if (ns.size() > 0xffff) { throw Error{ TSP_ERR_CAPACITY, "more than 65535 scenes in one context" }; }
+        const uint32_t nF = static_cast<uint32_t>(oF), nB = static_cast<uint32_t>(oB);
+        const uint32_t nR = static_cast<uint32_t>(oR), nRB = static_cast<uint32_t>(oRB);
+        const uint32_t S = static_cast<uint32_t>(ns.size());
+        const size_t nRB1 = std::max<uint32_t>(nRB, 1), oldRB1 = std::max<uint32_t>(d.nRB, 1);
 
-        // dense cell tables over the frame
-        releaseList(frameAllocs);
-        const uint64_t nCells = static_cast<uint64_t>(f.dim[0]) * f.dim[1] * f.dim[2];
-        d.cellF = alloc<uint2>(nCells, frameAllocs);
-        d.cellB = alloc<uint2>(nCells, frameAllocs);
-        d.cellR = alloc<uint2>(nCells, frameAllocs);
-        launchClearAll(stream, d.cellF, nCells);
-        launchClearAll(stream, d.cellB, nCells);
-        launchClearAll(stream, d.cellR, nCells);
+        std::vector<void*> A;
+        Dev n = d;
+        float4 *npos2, *nvel2, *nvp2, *nap2, *nbpos2;
+        float *nsource2, *ndiag2, *nbvol2;
+        uint64_t *nidF2, *nidB, *nidB2, *nidR;
+        try
+        {
+            n.nF = nF; n.nB = nB; n.nR = nR; n.nRB = nRB; n.nScenes = S;
+            n.pos = alloc<float4>(nF, A); npos2 = alloc<float4>(nF, A);
+            n.vel = alloc<float4>(nF, A); nvel2 = alloc<float4>(nF, A);
+            n.vp = alloc<float4>(nF, A); nvp2 = alloc<float4>(nF, A);
+            n.ap = alloc<float4>(nF, A); nap2 = alloc<float4>(nF, A);
+            n.source = alloc<float>(nF, A); nsource2 = alloc<float>(nF, A);
+            n.diag = alloc<float>(nF, A); ndiag2 = alloc<float>(nF, A);
+            n.id = alloc<uint64_t>(nF, A); nidF2 = alloc<uint64_t>(nF, A);
+            // the padding slots of the fluid segments hold zeros
+            zero(n.pos, sizeof(float4) * nF); zero(n.vel, sizeof(float4) * nF);
+            zero(n.vp, sizeof(float4) * nF); zero(n.ap, sizeof(float4) * nF);
+            zero(n.source, sizeof(float) * nF); zero(n.diag, sizeof(float) * nF);
+            zero(n.id, sizeof(uint64_t) * nF);
+            n.bpos = alloc<float4>(nB, A); nbpos2 = alloc<float4>(nB, A);
+            n.bvol = alloc<float>(nB, A); nbvol2 = alloc<float>(nB, A);
+            nidB = alloc<uint64_t>(nB, A); nidB2 = alloc<uint64_t>(nB, A);
+            n.rxrel = alloc<float>(nR, A); n.ryrel = alloc<float>(nR, A); n.rzrel = alloc<float>(nR, A);
+            n.rpos = alloc<float4>(nR, A); n.rvel = alloc<float4>(nR, A); n.rvp = alloc<float4>(nR, A);
+            n.rfx = alloc<float>(nR, A); n.rfy = alloc<float>(nR, A); n.rfz = alloc<float>(nR, A);
+            n.rbody = alloc<uint32_t>(nR, A);
+            nidR = alloc<uint64_t>(nR, A);
+            n.rb = alloc<float>(static_cast<size_t>(TSP_RB_COUNT) * nRB1, A);
+            n.rbPred = alloc<float>(9 * nRB1, A);
+            n.rbForce = alloc<float>(6 * nRB1, A);
+            zero(n.rbPred, sizeof(float) * 9 * nRB1);
+            zero(n.rbForce, sizeof(float) * 6 * nRB1);
+            n.byBodyOff = alloc<uint32_t>(nRB + 1ull, A);
+            n.byBodyIdx = alloc<uint32_t>(nR, A);
+            n.blockScene = alloc<uint32_t>(nF / kBlock, A);
+            n.bodyScene = alloc<uint32_t>(nRB, A);
+            n.sceneB = alloc<uint16_t>(nB, A);
+
+            const auto field = [](const float* f, uint32_t i, float dflt) { return f ? f[i] : dflt; };
+            std::vector<uint64_t> seq;
+            const auto ids = [&](const uint64_t* given, uint32_t cnt) -> const uint64_t* {
+                if (given) { return given; }
+                seq.resize(cnt);
+                for (uint32_t i = 0; i < cnt; i++) { seq[i] = i; }
+                return seq.data();
+            };
+            for (size_t q = 0; q < src.size(); q++)
+            {
+                const SceneHost& h = ns[q];
+                if (src[q].keep >= 0)
+                {
+                    const SceneHost& o = scenes[static_cast<size_t>(src[q].keep)];
+                    d2d(n.pos + h.offF, d.pos + o.offF, sizeof(float4) * h.nF);
+                    d2d(n.vel + h.offF, d.vel + o.offF, sizeof(float4) * h.nF);
+                    d2d(n.vp + h.offF, d.vp + o.offF, sizeof(float4) * h.nF);
+                    d2d(n.ap + h.offF, d.ap + o.offF, sizeof(float4) * h.nF);
+                    d2d(n.source + h.offF, d.source + o.offF, sizeof(float) * h.nF);
+                    d2d(n.diag + h.offF, d.diag + o.offF, sizeof(float) * h.nF);
+                    d2d(n.id + h.offF, d.id + o.offF, sizeof(uint64_t) * h.nF);
+                    d2d(n.bpos + h.offB, d.bpos + o.offB, sizeof(float4) * h.nB);
+                    d2d(n.bvol + h.offB, d.bvol + o.offB, sizeof(float) * h.nB);
+                    d2d(nidB + h.offB, idB + o.offB, sizeof(uint64_t) * h.nB);
+                    d2d(n.rxrel + h.offR, d.rxrel + o.offR, sizeof(float) * h.nR);
+                    d2d(n.ryrel + h.offR, d.ryrel + o.offR, sizeof(float) * h.nR);
+                    d2d(n.rzrel + h.offR, d.rzrel + o.offR, sizeof(float) * h.nR);
+                    d2d(n.rpos + h.offR, d.rpos + o.offR, sizeof(float4) * h.nR);
+                    d2d(n.rvel + h.offR, d.rvel + o.offR, sizeof(float4) * h.nR);
+                    d2d(n.rvp + h.offR, d.rvp + o.offR, sizeof(float4) * h.nR);
+                    d2d(nidR + h.offR, idR + o.offR, sizeof(uint64_t) * h.nR);
+                    for (size_t f = 0; f < TSP_RB_COUNT; f++)
+                    {
+                        d2d(n.rb + f * nRB1 + h.offRB, d.rb + f * oldRB1 + o.offRB, sizeof(float) * h.nRB);
+                    }
+                    for (size_t f = 0; f < 9; f++)
+                    {
+                        d2d(n.rbPred + f * nRB1 + h.offRB, d.rbPred + f * oldRB1 + o.offRB, sizeof(float) * h.nRB);
+                    }
+                    for (size_t f = 0; f < 6; f++)
+                    {
+                        d2d(n.rbForce + f * nRB1 + h.offRB, d.rbForce + f * oldRB1 + o.offRB, sizeof(float) * h.nRB);
+                    }
+                    continue;
+                }
+                const tsp_state_view* s = src[q].view;
+                // fluid: the SoA fields of the view packed into the float4 records
+                {
+                    std::vector<float4> r[4];
+                    for (auto& v : r) { v.resize(h.nF); }
+                    std::vector<float> src1(h.nF), diag1(h.nF);
+                    float* const* f = s->fluid;
+                    for (uint32_t i = 0; i < h.nF; i++)
+                    {
+                        r[0][i] = make_float4(field(f[TSP_F_X], i, 0), field(f[TSP_F_Y], i, 0), field(f[TSP_F_Z], i, 0),
+                                              field(f[TSP_F_PRESSURE], i, 0));
+                        r[1][i] = make_float4(field(f[TSP_F_VX], i, 0), field(f[TSP_F_VY], i, 0), field(f[TSP_F_VZ], i, 0),
+                                              field(f[TSP_F_DENSITY], i, 0));
+                        r[2][i] = make_float4(field(f[TSP_F_VPX], i, 0), field(f[TSP_F_VPY], i, 0), field(f[TSP_F_VPZ], i, 0), 0.0f);
+                        r[3][i] = make_float4(field(f[TSP_F_APX], i, 0), field(f[TSP_F_APY], i, 0), field(f[TSP_F_APZ], i, 0), 0.0f);
+                        src1[i] = field(f[TSP_F_SOURCE], i, 0);
+                        diag1[i] = field(f[TSP_F_DIAG], i, 0);
+                    }
+                    float4* dst[4] = { n.pos, n.vel, n.vp, n.ap };
+                    for (int k = 0; k < 4; k++) { h2d(dst[k] + h.offF, r[k].data(), sizeof(float4) * h.nF); }
+                    h2d(n.source + h.offF, src1.data(), sizeof(float) * h.nF);
+                    h2d(n.diag + h.offF, diag1.data(), sizeof(float) * h.nF);
+                    h2d(n.id + h.offF, ids(s->fluid_id, h.nF), sizeof(uint64_t) * h.nF);
+                }
+                // boundary (ensureBoundaryCoefficientsPresent defaults: psi = massB, V = psi / rho0)
+                {
+                    const float hh = h.params.particle_size, rho0 = h.params.rest_density;
+                    const float massB = rho0 * hh * hh * hh;
+                    const float volB = rho0 > 0.0f ? massB / rho0 : 0.0f;
+                    std::vector<float4> r(h.nB);
+                    std::vector<float> vol(h.nB);
+                    float* const* f = s->boundary;
+                    for (uint32_t i = 0; i < h.nB; i++)
+                    {
+                        r[i] = make_float4(field(f[TSP_B_X], i, 0), field(f[TSP_B_Y], i, 0), field(f[TSP_B_Z], i, 0),
+                                           field(f[TSP_B_PSI], i, massB));
+                        vol[i] = field(f[TSP_B_VOL], i, volB);
+                    }
+                    h2d(n.bpos + h.offB, r.data(), sizeof(float4) * h.nB);
+                    h2d(n.bvol + h.offB, vol.data(), sizeof(float) * h.nB);
+                    h2d(nidB + h.offB, ids(s->boundary_id, h.nB), sizeof(uint64_t) * h.nB);
+                }
+                // rigid particles: body-frame offsets; the world records follow from the body state
+                {
+                    float* const* f = s->rigid;
+                    std::vector<float> rel[3];
+                    std::vector<float4> wp(h.nR), wv(h.nR), wvp(h.nR);
+                    for (int a = 0; a < 3; a++) { rel[a].resize(h.nR); }
+                    for (uint32_t i = 0; i < h.nR; i++)
+                    {
+                        rel[0][i] = field(f[TSP_R_XREL], i, 0);
+                        rel[1][i] = field(f[TSP_R_YREL], i, 0);
+                        rel[2][i] = field(f[TSP_R_ZREL], i, 0);
+                        wp[i] = make_float4(0.0f, 0.0f, 0.0f, field(f[TSP_R_PSI], i, 0));
+                        wv[i] = make_float4(0.0f, 0.0f, 0.0f, field(f[TSP_R_VOL], i, 0));
+                        wvp[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    }
+                    h2d(n.rxrel + h.offR, rel[0].data(), sizeof(float) * h.nR);
+                    h2d(n.ryrel + h.offR, rel[1].data(), sizeof(float) * h.nR);
+                    h2d(n.rzrel + h.offR, rel[2].data(), sizeof(float) * h.nR);
+                    h2d(n.rpos + h.offR, wp.data(), sizeof(float4) * h.nR);
+                    h2d(n.rvel + h.offR, wv.data(), sizeof(float4) * h.nR);
+                    h2d(n.rvp + h.offR, wvp.data(), sizeof(float4) * h.nR);
+                    h2d(nidR + h.offR, ids(s->rigid_id, h.nR), sizeof(uint64_t) * h.nR);
+                }
+                // bodies (a missing orientation field is the identity)
+                for (size_t f = 0; f < TSP_RB_COUNT; f++)
+                {
+                    std::vector<float> v(h.nRB);
+                    const bool diagonal = f == TSP_RB_A00 || f == TSP_RB_A11 || f == TSP_RB_A22;
+                    for (uint32_t b = 0; b < h.nRB; b++) { v[b] = s->body[f] ? s->body[f][b] : (diagonal ? 1.0f : 0.0f); }
+                    h2d(n.rb + f * nRB1 + h.offRB, v.data(), sizeof(float) * h.nRB);
+                }
+            }
+            // tables rebuilt from the host records: body of each rigid particle, the particles of
+            // each body (BuildRigidParticleIndicesByBody, sync/state_sim.h:609-631), scene maps
+            std::vector<uint32_t> rbody(nR), off(nRB + 1ull, 0), idx(nR);
+            std::vector<uint32_t> blockScene(nF / kBlock), bodyScene(nRB);
+            std::vector<uint16_t> sceneB(nB);
+            for (uint32_t s = 0; s < S; s++)
+            {
+                const SceneHost& h = ns[s];
+                for (uint32_t i = 0; i < h.nR; i++) { rbody[h.offR + i] = h.offRB + h.rbodyLocal[i]; }
+                for (uint32_t b = 0; b < h.nBlk; b++) { blockScene[h.blk0 + b] = s; }
+                for (uint32_t b = 0; b < h.nRB; b++) { bodyScene[h.offRB + b] = s; }
+                for (uint32_t i = 0; i < h.nB; i++) { sceneB[h.offB + i] = static_cast<uint16_t>(s); }
+            }
+            for (uint32_t i = 0; i < nR; i++) { off[rbody[i] + 1]++; }
+            for (uint32_t b = 0; b < nRB; b++) { off[b + 1] += off[b]; }
+            std::vector<uint32_t> w(off);
+            for (uint32_t i = 0; i < nR; i++) { idx[w[rbody[i]]++] = i; }
+            h2d(n.rbody, rbody.data(), sizeof(uint32_t) * nR);
+            h2d(n.byBodyOff, off.data(), sizeof(uint32_t) * (nRB + 1ull));
+            h2d(n.byBodyIdx, idx.data(), sizeof(uint32_t) * nR);
+            h2d(n.blockScene, blockScene.data(), sizeof(uint32_t) * blockScene.size());
+            h2d(n.bodyScene, bodyScene.data(), sizeof(uint32_t) * nRB);
+            h2d(n.sceneB, sceneB.data(), sizeof(uint16_t) * nB);
+            sync();
+        }
+        catch (...)
+        {
+            (void)hipStreamSynchronize(stream);
+            releaseList(A);
+            throw;
+        }
+
+        // install the new arena
+        releaseList(arena);
+        arena = std::move(A);
+        d = n;
+        pos2 = npos2; vel2 = nvel2; vp2 = nvp2; ap2 = nap2; source2 = nsource2; diag2 = ndiag2; idF2 = nidF2;
+        bpos2 = nbpos2; bvol2 = nbvol2; idB = nidB; idB2 = nidB2; idR = nidR;
+        scenes = std::move(ns);
+        layoutDirty = true;
+        cellsClean = false;
+
+        // buffers sized by the layout
+        releaseList(work);
+        const uint32_t nSort = std::max(nF, std::max(nB, nR));
+        d.keyF = alloc<uint32_t>(nSort, work);
+        d.keyF2 = alloc<uint32_t>(nSort, work);
+        d.idxF = alloc<uint32_t>(nSort, work);
+        d.idxF2 = alloc<uint32_t>(nSort, work);
+        d.keyR = alloc<uint32_t>(nR, work);
+        d.keyR2 = alloc<uint32_t>(nR, work);
+        d.idxR = alloc<uint32_t>(nR, work);
+        d.idxR2 = alloc<uint32_t>(nR, work);
+        const uint64_t nTiles = (nSort + 2047ull) / 2048ull;
+        radixHist = alloc<uint32_t>(256ull * nTiles, work);
+        aabb = alloc<uint32_t>(6ull * S, work);
+        const uint32_t nSlices = (nF + kSlice - 1) / kSlice;
+        for (int r = 0; r < 3; r++)
+        {
+            d.cnt[r] = alloc<uint16_t>(nF, work);
+            d.meta[r] = alloc<SliceMeta>(nSlices, work);
+            zero(d.cnt[r], sizeof(uint16_t) * nF);
+        }
+        sliceUnits = alloc<uint32_t>(3ull * (nSlices + 1), work);
+        {
+            // block sums of the scans: radix digit offsets, slice units, RF offsets
+            const uint64_t m = std::max<uint64_t>(256ull * nTiles, std::max<uint64_t>(nSlices + 1ull, nR + 1ull));
+            d.scanTmp = alloc<uint32_t>((m + 2047ull) / 2048ull + 1ull, work);
+        }
+        d.ffWideMin = alloc<uint32_t>(nF, work);
+        {
+            const uint64_t nBlocks = nF / kBlock;
+            d.haloIdx = alloc<uint32_t>(nBlocks * kHaloMax, work);
+            d.haloCount = alloc<uint32_t>(nBlocks, work);
+            // no block has a halo until the first search builds one
+            hipCheck(hipMemsetAsync(d.haloCount, 0xff, sizeof(uint32_t) * std::max<uint64_t>(nBlocks, 1), stream), "memset");
+        }
+        d.rfCnt = alloc<uint32_t>(nR, work);
+        d.rbCnt = alloc<uint32_t>(nR, work);
+        d.rrCnt = alloc<uint32_t>(nR, work);
+        d.rfOff = alloc<uint32_t>(nR + 1ull, work);
+        d.errPartial = alloc<double>(3ull * (nF / kBlock), work);
+        d.status = alloc<Status>(1, work);
+        zero(d.status, sizeof(Status));
+        d.scenes = alloc<Scene>(S, work);
+        d.frames = alloc<Frame>(S, work);
+        d.consts = alloc<Consts>(S, work);
+        d.solve = alloc<Solve>(S, work);
+        d.sceneKey0 = alloc<uint32_t>(S + 1ull, work);
+        zero(d.solve, sizeof(Solve) * S);
+        {
+            std::vector<Scene> t(S);
+            for (uint32_t s = 0; s < S; s++)
+            {
+                const SceneHost& h = scenes[s];
+                t[s] = Scene{ h.offF, h.nF, h.blk0, h.nBlk, h.offB, h.nB, h.offR, h.nR, h.offRB, h.nRB };
+            }
+            h2d(d.scenes, t.data(), sizeof(Scene) * S);
+        }
+        uploadConsts();
+        if (solveCap < S)
+        {
+            if (hostSolve) { (void)hipHostFree(hostSolve); hostSolve = nullptr; solveCap = 0; }
+            hipCheck(hipHostMalloc(reinterpret_cast<void**>(&hostSolve), sizeof(Solve) * S), "hipHostMalloc");
+            solveCap = S;
+        }
+        if (S) { std::memset(hostSolve, 0, sizeof(Solve) * S); }
+        // pools: kept when large enough (about 48 FF, 16 FB and 8 FR entries per particle to
+        // start, grown on demand)
+        {
+            uint64_t cap[3] = { 48ull * nF + 1024, 16ull * nF + 1024, 8ull * nF + 1024 };
+            uint64_t rfCap = 64ull * nR + 1024;
+            const bool have = !poolAllocs.empty();
+            if (!have || cap[0] > d.poolCap[0] || cap[1] > d.poolCap[1] || cap[2] > d.poolCap[2] || rfCap > d.rfCap)
+            {
+                if (have)
+                {
+                    for (int r = 0; r < 3; r++) { cap[r] = std::max(cap[r], d.poolCap[r]); }
+                    rfCap = std::max(rfCap, d.rfCap);
+                }
+                allocPools(cap, rfCap);
+            }
+        }
+        // world records of the rigid particles from the body state
+        launchRigidUpdate(stream, d, false);
+        sync();
+    }
+
+    // a scene record for a new scene from a host state (validates it)
+    SceneHost newScene(const tsp_params& p, const tsp_state_view* s, const float* mn, const float* mx)
+    {
+        validateParams(p);
+        if (s->n_fluid > 0x7fffffffu || s->n_boundary > 0x7fffffffu || s->n_rigid > 0x7fffffffu)
+        {
+            throw Error{ TSP_ERR_CAPACITY, "particle count exceeds the local particle limit" };
+        }
+        if (s->n_rigid && !s->rigid_body) { throw Error{ TSP_ERR_INVALID, "rigid particles need body ids" }; }
+        SceneHost h;
+        h.params = p;
+        h.domainSet = mn && mx;
+        if (h.domainSet) { for (int a = 0; a < 3; a++) { h.domMin[a] = mn[a]; h.domMax[a] = mx[a]; } }
+        h.nF = s->n_fluid; h.nB = s->n_boundary; h.nR = s->n_rigid; h.nRB = s->n_bodies;
+        h.rbodyLocal.resize(h.nR);
+        for (uint32_t i = 0; i < h.nR; i++)
+        {
+            if (s->rigid_body[i] >= h.nRB) { throw Error{ TSP_ERR_INVALID, "Invalid rigid body index." }; }
+            h.rbodyLocal[i] = s->rigid_body[i];
+        }
+        h.bodyId.resize(h.nRB);
+        h.bodyCount.resize(h.nRB);
+        for (uint32_t b = 0; b < h.nRB; b++)
+        {
+            h.bodyId[b] = s->body_id ? s->body_id[b] : b;
+            h.bodyCount[b] = s->body_count ? s->body_count[b] : 0;
+        }
+        h.step = s->step;
+        h.time = s->time;
+        return h;
+    }
+
+    // --- cell frames: UpdateCellFrame (CPU/backend/neighbor_search.cpp:1902-2006) for every scene
+    //     whose frame is invalid; then every scene's key range and cell range, the cleared cell
+    //     tables and the boundary order ---
+    void layoutFrames()
+    {
+        const uint32_t S = nScenes();
+        bool rebuild = false;
+        for (const SceneHost& h : scenes) { rebuild = rebuild || !h.frameValid; }
+        if (rebuild)
+        {
+            // per-scene bounds of F, B and R on the device (order-free min/max)
+            std::vector<uint32_t> bits(6ull * S);
+            for (uint32_t s = 0; s < S; s++)
+            {
+                for (int a = 0; a < 3; a++) { bits[6 * s + a] = 0xffffffffu; bits[6 * s + 3 + a] = 0u; }
+            }
+            h2d(aabb, bits.data(), sizeof(uint32_t) * bits.size());
+            launchAabb(stream, d, kPopF, d.pos, d.nF, aabb);
+            launchAabb(stream, d, kPopB, d.bpos, d.nB, aabb);
+            launchAabb(stream, d, kPopR, d.rpos, d.nR, aabb);
+            hipCheck(hipMemcpyAsync(bits.data(), aabb, sizeof(uint32_t) * bits.size(), hipMemcpyDeviceToHost, stream), "aabb D2H");
+            sync();
+            const auto toFloat = [](uint32_t u)
+            {
+                const uint32_t v = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+                float f;
+                std::memcpy(&f, &v, 4);
+                return f;
+            };
+            const auto floorToIntHost = [](float v)
+            {
+                const int i = static_cast<int>(v);
+                return (v < static_cast<float>(i)) ? i - 1 : i;
+            };
+            for (uint32_t s = 0; s < S; s++)
+            {
+                SceneHost& h = scenes[s];
+                if (h.frameValid) { continue; }
+                const tsp_params& p = h.params;
+                const float cs = p.kernel_radius * p.grid_cell_size_ratio;
+                const float span = p.kernel_radius / cs;
+                const float sm = std::max(0.0f, span - 1e-6f);
+                const int32_t reach = std::max<int32_t>(1, static_cast<int32_t>(std::ceil(sm)));
+                const float inv = 1.0f / cs;
+                const bool empty = bits[6 * s] == 0xffffffffu && bits[6 * s + 3] == 0u;
+                float mn[3], mx[3];
+                for (int a = 0; a < 3; a++)
+                {
+                    mn[a] = empty ? 0.0f : toFloat(bits[6 * s + a]);
+                    mx[a] = empty ? 0.0f : toFloat(bits[6 * s + 3 + a]);
+                }
+                const bool domOk = h.domainSet && h.domMax[0] > h.domMin[0] && h.domMax[1] > h.domMin[1] && h.domMax[2] > h.domMin[2];
+                if (domOk)
+                {
+                    for (int a = 0; a < 3; a++)
+                    {
+                        mn[a] = std::clamp(mn[a], h.domMin[a], h.domMax[a]);
+                        mx[a] = std::clamp(mx[a], h.domMin[a], h.domMax[a]);
+                    }
+                }
+                const int64_t margin = static_cast<int64_t>(reach) + 8;
+                Frame f{};
+                for (int a = 0; a < 3; a++)
+                {
+                    const int64_t o = static_cast<int64_t>(floorToIntHost(mn[a] * inv)) - margin;
+                    const int64_t e = static_cast<int64_t>(floorToIntHost(mx[a] * inv)) + margin;
+                    if (o > e || e - o + 1 > 1024)
+                    {
+                        throw Error{ TSP_ERR_CAPACITY, "cell frame exceeds 1024 cells per axis (32-bit Morton keys)" };
+                    }
+                    f.orig[a] = static_cast<int32_t>(o);
+                    f.dim[a] = static_cast<uint32_t>(e - o + 1);
+                    f.innerLo[a] = f.orig[a] + reach;
+                    f.innerHi[a] = f.orig[a] + static_cast<int32_t>(f.dim[a]) - 1 - reach;
+                }
+                f.cellSizeInv = inv;
+                f.reach = reach;
+                f.domainSet = domOk ? 1 : 0;
+                for (int a = 0; a < 3; a++) { f.domMin[a] = h.domMin[a]; f.domMax[a] = h.domMax[a]; }
+                h.frame = f;
+                h.frameValid = true;
+                h.frameBuilds++;
+                h.flags |= TSP_FLAG_FRAME_REBUILT;
+            }
+        }
+        // key range of each scene: its Morton ids, then its empty key (outside the frame) and its
+        // dead key (padding slots), so a sort of all scenes keeps every scene's segment in place
+        uint64_t key = 0, cells = 0;
+        std::vector<uint32_t> key0(S + 1ull);
+        std::vector<Frame> frames(S);
+        for (uint32_t s = 0; s < S; s++)
+        {
+            Frame& f = scenes[s].frame;
+            const uint32_t dimMax = std::max(f.dim[0], std::max(f.dim[1], f.dim[2]));
+            uint32_t axisBits = 1;
+            if (dimMax >= 2) { axisBits = 0; for (uint32_t v = dimMax - 1; v; v >>= 1) { axisBits++; } }
+            f.keyBase = static_cast<uint32_t>(key);
+            f.emptyKey = static_cast<uint32_t>(key + (1ull << (3 * axisBits)));
+            f.deadKey = f.emptyKey + 1;
+            f.cellOff = static_cast<uint32_t>(cells);
+            key0[s] = f.keyBase;
+            key += (1ull << (3 * axisBits)) + 2;
+            cells += static_cast<uint64_t>(f.dim[0]) * f.dim[1] * f.dim[2];
+            if (key > 0xffffffffull || cells > 0xffffffffull)
+            {
+                throw Error{ TSP_ERR_CAPACITY, "the scenes' cell frames exceed the 32-bit key space of one context" };
+            }
+            frames[s] = f;
+        }
+        key0[S] = static_cast<uint32_t>(key);
+        keyBits = 1;
+        for (uint64_t v = key > 0 ? key - 1 : 0; v >> 1; v >>= 1) { keyBits++; }
+        h2d(d.frames, frames.data(), sizeof(Frame) * S);
+        h2d(d.sceneKey0, key0.data(), sizeof(uint32_t) * key0.size());
+        if (cells > cellCap || frameAllocs.empty())
+        {
+            releaseList(frameAllocs);
+            cellCap = std::max<uint64_t>(cells, 1);
+            d.cellF = alloc<uint2>(cellCap, frameAllocs);
+            d.cellB = alloc<uint2>(cellCap, frameAllocs);
+            d.cellR = alloc<uint2>(cellCap, frameAllocs);
+        }
+        launchClearAll(stream, d.cellF, cells);
+        launchClearAll(stream, d.cellB, cells);
+        launchClearAll(stream, d.cellR, cells);
         cellsClean = true;
 
         // boundary: re-sorted only if its ids are not already non-decreasing (neighbor_search.cpp:1755-1811)
         if (d.nB)
         {
-            launchResetSearch(stream, d.status);
-            launchKeysSorted(stream, d.bpos, d.nB, d.frame, d.status, d.keyF, d.idxF);
+            launchResetSearch(stream, d);
+            launchKeysSorted(stream, d, d.bpos, d.nB, d.keyF, d.idxF);
             uint32_t unsorted = 0;
             hipCheck(hipMemcpyAsync(&unsorted, &d.status->bUnsorted, 4, hipMemcpyDeviceToHost, stream), "flag D2H");
             sync();
@@ -376,39 +846,39 @@ struct tsp_ctx
                 std::swap(d.bvol, bvol2);
                 std::swap(idB, idB2);
             }
-            launchBuildCells(stream, d.keyF, d.nB, d.frame, d.cellB);
+            launchBuildCells(stream, d, d.keyF, d.nB, d.cellB);
             sync();
         }
-        frameValid = true;
-        frameBuilds++;
-        flags |= TSP_FLAG_FRAME_REBUILT;
+        layoutDirty = false;
     }
 
-    // --- re-order, construct and query (DataReOrder + Construct + Query_Count/Fill) ---
-    void neighborSearch(bool allFields, const Consts& k)
+    // --- re-order, construct and query (DataReOrder + Construct + Query_Count/Fill), all scenes ---
+    void neighborSearch(bool allFields)
     {
-        if (!frameValid) { buildFrame(); }
+        bool rebuild = layoutDirty;
+        for (const SceneHost& h : scenes) { rebuild = rebuild || !h.frameValid; }
+        if (rebuild) { layoutFrames(); }
         for (int attempt = 0; attempt < 4; attempt++)
         {
             {
                 Scope sc(prof, stream, "reset");
-                launchResetSearch(stream, d.status);
+                launchResetSearch(stream, d);
                 // the runs of the previous sorted keys are the only non-empty cells
                 if (!cellsClean)
                 {
-                    launchClearCells(stream, d.keyF, d.nF, d.frame, d.cellF);
-                    launchClearCells(stream, d.keyR, d.nR, d.frame, d.cellR);
+                    launchClearCells(stream, d, d.keyF, d.nF, d.cellF);
+                    launchClearCells(stream, d, d.keyR, d.nR, d.cellR);
                 }
             }
             if (d.nR)
             {
                 Scope sc(prof, stream, "keys_sort_rigid");
-                launchKeys(stream, d.rpos, d.nR, d.frame, d.status, true, d.keyR, d.idxR);
+                launchKeys(stream, d, kPopR, d.rpos, d.nR, true, d.keyR, d.idxR);
                 launchRadixSort(stream, d.keyR, d.idxR, d.keyR2, d.idxR2, d.nR, keyBits, radixHist, d.scanTmp);
             }
             {
                 Scope sc(prof, stream, "keys_sort");
-                launchKeys(stream, d.pos, d.nF, d.frame, d.status, true, d.keyF, d.idxF);
+                launchKeys(stream, d, kPopF, d.pos, d.nF, true, d.keyF, d.idxF);
                 launchRadixSort(stream, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nF, keyBits, radixHist, d.scanTmp);
             }
             // the step re-orders positions, velocities (with density) and pressure (pos.w, the
@@ -448,14 +918,14 @@ struct tsp_ctx
             swapFields();
             {
                 Scope sc(prof, stream, "cells");
-                launchBuildCells(stream, d.keyF, d.nF, d.frame, d.cellF);
-                launchBuildCells(stream, d.keyR, d.nR, d.frame, d.cellR);
+                launchBuildCells(stream, d, d.keyF, d.nF, d.cellF);
+                launchBuildCells(stream, d, d.keyR, d.nR, d.cellR);
                 cellsClean = false;
             }
             {
                 Scope sc(prof, stream, "query_count");
-                launchQueryCount(stream, d, params.kernel_radius_sq, static_cast<int>(reach));
-                launchQueryRigid(stream, d, params.kernel_radius_sq, static_cast<int>(reach), false, d.scanTmp);
+                launchQueryCount(stream, d);
+                launchQueryRigid(stream, d, false, d.scanTmp);
             }
             {
                 Scope sc(prof, stream, "scan");
@@ -464,10 +934,14 @@ struct tsp_ctx
             readStatus();
             if (last.abort)
             {
-                // a particle left the inner frame: undo the permutation and rebuild the frame first
+                // a particle left its scene's inner frame: undo the permutation, rebuild that
+                // scene's frame first
                 swapFields();
-                frameValid = false;
-                buildFrame();
+                for (uint32_t s = 0; s < nScenes(); s++)
+                {
+                    if (hostSolve[s].abort) { scenes[s].frameValid = false; }
+                }
+                layoutFrames();
                 continue;
             }
             if (last.overflow & 2u) { throw Error{ TSP_ERR_CAPACITY, "neighbor count exceeds the u16 count storage" }; }
@@ -481,57 +955,66 @@ struct tsp_ctx
                     if (cap[r] > 0xffffffffull) { throw Error{ TSP_ERR_CAPACITY, "SELL pool exceeds 2^32 u16 units" }; }
                 }
                 allocPools(cap, rfCap);
-                flags |= TSP_FLAG_LIST_REGROWN;
+                for (SceneHost& h : scenes) { h.flags |= TSP_FLAG_LIST_REGROWN; }
             }
             {
                 Scope sc(prof, stream, "query_fill");
-                launchQueryFill(stream, d, params.kernel_radius_sq, static_cast<int>(reach));
-                launchQueryRigid(stream, d, params.kernel_radius_sq, static_cast<int>(reach), true, d.scanTmp);
+                launchQueryFill(stream, d);
+                launchQueryRigid(stream, d, true, d.scanTmp);
             }
             {
                 Scope sc(prof, stream, "halo");
                 launchHaloBuild(stream, d);
             }
-            if (last.rbTotal + last.rrTotal > 0) { flags |= TSP_FLAG_RIGID_CONTACT; }
-            (void)k;
+            if (last.rbTotal + last.rrTotal > 0)
+            {
+                for (SceneHost& h : scenes) { if (h.nR) { h.flags |= TSP_FLAG_RIGID_CONTACT; } }
+            }
             return;
         }
         throw Error{ TSP_ERR_STATE, "cell frame rebuild did not converge" };
     }
 
-    // --- one IISPH step (Step_IISPH, sph_solver.cpp:34-319) ---
+    // --- one IISPH step of every scene (Step_IISPH, sph_solver.cpp:34-319) ---
     void stepOnce()
     {
-        const Consts k = consts();
-        neighborSearch(false, k);
+        neighborSearch(false);
         {
             Scope sc(prof, stream, "density");
-            launchDensity(stream, d, k);
+            launchDensity(stream, d);
         }
-        if (d.nRB) { launchRigidClearGravity(stream, d, k); }
+        if (d.nRB) { launchRigidClearGravity(stream, d); }
         {
             Scope sc(prof, stream, "nonpressure");
-            launchNonPressure(stream, d, k);
+            launchNonPressure(stream, d);
         }
         if (d.nRB)
         {
             Scope sc(prof, stream, "rigid");
-            launchRigidForces(stream, d, k, false);
+            launchRigidForces(stream, d, false);
             launchRigidReduce(stream, d);
-            launchRigidPredict(stream, d, k);
+            launchRigidPredict(stream, d);
             launchRigidUpdate(stream, d, true);
         }
         {
             Scope sc(prof, stream, "source_diag");
-            launchSourceDiag(stream, d, k);
+            launchSourceDiag(stream, d);
         }
-        // relaxed Jacobi sweeps; each group is followed by one status read, sweeps queued past
-        // convergence return at once
+        // relaxed Jacobi sweeps of all scenes; each group is followed by one read of the solver
+        // states; sweeps queued past a scene's convergence return at once for that scene
+        uint32_t maxIters = 1, lastMax = 0;
+        bool anyFluid = false;
+        for (const SceneHost& h : scenes)
+        {
+            if (!h.nF) { continue; }
+            anyFluid = true;
+            maxIters = std::max(maxIters, h.params.max_solver_iters);
+            lastMax = std::max(lastMax, h.lastSweeps);
+        }
+        const uint32_t maxSweeps = maxIters > 1 ? maxIters - 1 : 1;
+        uint32_t group = std::max<uint32_t>(1u, lastMax > 1 ? lastMax - 1 : 1u);
         uint32_t queued = 0;
-        const uint32_t maxSweeps = params.max_solver_iters > 1 ? params.max_solver_iters - 1 : 1;
-        uint32_t group = std::max<uint32_t>(1u, lastSweeps > 1 ? lastSweeps - 1 : 1u);
-        if (d.nF == 0) { group = 0; }
-        while (d.nF > 0)
+        while (anyFluid)
         {
             const uint32_t n = std::min(group, maxSweeps - queued);
             if (n == 0) { throw Error{ TSP_ERR_STATE, "pressure solve did not stop at max_solver_iters" }; }
@@ -539,51 +1022,245 @@ struct tsp_ctx
             {
                 {
                     Scope sc(prof, stream, "pressure_acc");
-                    launchPressureAcc(stream, d, k, true);
+                    launchPressureAcc(stream, d, true);
                 }
                 Scope sc(prof, stream, "jacobi");
-                launchJacobi(stream, d, k);
+                launchJacobi(stream, d);
             }
             {
                 Scope sc(prof, stream, "finalize");
-                launchFinalize(stream, d, k);
+                launchFinalize(stream, d);
             }
             queued += n;
             readStatus();
-            if (last.converged) { break; }
+            bool all = true;
+            for (uint32_t s = 0; s < nScenes(); s++) { all = all && (scenes[s].nF == 0 || hostSolve[s].converged); }
+            if (all) { break; }
             group = 1;
         }
-        lastSweeps = d.nF ? last.sweeps : 0;
-        sweepsTotal += lastSweeps;
+        if (!anyFluid) { readStatus(); }
         {
             Scope sc(prof, stream, "pressure_acc_final");
-            launchPressureAcc(stream, d, k, false);
+            launchPressureAcc(stream, d, false);
         }
         if (d.nRB)
         {
             Scope sc(prof, stream, "rigid");
-            launchRigidForces(stream, d, k, true);
-            launchRigidClearGravity(stream, d, k);
+            launchRigidForces(stream, d, true);
+            launchRigidClearGravity(stream, d);
             launchRigidReduce(stream, d);
         }
         {
             Scope sc(prof, stream, "integration");
-            launchIntegration(stream, d, k);
+            launchIntegration(stream, d);
         }
         if (d.nRB)
         {
             Scope sc(prof, stream, "rigid");
-            launchRigidPredict(stream, d, k);
-            launchRigidIntegrate(stream, d, k);
+            launchRigidPredict(stream, d);
+            launchRigidIntegrate(stream, d);
             launchRigidUpdate(stream, d, false);
         }
-        if (d.nF && params.termination_cond != 0 && last.iters >= params.max_solver_iters)
+        for (uint32_t s = 0; s < nScenes(); s++)
         {
-            const float e = params.termination_cond == 1 ? last.avgRaw : params.termination_cond == 2 ? last.avgPos : last.avgAbs;
-            if (!(e <= params.density_dev_target)) { flags |= TSP_FLAG_MAX_ITERS; }
+            SceneHost& h = scenes[s];
+            const Solve& so = hostSolve[s];
+            h.last = so;
+            h.lastSweeps = h.nF ? so.sweeps : 0;
+            h.sweepsTotal += h.lastSweeps;
+            const tsp_params& p = h.params;
+            if (h.nF && p.termination_cond != 0 && so.iters >= p.max_solver_iters)
+            {
+                const float e = p.termination_cond == 1 ? so.avgRaw : p.termination_cond == 2 ? so.avgPos : so.avgAbs;
+                if (!(e <= p.density_dev_target)) { h.flags |= TSP_FLAG_MAX_ITERS; }
+            }
+            h.step++;
+            h.time += static_cast<double>(p.time_step);
         }
-        step++;
-        time += static_cast<double>(params.time_step);
+    }
+
+    void steps(uint32_t n)
+    {
+        for (SceneHost& h : scenes) { h.flags = 0; }
+        bool work = false;
+        for (const SceneHost& h : scenes) { work = work || h.nF > 0 || h.nRB > 0; }
+        for (uint32_t s = 0; s < n; s++)
+        {
+            if (!work)
+            {
+                for (SceneHost& h : scenes) { h.step++; h.time += static_cast<double>(h.params.time_step); }
+                continue;
+            }
+            stepOnce();
+        }
+        readStatus();
+        for (uint32_t s = 0; s < nScenes(); s++)
+        {
+            if (hostSolve[s].rigidDegenerate) { throw Error{ TSP_ERR_NUMERIC, "Degenerate rigid body orientation." }; }
+        }
+    }
+
+    // --- host copies of one scene ---
+    void pushScene(uint32_t si, tsp_state_view* s)
+    {
+        const SceneHost& h = scenes[si];
+        if (s->n_fluid != h.nF || s->n_boundary != h.nB || s->n_rigid != h.nR || s->n_bodies != h.nRB)
+        {
+            throw Error{ TSP_ERR_INVALID, "state view counts differ from the backend" };
+        }
+        const size_t nRB1 = std::max<uint32_t>(d.nRB, 1);
+        sync();
+        const auto d2h = [&](void* dst, const void* src, size_t bytes) {
+            if (dst && bytes) { hipCheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream), "D2H"); }
+        };
+        // fluid records unpacked into the SoA fields of the view
+        {
+            const uint32_t n = h.nF;
+            std::vector<float4> q[4];
+            const float4* srcs[4] = { d.pos, d.vel, d.vp, d.ap };
+            for (int r = 0; r < 4; r++)
+            {
+                q[r].resize(n);
+                d2h(q[r].data(), srcs[r] + h.offF, sizeof(float4) * n);
+            }
+            sync();
+            float* const* f = s->fluid;
+            const int comp[4][4] = { { TSP_F_X, TSP_F_Y, TSP_F_Z, TSP_F_PRESSURE }, { TSP_F_VX, TSP_F_VY, TSP_F_VZ, TSP_F_DENSITY },
+                                     { TSP_F_VPX, TSP_F_VPY, TSP_F_VPZ, -1 }, { TSP_F_APX, TSP_F_APY, TSP_F_APZ, -1 } };
+            for (int r = 0; r < 4; r++)
+            {
+                for (int c = 0; c < 4; c++)
+                {
+                    float* out = comp[r][c] >= 0 ? f[comp[r][c]] : nullptr;
+                    if (!out) { continue; }
+                    for (uint32_t i = 0; i < n; i++) { out[i] = reinterpret_cast<const float*>(&q[r][i])[c]; }
+                }
+            }
+            d2h(f[TSP_F_SOURCE], d.source + h.offF, sizeof(float) * n);
+            d2h(f[TSP_F_DIAG], d.diag + h.offF, sizeof(float) * n);
+        }
+        d2h(s->fluid_id, d.id + h.offF, sizeof(uint64_t) * h.nF);
+        {
+            const uint32_t n = h.nB;
+            std::vector<float4> q(n);
+            d2h(q.data(), d.bpos + h.offB, sizeof(float4) * n);
+            sync();
+            float* const* f = s->boundary;
+            const int comp[4] = { TSP_B_X, TSP_B_Y, TSP_B_Z, TSP_B_PSI };
+            for (int c = 0; c < 4; c++)
+            {
+                if (!f[comp[c]]) { continue; }
+                for (uint32_t i = 0; i < n; i++) { f[comp[c]][i] = reinterpret_cast<const float*>(&q[i])[c]; }
+            }
+            d2h(f[TSP_B_VOL], d.bvol + h.offB, sizeof(float) * n);
+        }
+        d2h(s->boundary_id, idB + h.offB, sizeof(uint64_t) * h.nB);
+        {
+            const uint32_t n = h.nR;
+            float* const* f = s->rigid;
+            d2h(f[TSP_R_XREL], d.rxrel + h.offR, sizeof(float) * n);
+            d2h(f[TSP_R_YREL], d.ryrel + h.offR, sizeof(float) * n);
+            d2h(f[TSP_R_ZREL], d.rzrel + h.offR, sizeof(float) * n);
+            std::vector<float4> wp(n), wv(n);
+            d2h(wp.data(), d.rpos + h.offR, sizeof(float4) * n);
+            d2h(wv.data(), d.rvel + h.offR, sizeof(float4) * n);
+            sync();
+            for (uint32_t i = 0; i < n; i++)
+            {
+                if (f[TSP_R_PSI]) { f[TSP_R_PSI][i] = wp[i].w; }
+                if (f[TSP_R_VOL]) { f[TSP_R_VOL][i] = wv[i].w; }
+                if (s->rigid_body) { s->rigid_body[i] = h.rbodyLocal[i]; }
+            }
+        }
+        d2h(s->rigid_id, idR + h.offR, sizeof(uint64_t) * h.nR);
+        for (size_t f = 0; f < TSP_RB_COUNT; f++) { d2h(s->body[f], d.rb + f * nRB1 + h.offRB, sizeof(float) * h.nRB); }
+        sync();
+        for (uint32_t b = 0; b < h.nRB; b++)
+        {
+            if (s->body_id) { s->body_id[b] = h.bodyId[b]; }
+            if (s->body_count) { s->body_count[b] = h.bodyCount[b]; }
+        }
+        s->step = h.step;
+        s->time = h.time;
+    }
+
+    void sceneInfo(uint32_t si, tsp_step_info* info)
+    {
+        std::memset(info, 0, sizeof(*info));
+        const SceneHost& h = scenes[si];
+        const Solve& s = h.last;
+        info->step = h.step;
+        info->time = h.time;
+        info->solver_iters = h.nF ? s.iters : 0;
+        info->flags = h.flags;
+        info->avg_err_raw = s.avgRaw; info->avg_err_pos = s.avgPos; info->avg_err_abs = s.avgAbs;
+        info->err_sum_raw = s.sumRaw; info->err_sum_pos = s.sumPos; info->err_sum_abs = s.sumAbs;
+        info->sweeps_total = h.sweepsTotal;
+        info->frame_builds = h.frameBuilds;
+        for (int a = 0; a < 3; a++) { info->frame_orig[a] = h.frame.orig[a]; info->frame_dim[a] = h.frame.dim[a]; }
+        // list totals count every scene of the context
+        info->n_ff = last.entries[0]; info->n_fb = last.entries[1]; info->n_fr = last.entries[2];
+        info->n_rf = last.rfTotal; info->n_rb = last.rbTotal; info->n_rr = last.rrTotal;
+        info->list_bytes = 2ull * (last.poolUnits[0] + last.poolUnits[1] + last.poolUnits[2]);
+        if (h.nRB)
+        {
+            sync();
+            const size_t nRB1 = std::max<uint32_t>(d.nRB, 1);
+            for (size_t f = 0; f < 6; f++)
+            {
+                hipCheck(hipMemcpy(&info->rigid_force_torque[f], d.rbForce + f * nRB1 + h.offRB, sizeof(float), hipMemcpyDeviceToHost),
+                         "D2H");
+            }
+        }
+    }
+
+    // one relayout: the listed scenes leave, new scenes join at the end
+    void update(uint32_t nRemove, const uint32_t* removeIds, uint32_t nAdd, const tsp_params* params,
+                const tsp_state_view* const* views, const float* domains, uint32_t* outIds)
+    {
+        if ((nRemove && !removeIds) || (nAdd && (!params || !views))) { throw Error{ TSP_ERR_INVALID, "null scene arrays" }; }
+        std::vector<bool> drop(nScenes(), false);
+        for (uint32_t q = 0; q < nRemove; q++)
+        {
+            const int at = indexOf(removeIds[q]);
+            if (at < 0) { throw Error{ TSP_ERR_INVALID, "unknown scene id" }; }
+            drop[static_cast<size_t>(at)] = true;
+        }
+        std::vector<Source> src;
+        for (uint32_t q = 0; q < nScenes(); q++)
+        {
+            if (drop[q]) { continue; }
+            Source x;
+            x.keep = static_cast<int>(q);
+            src.push_back(std::move(x));
+        }
+        for (uint32_t q = 0; q < nAdd; q++)
+        {
+            if (!views[q]) { throw Error{ TSP_ERR_INVALID, "null state" }; }
+            Source x;
+            x.view = views[q];
+            x.host = newScene(params[q], views[q], domains ? domains + 6 * q : nullptr, domains ? domains + 6 * q + 3 : nullptr);
+            src.push_back(std::move(x));
+        }
+        std::vector<uint32_t> ids;
+        for (Source& x : src)
+        {
+            if (x.keep < 0) { x.host.id = nextId++; ids.push_back(x.host.id); }
+        }
+        relayout(src);
+        if (outIds) { for (uint32_t q = 0; q < nAdd; q++) { outIds[q] = ids[q]; } }
+    }
+
+    uint32_t sceneAt(uint32_t id) const
+    {
+        const int at = indexOf(id);
+        if (at < 0) { throw Error{ TSP_ERR_INVALID, "unknown scene id" }; }
+        return static_cast<uint32_t>(at);
+    }
+
+    void requireScene() const
+    {
+        if (scenes.empty()) { throw Error{ TSP_ERR_STATE, "no state" }; }
     }
 };
 
@@ -608,16 +1285,6 @@ namespace
         {
             ctx->err = e.what();
             return TSP_ERR_HIP;
-        }
-    }
-
-    void validateParams(const tsp_params& p)
-    {
-        if (!(p.particle_size > 0.0f) || !(p.kernel_radius > 0.0f) || !(p.kernel_radius_sq > 0.0f)
-            || !(p.rest_density > 0.0f) || !(p.time_step > 0.0f) || !(p.grid_cell_size_ratio > 0.0f)
-            || p.max_solver_iters < 1)
-        {
-            throw Error{ TSP_ERR_INVALID, "invalid solver parameters" };
         }
     }
 }
@@ -660,11 +1327,22 @@ int tsp_set_params(tsp_ctx* ctx, const tsp_params* p)
     return guarded(ctx, [&]() {
         if (!p) { throw Error{ TSP_ERR_INVALID, "null params" }; }
         validateParams(*p);
-        const bool frameChanged = ctx->paramsSet && (p->kernel_radius != ctx->params.kernel_radius
-                                                     || p->grid_cell_size_ratio != ctx->params.grid_cell_size_ratio);
-        ctx->params = *p;
+        ctx->defaults = *p;
         ctx->paramsSet = true;
-        if (frameChanged) { ctx->frameValid = false; }
+        // every resident scene takes the parameters; a new kernel radius or cell ratio needs a
+        // new cell frame
+        bool changed = false;
+        for (SceneHost& h : ctx->scenes)
+        {
+            if (std::memcmp(&h.params, p, sizeof(tsp_params)) == 0) { continue; }
+            if (p->kernel_radius != h.params.kernel_radius || p->grid_cell_size_ratio != h.params.grid_cell_size_ratio)
+            {
+                h.frameValid = false;
+            }
+            h.params = *p;
+            changed = true;
+        }
+        if (changed) { ctx->uploadConsts(); }
         return TSP_OK;
     });
 }
@@ -675,15 +1353,21 @@ int tsp_set_domain(tsp_ctx* ctx, const float mn[3], const float mx[3])
         if (!mn || !mx) { throw Error{ TSP_ERR_INVALID, "null domain" }; }
         for (int a = 0; a < 3; a++) { ctx->domMin[a] = mn[a]; ctx->domMax[a] = mx[a]; }
         ctx->domainSet = true;
-        // the hysteresis test reads the domain of every step (IsInsideSimDomain); the frame
-        // extent itself changes only at the next rebuild
-        if (ctx->frameValid)
+        // the hysteresis test reads the domain of every step (IsInsideSimDomain); a frame's
+        // extent changes only at its next rebuild
+        for (SceneHost& h : ctx->scenes)
         {
-            Frame& f = ctx->frame;
+            for (int a = 0; a < 3; a++) { h.domMin[a] = mn[a]; h.domMax[a] = mx[a]; }
+            h.domainSet = true;
+            Frame& f = h.frame;
             f.domainSet = (mx[0] > mn[0] && mx[1] > mn[1] && mx[2] > mn[2]) ? 1 : 0;
             for (int a = 0; a < 3; a++) { f.domMin[a] = mn[a]; f.domMax[a] = mx[a]; }
-            hipCheck(hipMemcpyAsync(ctx->d.frame, &f, sizeof(Frame), hipMemcpyHostToDevice, ctx->stream), "frame H2D");
-            ctx->sync();
+        }
+        if (!ctx->scenes.empty() && !ctx->layoutDirty)
+        {
+            std::vector<Frame> frames;
+            for (const SceneHost& h : ctx->scenes) { frames.push_back(h.frame); }
+            ctx->h2d(ctx->d.frames, frames.data(), sizeof(Frame) * frames.size());
         }
         return TSP_OK;
     });
@@ -693,223 +1377,13 @@ int tsp_pull(tsp_ctx* ctx, const tsp_state_view* s)
 {
     return guarded(ctx, [&]() {
         if (!s) { throw Error{ TSP_ERR_INVALID, "null state" }; }
-        if (s->n_fluid > 0x7fffffffu || s->n_boundary > 0x7fffffffu || s->n_rigid > 0x7fffffffu)
-        {
-            throw Error{ TSP_ERR_CAPACITY, "particle count exceeds the local particle limit" };
-        }
-        if (s->n_rigid && !s->rigid_body) { throw Error{ TSP_ERR_INVALID, "rigid particles need body ids" }; }
-        for (uint32_t i = 0; i < s->n_rigid; i++)
-        {
-            if (s->rigid_body[i] >= s->n_bodies) { throw Error{ TSP_ERR_INVALID, "Invalid rigid body index." }; }
-        }
-        ctx->sync();
-        ctx->releaseAll();
-        ctx->d = Dev{};
-        Dev& d = ctx->d;
-        d.nF = s->n_fluid; d.nB = s->n_boundary; d.nR = s->n_rigid; d.nRB = s->n_bodies;
-        auto& A = ctx->allocs;
-        hipStream_t st = ctx->stream;
-        const auto h2d = [&](void* dst, const void* src, size_t bytes) {
-            if (bytes) { hipCheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), "H2D"); }
-            ctx->sync();
-        };
-        const auto field = [](const float* src, uint32_t i, float dflt) { return src ? src[i] : dflt; };
-        // fluid: the SoA fields of the view packed into the float4 records
-        {
-            const uint32_t n = d.nF;
-            std::vector<float4> q[4];
-            for (auto& v : q) { v.resize(n); }
-            std::vector<float> src1(n), diag1(n);
-            float* const* f = s->fluid;
-            for (uint32_t i = 0; i < n; i++)
-            {
-                q[0][i] = make_float4(field(f[TSP_F_X], i, 0), field(f[TSP_F_Y], i, 0), field(f[TSP_F_Z], i, 0), field(f[TSP_F_PRESSURE], i, 0));
-                q[1][i] = make_float4(field(f[TSP_F_VX], i, 0), field(f[TSP_F_VY], i, 0), field(f[TSP_F_VZ], i, 0), field(f[TSP_F_DENSITY], i, 0));
-                q[2][i] = make_float4(field(f[TSP_F_VPX], i, 0), field(f[TSP_F_VPY], i, 0), field(f[TSP_F_VPZ], i, 0), 0.0f);
-                q[3][i] = make_float4(field(f[TSP_F_APX], i, 0), field(f[TSP_F_APY], i, 0), field(f[TSP_F_APZ], i, 0), 0.0f);
-                src1[i] = field(f[TSP_F_SOURCE], i, 0);
-                diag1[i] = field(f[TSP_F_DIAG], i, 0);
-            }
-            float4** dst[4] = { &d.pos, &d.vel, &d.vp, &d.ap };
-            float4** dst2[4] = { &ctx->pos2, &ctx->vel2, &ctx->vp2, &ctx->ap2 };
-            for (int r = 0; r < 4; r++)
-            {
-                *dst[r] = ctx->alloc<float4>(n, A);
-                *dst2[r] = ctx->alloc<float4>(n, A);
-                h2d(*dst[r], q[r].data(), sizeof(float4) * n);
-            }
-            d.source = ctx->alloc<float>(n, A);
-            d.diag = ctx->alloc<float>(n, A);
-            ctx->source2 = ctx->alloc<float>(n, A);
-            ctx->diag2 = ctx->alloc<float>(n, A);
-            h2d(d.source, src1.data(), sizeof(float) * n);
-            h2d(d.diag, diag1.data(), sizeof(float) * n);
-        }
-        std::vector<uint64_t> seq;
-        const auto ids = [&](const uint64_t* src, uint32_t n) -> const uint64_t* {
-            if (src) { return src; }
-            seq.resize(n);
-            for (uint32_t i = 0; i < n; i++) { seq[i] = i; }
-            return seq.data();
-        };
-        d.id = ctx->alloc<uint64_t>(d.nF, A);
-        ctx->idF2 = ctx->alloc<uint64_t>(d.nF, A);
-        h2d(d.id, ids(s->fluid_id, d.nF), sizeof(uint64_t) * d.nF);
-        // boundary (ensureBoundaryCoefficientsPresent defaults: psi = massB, V = psi / rho0)
-        {
-            const uint32_t n = d.nB;
-            const float h = ctx->params.particle_size, rho0 = ctx->params.rest_density;
-            const float massB = rho0 * h * h * h;
-            const float volB = rho0 > 0.0f ? massB / rho0 : 0.0f;
-            std::vector<float4> q(n);
-            std::vector<float> vol(n);
-            float* const* f = s->boundary;
-            for (uint32_t i = 0; i < n; i++)
-            {
-                q[i] = make_float4(field(f[TSP_B_X], i, 0), field(f[TSP_B_Y], i, 0), field(f[TSP_B_Z], i, 0), field(f[TSP_B_PSI], i, massB));
-                vol[i] = field(f[TSP_B_VOL], i, volB);
-            }
-            d.bpos = ctx->alloc<float4>(n, A);
-            ctx->bpos2 = ctx->alloc<float4>(n, A);
-            d.bvol = ctx->alloc<float>(n, A);
-            ctx->bvol2 = ctx->alloc<float>(n, A);
-            h2d(d.bpos, q.data(), sizeof(float4) * n);
-            h2d(d.bvol, vol.data(), sizeof(float) * n);
-        }
-        ctx->idB = ctx->alloc<uint64_t>(d.nB, A);
-        ctx->idB2 = ctx->alloc<uint64_t>(d.nB, A);
-        h2d(ctx->idB, ids(s->boundary_id, d.nB), sizeof(uint64_t) * d.nB);
-        // rigid particles: body-frame offsets; world records are written from the body state below
-        {
-            const uint32_t n = d.nR;
-            float* const* f = s->rigid;
-            std::vector<float> rel[3];
-            std::vector<float4> wp(n), wv(n), wvp(n);
-            for (int a2 = 0; a2 < 3; a2++) { rel[a2].resize(n); }
-            for (uint32_t i = 0; i < n; i++)
-            {
-                rel[0][i] = field(f[TSP_R_XREL], i, 0);
-                rel[1][i] = field(f[TSP_R_YREL], i, 0);
-                rel[2][i] = field(f[TSP_R_ZREL], i, 0);
-                wp[i] = make_float4(0.0f, 0.0f, 0.0f, field(f[TSP_R_PSI], i, 0));
-                wv[i] = make_float4(0.0f, 0.0f, 0.0f, field(f[TSP_R_VOL], i, 0));
-                wvp[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            }
-            float** rr[3] = { &d.rxrel, &d.ryrel, &d.rzrel };
-            for (int a2 = 0; a2 < 3; a2++)
-            {
-                *rr[a2] = ctx->alloc<float>(n, A);
-                h2d(*rr[a2], rel[a2].data(), sizeof(float) * n);
-            }
-            d.rpos = ctx->alloc<float4>(n, A);
-            d.rvel = ctx->alloc<float4>(n, A);
-            d.rvp = ctx->alloc<float4>(n, A);
-            h2d(d.rpos, wp.data(), sizeof(float4) * n);
-            h2d(d.rvel, wv.data(), sizeof(float4) * n);
-            h2d(d.rvp, wvp.data(), sizeof(float4) * n);
-        }
-        d.rbody = ctx->alloc<uint32_t>(d.nR, A);
-        h2d(d.rbody, s->rigid_body, sizeof(uint32_t) * d.nR);
-        ctx->idR = ctx->alloc<uint64_t>(d.nR, A);
-        h2d(ctx->idR, ids(s->rigid_id, d.nR), sizeof(uint64_t) * d.nR);
-        d.rfx = ctx->alloc<float>(d.nR, A);
-        d.rfy = ctx->alloc<float>(d.nR, A);
-        d.rfz = ctx->alloc<float>(d.nR, A);
-        // bodies
-        d.rb = ctx->alloc<float>(static_cast<size_t>(TSP_RB_COUNT) * d.nRB, A);
-        d.rbPred = ctx->alloc<float>(9ull * std::max<uint32_t>(d.nRB, 1), A);
-        d.rbForce = ctx->alloc<float>(6ull * std::max<uint32_t>(d.nRB, 1), A);
-        {
-            std::vector<float> zeros(9ull * std::max<uint32_t>(d.nRB, 1), 0.0f);
-            h2d(d.rbForce, zeros.data(), sizeof(float) * 6 * std::max<uint32_t>(d.nRB, 1));
-            h2d(d.rbPred, zeros.data(), sizeof(float) * 9 * std::max<uint32_t>(d.nRB, 1));
-        }
-        for (int f = 0; f < TSP_RB_COUNT; f++)
-        {
-            if (s->body[f]) { h2d(d.rb + static_cast<size_t>(f) * d.nRB, s->body[f], sizeof(float) * d.nRB); }
-            else
-            {
-                std::vector<float> v(d.nRB, (f == TSP_RB_A00 || f == TSP_RB_A11 || f == TSP_RB_A22) ? 1.0f : 0.0f);
-                h2d(d.rb + static_cast<size_t>(f) * d.nRB, v.data(), sizeof(float) * d.nRB);
-            }
-        }
-        ctx->hostIdRB.assign(d.nRB, 0);
-        ctx->hostCountRB.assign(d.nRB, 0);
-        for (uint32_t b = 0; b < d.nRB; b++)
-        {
-            ctx->hostIdRB[b] = s->body_id ? s->body_id[b] : b;
-            ctx->hostCountRB[b] = s->body_count ? s->body_count[b] : 0;
-        }
-        // BuildRigidParticleIndicesByBody (sync/state_sim.h:609-631)
-        {
-            std::vector<uint32_t> off(d.nRB + 1, 0), idx(d.nR);
-            for (uint32_t i = 0; i < d.nR; i++) { off[s->rigid_body[i] + 1]++; }
-            for (uint32_t b = 0; b < d.nRB; b++) { off[b + 1] += off[b]; }
-            std::vector<uint32_t> w(off);
-            for (uint32_t i = 0; i < d.nR; i++) { idx[w[s->rigid_body[i]]++] = i; }
-            d.byBodyOff = ctx->alloc<uint32_t>(d.nRB + 1, A);
-            d.byBodyIdx = ctx->alloc<uint32_t>(d.nR, A);
-            h2d(d.byBodyOff, off.data(), sizeof(uint32_t) * (d.nRB + 1));
-            h2d(d.byBodyIdx, idx.data(), sizeof(uint32_t) * d.nR);
-        }
-        // neighbor search buffers
-        const uint32_t nSort = std::max(d.nF, std::max(d.nB, d.nR));
-        d.keyF = ctx->alloc<uint32_t>(nSort, A);
-        d.keyF2 = ctx->alloc<uint32_t>(nSort, A);
-        d.idxF = ctx->alloc<uint32_t>(nSort, A);
-        d.idxF2 = ctx->alloc<uint32_t>(nSort, A);
-        d.keyR = ctx->alloc<uint32_t>(d.nR, A);
-        d.keyR2 = ctx->alloc<uint32_t>(d.nR, A);
-        d.idxR = ctx->alloc<uint32_t>(d.nR, A);
-        d.idxR2 = ctx->alloc<uint32_t>(d.nR, A);
-        const uint64_t nTiles = (nSort + 2047ull) / 2048ull;
-        ctx->radixHist = ctx->alloc<uint32_t>(256ull * nTiles, A);
-        ctx->aabb = ctx->alloc<uint32_t>(6, A);
-        const uint32_t nSlices = (d.nF + kSlice - 1) / kSlice;
-        for (int r = 0; r < 3; r++)
-        {
-            d.cnt[r] = ctx->alloc<uint16_t>(d.nF, A);
-            d.meta[r] = ctx->alloc<SliceMeta>(nSlices, A);
-        }
-        ctx->sliceUnits = ctx->alloc<uint32_t>(3ull * (nSlices + 1), A);
-        {
-            // block sums of the scans: radix digit offsets, slice units, RF offsets
-            const uint64_t m = std::max<uint64_t>(256ull * nTiles, std::max<uint64_t>(nSlices + 1ull, d.nR + 1ull));
-            d.scanTmp = ctx->alloc<uint32_t>((m + 2047ull) / 2048ull + 1ull, A);
-        }
-        d.ffWideMin = ctx->alloc<uint32_t>(d.nF, A);
-        {
-            const uint64_t nBlocks = (d.nF + kBlock - 1) / kBlock;
-            d.haloIdx = ctx->alloc<uint32_t>(nBlocks * kHaloMax, A);
-            d.haloCount = ctx->alloc<uint32_t>(nBlocks, A);
-            // no block has a halo until the first search builds one
-            hipCheck(hipMemsetAsync(d.haloCount, 0xff, sizeof(uint32_t) * std::max<uint64_t>(nBlocks, 1), st), "memset");
-        }
-        d.rfCnt = ctx->alloc<uint32_t>(d.nR, A);
-        d.rbCnt = ctx->alloc<uint32_t>(d.nR, A);
-        d.rrCnt = ctx->alloc<uint32_t>(d.nR, A);
-        d.rfOff = ctx->alloc<uint32_t>(d.nR + 1ull, A);
-        d.errPartial = ctx->alloc<double>(3ull * ((d.nF + kBlock - 1) / kBlock), A);
-        d.status = ctx->alloc<Status>(1, A);
-        d.frame = ctx->alloc<Frame>(1, A);
-        {
-            const Status zero{};
-            h2d(d.status, &zero, sizeof(Status));
-        }
-        // initial pool sizes: about 40 FF, 16 FB and 8 FR entries per particle, grown on demand
-        const uint64_t cap[3] = { 48ull * d.nF + 1024, 16ull * d.nF + 1024, 8ull * d.nF + 1024 };
-        ctx->allocPools(cap, 64ull * d.nR + 1024);
-        ctx->sync();
-        launchRigidUpdate(st, d, false);
-        ctx->sync();
-        ctx->frameValid = false;
-        ctx->step = s->step;
-        ctx->time = s->time;
-        ctx->sweepsTotal = 0;
-        ctx->lastSweeps = 0;
-        ctx->flags = 0;
-        ctx->last = Status{};
-        ctx->pulled = true;
+        if (!ctx->paramsSet) { throw Error{ TSP_ERR_STATE, "pull before params" }; }
+        // the context holds exactly this one scene afterwards
+        std::vector<uint32_t> all;
+        for (const SceneHost& h : ctx->scenes) { all.push_back(h.id); }
+        float dom[6];
+        for (int a = 0; a < 3; a++) { dom[a] = ctx->domMin[a]; dom[3 + a] = ctx->domMax[a]; }
+        ctx->update(static_cast<uint32_t>(all.size()), all.data(), 1, &ctx->defaults, &s, ctx->domainSet ? dom : nullptr, nullptr);
         return TSP_OK;
     });
 }
@@ -918,85 +1392,8 @@ int tsp_push(tsp_ctx* ctx, tsp_state_view* s)
 {
     return guarded(ctx, [&]() {
         if (!s) { throw Error{ TSP_ERR_INVALID, "null state" }; }
-        if (!ctx->pulled) { throw Error{ TSP_ERR_STATE, "push before pull" }; }
-        const Dev& d = ctx->d;
-        if (s->n_fluid != d.nF || s->n_boundary != d.nB || s->n_rigid != d.nR || s->n_bodies != d.nRB)
-        {
-            throw Error{ TSP_ERR_INVALID, "state view counts differ from the backend" };
-        }
-        hipStream_t st = ctx->stream;
-        const auto d2h = [&](void* dst, const void* src, size_t bytes) {
-            if (dst && bytes) { hipCheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st), "D2H"); }
-        };
-        // fluid records unpacked into the SoA fields of the view
-        {
-            const uint32_t n = d.nF;
-            std::vector<float4> q[4];
-            const float4* srcs[4] = { d.pos, d.vel, d.vp, d.ap };
-            for (int r = 0; r < 4; r++)
-            {
-                q[r].resize(n);
-                d2h(q[r].data(), srcs[r], sizeof(float4) * n);
-            }
-            ctx->sync();
-            float* const* f = s->fluid;
-            const int comp[4][4] = { { TSP_F_X, TSP_F_Y, TSP_F_Z, TSP_F_PRESSURE }, { TSP_F_VX, TSP_F_VY, TSP_F_VZ, TSP_F_DENSITY },
-                                     { TSP_F_VPX, TSP_F_VPY, TSP_F_VPZ, -1 }, { TSP_F_APX, TSP_F_APY, TSP_F_APZ, -1 } };
-            for (int r = 0; r < 4; r++)
-            {
-                for (int c = 0; c < 4; c++)
-                {
-                    float* out = comp[r][c] >= 0 ? f[comp[r][c]] : nullptr;
-                    if (!out) { continue; }
-                    for (uint32_t i = 0; i < n; i++) { out[i] = reinterpret_cast<const float*>(&q[r][i])[c]; }
-                }
-            }
-            d2h(f[TSP_F_SOURCE], d.source, sizeof(float) * n);
-            d2h(f[TSP_F_DIAG], d.diag, sizeof(float) * n);
-        }
-        d2h(s->fluid_id, d.id, sizeof(uint64_t) * d.nF);
-        {
-            const uint32_t n = d.nB;
-            std::vector<float4> q(n);
-            d2h(q.data(), d.bpos, sizeof(float4) * n);
-            ctx->sync();
-            float* const* f = s->boundary;
-            const int comp[4] = { TSP_B_X, TSP_B_Y, TSP_B_Z, TSP_B_PSI };
-            for (int c = 0; c < 4; c++)
-            {
-                if (!f[comp[c]]) { continue; }
-                for (uint32_t i = 0; i < n; i++) { f[comp[c]][i] = reinterpret_cast<const float*>(&q[i])[c]; }
-            }
-            d2h(f[TSP_B_VOL], d.bvol, sizeof(float) * n);
-        }
-        d2h(s->boundary_id, ctx->idB, sizeof(uint64_t) * d.nB);
-        {
-            const uint32_t n = d.nR;
-            float* const* f = s->rigid;
-            d2h(f[TSP_R_XREL], d.rxrel, sizeof(float) * n);
-            d2h(f[TSP_R_YREL], d.ryrel, sizeof(float) * n);
-            d2h(f[TSP_R_ZREL], d.rzrel, sizeof(float) * n);
-            std::vector<float4> wp(n), wv(n);
-            d2h(wp.data(), d.rpos, sizeof(float4) * n);
-            d2h(wv.data(), d.rvel, sizeof(float4) * n);
-            ctx->sync();
-            for (uint32_t i = 0; i < n; i++)
-            {
-                if (f[TSP_R_PSI]) { f[TSP_R_PSI][i] = wp[i].w; }
-                if (f[TSP_R_VOL]) { f[TSP_R_VOL][i] = wv[i].w; }
-            }
-        }
-        d2h(s->rigid_body, d.rbody, sizeof(uint32_t) * d.nR);
-        d2h(s->rigid_id, ctx->idR, sizeof(uint64_t) * d.nR);
-        for (int f = 0; f < TSP_RB_COUNT; f++) { d2h(s->body[f], d.rb + static_cast<size_t>(f) * d.nRB, sizeof(float) * d.nRB); }
-        ctx->sync();
-        for (uint32_t b = 0; b < d.nRB; b++)
-        {
-            if (s->body_id) { s->body_id[b] = ctx->hostIdRB[b]; }
-            if (s->body_count) { s->body_count[b] = ctx->hostCountRB[b]; }
-        }
-        s->step = ctx->step;
-        s->time = ctx->time;
+        if (ctx->scenes.empty()) { throw Error{ TSP_ERR_STATE, "push before pull" }; }
+        ctx->pushScene(0, s);
         return TSP_OK;
     });
 }
@@ -1004,10 +1401,10 @@ int tsp_push(tsp_ctx* ctx, tsp_state_view* s)
 int tsp_neighbor_search(tsp_ctx* ctx)
 {
     return guarded(ctx, [&]() {
-        if (!ctx->pulled || !ctx->paramsSet) { throw Error{ TSP_ERR_STATE, "neighbor search before pull/params" }; }
-        ctx->flags = 0;
+        if (ctx->scenes.empty() || !ctx->paramsSet) { throw Error{ TSP_ERR_STATE, "neighbor search before pull/params" }; }
+        for (SceneHost& h : ctx->scenes) { h.flags = 0; }
         if (ctx->d.nF == 0 && ctx->d.nB == 0 && ctx->d.nR == 0) { return TSP_OK; }
-        ctx->neighborSearch(true, ctx->consts());
+        ctx->neighborSearch(true);
         ctx->sync();
         ctx->prof.resolve();
         return TSP_OK;
@@ -1017,20 +1414,8 @@ int tsp_neighbor_search(tsp_ctx* ctx)
 int tsp_step(tsp_ctx* ctx, uint32_t n)
 {
     return guarded(ctx, [&]() {
-        if (!ctx->pulled || !ctx->paramsSet) { throw Error{ TSP_ERR_STATE, "step before pull/params" }; }
-        ctx->flags = 0;
-        for (uint32_t s = 0; s < n; s++)
-        {
-            if (ctx->d.nF == 0 && ctx->d.nRB == 0)
-            {
-                ctx->step++;
-                ctx->time += static_cast<double>(ctx->params.time_step);
-                continue;
-            }
-            ctx->stepOnce();
-        }
-        ctx->readStatus();
-        if (ctx->last.rigidDegenerate) { throw Error{ TSP_ERR_NUMERIC, "Degenerate rigid body orientation." }; }
+        if (ctx->scenes.empty() || !ctx->paramsSet) { throw Error{ TSP_ERR_STATE, "step before pull/params" }; }
+        ctx->steps(n);
         return TSP_OK;
     });
 }
@@ -1039,30 +1424,8 @@ int tsp_get_step_info(tsp_ctx* ctx, tsp_step_info* info)
 {
     return guarded(ctx, [&]() {
         if (!info) { throw Error{ TSP_ERR_INVALID, "null info" }; }
-        std::memset(info, 0, sizeof(*info));
-        const Status& s = ctx->last;
-        info->step = ctx->step;
-        info->time = ctx->time;
-        info->solver_iters = ctx->d.nF ? s.iters : 0;
-        info->flags = ctx->flags;
-        info->avg_err_raw = s.avgRaw; info->avg_err_pos = s.avgPos; info->avg_err_abs = s.avgAbs;
-        info->err_sum_raw = s.sumRaw; info->err_sum_pos = s.sumPos; info->err_sum_abs = s.sumAbs;
-        info->sweeps_total = ctx->sweepsTotal;
-        info->frame_builds = ctx->frameBuilds;
-        for (int a = 0; a < 3; a++) { info->frame_orig[a] = ctx->frame.orig[a]; info->frame_dim[a] = ctx->frame.dim[a]; }
-        info->n_ff = s.entries[0]; info->n_fb = s.entries[1]; info->n_fr = s.entries[2];
-        info->n_rf = s.rfTotal; info->n_rb = s.rbTotal; info->n_rr = s.rrTotal;
-        info->list_bytes = 2ull * (s.poolUnits[0] + s.poolUnits[1] + s.poolUnits[2]);
-        if (ctx->d.nRB)
-        {
-            float ft[6];
-            for (int f = 0; f < 6; f++)
-            {
-                hipCheck(hipMemcpy(&ft[f], ctx->d.rbForce + static_cast<size_t>(f) * ctx->d.nRB, sizeof(float),
-                                   hipMemcpyDeviceToHost), "D2H");
-                info->rigid_force_torque[f] = ft[f];
-            }
-        }
+        ctx->requireScene();
+        ctx->sceneInfo(0, info);
         return TSP_OK;
     });
 }
@@ -1071,7 +1434,9 @@ int tsp_get_counts(tsp_ctx* ctx, uint32_t counts[4])
 {
     return guarded(ctx, [&]() {
         if (!counts) { throw Error{ TSP_ERR_INVALID, "null counts" }; }
-        counts[0] = ctx->d.nF; counts[1] = ctx->d.nB; counts[2] = ctx->d.nR; counts[3] = ctx->d.nRB;
+        if (ctx->scenes.empty()) { counts[0] = counts[1] = counts[2] = counts[3] = 0; return TSP_OK; }
+        const SceneHost& h = ctx->scenes[0];
+        counts[0] = h.nF; counts[1] = h.nB; counts[2] = h.nR; counts[3] = h.nRB;
         return TSP_OK;
     });
 }
@@ -1080,34 +1445,39 @@ int tsp_get_neighbors(tsp_ctx* ctx, int rel, uint32_t* counts, uint32_t* idx, ui
 {
     return guarded(ctx, [&]() {
         if (rel < 0 || rel > 5) { throw Error{ TSP_ERR_INVALID, "relation must be 0..5" }; }
-        if (!ctx->pulled) { throw Error{ TSP_ERR_STATE, "no state" }; }
+        ctx->requireScene();
         const Dev& d = ctx->d;
+        const SceneHost& h = ctx->scenes[0];
         ctx->sync();
         if (rel >= 3)
         {
             // RF is stored as CSR; RB and RR are only counted (they drive the contact flag)
-            const uint32_t n = d.nR;
+            const uint32_t n = h.nR;
             std::vector<uint32_t> cnt(n);
             const uint32_t* src = rel == 3 ? d.rfCnt : (rel == 4 ? d.rbCnt : d.rrCnt);
-            if (n) { hipCheck(hipMemcpy(cnt.data(), src, sizeof(uint32_t) * n, hipMemcpyDeviceToHost), "D2H"); }
+            if (n) { hipCheck(hipMemcpy(cnt.data(), src + h.offR, sizeof(uint32_t) * n, hipMemcpyDeviceToHost), "D2H"); }
             uint64_t tot = 0;
             for (uint32_t i = 0; i < n; i++) { tot += cnt[i]; if (counts) { counts[i] = cnt[i]; } }
             if (total) { *total = tot; }
             if (rel == 3 && idx && cap >= tot && tot)
             {
-                hipCheck(hipMemcpy(idx, d.rfIdx, sizeof(uint32_t) * tot, hipMemcpyDeviceToHost), "D2H");
+                uint32_t first = 0;
+                hipCheck(hipMemcpy(&first, d.rfOff + h.offR, sizeof(uint32_t), hipMemcpyDeviceToHost), "D2H");
+                hipCheck(hipMemcpy(idx, d.rfIdx + first, sizeof(uint32_t) * tot, hipMemcpyDeviceToHost), "D2H");
+                for (uint64_t q = 0; q < tot; q++) { idx[q] -= h.offF; }
             }
             else if (rel != 3 && idx && tot) { throw Error{ TSP_ERR_INVALID, "RB/RR lists are counted, not stored" }; }
             return TSP_OK;
         }
-        const uint32_t n = d.nF;
-        const uint32_t nSlices = (n + kSlice - 1) / kSlice;
+        // the scene's slices (its fluid segment starts at a multiple of the slice height)
+        const uint32_t n = h.nF;
+        const uint32_t s0 = h.offF / kSlice, nSlices = (n + kSlice - 1) / kSlice;
         std::vector<uint16_t> cnt(n);
         std::vector<SliceMeta> meta(nSlices);
         if (n)
         {
-            hipCheck(hipMemcpy(cnt.data(), d.cnt[rel], sizeof(uint16_t) * n, hipMemcpyDeviceToHost), "D2H");
-            hipCheck(hipMemcpy(meta.data(), d.meta[rel], sizeof(SliceMeta) * nSlices, hipMemcpyDeviceToHost), "D2H");
+            hipCheck(hipMemcpy(cnt.data(), d.cnt[rel] + h.offF, sizeof(uint16_t) * n, hipMemcpyDeviceToHost), "D2H");
+            hipCheck(hipMemcpy(meta.data(), d.meta[rel] + s0, sizeof(SliceMeta) * nSlices, hipMemcpyDeviceToHost), "D2H");
         }
         const uint32_t mask = rel == 0 ? 0x7fffu : 0xffffu;
         uint64_t tot = 0;
@@ -1118,15 +1488,17 @@ int tsp_get_neighbors(tsp_ctx* ctx, int rel, uint32_t* counts, uint32_t* idx, ui
         std::vector<uint16_t> pool(units);
         if (units) { hipCheck(hipMemcpy(pool.data(), d.pool[rel], sizeof(uint16_t) * units, hipMemcpyDeviceToHost), "D2H"); }
         // FF entries of blocks with a halo are halo slots
-        const uint32_t nBlocks = (n + kBlock - 1) / kBlock;
+        const uint32_t b0 = h.blk0, nBlocks = h.nBlk;
         std::vector<uint32_t> haloCount, haloIdx;
-        if (rel == REL_FF && n)
+        if (rel == REL_FF && nBlocks)
         {
             haloCount.resize(nBlocks);
             haloIdx.resize(static_cast<size_t>(nBlocks) * kHaloMax);
-            hipCheck(hipMemcpy(haloCount.data(), d.haloCount, sizeof(uint32_t) * nBlocks, hipMemcpyDeviceToHost), "D2H");
-            hipCheck(hipMemcpy(haloIdx.data(), d.haloIdx, sizeof(uint32_t) * haloIdx.size(), hipMemcpyDeviceToHost), "D2H");
+            hipCheck(hipMemcpy(haloCount.data(), d.haloCount + b0, sizeof(uint32_t) * nBlocks, hipMemcpyDeviceToHost), "D2H");
+            hipCheck(hipMemcpy(haloIdx.data(), d.haloIdx + static_cast<size_t>(b0) * kHaloMax, sizeof(uint32_t) * haloIdx.size(),
+                               hipMemcpyDeviceToHost), "D2H");
         }
+        const uint32_t sub = rel == REL_FF ? h.offF : (rel == REL_FB ? h.offB : h.offR);
         // host decoder of the SELL-32 storage (ReadCompressedSell32Neighbor, gpu_backend.cpp:42-65)
         uint64_t w = 0;
         for (uint32_t i = 0; i < n; i++)
@@ -1141,7 +1513,7 @@ int tsp_get_neighbors(tsp_ctx* ctx, int rel, uint32_t* counts, uint32_t* idx, ui
                 if (m.format == kFmtDelta16) { j = (slots ? 0u : m.base) + pool[m.offset + k * kSlice + lane]; }
                 else { std::memcpy(&j, &pool[m.offset + 2ull * (k * kSlice + lane)], 4); }
                 if (slots) { j = haloIdx[static_cast<size_t>(blk) * kHaloMax + j]; }
-                idx[w++] = j;
+                idx[w++] = j - sub;
             }
         }
         return TSP_OK;
@@ -1151,27 +1523,33 @@ int tsp_get_neighbors(tsp_ctx* ctx, int rel, uint32_t* counts, uint32_t* idx, ui
 int tsp_get_sort(tsp_ctx* ctx, uint32_t* orderF, uint64_t* keysF, uint32_t* orderB, uint64_t* keysB)
 {
     return guarded(ctx, [&]() {
+        ctx->requireScene();
         const Dev& d = ctx->d;
+        const SceneHost& h = ctx->scenes[0];
         ctx->sync();
+        // scene-local Morton ids; out-of-frame ids as all ones
         const auto widen = [&](uint64_t* dst, const uint32_t* src, uint32_t n) {
             std::vector<uint32_t> k(n);
             if (n) { hipCheck(hipMemcpy(k.data(), src, sizeof(uint32_t) * n, hipMemcpyDeviceToHost), "D2H"); }
-            for (uint32_t i = 0; i < n; i++) { dst[i] = k[i] == ctx->frame.emptyKey ? ~0ull : k[i]; }
+            for (uint32_t i = 0; i < n; i++) { dst[i] = k[i] >= h.frame.emptyKey ? ~0ull : k[i] - h.frame.keyBase; }
         };
-        if (orderF && d.nF) { hipCheck(hipMemcpy(orderF, d.idxF, sizeof(uint32_t) * d.nF, hipMemcpyDeviceToHost), "D2H"); }
-        if (keysF) { widen(keysF, d.keyF, d.nF); }
-        if ((orderB || keysB) && d.nB)
+        if (orderF && h.nF)
         {
-            // the boundary order relative to the pulled order, from the ids (stable sequence ids on pull)
-            std::vector<uint32_t> kb(d.nB), ib(d.nB);
-            launchKeys(ctx->stream, d.bpos, d.nB, d.frame, d.status, false, d.keyF2, d.idxF2);
+            hipCheck(hipMemcpy(orderF, d.idxF + h.offF, sizeof(uint32_t) * h.nF, hipMemcpyDeviceToHost), "D2H");
+            for (uint32_t i = 0; i < h.nF; i++) { orderF[i] -= h.offF; }
+        }
+        if (keysF) { widen(keysF, d.keyF + h.offF, h.nF); }
+        if ((orderB || keysB) && h.nB)
+        {
+            // the boundary order relative to the pulled order, from the ids (sequence ids on pull)
+            launchKeys(ctx->stream, d, kPopB, d.bpos, d.nB, false, d.keyF2, d.idxF2);
             ctx->sync();
-            if (keysB) { widen(keysB, d.keyF2, d.nB); }
+            if (keysB) { widen(keysB, d.keyF2 + h.offB, h.nB); }
             if (orderB)
             {
-                std::vector<uint64_t> idb(d.nB);
-                hipCheck(hipMemcpy(idb.data(), ctx->idB, sizeof(uint64_t) * d.nB, hipMemcpyDeviceToHost), "D2H");
-                for (uint32_t i = 0; i < d.nB; i++) { orderB[i] = static_cast<uint32_t>(idb[i]); }
+                std::vector<uint64_t> idb(h.nB);
+                hipCheck(hipMemcpy(idb.data(), ctx->idB + h.offB, sizeof(uint64_t) * h.nB, hipMemcpyDeviceToHost), "D2H");
+                for (uint32_t i = 0; i < h.nB; i++) { orderB[i] = static_cast<uint32_t>(idb[i]); }
             }
         }
         return TSP_OK;
@@ -1181,16 +1559,18 @@ int tsp_get_sort(tsp_ctx* ctx, uint32_t* orderF, uint64_t* keysF, uint32_t* orde
 int tsp_get_rigid_world(tsp_ctx* ctx, float* xyz, float* vel)
 {
     return guarded(ctx, [&]() {
+        ctx->requireScene();
         const Dev& d = ctx->d;
+        const SceneHost& h = ctx->scenes[0];
         ctx->sync();
-        std::vector<float4> q(d.nR);
+        std::vector<float4> q(h.nR);
         const float4* src[2] = { d.rpos, d.rvel };
         float* dst[2] = { xyz, vel };
         for (int r = 0; r < 2; r++)
         {
-            if (!dst[r] || !d.nR) { continue; }
-            hipCheck(hipMemcpy(q.data(), src[r], sizeof(float4) * d.nR, hipMemcpyDeviceToHost), "D2H");
-            for (uint32_t i = 0; i < d.nR; i++) { dst[r][3 * i] = q[i].x; dst[r][3 * i + 1] = q[i].y; dst[r][3 * i + 2] = q[i].z; }
+            if (!dst[r] || !h.nR) { continue; }
+            hipCheck(hipMemcpy(q.data(), src[r] + h.offR, sizeof(float4) * h.nR, hipMemcpyDeviceToHost), "D2H");
+            for (uint32_t i = 0; i < h.nR; i++) { dst[r][3 * i] = q[i].x; dst[r][3 * i + 1] = q[i].y; dst[r][3 * i + 2] = q[i].z; }
         }
         return TSP_OK;
     });
@@ -1238,6 +1618,86 @@ int tsp_synchronize(tsp_ctx* ctx)
         ctx->sync();
         return TSP_OK;
     });
+}
+
+// --- several scenes in one context ---
+
+int tsp_batch_update(tsp_ctx* ctx, uint32_t n_remove, const uint32_t* remove_ids, uint32_t n_add, const tsp_params* params,
+                     const tsp_state_view* const* views, const float* domains, uint32_t* out_ids)
+{
+    return guarded(ctx, [&]() {
+        ctx->update(n_remove, remove_ids, n_add, params, views, domains, out_ids);
+        if (!ctx->paramsSet && n_add) { ctx->defaults = params[0]; ctx->paramsSet = true; }
+        return TSP_OK;
+    });
+}
+
+int tsp_batch_add_scene(tsp_ctx* ctx, const tsp_params* params, const tsp_state_view* state, const float domain[6],
+                        uint32_t* scene_id)
+{
+    return tsp_batch_update(ctx, 0, nullptr, 1, params, &state, domain, scene_id);
+}
+
+int tsp_batch_remove_scene(tsp_ctx* ctx, uint32_t scene_id)
+{
+    return tsp_batch_update(ctx, 1, &scene_id, 0, nullptr, nullptr, nullptr, nullptr);
+}
+
+int tsp_batch_scene_ids(tsp_ctx* ctx, uint32_t* ids, uint32_t cap, uint32_t* n)
+{
+    return guarded(ctx, [&]() {
+        if (n) { *n = ctx->nScenes(); }
+        for (uint32_t q = 0; ids && q < ctx->nScenes() && q < cap; q++) { ids[q] = ctx->scenes[q].id; }
+        return TSP_OK;
+    });
+}
+
+int tsp_batch_scene_counts(tsp_ctx* ctx, uint32_t scene_id, uint32_t counts[4])
+{
+    return guarded(ctx, [&]() {
+        if (!counts) { throw Error{ TSP_ERR_INVALID, "null counts" }; }
+        const SceneHost& h = ctx->scenes[ctx->sceneAt(scene_id)];
+        counts[0] = h.nF; counts[1] = h.nB; counts[2] = h.nR; counts[3] = h.nRB;
+        return TSP_OK;
+    });
+}
+
+int tsp_batch_push_scene(tsp_ctx* ctx, uint32_t scene_id, tsp_state_view* state)
+{
+    return guarded(ctx, [&]() {
+        if (!state) { throw Error{ TSP_ERR_INVALID, "null state" }; }
+        ctx->pushScene(ctx->sceneAt(scene_id), state);
+        return TSP_OK;
+    });
+}
+
+int tsp_batch_scene_info(tsp_ctx* ctx, uint32_t scene_id, tsp_step_info* info)
+{
+    return guarded(ctx, [&]() {
+        if (!info) { throw Error{ TSP_ERR_INVALID, "null info" }; }
+        ctx->sceneInfo(ctx->sceneAt(scene_id), info);
+        return TSP_OK;
+    });
+}
+
+int tsp_batch_step(tsp_ctx* ctx, uint32_t n_steps)
+{
+    return guarded(ctx, [&]() {
+        ctx->requireScene();
+        ctx->steps(n_steps);
+        return TSP_OK;
+    });
+}
+
+uint64_t tsp_batch_footprint(uint32_t n_fluid, uint32_t n_boundary, uint32_t n_rigid, uint32_t n_bodies)
+{
+    // device bytes a scene holds while resident: per fluid slot (segment rounded to 256) about
+    // 190 B of records and second buffers, 16 B of sort keys, 150 B of SELL-32 lists at the
+    // initial pool sizes, 20 B of halo and counts, and up to 24 B of cell tables; per boundary
+    // particle 64 B; per rigid particle about 400 B (records, keys, the RF list); per body 256 B
+    const uint64_t f = static_cast<uint64_t>(roundUp(n_fluid, kBlock));
+    return f * 400ull + static_cast<uint64_t>(n_boundary) * 64ull + static_cast<uint64_t>(n_rigid) * 400ull
+         + static_cast<uint64_t>(n_bodies) * 256ull + (64ull << 10);
 }
 
 } // extern "C"

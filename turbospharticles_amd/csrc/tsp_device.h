@@ -45,18 +45,22 @@ namespace tsp
         uint32_t entries; // list entries of the slice without padding
     };
 
-    // solver status shared by the kernels of one step and read back by the host
+    // flags and totals of one neighbor search over all scenes of a context, read back by the host
     struct Status
     {
-        // neighbor search
-        uint32_t abort;          // a particle left the inner cell frame: the step must restart
+        uint32_t abort;          // some scene's particle left its inner cell frame: restart the search
         uint32_t overflow;       // a SELL pool is too small
-        uint32_t bUnsorted;      // boundary ids are not non-decreasing under the frame
+        uint32_t bUnsorted;      // boundary ids are not non-decreasing under the frames
         uint32_t pad0[3];
         uint64_t poolUnits[3];   // u16 units used by FF, FB, FR
         uint64_t entries[3];     // entries (without padding) of FF, FB, FR
         uint64_t rfTotal, rbTotal, rrTotal;
-        // pressure solve (SimBackend_CPU members N_solverIters_, avgRelDensityErrorF_*)
+    };
+
+    // pressure solve state of one scene (SimBackend_CPU members N_solverIters_,
+    // avgRelDensityErrorF_*), decided on the device and read back after groups of sweeps
+    struct Solve
+    {
         uint32_t iters;
         uint32_t compute;        // the current sweep reduces errors
         uint32_t converged;
@@ -64,9 +68,21 @@ namespace tsp
         float avgRaw, avgPos, avgAbs;
         uint32_t pending;        // a Jacobi sweep ran whose errors are not yet summed
         double sumRaw, sumPos, sumAbs;
-        // rigid
+        uint32_t abort;          // a particle of this scene left the inner frame
         uint32_t rigidDegenerate;
-        uint32_t pad2;
+        uint32_t pad[2];
+    };
+
+    // where a scene's particles live in the arrays of a context. Fluid segments start at a
+    // multiple of kBlock and are padded to one, so a thread block never spans two scenes;
+    // the padding slots are dead (they sort last, own no cell, and no kernel updates them).
+    struct Scene
+    {
+        uint32_t offF, nF;       // fluid: first slot, active particles
+        uint32_t blk0, nBlk;     // fluid thread blocks
+        uint32_t offB, nB;
+        uint32_t offR, nR;
+        uint32_t offRB, nRB;
     };
 
     struct Frame
@@ -74,7 +90,11 @@ namespace tsp
         int32_t orig[3];
         uint32_t dim[3];
         float cellSizeInv;
-        uint32_t emptyKey;   // 1 << (3 * axisBits): sorts behind every in-frame id
+        int32_t reach;       // stencil half width in cells
+        uint32_t keyBase;    // the scene's Morton ids are keyBase + morton(local cell)
+        uint32_t emptyKey;   // keyBase + (1 << 3 axisBits): particles outside the frame sort last
+        uint32_t deadKey;    // emptyKey + 1: the padding slots of the fluid segment
+        uint32_t cellOff;    // the scene's first cell in the cell tables
         int32_t innerLo[3], innerHi[3];
         int32_t domainSet;
         float domMin[3], domMax[3];
@@ -93,7 +113,7 @@ namespace tsp
         float dtsq;
         float densityDevTarget;
         uint32_t minIters, maxIters, fixedIters, termCond;
-        uint32_t nF;
+        uint32_t nF;           // active fluid particles of the scene (the error averages divide by it)
     };
 
     // records gathered by one permutation launch
@@ -113,7 +133,16 @@ namespace tsp
     // device pointers of one context (passed by value to every kernel)
     struct Dev
     {
-        uint32_t nF, nB, nR, nRB;
+        uint32_t nF, nB, nR, nRB;    // array lengths over all scenes (nF counts padding slots)
+        uint32_t nScenes;
+        Scene *scenes;
+        Frame *frames;
+        Consts *consts;
+        Solve *solve;
+        uint32_t *sceneKey0;         // nScenes + 1 ascending key bases (the last one ends the key space)
+        uint32_t *blockScene;        // scene of each fluid thread block
+        uint16_t *sceneB;            // scene of each boundary particle
+        uint32_t *bodyScene;         // scene of each rigid body
         // fluid, packed so that one neighbor costs one 16-byte gather per quantity:
         //   pos (x, y, z, pressure), vel (vx, vy, vz, density), vp (v*, 0), ap (a^p, 0)
         float4 *pos, *vel, *vp, *ap;
@@ -151,8 +180,19 @@ namespace tsp
         double *errPartial;      // Jacobi error partials, [3][blocks of the fluid grid]
         uint32_t *scanTmp;       // block sums of the multi-block scans
         Status *status;
-        Frame *frame;
     };
+
+    // scene of a key: the last scene whose key base is not above it
+    __device__ __forceinline__ uint32_t sceneOfKey(const Dev& d, uint32_t key)
+    {
+        uint32_t lo = 0, hi = d.nScenes;
+        while (hi - lo > 1)
+        {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (d.sceneKey0[mid] <= key) { lo = mid; } else { hi = mid; }
+        }
+        return lo;
+    }
 
     // --- FP32 helpers matching the reference's operation order (no contraction) ---
 
@@ -192,9 +232,10 @@ namespace tsp
         return spread10(lx) | (spread10(ly) << 1) | (spread10(lz) << 2);
     }
 
+    // index of a local cell of a scene's frame in the cell tables
     __device__ __forceinline__ uint32_t linearCell(const Frame& f, int lx, int ly, int lz)
     {
-        return static_cast<uint32_t>(lx) + f.dim[0] * (static_cast<uint32_t>(ly) + f.dim[1] * static_cast<uint32_t>(lz));
+        return f.cellOff + static_cast<uint32_t>(lx) + f.dim[0] * (static_cast<uint32_t>(ly) + f.dim[1] * static_cast<uint32_t>(lz));
     }
 
     // KernelWeight_CubicSpline_SIMD, CPU/simd.h:279-309
