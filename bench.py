@@ -55,6 +55,10 @@ def measured_traffic(kernel, n_side):
         return int(json.load(f)["hbm_bytes_per_launch"])
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(state, params, steps, threads):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
@@ -67,6 +71,101 @@ def cpu_baseline(state, params, steps, threads):
         sweeps += max(o.info().solver_iters - 1, 0)
     dt = time.perf_counter() - t0
     return state.n[0] * steps / dt, dt, sweeps / steps
+
+
+def batched_bench(args):
+    """Mixed-size scenes in one context against the same scenes one context each: 256 dam breaks,
+    fluid counts log-uniform in [2,000, 200,000] (seeded), each tank scaled to its block, h = 0.01.
+    Both forms take the same warm-up steps, then the same timed steps."""
+    import numpy as np
+    from turbospharticles_amd import scenes
+    from turbospharticles_amd.backend import HipBackend, HipBatch
+
+    sides = scenes.log_uniform_sides(args.batch_scenes, 2000, 200000, seed=2026)
+    params = scenes.make_params(0.01, termination_cond=scenes.TERM_POS)
+    states = [scenes.build_scene(scenes.scaled_dam_break_spec(n, 0.01, jitter_seed=67 + q)) for q, n in enumerate(sides)]
+    n_total = sum(st.n[0] for st in states)
+    progress(f"batched: {len(states)} scenes, {n_total} fluid")
+    b = HipBatch(0)
+    ids = b.update(add=[(params, st) for st in states])
+    b.step(args.batch_warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(args.batch_steps)
+    b.synchronize()
+    t_batch = time.perf_counter() - t0
+    sweeps = sum(b.info(i).sweeps_total for i in ids)
+    b.close()
+    progress(f"batched: {t_batch:.2f} s for {args.batch_steps} steps; plain form next")
+    t_plain = 0.0
+    for st in states:
+        g = HipBackend(0)
+        g.set_params(params)
+        g.pull(st)
+        g.step(args.batch_warmup)
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.step(args.batch_steps)
+        g.synchronize()
+        t_plain += time.perf_counter() - t0
+        g.close()
+    v_b = n_total * args.batch_steps / t_batch
+    v_p = n_total * args.batch_steps / t_plain
+    return {"workload": f"{len(states)} dam-break scenes, fluid counts log-uniform in [2000, 200000] "
+                        f"(lattice sides {min(sides)}..{max(sides)}, {n_total} fluid in total), tanks scaled to "
+                        f"their blocks, h 0.01, steps {args.batch_warmup + 1}..{args.batch_warmup + args.batch_steps}",
+            "value": round(v_b, 1), "unit": "particle-steps/s", "plain_value": round(v_p, 1),
+            "speedup_vs_plain": round(v_b / v_p, 2), "ms_per_step": round(t_batch * 1e3 / args.batch_steps, 3),
+            "plain_ms_per_step_all_scenes": round(t_plain * 1e3 / args.batch_steps, 3),
+            "mean_jacobi_sweeps_per_scene_step": round(sweeps / len(ids) / (args.batch_warmup + args.batch_steps), 2)}
+
+
+def served_bench(args):
+    """A Poisson stream of dam-break / dropped-cube scenes (50/50), sizes log-uniform in
+    [5,000, 1,000,000] fluid, steps uniform in [200, 2,000], admitted against a 200 GB budget;
+    served with continuous batching and, as the plain form, one scene at a time."""
+    import numpy as np
+    from turbospharticles_amd import scenes, serving
+
+    rng = np.random.default_rng(args.serve_seed)
+    n = args.serve_requests
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, n))
+    arrivals -= arrivals[0]
+    sides = scenes.log_uniform_sides(n, 5000, 1000000, seed=args.serve_seed + 1)
+    steps = rng.integers(200, 2001, n)
+    kinds = rng.integers(0, 2, n)
+    params = scenes.make_params(0.01, termination_cond=scenes.TERM_POS)
+
+    def make():
+        reqs = []
+        for q in range(n):
+            spec = (scenes.scaled_cube_drop_spec(sides[q], 0.01, rotation_seed=q) if kinds[q]
+                    else scenes.scaled_dam_break_spec(sides[q], 0.01, jitter_seed=67 + q))
+            reqs.append(serving.Request(q, arrivals[q], params, scenes.build_scene(spec), steps[q]))
+        return reqs
+
+    budget = 200e9
+    rb = make()
+    progress(f"served: {n} requests, continuous batching")
+    wall_b = serving.serve_batched(rb, budget, chunk_steps=args.serve_chunk)
+    rp = make()
+    progress(f"served: batched {wall_b:.1f} s; plain form next")
+    wall_p = serving.serve_plain(rp)
+    work = sum(r.particle_steps for r in rb)
+    same = 0
+    for a, c in zip(rb, rp):
+        same += int(all(np.array_equal(a.result.fluid[f], c.result.fluid[f]) for f in a.result.fluid)
+                    and np.array_equal(a.result.fluid_id, c.result.fluid_id))
+    lb, lp = serving.latency_summary(rb), serving.latency_summary(rp)
+    return {"workload": f"{n} requests, Poisson arrivals at {args.serve_rate}/s (seed {args.serve_seed}), "
+                        f"dam break / cube drop 50/50, fluid {min(s ** 3 for s in sides)}..{max(s ** 3 for s in sides)}, "
+                        f"steps {int(steps.min())}..{int(steps.max())}, 200 GB admission budget, joins and leaves "
+                        f"every <= {args.serve_chunk} steps",
+            "value": round(work / wall_b, 1), "unit": "particle-steps/s",
+            "p50_latency_s": lb["p50_s"], "p99_latency_s": lb["p99_s"],
+            "plain_value": round(work / wall_p, 1), "plain_p50_latency_s": lp["p50_s"],
+            "plain_p99_latency_s": lp["p99_s"], "speedup_vs_plain": round(wall_p / wall_b, 2),
+            "results_identical_to_plain": f"{same}/{n}"}
 
 
 def copy_probe():
@@ -90,6 +189,15 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n-side", type=int, default=N_SIDE)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain mixed scenes")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-warmup", type=int, default=5)
+    ap.add_argument("--no-serve", action="store_true", help="skip the served request stream")
+    ap.add_argument("--serve-requests", type=int, default=8)
+    ap.add_argument("--serve-rate", type=float, default=0.5, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-seed", type=int, default=5)
+    ap.add_argument("--serve-chunk", type=int, default=10)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 1 GiB device buffer 3 times (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -154,6 +262,10 @@ def main():
                "sample": f"oracle (C restatement of the TBB path, OpenMP) on the first {args.cpu_steps} steps "
                          f"of the same workload from its initial state: {dt:.1f} s, mean {kc:.1f} Jacobi sweeps/step"}
 
+    progress(f"1M step: {value / 1e6:.1f} M particle-steps/s")
+    batched = None if args.no_batch else batched_bench(args)
+    served = None if args.no_serve else served_bench(args)
+
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
@@ -166,6 +278,8 @@ def main():
                    "kernel_ms": {k: round(v[1], 3) for k, v in prof.items()}},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(line))
 

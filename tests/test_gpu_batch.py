@@ -114,3 +114,27 @@ def test_scenes_joining_and_leaving_leave_the_others_unchanged():
     _same_state(g.push(), b.push(ib), "ragged_abs (8 steps)")
     g, _ = _alone(pc, sc, 5)
     _same_state(g.push(), b.push(ic), "rigid_cube (joined after 3 steps)")
+
+
+def test_served_stream_with_a_tight_budget_matches_the_plain_form():
+    # five requests arriving at once; the budget holds at most two of them, so the rest wait and
+    # join as others finish; every result must equal the plain one-scene-per-context run
+    from turbospharticles_amd import serving
+    print("case: served stream, 5 requests, budget for 2", flush=True)
+    p = scenes.make_params(0.01, termination_cond=scenes.TERM_POS)
+    specs = [(scenes.scaled_dam_break_spec(8, 0.01), 7), (scenes.scaled_cube_drop_spec(10, 0.01), 12),
+             (scenes.scaled_dam_break_spec(12, 0.01, jitter_seed=3), 5), (scenes.scaled_dam_break_spec(9, 0.01), 9),
+             (scenes.scaled_cube_drop_spec(8, 0.01, rotation_seed=2), 4)]
+
+    def make():
+        return [serving.Request(q, 0.0, p, scenes.build_scene(s), n) for q, (s, n) in enumerate(specs)]
+
+    rb = make()
+    budget = 2.5 * max(r.footprint for r in rb)
+    serving.serve_batched(rb, budget, chunk_steps=3)
+    rp = make()
+    serving.serve_plain(rp)
+    assert max(r.admitted_s for r in rb) > min(r.done_s for r in rb), "some request must wait for another to finish"
+    for a, c in zip(rb, rp):
+        _same_state(a.result, c.result, f"request {a.rid}")
+        assert a.result.step == a.steps

@@ -3,6 +3,7 @@
   lib/libtsp_hip.so   hipcc --offload-arch=gfx950: csrc/*.hip (kernels + C ABI)
   lib/libtsp_host.so  g++: host/*.cpp (scene builder, SimBackend_HIP), linked to libtsp_hip.so
   lib/sim_backend_roundtrip  g++: the C++ driver of SimBackend_HIP that the tests run
+  lib/math_selftest  hipcc: GPU check of the short exact sqrt / quotient against the general lowering
 
 Floating point: -ffp-contract=off everywhere (the reference's ISO C++ build contracts nothing;
 every fused multiply-add is an explicit fmaf), correctly rounded f32 division and square root,
@@ -75,10 +76,24 @@ def build_driver(verbose=False):
     return out
 
 
+def build_math_selftest(verbose=False):
+    """lib/math_selftest: the GPU check of the short exact sqrt / quotient against the compiler's"""
+    out = os.path.join(LIB, "math_selftest")
+    hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
+    src = os.path.join(ROOT, "tests", "cpp", "math_selftest.hip")
+    cmd = [hipcc] + HIPCC_FLAGS + ["-I" + INC, "-I" + os.path.join(HERE, "csrc"), "-o", out + ".tmp", src]
+    r = _run(cmd)
+    os.replace(out + ".tmp", out)
+    if verbose and r.stderr:
+        print(r.stderr)
+    return out
+
+
 def build_all(verbose=False):
     build_hip(verbose)
     build_host(verbose)
     build_driver(verbose)
+    build_math_selftest(verbose)
 
 
 if __name__ == "__main__":

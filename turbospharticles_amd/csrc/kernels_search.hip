@@ -366,6 +366,7 @@ namespace tsp
         const Frame& f = d.frames[sc];
         const float r2 = d.consts[sc].r2;
         uint32_t cnt[3] = { 0, 0, 0 };
+        bool tiny = false;
         uint32_t mn[3] = { kEmpty32, kEmpty32, kEmpty32 };
         uint32_t mx[3] = { 0, 0, 0 };
         Stencil s;
@@ -382,9 +383,11 @@ namespace tsp
                         const uint2 rf = d.cellF[lin];
                         for (uint32_t j = rf.x; j < rf.y; j++)
                         {
-                            if (within(pi.x, pi.y, pi.z, d.pos[j], r2))
+                            const float4 pj = d.pos[j];
+                            if (within(pi.x, pi.y, pi.z, pj, r2))
                             {
                                 cnt[0]++; mn[0] = min(mn[0], j); mx[0] = max(mx[0], j);
+                                tiny = tiny || tinyDist2(dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z));
                             }
                         }
                         const uint2 rb = d.cellB[lin];
@@ -419,6 +422,7 @@ namespace tsp
             d.cnt[REL_FR][i] = static_cast<uint16_t>(cnt[2]);
             if (wide) { d.ffWideMin[i] = mn[0]; }
             if (cnt[0] >= kWideSpanBit || cnt[1] > 0xffffu || cnt[2] > 0xffffu) { atomicOr(&d.status->overflow, 2u); }
+            if (tiny) { atomicOr(&d.status->ffTiny, 1u); }
         }
         // per slice: pitch, base (min index), format, entries
         const uint32_t slice = i / kSlice;

@@ -100,11 +100,6 @@ namespace tsp
         __syncthreads();
     }
 
-    __device__ __forceinline__ float dist2(float dx, float dy, float dz)
-    {
-        return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-    }
-
     __global__ void __launch_bounds__(kBlock) kDensity(Dev d)
     {
         __shared__ float4 hp[kHaloMax];
@@ -391,6 +386,7 @@ namespace tsp
     // a sweep's pressure accelerations; block 0 of a sweep launch first finishes the bookkeeping of
     // the previous Jacobi sweep, which the next Jacobi launch reads. Blocks that see the solve
     // converged skip their particles: the final launch (sweep = false) recomputes every a^p.
+    template <bool kFast>
     __global__ void __launch_bounds__(kBlock) kPressureAcc(Dev d, bool sweep)
     {
         __shared__ float4 hp[kHaloMax];
@@ -418,7 +414,7 @@ namespace tsp
         const auto ffTerm = [&](const float4& pj)
         {
             const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
-            const float kg = kgFactor(dist2(dx, dy, dz), k);
+            const float kg = kgFactor<kFast>(dist2(dx, dy, dz), k);
             const float coeff = (pi + pj.w) * kg;
             sx = __builtin_fmaf(coeff, dx, sx);
             sy = __builtin_fmaf(coeff, dy, sy);
@@ -457,6 +453,7 @@ namespace tsp
         return v;
     }
 
+    template <bool kFast>
     __global__ void __launch_bounds__(kBlock) kJacobi(Dev d)
     {
         __shared__ double red[3][kBlock / 64];
@@ -483,7 +480,7 @@ namespace tsp
             const auto ffTerm = [&](const float4& pj, const float4& aj)
             {
                 const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
-                const float kg = kgFactor(dist2(dx, dy, dz), k);
+                const float kg = kgFactor<kFast>(dist2(dx, dy, dz), k);
                 const float sv = __builtin_fmaf(axi - aj.x, dx, __builtin_fmaf(ayi - aj.y, dy, (azi - aj.z) * dz));
                 sf = __builtin_fmaf(kg, sv, sf);
             };
@@ -796,11 +793,15 @@ namespace tsp
     }
     void launchPressureAcc(hipStream_t s, const Dev& d, bool sweep)
     {
-        if (d.nF) { kPressureAcc<<<blocksOf(d.nF), kBlock, 0, s>>>(d, sweep); }
+        if (!d.nF) { return; }
+        if (d.fastFF) { kPressureAcc<true><<<blocksOf(d.nF), kBlock, 0, s>>>(d, sweep); }
+        else { kPressureAcc<false><<<blocksOf(d.nF), kBlock, 0, s>>>(d, sweep); }
     }
     void launchJacobi(hipStream_t s, const Dev& d)
     {
-        if (d.nF) { kJacobi<<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
+        if (!d.nF) { return; }
+        if (d.fastFF) { kJacobi<true><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
+        else { kJacobi<false><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
     }
     void launchFinalize(hipStream_t s, const Dev& d)
     {

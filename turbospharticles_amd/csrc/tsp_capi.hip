@@ -333,7 +333,15 @@ struct tsp_ctx
     void uploadConsts()
     {
         std::vector<Consts> k(nScenes());
-        for (uint32_t s = 0; s < nScenes(); s++) { k[s] = makeConsts(scenes[s].params, scenes[s].nF); }
+        bool fast = true;
+        for (uint32_t s = 0; s < nScenes(); s++)
+        {
+            k[s] = makeConsts(scenes[s].params, scenes[s].nF);
+            // the short exact quotient of the gradient factor needs its operands in range
+            const float a = std::fabs(k[s].koutScale);
+            fast = fast && a >= 0x1p-60f && a <= 0x1p60f && k[s].ksize >= 0x1p-30f && k[s].ktwo <= 0x1p31f;
+        }
+        d.fastDiv = fast ? 1u : 0u;
         h2d(d.consts, k.data(), sizeof(Consts) * k.size());
     }
 
@@ -944,6 +952,8 @@ struct tsp_ctx
                 layoutFrames();
                 continue;
             }
+            // the short exact root and quotient cover this search's FF pairs unless one is tiny
+            d.fastFF = (d.fastDiv && !last.ffTiny) ? 1u : 0u;
             if (last.overflow & 2u) { throw Error{ TSP_ERR_CAPACITY, "neighbor count exceeds the u16 count storage" }; }
             if (last.overflow & 5u)
             {

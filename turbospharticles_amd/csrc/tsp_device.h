@@ -51,7 +51,8 @@ namespace tsp
         uint32_t abort;          // some scene's particle left its inner cell frame: restart the search
         uint32_t overflow;       // a SELL pool is too small
         uint32_t bUnsorted;      // boundary ids are not non-decreasing under the frames
-        uint32_t pad0[3];
+        uint32_t ffTiny;         // some FF pair has 0 < d2 < 2^-96: the sweeps take the general sqrt
+        uint32_t pad0[2];
         uint64_t poolUnits[3];   // u16 units used by FF, FB, FR
         uint64_t entries[3];     // entries (without padding) of FF, FB, FR
         uint64_t rfTotal, rbTotal, rrTotal;
@@ -135,6 +136,8 @@ namespace tsp
     {
         uint32_t nF, nB, nR, nRB;    // array lengths over all scenes (nF counts padding slots)
         uint32_t nScenes;
+        uint32_t fastDiv;            // every scene's kernel constants lie in divCR's range
+        uint32_t fastFF;             // this search's FF pairs may take sqrtCR / divCR (kgFactor<true>)
         Scene *scenes;
         Frame *frames;
         Consts *consts;
@@ -238,6 +241,54 @@ namespace tsp
         return f.cellOff + static_cast<uint32_t>(lx) + f.dim[0] * (static_cast<uint32_t>(ly) + f.dim[1] * static_cast<uint32_t>(lz));
     }
 
+    // --- correctly rounded f32 square root and quotient, in fewer instructions than the
+    //     general lowering: the pair kernels must round exactly like the CPU restatement ---
+
+    // sqrt(x) for x = 0 or x >= 2^-96: the hardware estimate (within 1 ulp) moved to the
+    // neighbour whose residual has the right sign -- the correction of the general lowering,
+    // without its input scaling (needed only below 2^-96) and its +-0 / inf cases (x = 0 gives
+    // +0 here). Checked bit for bit against the general lowering over every such float.
+    __device__ __forceinline__ float sqrtCR(float x)
+    {
+        const float s = __builtin_amdgcn_sqrtf(x);
+        const float sDn = __int_as_float(__float_as_int(s) - 1);
+        const float sUp = __int_as_float(__float_as_int(s) + 1);
+        const float rDn = __builtin_fmaf(-sDn, s, x);
+        const float rUp = __builtin_fmaf(-sUp, s, x);
+        const float t = rDn <= 0.0f ? sDn : s;
+        return rUp > 0.0f ? sUp : t;
+    }
+
+    // squared distance in the reference's operation order
+    __device__ __forceinline__ float dist2(float dx, float dy, float dz)
+    {
+        return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+    }
+
+    // squared distances that sqrtCR does not cover (0 < d2 < 2^-96), as one unsigned compare
+    __device__ __forceinline__ bool tinyDist2(float d2)
+    {
+        return static_cast<uint32_t>(__float_as_int(d2)) - 1u < 0x0f800000u - 1u;
+    }
+
+    // a / b: the reciprocal estimate refined by one Newton step, then two residual corrections
+    // of the quotient -- the general lowering's core, which is the whole computation when it
+    // needs no operand scaling. Callers guarantee 2^-60 <= |a| <= 2^60 and
+    // 2^-30 <= |b| <= 2^31 (the exponents then differ by less than the 96 that trigger
+    // scaling, and every intermediate is a normal number); checked bit for bit against the
+    // general lowering over such operands.
+    __device__ __forceinline__ float divCR(float a, float b)
+    {
+        float y = __builtin_amdgcn_rcpf(b);
+        const float e = __builtin_fmaf(-b, y, 1.0f);
+        y = __builtin_fmaf(e, y, y);
+        float q = a * y;
+        float r = __builtin_fmaf(-b, q, a);
+        q = __builtin_fmaf(r, y, q);
+        r = __builtin_fmaf(-b, q, a);
+        return __builtin_fmaf(r, y, q);
+    }
+
     // KernelWeight_CubicSpline_SIMD, CPU/simd.h:279-309
     __device__ __forceinline__ float kwNormal(float dx, float dy, float dz, const Consts& k)
     {
@@ -261,13 +312,17 @@ namespace tsp
         return __builtin_fmaf(-4.0f, t1 * t1 * t1, t2 * t2 * t2);
     }
 
-    // KernelGradientFactorNoMinDist_CubicSpline_SIMD, CPU/simd.h:374-388
+    // KernelGradientFactorNoMinDist_CubicSpline_SIMD, CPU/simd.h:374-388. With kFast the root
+    // and the quotient take sqrtCR and divCR. The host allows it for a launch when no FF pair of
+    // the search has 0 < d2 < 2^-96 and every scene has |koutScale| and h in divCR's range (the
+    // quotient's operands are koutScale and dist, and it only matters where dist >= h).
+    template <bool kFast = false>
     __device__ __forceinline__ float kgFactor(float d2, const Consts& k)
     {
-        const float dist = __builtin_sqrtf(d2);
+        const float dist = kFast ? sqrtCR(d2) : __builtin_sqrtf(d2);
         const float inner = __builtin_fmaf(k.kinSlope, dist, k.kinOff);
         const float t2 = vmax(k.ktwo - dist, 0.0f);
-        const float outer = (t2 * t2) * (k.koutScale / dist);
+        const float outer = (t2 * t2) * (kFast ? divCR(k.koutScale, dist) : k.koutScale / dist);
         return dist < k.ksize ? inner : outer;
     }
 

@@ -106,3 +106,30 @@ def build_scene(spec):
     finally:
         lib.tsp_scene_free(handle)
     return st
+
+
+# --- mixed-size scene sets: each scene's tank scaled to its own fluid block ---------------------
+
+TANK_RATIO = (2.5, 1.25, 1.25)   # tank / fluid block edge, as the 0.4 m block in the 1.0 x 0.5 x 0.5 tank
+
+
+def scaled_dam_break_spec(n_side, h, jitter_seed=67):
+    """n_side^3 fluid lattice at spacing h in a tank of TANK_RATIO x the block."""
+    block = n_side * h
+    return dam_break_spec(n_side, h, tank=tuple(r * block for r in TANK_RATIO), jitter_seed=jitter_seed)
+
+
+def scaled_cube_drop_spec(n_side, h, rotation_seed=7, jitter_seed=67):
+    """The rigid-cube scene scaled to an n_side^3 block: cube side block/4 and density 500, released
+    at rest at (1.75, 0.75, 0.625) x block (the 1M scene's 0.1 m cube at (0.7, 0.3, 0.25))."""
+    block = n_side * h
+    return rigid_cube_spec(n_side, h, side=0.25 * block, density=500.0,
+                           center=(1.75 * block, 0.75 * block, 0.625 * block), rotation_seed=rotation_seed,
+                           tank=tuple(r * block for r in TANK_RATIO), jitter_seed=jitter_seed)
+
+
+def log_uniform_sides(n, lo, hi, seed):
+    """n lattice sides round(N^(1/3)) for fluid counts N log-uniform in [lo, hi] (seeded)."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(lo), np.log(hi), n))
+    return [max(2, int(round(c ** (1.0 / 3.0)))) for c in counts]

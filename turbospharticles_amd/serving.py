@@ -1,0 +1,132 @@
+"""Request loop for independent scenes on one GPU: continuous batching over a HipBatch context.
+
+A request is one scene (parameters, initial state) and a number of IISPH steps. Requests are
+admitted in arrival order while their device footprint (tsp_batch_footprint) fits the HBM
+budget; admitted scenes stay resident and advance together, one launch per kernel for all of
+them; a scene leaves the batch after its last step, its final state is copied out, and the
+freed memory admits the next waiting requests. Scenes join and leave only between groups of
+steps (at most `chunk_steps`, cut short by the scene that finishes first), so every scene
+takes exactly its own number of steps, and its result is bit-identical to the scene run alone.
+
+The plain form (`serve_plain`) runs the same stream one scene at a time, one context per scene,
+in arrival order: the reference's one-simulation-per-backend model.
+"""
+import collections
+import time
+
+from .backend import HipBackend, HipBatch
+
+
+class Request:
+    def __init__(self, rid, arrival_s, params, state, steps):
+        self.rid = rid
+        self.arrival_s = float(arrival_s)
+        self.params = params
+        self.state = state
+        self.steps = int(steps)
+        self.footprint = HipBatch.footprint(state.n)
+        # filled in by the server
+        self.admitted_s = None
+        self.done_s = None
+        self.result = None
+
+    @property
+    def particle_steps(self):
+        return self.state.n[0] * self.steps
+
+    @property
+    def latency_s(self):
+        return None if self.done_s is None else self.done_s - self.arrival_s
+
+
+def _wait_until(t0, t):
+    dt = t - (time.perf_counter() - t0)
+    if dt > 0:
+        time.sleep(dt)
+
+
+def serve_batched(requests, hbm_budget_bytes, chunk_steps=10, device=0, keep_results=True):
+    """Serves `requests` (arrival times relative to the call) with continuous batching; returns
+    the wall time from the start to the last completion. Fills admitted_s, done_s and result."""
+    batch = HipBatch(device)
+    pending = collections.deque(sorted(requests, key=lambda r: r.arrival_s))
+    waiting = collections.deque()
+    active = {}            # scene id -> [request, steps left]
+    used = 0
+    t0 = time.perf_counter()
+    try:
+        while pending or waiting or active:
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival_s <= now:
+                waiting.append(pending.popleft())
+            admit = []
+            while waiting and used + waiting[0].footprint <= hbm_budget_bytes:
+                r = waiting.popleft()
+                used += r.footprint
+                admit.append(r)
+            if waiting and not active and not admit:
+                r = waiting[0]
+                raise RuntimeError(f"request {r.rid} needs {r.footprint} B, more than the {hbm_budget_bytes} B budget")
+            if admit:
+                ids = batch.update(add=[(r.params, r.state) for r in admit])
+                now = time.perf_counter() - t0
+                for sid, r in zip(ids, admit):
+                    r.admitted_s = now
+                    active[sid] = [r, r.steps]
+            if not active:
+                if pending:
+                    _wait_until(t0, pending[0].arrival_s)
+                continue
+            n = min(chunk_steps, min(left for _, left in active.values()))
+            batch.step(n)
+            done = []
+            for sid, entry in active.items():
+                entry[1] -= n
+                if entry[1] == 0:
+                    done.append(sid)
+            if done:
+                for sid in done:
+                    r = active[sid][0]
+                    if keep_results:
+                        r.result = batch.push(sid)
+                batch.update(remove=done)
+                now = time.perf_counter() - t0
+                for sid in done:
+                    r = active.pop(sid)[0]
+                    r.done_s = now
+                    used -= r.footprint
+        return time.perf_counter() - t0
+    finally:
+        batch.close()
+
+
+def serve_plain(requests, device=0, keep_results=True):
+    """The same stream one scene at a time (one context per scene, arrival order)."""
+    t0 = time.perf_counter()
+    for r in sorted(requests, key=lambda q: q.arrival_s):
+        _wait_until(t0, r.arrival_s)
+        r.admitted_s = time.perf_counter() - t0
+        g = HipBackend(device)
+        try:
+            g.set_params(r.params)
+            g.pull(r.state)
+            g.step(r.steps)
+            out = g.push()
+            if keep_results:
+                r.result = out
+        finally:
+            g.close()
+        r.done_s = time.perf_counter() - t0
+    return time.perf_counter() - t0
+
+
+def latency_summary(requests):
+    """p50 / p99 completion latency (seconds, nearest rank) and the mean."""
+    lat = sorted(r.latency_s for r in requests)
+    if not lat:
+        return {"p50_s": None, "p99_s": None, "mean_s": None}
+
+    def rank(p):
+        k = max(0, min(len(lat) - 1, int(-(-p * len(lat) // 1)) - 1))
+        return lat[k]
+    return {"p50_s": round(rank(0.50), 3), "p99_s": round(rank(0.99), 3), "mean_s": round(sum(lat) / len(lat), 3)}
