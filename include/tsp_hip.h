@@ -180,8 +180,9 @@ int tsp_synchronize(tsp_ctx *ctx);
  * Scene ids are assigned by the context and stay valid until the scene is removed.
  */
 /* one relayout: removes n_remove scenes, then appends n_add new ones (params[q], *views[q],
-   domains[6q..6q+5] = min xyz, max xyz, or domains null for no domain); out_ids receives the
-   new ids. Resident scenes keep their device state. */
+   domains[6q..6q+5] = min xyz, max xyz, or domains null for none; a domain with min >= max on
+   an axis is no domain, as in SetSimDomainBounds); out_ids receives the new ids. Resident scenes
+   keep their device state. */
 int tsp_batch_update(tsp_ctx *ctx, uint32_t n_remove, const uint32_t *remove_ids, uint32_t n_add,
                      const tsp_params *params, const tsp_state_view *const *views,
                      const float *domains, uint32_t *out_ids);

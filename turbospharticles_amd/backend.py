@@ -179,20 +179,17 @@ class HipBatch:
         params = (_capi.TspParams * max(n, 1))()
         views = [st.view() for _, st in add]
         vptrs = (ctypes.POINTER(_capi.TspStateView) * max(n, 1))()
-        dom = np.zeros(6 * max(n, 1), np.float32)
-        any_dom = False
+        # a scene without a domain gets an empty one (min > max), which the context treats the same
+        dom = np.tile(np.array([np.inf] * 3 + [-np.inf] * 3, np.float32), max(n, 1))
         for q, (p, st) in enumerate(add):
             params[q] = p
             vptrs[q] = ctypes.pointer(views[q])
             if st.domain is not None:
-                any_dom = True
                 dom[6 * q:6 * q + 3] = st.domain[0]
                 dom[6 * q + 3:6 * q + 6] = st.domain[1]
-        if any_dom and not all(st.domain is not None for _, st in add):
-            raise ValueError("either every added scene has a domain or none has")
         ids = np.zeros(max(n, 1), np.uint32)
         self._check(self.lib.tsp_batch_update(self.h, len(rm), _capi.u32ptr(rm) if len(rm) else None, n, params,
-                                              vptrs, _capi.fptr(dom) if any_dom else None, _capi.u32ptr(ids)),
+                                              vptrs, _capi.fptr(dom), _capi.u32ptr(ids)),
                     "tsp_batch_update")
         for r in rm:
             self.counts.pop(int(r), None)
