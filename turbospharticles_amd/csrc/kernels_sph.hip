@@ -18,6 +18,14 @@
 // prediction and semi-implicit integration with Gram-Schmidt re-orthonormalization.
 #include "tsp_device.h"
 
+// build-time variants (see turbospharticles_amd/build.py); 0 is the default build
+#ifndef TSP_TUNE_PREFETCH
+#define TSP_TUNE_PREFETCH 0
+#endif
+#ifndef TSP_TUNE_JACOBI_REC24
+#define TSP_TUNE_JACOBI_REC24 0
+#endif
+
 namespace tsp
 {
     struct List
@@ -52,6 +60,23 @@ namespace tsp
             const uint32_t e0 = l.m.offset + l.lane;
             const uint16_t* p = l.pool + e0;
             const uint32_t base = kSlots ? 0u : l.m.base;
+#if TSP_TUNE_PREFETCH
+            // the next four indices load while the current four are summed
+            uint32_t j0 = 0, j1 = 0, j2 = 0, j3 = 0;
+            if (l.n >= 4) { j0 = p[0]; j1 = p[kSlice]; j2 = p[2 * kSlice]; j3 = p[3 * kSlice]; }
+            for (; q + 4 <= l.n; q += 4)
+            {
+                uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+                if (q + 8 <= l.n)
+                {
+                    n0 = p[(q + 4) * kSlice]; n1 = p[(q + 5) * kSlice];
+                    n2 = p[(q + 6) * kSlice]; n3 = p[(q + 7) * kSlice];
+                }
+                fn(base + j0, e0 + (q + 0) * kSlice); fn(base + j1, e0 + (q + 1) * kSlice);
+                fn(base + j2, e0 + (q + 2) * kSlice); fn(base + j3, e0 + (q + 3) * kSlice);
+                j0 = n0; j1 = n1; j2 = n2; j3 = n3;
+            }
+#else
             for (; q + 4 <= l.n; q += 4)
             {
                 const uint32_t j0 = p[(q + 0) * kSlice], j1 = p[(q + 1) * kSlice];
@@ -59,6 +84,7 @@ namespace tsp
                 fn(base + j0, e0 + (q + 0) * kSlice); fn(base + j1, e0 + (q + 1) * kSlice);
                 fn(base + j2, e0 + (q + 2) * kSlice); fn(base + j3, e0 + (q + 3) * kSlice);
             }
+#endif
             for (; q < l.n; q++) { fn(base + p[q * kSlice], e0 + q * kSlice); }
         }
         else
@@ -457,7 +483,13 @@ namespace tsp
     __global__ void __launch_bounds__(kBlock) kJacobi(Dev d)
     {
         __shared__ double red[3][kBlock / 64];
+#if TSP_TUNE_JACOBI_REC24
+        // 24-byte halo records (x, y, z, a^p x | a^p y, a^p z): 24 KB per block, six blocks per CU
+        __shared__ float4 hp[kHaloMax];
+        __shared__ float2 ha[kHaloMax];
+#else
         __shared__ float4 hp[kHaloMax], ha[kHaloMax];
+#endif
         const uint32_t blk = xcdBlock();
         const uint32_t i = blk * kBlock + threadIdx.x;
         const uint32_t sc = d.blockScene[blk];
@@ -469,7 +501,16 @@ namespace tsp
         if (blk == S.blk0 && threadIdx.x == 0) { d.solve[sc].pending = 1u; }
         const uint32_t hc = d.haloCount[blk];
         // (positions only: the pressures this launch writes into pos.w are not read here)
+#if TSP_TUNE_JACOBI_REC24
+        stageHalo(d, blk, hc, [&](uint32_t q, uint32_t j)
+        {
+            const float4 pj = d.pos[j], aj = d.ap[j];
+            hp[q] = make_float4(pj.x, pj.y, pj.z, aj.x);
+            ha[q] = make_float2(aj.y, aj.z);
+        });
+#else
         stageHalo(d, blk, hc, [&](uint32_t q, uint32_t j) { hp[q] = d.pos[j]; ha[q] = d.ap[j]; });
+#endif
         double eRaw = 0.0, ePos = 0.0, eAbs = 0.0;
         if (i - S.offF < S.nF)
         {
@@ -477,15 +518,33 @@ namespace tsp
             const float xi = pi.x, yi = pi.y, zi = pi.z, axi = ai.x, ayi = ai.y, azi = ai.z;
             float sf = 0.0f;
             const List ff = listOf(d, REL_FF, i);
-            const auto ffTerm = [&](const float4& pj, const float4& aj)
+            const auto ffTerm = [&](float xj, float yj, float zj, float axj, float ayj, float azj)
             {
-                const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
+                const float dx = xi - xj, dy = yi - yj, dz = zi - zj;
                 const float kg = kgFactor<kFast>(dist2(dx, dy, dz), k);
-                const float sv = __builtin_fmaf(axi - aj.x, dx, __builtin_fmaf(ayi - aj.y, dy, (azi - aj.z) * dz));
+                const float sv = __builtin_fmaf(axi - axj, dx, __builtin_fmaf(ayi - ayj, dy, (azi - azj) * dz));
                 sf = __builtin_fmaf(kg, sv, sf);
             };
-            visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(hp[q], ha[q]); },
-                    [&](uint32_t j, uint32_t) { ffTerm(d.pos[j], d.ap[j]); });
+#if TSP_TUNE_JACOBI_REC24
+            const auto fromHalo = [&](uint32_t q, uint32_t)
+            {
+                const float4 pj = hp[q];
+                const float2 aj = ha[q];
+                ffTerm(pj.x, pj.y, pj.z, pj.w, aj.x, aj.y);
+            };
+#else
+            const auto fromHalo = [&](uint32_t q, uint32_t)
+            {
+                const float4 pj = hp[q], aj = ha[q];
+                ffTerm(pj.x, pj.y, pj.z, aj.x, aj.y, aj.z);
+            };
+#endif
+            visitFF(ff, hc <= kHaloMax, fromHalo,
+                    [&](uint32_t j, uint32_t)
+                    {
+                        const float4 pj = d.pos[j], aj = d.ap[j];
+                        ffTerm(pj.x, pj.y, pj.z, aj.x, aj.y, aj.z);
+                    });
             sf *= k.mass;
             float sb = 0.0f;
             const List fb = listOf(d, REL_FB, i);

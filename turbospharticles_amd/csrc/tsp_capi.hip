@@ -1257,7 +1257,22 @@ struct tsp_ctx
         {
             if (x.keep < 0) { x.host.id = nextId++; ids.push_back(x.host.id); }
         }
+        const uint32_t nKept = static_cast<uint32_t>(src.size()) - nAdd;
         relayout(src);
+        // the new scenes' frames and key ranges are laid out now, so a context that cannot hold
+        // them refuses them here rather than at the next step
+        try
+        {
+            layoutFrames();
+        }
+        catch (const Error& e)
+        {
+            if (e.code != TSP_ERR_CAPACITY || nAdd == 0) { throw; }
+            std::vector<Source> back(nKept);
+            for (uint32_t q = 0; q < nKept; q++) { back[q].keep = static_cast<int>(q); }
+            relayout(back);
+            throw;
+        }
         if (outIds) { for (uint32_t q = 0; q < nAdd; q++) { outIds[q] = ids[q]; } }
     }
 
