@@ -254,6 +254,82 @@ namespace tsp
         }
     }
 
+    // the whole stable LSD sort of up to kSmallSort keys in one block of 1024 threads (one launch
+    // instead of five per digit): per digit, a histogram and its scan in LDS, then the keys in
+    // order, 1024 at a time, ranked by wave ballots as in kRadixScatter; the passes alternate
+    // between the two buffers in global memory, and an odd pass count copies back at the end
+    constexpr uint32_t kSmallThreads = 1024;
+    constexpr uint32_t kSmallSort = 16384;
+
+    __global__ void __launch_bounds__(kSmallThreads) kSortSmall(uint32_t* __restrict__ key, uint32_t* __restrict__ idx,
+        uint32_t* __restrict__ key2, uint32_t* __restrict__ idx2, uint32_t n, uint32_t bits)
+    {
+        constexpr uint32_t kWaves = kSmallThreads / 64;
+        __shared__ uint32_t run[256];
+        __shared__ uint32_t waveCnt[kWaves][256];
+        const uint32_t t = threadIdx.x;
+        const uint32_t wave = t >> 6, lane = t & 63u;
+        const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+        uint32_t* kIn = key; uint32_t* iIn = idx; uint32_t* kOut = key2; uint32_t* iOut = idx2;
+        for (uint32_t shift = 0; shift < bits; shift += 8)
+        {
+            // digit counts, then their exclusive scan (one thread per digit)
+            if (t < 256) { run[t] = 0; }
+            for (uint32_t w = 0; w < kWaves; w++) { if (t < 256) { waveCnt[w][t] = 0; } }
+            __syncthreads();
+            for (uint32_t i = t; i < n; i += kSmallThreads) { atomicAdd(&run[(kIn[i] >> shift) & 255u], 1u); }
+            __syncthreads();
+            if (t == 0)
+            {
+                uint32_t acc = 0;
+                for (uint32_t dg = 0; dg < 256; dg++) { const uint32_t c = run[dg]; run[dg] = acc; acc += c; }
+            }
+            __syncthreads();
+            for (uint32_t base = 0; base < n; base += kSmallThreads)
+            {
+                const uint32_t i = base + t;
+                const bool valid = i < n;
+                const uint32_t k = valid ? kIn[i] : 0u;
+                const uint32_t dg = (k >> shift) & 255u;
+                uint64_t match = __ballot(valid);
+                for (uint32_t bit = 0; bit < 8; bit++)
+                {
+                    const bool set = (dg >> bit) & 1u;
+                    const uint64_t bs = __ballot(set);
+                    match &= set ? bs : ~bs;
+                }
+                const uint32_t rankInWave = __popcll(match & lanesBelow);
+                if (valid && rankInWave == 0) { waveCnt[wave][dg] = __popcll(match); }
+                __syncthreads();
+                uint32_t rank = 0;
+                if (valid)
+                {
+                    rank = run[dg] + rankInWave;
+                    for (uint32_t w = 0; w < wave; w++) { rank += waveCnt[w][dg]; }
+                }
+                __syncthreads();
+                if (t < 256)
+                {
+                    uint32_t tot = 0;
+                    for (uint32_t w = 0; w < kWaves; w++) { tot += waveCnt[w][t]; waveCnt[w][t] = 0; }
+                    run[t] += tot;
+                }
+                if (valid)
+                {
+                    kOut[rank] = k;
+                    iOut[rank] = iIn[i];
+                }
+                __syncthreads();
+            }
+            uint32_t* tk = kIn; kIn = kOut; kOut = tk;
+            uint32_t* ti = iIn; iIn = iOut; iOut = ti;
+        }
+        if (kIn != key)
+        {
+            for (uint32_t i = t; i < n; i += kSmallThreads) { key[i] = kIn[i]; idx[i] = iIn[i]; }
+        }
+    }
+
     // ----------------------------------------------------------------------------------------
     // permutation (gather into the second buffers)
 
@@ -881,6 +957,11 @@ namespace tsp
                          uint32_t bits, uint32_t* hist, uint32_t* tmp)
     {
         if (n < 2) { return; }
+        if (n <= kSmallSort)
+        {
+            kSortSmall<<<1, kSmallThreads, 0, s>>>(key, idx, key2, idx2, n, bits);
+            return;
+        }
         const uint32_t nTiles = (n + kTile - 1) / kTile;
         uint32_t* kIn = key; uint32_t* iIn = idx; uint32_t* kOut = key2; uint32_t* iOut = idx2;
         for (uint32_t shift = 0; shift < bits; shift += 8)

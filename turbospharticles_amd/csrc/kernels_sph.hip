@@ -18,14 +18,6 @@
 // prediction and semi-implicit integration with Gram-Schmidt re-orthonormalization.
 #include "tsp_device.h"
 
-// build-time variants (see turbospharticles_amd/build.py); 0 is the default build
-#ifndef TSP_TUNE_PREFETCH
-#define TSP_TUNE_PREFETCH 0
-#endif
-#ifndef TSP_TUNE_JACOBI_REC24
-#define TSP_TUNE_JACOBI_REC24 0
-#endif
-
 namespace tsp
 {
     struct List
@@ -241,12 +233,18 @@ namespace tsp
         const float xi = pi.x, yi = pi.y, zi = pi.z, vxi = vi.x, vyi = vi.y, vzi = vi.z;
         float sF = 0.0f, cFx = 0.0f, cFy = 0.0f, cFz = 0.0f, g2 = 0.0f;
         const List ff = listOf(d, REL_FF, i);
-        const auto ffTerm = [&](const float4& pj, const float4& vj)
+        const auto ffTerm = [&](const float4& pj, const float4& vj, uint32_t e)
         {
             const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
             const float dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
             const float d2 = dist2(dx, dy, dz);
             const float kg = kgFactor(d2, k);
+#if TSP_TUNE_KG_STORE
+            // the sweeps of this step read the pair's gradient factor instead of recomputing it
+            d.kgFF[e] = kg;
+#else
+            (void)e;
+#endif
             const float st = __builtin_fmaf(dvx, dx, __builtin_fmaf(dvy, dy, dvz * dz));
             sF = __builtin_fmaf(kg, st, sF);
             cFx = __builtin_fmaf(kg, dx, cFx);
@@ -254,8 +252,8 @@ namespace tsp
             cFz = __builtin_fmaf(kg, dz, cFz);
             g2 = __builtin_fmaf(kg, kg * d2, g2);
         };
-        visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(hp[q], hv[q]); },
-                [&](uint32_t j, uint32_t) { ffTerm(d.pos[j], d.vp[j]); });
+        visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t e) { ffTerm(hp[q], hv[q], e); },
+                [&](uint32_t j, uint32_t e) { ffTerm(d.pos[j], d.vp[j], e); });
         float sB = 0.0f, cBx = 0.0f, cBy = 0.0f, cBz = 0.0f;
         const List fb = listOf(d, REL_FB, i);
         visit(fb, [&](uint32_t j, uint32_t)
@@ -437,16 +435,22 @@ namespace tsp
         const float xi = p4.x, yi = p4.y, zi = p4.z, pi = p4.w;
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         const List ff = listOf(d, REL_FF, i);
-        const auto ffTerm = [&](const float4& pj)
+        const auto ffTerm = [&](const float4& pj, uint32_t e)
         {
             const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
+#if TSP_TUNE_KG_STORE
+            const float kg = d.kgFF[e];
+#else
+            (void)e;
             const float kg = kgFactor<kFast>(dist2(dx, dy, dz), k);
+#endif
             const float coeff = (pi + pj.w) * kg;
             sx = __builtin_fmaf(coeff, dx, sx);
             sy = __builtin_fmaf(coeff, dy, sy);
             sz = __builtin_fmaf(coeff, dz, sz);
         };
-        visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(hp[q]); }, [&](uint32_t j, uint32_t) { ffTerm(d.pos[j]); });
+        visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t e) { ffTerm(hp[q], e); },
+                [&](uint32_t j, uint32_t e) { ffTerm(d.pos[j], e); });
         float ax = sx * k.presOuterFF, ay = sy * k.presOuterFF, az = sz * k.presOuterFF;
         const List fb = listOf(d, REL_FB, i);
         visit(fb, [&](uint32_t j, uint32_t)
@@ -518,32 +522,37 @@ namespace tsp
             const float xi = pi.x, yi = pi.y, zi = pi.z, axi = ai.x, ayi = ai.y, azi = ai.z;
             float sf = 0.0f;
             const List ff = listOf(d, REL_FF, i);
-            const auto ffTerm = [&](float xj, float yj, float zj, float axj, float ayj, float azj)
+            const auto ffTerm = [&](float xj, float yj, float zj, float axj, float ayj, float azj, uint32_t e)
             {
                 const float dx = xi - xj, dy = yi - yj, dz = zi - zj;
+#if TSP_TUNE_KG_STORE
+                const float kg = d.kgFF[e];
+#else
+                (void)e;
                 const float kg = kgFactor<kFast>(dist2(dx, dy, dz), k);
+#endif
                 const float sv = __builtin_fmaf(axi - axj, dx, __builtin_fmaf(ayi - ayj, dy, (azi - azj) * dz));
                 sf = __builtin_fmaf(kg, sv, sf);
             };
 #if TSP_TUNE_JACOBI_REC24
-            const auto fromHalo = [&](uint32_t q, uint32_t)
+            const auto fromHalo = [&](uint32_t q, uint32_t e)
             {
                 const float4 pj = hp[q];
                 const float2 aj = ha[q];
-                ffTerm(pj.x, pj.y, pj.z, pj.w, aj.x, aj.y);
+                ffTerm(pj.x, pj.y, pj.z, pj.w, aj.x, aj.y, e);
             };
 #else
-            const auto fromHalo = [&](uint32_t q, uint32_t)
+            const auto fromHalo = [&](uint32_t q, uint32_t e)
             {
                 const float4 pj = hp[q], aj = ha[q];
-                ffTerm(pj.x, pj.y, pj.z, aj.x, aj.y, aj.z);
+                ffTerm(pj.x, pj.y, pj.z, aj.x, aj.y, aj.z, e);
             };
 #endif
             visitFF(ff, hc <= kHaloMax, fromHalo,
-                    [&](uint32_t j, uint32_t)
+                    [&](uint32_t j, uint32_t e)
                     {
                         const float4 pj = d.pos[j], aj = d.ap[j];
-                        ffTerm(pj.x, pj.y, pj.z, aj.x, aj.y, aj.z);
+                        ffTerm(pj.x, pj.y, pj.z, aj.x, aj.y, aj.z, e);
                     });
             sf *= k.mass;
             float sb = 0.0f;

@@ -356,6 +356,11 @@ struct tsp_ctx
         }
         d.rfCap = rfCap;
         d.rfIdx = alloc<uint32_t>(rfCap, poolAllocs);
+#if TSP_TUNE_KG_STORE
+        d.kgFF = alloc<float>(cap[REL_FF], poolAllocs);
+#else
+        d.kgFF = nullptr;
+#endif
     }
 
     // --- the arena: the particles and bodies of every scene, laid out anew. Resident scenes

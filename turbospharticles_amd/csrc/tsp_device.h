@@ -21,6 +21,20 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// build-time variants of the solver sweeps (all give the same bits; the build picks one):
+//   TSP_TUNE_PREFETCH      the next four list entries load while the current four are summed
+//   TSP_TUNE_JACOBI_REC24  kJacobi's halo records take 24 bytes instead of 32 (six blocks per CU)
+//   TSP_TUNE_KG_STORE      kSourceDiag stores each FF pair's gradient factor; the sweeps read it
+#ifndef TSP_TUNE_PREFETCH
+#define TSP_TUNE_PREFETCH 0
+#endif
+#ifndef TSP_TUNE_JACOBI_REC24
+#define TSP_TUNE_JACOBI_REC24 0
+#endif
+#ifndef TSP_TUNE_KG_STORE
+#define TSP_TUNE_KG_STORE 0
+#endif
+
 namespace tsp
 {
     constexpr int kBlock = 256;          // threads per block: 4 waves of 64
@@ -172,6 +186,7 @@ namespace tsp
         SliceMeta *meta[3];
         uint16_t *pool[3];
         uint64_t poolCap[3];     // u16 units
+        float *kgFF;             // per FF entry (by its u16 position): gradient factor of the step (build variant)
         uint32_t *ffWideMin;     // per particle: min FF index when the wide-span bit is set
         uint32_t *haloIdx;       // per block: kHaloMax particle indices, ascending
         uint32_t *haloCount;     // per block: halo size, or kNoHalo
