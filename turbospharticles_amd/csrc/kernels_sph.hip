@@ -343,12 +343,19 @@ namespace tsp
         }
     }
 
+    // sum over the 64 lanes of a wave (a fixed xor tree: the same order on every run)
+    __device__ __forceinline__ double waveSum(double v)
+    {
+        for (int o = 32; o > 0; o >>= 1) { v += __shfl_xor(v, o, 64); }
+        return v;
+    }
+
     // the bookkeeping of one scene after a Jacobi sweep (sph_solver.cpp:207-261): the scene's block
     // partials summed in a fixed order, the averages, TerminationConditionSatisfied and the error
     // decision of the next sweep. Run by one whole block; nTot is the grid size of the sweep.
     __device__ void finalizeSweep(const Dev& d, uint32_t sc, uint32_t nTot)
     {
-        __shared__ double red[3][kBlock];
+        __shared__ double red[3][kBlock / 64];
         __shared__ uint32_t flags[2];
         Solve& s = d.solve[sc];
         const Consts& k = d.consts[sc];
@@ -373,13 +380,18 @@ namespace tsp
                     a += v0; a += v1; a += v2; a += v3;
                 }
                 for (; b < nBlocks; b += kBlock) { a += src[b]; }
-                red[r][threadIdx.x] = a;
+                a = waveSum(a);
+                if ((threadIdx.x & 63u) == 0) { red[r][threadIdx.x >> 6] = a; }
             }
             __syncthreads();
-            for (uint32_t o = kBlock / 2; o > 0; o >>= 1)
+            if (threadIdx.x == 0)
             {
-                if (threadIdx.x < o) { for (int r = 0; r < 3; r++) { red[r][threadIdx.x] += red[r][threadIdx.x + o]; } }
-                __syncthreads();
+                for (int r = 0; r < 3; r++)
+                {
+                    double a = red[r][0];
+                    for (uint32_t w = 1; w < kBlock / 64; w++) { a += red[r][w]; }
+                    red[r][0] = a;
+                }
             }
         }
         if (threadIdx.x == 0)
@@ -411,7 +423,7 @@ namespace tsp
     // the previous Jacobi sweep, which the next Jacobi launch reads. Blocks that see the solve
     // converged skip their particles: the final launch (sweep = false) recomputes every a^p.
     template <bool kFast>
-    __global__ void __launch_bounds__(kBlock) kPressureAcc(Dev d, bool sweep)
+    __global__ void __launch_bounds__(kBlock) TSP_WAVES_ATTR(TSP_TUNE_PA_WAVES) kPressureAcc(Dev d, bool sweep)
     {
         __shared__ float4 hp[kHaloMax];
         __shared__ uint32_t skip;
@@ -477,14 +489,9 @@ namespace tsp
         d.ap[i] = make_float4(0.0f - ax, 0.0f - ay, 0.0f - az, 0.0f);
     }
 
-    __device__ __forceinline__ double waveSum(double v)
-    {
-        for (int o = 32; o > 0; o >>= 1) { v += __shfl_xor(v, o, 64); }
-        return v;
-    }
 
     template <bool kFast>
-    __global__ void __launch_bounds__(kBlock) kJacobi(Dev d)
+    __global__ void __launch_bounds__(kBlock) TSP_WAVES_ATTR(TSP_TUNE_JACOBI_WAVES) kJacobi(Dev d)
     {
         __shared__ double red[3][kBlock / 64];
 #if TSP_TUNE_JACOBI_REC24

@@ -25,15 +25,38 @@
 //   TSP_TUNE_PREFETCH      the next four list entries load while the current four are summed
 //   TSP_TUNE_JACOBI_REC24  kJacobi's halo records take 24 bytes instead of 32 (six blocks per CU)
 //   TSP_TUNE_KG_STORE      kSourceDiag stores each FF pair's gradient factor; the sweeps read it
+//   TSP_TUNE_HALO_MAX      FF halo records per 256-particle block (LDS per sweep block)
+//   TSP_TUNE_PA_WAVES, TSP_TUNE_JACOBI_WAVES   waves per SIMD the compiler must leave room for
+//                          (register budget of the sweep kernels); 0 leaves the choice to it
 #ifndef TSP_TUNE_PREFETCH
-#define TSP_TUNE_PREFETCH 0
+#define TSP_TUNE_PREFETCH 1
 #endif
 #ifndef TSP_TUNE_JACOBI_REC24
-#define TSP_TUNE_JACOBI_REC24 0
+#define TSP_TUNE_JACOBI_REC24 1
 #endif
 #ifndef TSP_TUNE_KG_STORE
 #define TSP_TUNE_KG_STORE 0
 #endif
+#ifndef TSP_TUNE_HALO_MAX
+#define TSP_TUNE_HALO_MAX 1000
+#endif
+#ifndef TSP_TUNE_PA_WAVES
+#define TSP_TUNE_PA_WAVES 0
+#endif
+#ifndef TSP_TUNE_JACOBI_WAVES
+#define TSP_TUNE_JACOBI_WAVES 0
+#endif
+#define TSP_WAVES_ATTR_0
+#define TSP_WAVES_ATTR_1 __attribute__((amdgpu_waves_per_eu(1)))
+#define TSP_WAVES_ATTR_2 __attribute__((amdgpu_waves_per_eu(2)))
+#define TSP_WAVES_ATTR_3 __attribute__((amdgpu_waves_per_eu(3)))
+#define TSP_WAVES_ATTR_4 __attribute__((amdgpu_waves_per_eu(4)))
+#define TSP_WAVES_ATTR_5 __attribute__((amdgpu_waves_per_eu(5)))
+#define TSP_WAVES_ATTR_6 __attribute__((amdgpu_waves_per_eu(6)))
+#define TSP_WAVES_ATTR_7 __attribute__((amdgpu_waves_per_eu(7)))
+#define TSP_WAVES_ATTR_8 __attribute__((amdgpu_waves_per_eu(8)))
+#define TSP_WAVES_CAT(n) TSP_WAVES_ATTR_##n
+#define TSP_WAVES_ATTR(n) TSP_WAVES_CAT(n)
 
 namespace tsp
 {
@@ -45,7 +68,7 @@ namespace tsp
     constexpr uint32_t kWideSpanBit = 0x8000u; // FF count flag: neighbors span more than 0xffff
     // FF neighbor halo of a 256-particle block: the distinct FF neighbors of its particles,
     // ascending; the FF entries of the block then hold halo slots instead of particle indices
-    constexpr uint32_t kHaloMax = 1000;         // 2 x 16 B x 1000 records + reductions: 5 blocks per CU
+    constexpr uint32_t kHaloMax = TSP_TUNE_HALO_MAX; // 24 B x 1000 records in kJacobi: six blocks per CU
     constexpr uint32_t kNoHalo = 0xffffffffu;   // halo count of a block that keeps particle indices
 
     enum Rel { REL_FF = 0, REL_FB = 1, REL_FR = 2 };
