@@ -255,11 +255,13 @@ namespace tsp
     }
 
     // the whole stable LSD sort of up to kSmallSort keys in one block of 1024 threads (one launch
-    // instead of five per digit): per digit, a histogram and its scan in LDS, then the keys in
-    // order, 1024 at a time, ranked by wave ballots as in kRadixScatter; the passes alternate
+    // instead of five per digit). Per digit: each thread loads its 16 elements (i = c*1024 + t) at
+    // once, counts digits in LDS, scans the 256 counts, then ranks the elements chunk by chunk in
+    // element order with wave ballots (as kRadixScatter) and scatters them; the passes alternate
     // between the two buffers in global memory, and an odd pass count copies back at the end
     constexpr uint32_t kSmallThreads = 1024;
-    constexpr uint32_t kSmallSort = 16384;
+    constexpr uint32_t kSmallPer = 16;
+    constexpr uint32_t kSmallSort = kSmallThreads * kSmallPer;
 
     __global__ void __launch_bounds__(kSmallThreads) kSortSmall(uint32_t* __restrict__ key, uint32_t* __restrict__ idx,
         uint32_t* __restrict__ key2, uint32_t* __restrict__ idx2, uint32_t n, uint32_t bits)
@@ -273,23 +275,36 @@ namespace tsp
         uint32_t* kIn = key; uint32_t* iIn = idx; uint32_t* kOut = key2; uint32_t* iOut = idx2;
         for (uint32_t shift = 0; shift < bits; shift += 8)
         {
-            // digit counts, then their exclusive scan (one thread per digit)
+            uint32_t kr[kSmallPer], ir[kSmallPer];
+#pragma unroll
+            for (uint32_t c = 0; c < kSmallPer; c++)
+            {
+                const uint32_t i = c * kSmallThreads + t;
+                kr[c] = i < n ? kIn[i] : 0u;
+                ir[c] = i < n ? iIn[i] : 0u;
+            }
             if (t < 256) { run[t] = 0; }
             for (uint32_t w = 0; w < kWaves; w++) { if (t < 256) { waveCnt[w][t] = 0; } }
             __syncthreads();
-            for (uint32_t i = t; i < n; i += kSmallThreads) { atomicAdd(&run[(kIn[i] >> shift) & 255u], 1u); }
+#pragma unroll
+            for (uint32_t c = 0; c < kSmallPer; c++)
+            {
+                if (c * kSmallThreads + t < n) { atomicAdd(&run[(kr[c] >> shift) & 255u], 1u); }
+            }
             __syncthreads();
             if (t == 0)
             {
                 uint32_t acc = 0;
-                for (uint32_t dg = 0; dg < 256; dg++) { const uint32_t c = run[dg]; run[dg] = acc; acc += c; }
+                for (uint32_t dg = 0; dg < 256; dg++) { const uint32_t cnt = run[dg]; run[dg] = acc; acc += cnt; }
             }
             __syncthreads();
-            for (uint32_t base = 0; base < n; base += kSmallThreads)
+#pragma unroll
+            for (uint32_t c = 0; c < kSmallPer; c++)
             {
-                const uint32_t i = base + t;
+                if (c * kSmallThreads >= n) { break; }
+                const uint32_t i = c * kSmallThreads + t;
                 const bool valid = i < n;
-                const uint32_t k = valid ? kIn[i] : 0u;
+                const uint32_t k = kr[c];
                 const uint32_t dg = (k >> shift) & 255u;
                 uint64_t match = __ballot(valid);
                 for (uint32_t bit = 0; bit < 8; bit++)
@@ -317,7 +332,7 @@ namespace tsp
                 if (valid)
                 {
                     kOut[rank] = k;
-                    iOut[rank] = iIn[i];
+                    iOut[rank] = ir[c];
                 }
                 __syncthreads();
             }
