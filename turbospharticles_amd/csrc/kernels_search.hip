@@ -256,9 +256,11 @@ namespace tsp
 
     // the whole stable LSD sort of up to kSmallSort keys in one block of 1024 threads (one launch
     // instead of five per digit). Per digit: each thread loads its 16 elements (i = c*1024 + t) at
-    // once, counts digits in LDS, scans the 256 counts, then ranks the elements chunk by chunk in
-    // element order with wave ballots (as kRadixScatter) and scatters them; the passes alternate
-    // between the two buffers in global memory, and an odd pass count copies back at the end
+    // once; the digit counts are scanned by the first 256 threads with wave shuffles; then the
+    // elements are ranked chunk by chunk in element order: wave ballots give the rank inside a
+    // wave, 256 digit threads turn the per-wave counts into per-wave offsets, and the elements
+    // scatter. The passes alternate between the two buffers in global memory, and an odd pass
+    // count copies back at the end.
     constexpr uint32_t kSmallThreads = 1024;
     constexpr uint32_t kSmallPer = 16;
     constexpr uint32_t kSmallSort = kSmallThreads * kSmallPer;
@@ -268,10 +270,13 @@ namespace tsp
     {
         constexpr uint32_t kWaves = kSmallThreads / 64;
         __shared__ uint32_t run[256];
-        __shared__ uint32_t waveCnt[kWaves][256];
+        __shared__ uint32_t waveTot[4];
+        __shared__ uint32_t cnt[kWaves][256];
+        __shared__ uint32_t pre[kWaves][256];
         const uint32_t t = threadIdx.x;
         const uint32_t wave = t >> 6, lane = t & 63u;
         const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+        for (uint32_t w = 0; w < kWaves; w++) { if (t < 256) { cnt[w][t] = 0; } }
         uint32_t* kIn = key; uint32_t* iIn = idx; uint32_t* kOut = key2; uint32_t* iOut = idx2;
         for (uint32_t shift = 0; shift < bits; shift += 8)
         {
@@ -284,7 +289,6 @@ namespace tsp
                 ir[c] = i < n ? iIn[i] : 0u;
             }
             if (t < 256) { run[t] = 0; }
-            for (uint32_t w = 0; w < kWaves; w++) { if (t < 256) { waveCnt[w][t] = 0; } }
             __syncthreads();
 #pragma unroll
             for (uint32_t c = 0; c < kSmallPer; c++)
@@ -292,10 +296,25 @@ namespace tsp
                 if (c * kSmallThreads + t < n) { atomicAdd(&run[(kr[c] >> shift) & 255u], 1u); }
             }
             __syncthreads();
-            if (t == 0)
+            // exclusive scan of the 256 digit counts: inclusive scan per wave, then the wave totals
+            uint32_t v = 0, incl = 0;
+            if (t < 256)
             {
-                uint32_t acc = 0;
-                for (uint32_t dg = 0; dg < 256; dg++) { const uint32_t cnt = run[dg]; run[dg] = acc; acc += cnt; }
+                v = run[t];
+                incl = v;
+                for (int o = 1; o < 64; o <<= 1)
+                {
+                    const uint32_t u = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), o, 64));
+                    if (lane >= static_cast<uint32_t>(o)) { incl += u; }
+                }
+                if (lane == 63) { waveTot[wave] = incl; }
+            }
+            __syncthreads();
+            if (t < 256)
+            {
+                uint32_t off = 0;
+                for (uint32_t w = 0; w < wave; w++) { off += waveTot[w]; }
+                run[t] = off + incl - v;
             }
             __syncthreads();
 #pragma unroll
@@ -314,28 +333,28 @@ namespace tsp
                     match &= set ? bs : ~bs;
                 }
                 const uint32_t rankInWave = __popcll(match & lanesBelow);
-                if (valid && rankInWave == 0) { waveCnt[wave][dg] = __popcll(match); }
+                if (valid && rankInWave == 0) { cnt[wave][dg] = __popcll(match); }
                 __syncthreads();
-                uint32_t rank = 0;
-                if (valid)
-                {
-                    rank = run[dg] + rankInWave;
-                    for (uint32_t w = 0; w < wave; w++) { rank += waveCnt[w][dg]; }
-                }
-                __syncthreads();
+                // per digit: offsets of the waves' runs (and the counts cleared for the next chunk)
                 if (t < 256)
                 {
-                    uint32_t tot = 0;
-                    for (uint32_t w = 0; w < kWaves; w++) { tot += waveCnt[w][t]; waveCnt[w][t] = 0; }
-                    run[t] += tot;
+                    uint32_t cw[kWaves];
+#pragma unroll
+                    for (uint32_t w = 0; w < kWaves; w++) { cw[w] = cnt[w][t]; }
+                    uint32_t acc = run[t];
+#pragma unroll
+                    for (uint32_t w = 0; w < kWaves; w++) { pre[w][t] = acc; acc += cw[w]; cnt[w][t] = 0; }
+                    run[t] = acc;
                 }
+                __syncthreads();
                 if (valid)
                 {
+                    const uint32_t rank = pre[wave][dg] + rankInWave;
                     kOut[rank] = k;
                     iOut[rank] = ir[c];
                 }
-                __syncthreads();
             }
+            __syncthreads();
             uint32_t* tk = kIn; kIn = kOut; kOut = tk;
             uint32_t* ti = iIn; iIn = iOut; iOut = ti;
         }
@@ -906,29 +925,22 @@ namespace tsp
 
     // per-scene bounds of one population: out[6 s + a] holds the ordered bits of the min x, y, z
     // and then the max; a wave whose valid lanes share one scene reduces before its atomics
+    // per-scene bounds: each wave walks a contiguous range of 64-particle chunks, keeps per-lane
+    // running bounds while the chunks belong to one scene, and reduces them into that scene's
+    // bounds with one set of atomics when the scene changes or the range ends (chunks that mix
+    // scenes update lane by lane)
     __global__ void __launch_bounds__(kBlock) kAabb(Dev d, int pop, const float4* __restrict__ p, uint32_t n,
         uint32_t* __restrict__ out)
     {
-        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-        bool valid = i < n;
-        uint32_t sc = 0;
-        if (valid)
-        {
-            sc = sceneOfParticle(d, pop, i);
-            if (pop == POP_F && i - d.scenes[sc].offF >= d.scenes[sc].nF) { valid = false; }
-        }
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint32_t nWaves = gridDim.x * (kBlock / 64);
+        const uint32_t gw = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+        const uint32_t nChunks = (n + 63u) / 64u;
+        const uint32_t c0 = static_cast<uint32_t>(static_cast<uint64_t>(nChunks) * gw / nWaves);
+        const uint32_t c1 = static_cast<uint32_t>(static_cast<uint64_t>(nChunks) * (gw + 1) / nWaves);
+        uint32_t rs = 0xffffffffu;
         uint32_t v[6] = { 0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u };
-        if (valid)
-        {
-            const float4 q = p[i];
-            const float c[3] = { q.x, q.y, q.z };
-            for (int a = 0; a < 3; a++) { v[a] = orderedBits(c[a]); v[3 + a] = v[a]; }
-        }
-        const uint64_t vm = __ballot(valid);
-        if (vm == 0) { return; }
-        const int first = __ffsll(static_cast<unsigned long long>(vm)) - 1;
-        const uint32_t s0 = static_cast<uint32_t>(__shfl(static_cast<int>(sc), first, 64));
-        if (__ballot(valid && sc != s0) == 0)
+        const auto flush = [&]()
         {
             for (int o = 32; o > 0; o >>= 1)
             {
@@ -938,15 +950,46 @@ namespace tsp
                     v[3 + a] = max(v[3 + a], static_cast<uint32_t>(__shfl_xor(static_cast<int>(v[3 + a]), o, 64)));
                 }
             }
-            if ((threadIdx.x & 63u) == 0)
+            if (lane == 0 && rs != 0xffffffffu)
             {
-                for (int a = 0; a < 3; a++) { atomicMin(&out[6 * s0 + a], v[a]); atomicMax(&out[6 * s0 + 3 + a], v[3 + a]); }
+                for (int a = 0; a < 3; a++) { atomicMin(&out[6 * rs + a], v[a]); atomicMax(&out[6 * rs + 3 + a], v[3 + a]); }
+            }
+            for (int a = 0; a < 3; a++) { v[a] = 0xffffffffu; v[3 + a] = 0u; }
+        };
+        for (uint32_t c = c0; c < c1; c++)
+        {
+            const uint32_t i = c * 64u + lane;
+            bool valid = i < n;
+            uint32_t sc = 0;
+            if (valid)
+            {
+                sc = sceneOfParticle(d, pop, i);
+                if (pop == POP_F && i - d.scenes[sc].offF >= d.scenes[sc].nF) { valid = false; }
+            }
+            uint32_t b[3] = { 0, 0, 0 };
+            if (valid)
+            {
+                const float4 q = p[i];
+                b[0] = orderedBits(q.x); b[1] = orderedBits(q.y); b[2] = orderedBits(q.z);
+            }
+            const uint64_t vm = __ballot(valid);
+            if (vm == 0) { continue; }
+            const int first = __ffsll(static_cast<unsigned long long>(vm)) - 1;
+            const uint32_t s0 = static_cast<uint32_t>(__shfl(static_cast<int>(sc), first, 64));
+            if (__ballot(valid && sc != s0) == 0)
+            {
+                if (s0 != rs) { flush(); rs = s0; }
+                if (valid)
+                {
+                    for (int a = 0; a < 3; a++) { v[a] = min(v[a], b[a]); v[3 + a] = max(v[3 + a], b[a]); }
+                }
+            }
+            else if (valid)
+            {
+                for (int a = 0; a < 3; a++) { atomicMin(&out[6 * sc + a], b[a]); atomicMax(&out[6 * sc + 3 + a], b[a]); }
             }
         }
-        else if (valid)
-        {
-            for (int a = 0; a < 3; a++) { atomicMin(&out[6 * sc + a], v[a]); atomicMax(&out[6 * sc + 3 + a], v[3 + a]); }
-        }
+        flush();
     }
 
     // ----------------------------------------------------------------------------------------
@@ -1053,6 +1096,9 @@ namespace tsp
 
     void launchAabb(hipStream_t s, const Dev& d, int pop, const float4* p, uint32_t n, uint32_t* out)
     {
-        if (n) { kAabb<<<blocksFor(n), kBlock, 0, s>>>(d, pop, p, n, out); }
+        // about 8 chunks of 64 particles per wave, at most 256 blocks
+        const uint32_t chunks = (n + 63u) / 64u;
+        const uint32_t blocks = std::max(1u, std::min(256u, (chunks + 31u) / 32u));
+        if (n) { kAabb<<<blocks, kBlock, 0, s>>>(d, pop, p, n, out); }
     }
 }
