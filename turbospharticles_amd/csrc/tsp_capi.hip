@@ -258,21 +258,34 @@ struct tsp_ctx
     bool cellsClean = false;        // fluid and rigid cell tables hold no runs
     uint32_t keyBits = 1;
 
-    Status* hostStatus = nullptr;   // pinned
-    Solve* hostSolve = nullptr;     // pinned, solveCap entries
+    // the search status and the scenes' solver states share one device block and one pinned
+    // host block, so one copy reads them all back
+    static constexpr size_t kSolveOff = 256;
+    unsigned char* hostBlock = nullptr;
+    Status* hostStatus = nullptr;   // pinned, at the start of hostBlock
+    Solve* hostSolve = nullptr;     // pinned, solveCap entries from kSolveOff
     uint32_t solveCap = 0;
     Status last{};
 
     Profiler prof;
 
+    // instantiated graphs of sweep groups, keyed by the kernels' arguments and the sweep count
+    std::map<std::string, hipGraphExec_t> graphs;
+
+    void dropGraphs()
+    {
+        for (auto& g : graphs) { (void)hipGraphExecDestroy(g.second); }
+        graphs.clear();
+    }
+
     ~tsp_ctx()
     {
+        dropGraphs();
         releaseList(poolAllocs);
         releaseList(frameAllocs);
         releaseList(work);
         releaseList(arena);
-        if (hostStatus) { (void)hipHostFree(hostStatus); }
-        if (hostSolve) { (void)hipHostFree(hostSolve); }
+        if (hostBlock) { (void)hipHostFree(hostBlock); }
         if (stream) { (void)hipStreamDestroy(stream); }
     }
 
@@ -320,11 +333,8 @@ struct tsp_ctx
     void readStatus()
     {
         hipCheck(hipGetLastError(), "kernel launch");
-        hipCheck(hipMemcpyAsync(hostStatus, d.status, sizeof(Status), hipMemcpyDeviceToHost, stream), "status D2H");
-        if (nScenes())
-        {
-            hipCheck(hipMemcpyAsync(hostSolve, d.solve, sizeof(Solve) * nScenes(), hipMemcpyDeviceToHost, stream), "solve D2H");
-        }
+        hipCheck(hipMemcpyAsync(hostBlock, d.status, kSolveOff + sizeof(Solve) * nScenes(), hipMemcpyDeviceToHost, stream),
+                 "status D2H");
         sync();
         prof.resolve();
         last = *hostStatus;
@@ -633,14 +643,17 @@ struct tsp_ctx
         d.rrCnt = alloc<uint32_t>(nR, work);
         d.rfOff = alloc<uint32_t>(nR + 1ull, work);
         d.errPartial = alloc<double>(3ull * (nF / kBlock), work);
-        d.status = alloc<Status>(1, work);
-        zero(d.status, sizeof(Status));
+        {
+            static_assert(sizeof(Status) <= kSolveOff, "status block layout");
+            unsigned char* blk = alloc<unsigned char>(kSolveOff + sizeof(Solve) * S, work);
+            zero(blk, kSolveOff + sizeof(Solve) * S);
+            d.status = reinterpret_cast<Status*>(blk);
+            d.solve = reinterpret_cast<Solve*>(blk + kSolveOff);
+        }
         d.scenes = alloc<Scene>(S, work);
         d.frames = alloc<Frame>(S, work);
         d.consts = alloc<Consts>(S, work);
-        d.solve = alloc<Solve>(S, work);
         d.sceneKey0 = alloc<uint32_t>(S + 1ull, work);
-        zero(d.solve, sizeof(Solve) * S);
         {
             std::vector<Scene> t(S);
             for (uint32_t s = 0; s < S; s++)
@@ -653,8 +666,11 @@ struct tsp_ctx
         uploadConsts();
         if (solveCap < S)
         {
-            if (hostSolve) { (void)hipHostFree(hostSolve); hostSolve = nullptr; solveCap = 0; }
-            hipCheck(hipHostMalloc(reinterpret_cast<void**>(&hostSolve), sizeof(Solve) * S), "hipHostMalloc");
+            if (hostBlock) { (void)hipHostFree(hostBlock); hostBlock = nullptr; hostStatus = nullptr; hostSolve = nullptr; solveCap = 0; }
+            hipCheck(hipHostMalloc(reinterpret_cast<void**>(&hostBlock), kSolveOff + sizeof(Solve) * S), "hipHostMalloc");
+            std::memset(hostBlock, 0, kSolveOff);
+            hostStatus = reinterpret_cast<Status*>(hostBlock);
+            hostSolve = reinterpret_cast<Solve*>(hostBlock + kSolveOff);
             solveCap = S;
         }
         if (S) { std::memset(hostSolve, 0, sizeof(Solve) * S); }
@@ -990,6 +1006,59 @@ struct tsp_ctx
         throw Error{ TSP_ERR_STATE, "cell frame rebuild did not converge" };
     }
 
+    // --- n relaxed Jacobi sweeps of every scene (a^p, then p) and the bookkeeping of the last ---
+    void sweepLaunches(uint32_t n)
+    {
+        for (uint32_t s = 0; s < n; s++)
+        {
+            {
+                Scope sc(prof, stream, "pressure_acc");
+                launchPressureAcc(stream, d, true);
+            }
+            Scope sc(prof, stream, "jacobi");
+            launchJacobi(stream, d);
+        }
+        Scope sc(prof, stream, "finalize");
+        launchFinalize(stream, d);
+    }
+
+    // the group runs as one instantiated HIP graph (captured on first use for these kernel
+    // arguments and this count), unless launches are being timed one by one
+    void launchSweeps(uint32_t n)
+    {
+        if (!TSP_TUNE_GRAPHS || prof.on)
+        {
+            sweepLaunches(n);
+            return;
+        }
+        std::string key(reinterpret_cast<const char*>(&d), sizeof(Dev));
+        key.append(reinterpret_cast<const char*>(&n), sizeof(n));
+        auto it = graphs.find(key);
+        if (it == graphs.end())
+        {
+            if (graphs.size() >= 64) { dropGraphs(); }
+            hipCheck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            hipGraph_t g = nullptr;
+            try
+            {
+                sweepLaunches(n);
+            }
+            catch (...)
+            {
+                (void)hipStreamEndCapture(stream, &g);
+                if (g) { (void)hipGraphDestroy(g); }
+                throw;
+            }
+            hipCheck(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
+            hipGraphExec_t e = nullptr;
+            const hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            hipCheck(rc, "hipGraphInstantiate");
+            it = graphs.emplace(std::move(key), e).first;
+        }
+        hipCheck(hipGraphLaunch(it->second, stream), "hipGraphLaunch");
+    }
+
     // --- one IISPH step of every scene (Step_IISPH, sph_solver.cpp:34-319) ---
     void stepOnce()
     {
@@ -1033,19 +1102,7 @@ struct tsp_ctx
         {
             const uint32_t n = std::min(group, maxSweeps - queued);
             if (n == 0) { throw Error{ TSP_ERR_STATE, "pressure solve did not stop at max_solver_iters" }; }
-            for (uint32_t s = 0; s < n; s++)
-            {
-                {
-                    Scope sc(prof, stream, "pressure_acc");
-                    launchPressureAcc(stream, d, true);
-                }
-                Scope sc(prof, stream, "jacobi");
-                launchJacobi(stream, d);
-            }
-            {
-                Scope sc(prof, stream, "finalize");
-                launchFinalize(stream, d);
-            }
+            launchSweeps(n);
             queued += n;
             readStatus();
             bool all = true;
@@ -1332,12 +1389,14 @@ int tsp_create(int device, tsp_ctx** out)
     tsp_ctx* c = new tsp_ctx();
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess
-        || hipHostMalloc(reinterpret_cast<void**>(&c->hostStatus), sizeof(Status)) != hipSuccess)
+        || hipHostMalloc(reinterpret_cast<void**>(&c->hostBlock), tsp_ctx::kSolveOff) != hipSuccess)
     {
         delete c;
         return TSP_ERR_HIP;
     }
-    std::memset(c->hostStatus, 0, sizeof(Status));
+    std::memset(c->hostBlock, 0, tsp_ctx::kSolveOff);
+    c->hostStatus = reinterpret_cast<Status*>(c->hostBlock);
+    c->hostSolve = reinterpret_cast<Solve*>(c->hostBlock + tsp_ctx::kSolveOff);
     *out = c;
     return TSP_OK;
 }

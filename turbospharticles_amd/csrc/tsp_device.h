@@ -26,6 +26,7 @@
 //   TSP_TUNE_JACOBI_REC24  kJacobi's halo records take 24 bytes instead of 32 (six blocks per CU)
 //   TSP_TUNE_KG_STORE      kSourceDiag stores each FF pair's gradient factor; the sweeps read it
 //   TSP_TUNE_HALO_MAX      FF halo records per 256-particle block (LDS per sweep block)
+//   TSP_TUNE_GRAPHS        each group of solver sweeps runs as one instantiated HIP graph
 //   TSP_TUNE_PA_WAVES, TSP_TUNE_JACOBI_WAVES   waves per SIMD the compiler must leave room for
 //                          (register budget of the sweep kernels); 0 leaves the choice to it
 #ifndef TSP_TUNE_PREFETCH
@@ -39,6 +40,9 @@
 #endif
 #ifndef TSP_TUNE_HALO_MAX
 #define TSP_TUNE_HALO_MAX 1000
+#endif
+#ifndef TSP_TUNE_GRAPHS
+#define TSP_TUNE_GRAPHS 1
 #endif
 #ifndef TSP_TUNE_PA_WAVES
 #define TSP_TUNE_PA_WAVES 0
