@@ -62,11 +62,15 @@ def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(state, params, steps, threads):
+def cpu_baseline(state, params, warmup, steps, threads):
+    """The oracle on the host cores: the same warm-up steps untimed, then the first `steps` of the
+    GPU's timed window."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     o = O.Oracle(params, chunk_width=1, threads=threads)
     o.pull(state)
+    for _ in range(warmup):
+        o.step()
     t0 = time.perf_counter()
     sweeps = 0
     for _ in range(steps):
@@ -188,7 +192,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=5,
+                    help="oracle steps timed for the CPU baseline (after the same warm-up as the GPU)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n-side", type=int, default=N_SIDE)
@@ -198,7 +203,7 @@ def main():
     ap.add_argument("--batch-warmup", type=int, default=5)
     ap.add_argument("--no-serve", action="store_true", help="skip the served request stream")
     ap.add_argument("--serve-requests", type=int, default=8)
-    ap.add_argument("--serve-rate", type=float, default=0.5, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-seed", type=int, default=5)
     ap.add_argument("--serve-chunk", type=int, default=10)
     ap.add_argument("--copy-probe", action="store_true",
@@ -267,10 +272,13 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        v, dt, kc = cpu_baseline(state, params, args.cpu_steps, args.cpu_threads)
+        progress("CPU baseline (oracle)")
+        v, dt, kc = cpu_baseline(state, params, args.warmup, args.cpu_steps, args.cpu_threads)
         cpu = {"value": round(v, 1), "unit": "particle-steps/s", "cores": args.cpu_threads, "kind": "port",
-               "sample": f"oracle (C restatement of the TBB path, OpenMP) on the first {args.cpu_steps} steps "
-                         f"of the same workload from its initial state: {dt:.1f} s, mean {kc:.1f} Jacobi sweeps/step"}
+               "sample": f"oracle (C restatement of the TBB path, OpenMP) on steps {args.warmup + 1}.."
+                         f"{args.warmup + args.cpu_steps} of the same workload (the first {args.cpu_steps} of the GPU's "
+                         f"timed window, after the same {args.warmup} warm-up steps): {dt:.1f} s, "
+                         f"mean {kc:.1f} Jacobi sweeps/step"}
 
     progress(f"1M step: {value / 1e6:.1f} M particle-steps/s")
     batched = None if args.no_batch else batched_bench(args)

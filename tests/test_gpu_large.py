@@ -1,0 +1,61 @@
+"""The large single scene: 8,000,000 fluid particles (200^3 lattice at h = 0.002, seeded jitter)
+with 1,262,036 wall particles in the 1.0 x 0.5 x 0.5 (2:1:1) tank.
+
+One step of the HIP path against the oracle: identical to the FP32 restatement of the reference
+CPU path (ids, every fluid field, iteration count), and its error against the FP64 oracle stated
+for density, pressure and velocity (normwise, max |a - b| / max |b|; printed, and bounded by
+1e-3 as a sanity check -- the FP32 path itself differs from FP64 by rounding only).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from turbospharticles_amd import scenes
+from turbospharticles_amd.backend import HipBackend
+
+import oracle_lib as O
+from helpers import FIELDS_STEP, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.timeout(600)
+def test_8m_scene_one_step_matches_the_oracle():
+    print("case: 8M dam break (200^3, h 0.002), one step", flush=True)
+    st = scenes.build_scene(scenes.dam_break_spec(200, 0.002))
+    assert st.n[:2] == (8_000_000, 1_262_036)
+    p = scenes.make_params(0.002, termination_cond=scenes.TERM_POS)
+    g = HipBackend(0)
+    g.set_params(p)
+    g.pull(st)
+    g.step(1)
+    sg = g.push()
+    it_g = g.info().solver_iters
+    g.close()
+    print("  HIP step done", flush=True)
+    threads = min(16, os.cpu_count() or 1)
+    o = O.Oracle(p, chunk_width=1, threads=threads)
+    o.pull(st)
+    o.step()
+    so = o.push()
+    assert it_g == o.info().solver_iters
+    del o
+    print("  FP32 oracle step done", flush=True)
+    assert np.array_equal(sg.fluid_id, so.fluid_id)
+    for f in FIELDS_STEP:
+        assert np.array_equal(sg.fluid[f], so.fluid[f]), f
+    del so
+    o64 = O.Oracle(p, precision=64, chunk_width=1, threads=threads)
+    o64.pull(st)
+    o64.step()
+    s64 = o64.push()
+    del o64
+    assert np.array_equal(sg.fluid_id, s64.fluid_id)
+    errs = {f: rel_err(sg.fluid[f], s64.fluid[f]) for f in ("density", "pressure")}
+    vg = np.stack([sg.fluid[c] for c in ("vx", "vy", "vz")], 1)
+    v64 = np.stack([s64.fluid[c] for c in ("vx", "vy", "vz")], 1)
+    errs["velocity"] = rel_err(vg, v64)
+    print(f"  error against the FP64 oracle after one step: {errs}", flush=True)
+    for f, e in errs.items():
+        assert e <= 1e-3, (f, e)
