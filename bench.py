@@ -105,7 +105,9 @@ def batched_bench(args):
     b.close()
     progress(f"batched: {t_batch:.2f} s for {args.batch_steps} steps; plain form next")
     t_plain = 0.0
-    for st in states:
+    for q, st in enumerate(states):
+        if q % 64 == 0:
+            progress(f"batched: plain form, scene {q}/{len(states)}")
         g = HipBackend(0)
         g.set_params(params)
         g.pull(st)
