@@ -736,12 +736,12 @@ namespace tsp
         __shared__ uint32_t table[kHashSlots];
         __shared__ uint16_t rank[kHashSlots];
         __shared__ uint32_t keys[kSortKeys];
-        __shared__ uint32_t unique, filled;
+        __shared__ uint32_t unique, filled, bmin, notDelta, outOfSpan;
+        __shared__ uint32_t waveSum[kBlock / 64];
         const uint32_t blk = blockIdx.x;
         const uint32_t t = threadIdx.x;
         const uint32_t i = blk * kBlock + t;
-        for (uint32_t q = t; q < kHashSlots; q += kBlock) { table[q] = kEmpty32; }
-        if (t == 0) { unique = 0; filled = 0; }
+        if (t == 0) { unique = 0; filled = 0; bmin = kEmpty32; notDelta = 0; outOfSpan = 0; }
         __syncthreads();
         uint32_t n = 0;
         SliceMeta m{};
@@ -751,6 +751,80 @@ namespace tsp
             m = d.meta[REL_FF][i / kSlice];
         }
         const uint32_t lane = i & (kSlice - 1);
+        uint32_t* out = d.haloIdx + static_cast<size_t>(blk) * kHaloMax;
+        // When every FF entry of the block lies within 0xffff of the block's smallest (u16 delta
+        // slices whose bases are close), the halo is a bitmap over that window: the set bits in
+        // ascending order are the halo, and an entry's slot is the count of set bits below it.
+        if (n > 0)
+        {
+            if (m.format != kFmtDelta16) { atomicOr(&notDelta, 1u); }
+            else { atomicMin(&bmin, m.base); }
+        }
+        __syncthreads();
+        if (!notDelta)
+        {
+            constexpr uint32_t kWords = kHashSlots;   // 2048 x 32 bits: a window of 0x10000 indices
+            for (uint32_t q = t; q < kWords; q += kBlock) { table[q] = 0u; }
+            __syncthreads();
+            const uint32_t b0 = bmin;
+            for (uint32_t k = 0; k < n; k++)
+            {
+                const uint32_t off = sellRead(d.pool[REL_FF], m, lane, k) - b0;
+                if (off > 0xffffu) { atomicOr(&outOfSpan, 1u); break; }
+                atomicOr(&table[off >> 5], 1u << (off & 31u));
+            }
+            __syncthreads();
+            if (!outOfSpan)
+            {
+                // ranks of the words (eight consecutive words per thread), as u16 in rank[]
+                constexpr uint32_t kPer = kWords / kBlock;
+                uint32_t pc[kPer], sum = 0;
+#pragma unroll
+                for (uint32_t u = 0; u < kPer; u++) { pc[u] = static_cast<uint32_t>(__popc(table[kPer * t + u])); sum += pc[u]; }
+                uint32_t incl = sum;
+                for (int o = 1; o < 64; o <<= 1)
+                {
+                    const uint32_t v = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), o, 64));
+                    if ((t & 63u) >= static_cast<uint32_t>(o)) { incl += v; }
+                }
+                if ((t & 63u) == 63u) { waveSum[t >> 6] = incl; }
+                __syncthreads();
+                uint32_t count = 0, before = 0;
+                for (uint32_t w = 0; w < kBlock / 64; w++) { before += w < (t >> 6) ? waveSum[w] : 0u; count += waveSum[w]; }
+                if (count > kHaloMax)
+                {
+                    if (t == 0) { d.haloCount[blk] = kNoHalo; }
+                    return;
+                }
+                uint32_t r = before + incl - sum;
+#pragma unroll
+                for (uint32_t u = 0; u < kPer; u++)
+                {
+                    const uint32_t w = kPer * t + u;
+                    rank[w] = static_cast<uint16_t>(r);
+                    uint32_t bits = table[w];
+                    while (bits)
+                    {
+                        const uint32_t b = static_cast<uint32_t>(__ffs(static_cast<int>(bits))) - 1u;
+                        out[r++] = b0 + 32u * w + b;
+                        bits &= bits - 1u;
+                    }
+                }
+                if (t == 0) { d.haloCount[blk] = count; }
+                __syncthreads();
+                for (uint32_t k = 0; k < n; k++)
+                {
+                    const uint32_t off = sellRead(d.pool[REL_FF], m, lane, k) - b0;
+                    const uint32_t w = off >> 5;
+                    const uint32_t slot = rank[w] + static_cast<uint32_t>(__popc(table[w] & ((1u << (off & 31u)) - 1u)));
+                    d.pool[REL_FF][m.offset + k * kSlice + lane] = static_cast<uint16_t>(slot);
+                }
+                return;
+            }
+        }
+        // otherwise a hash set of the block's entries, sorted
+        for (uint32_t q = t; q < kHashSlots; q += kBlock) { table[q] = kEmpty32; }
+        __syncthreads();
         // insert: open addressing, a plain read before the compare-and-swap (most entries are
         // already present); a thread stops once the halo is known to be too large, so at most
         // kHaloMax + kBlock values enter the kHashSlots slots and a probe always ends
@@ -805,7 +879,6 @@ namespace tsp
                 __syncthreads();
             }
         }
-        uint32_t* out = d.haloIdx + static_cast<size_t>(blk) * kHaloMax;
         for (uint32_t q = t; q < count; q += kBlock)
         {
             out[q] = keys[q];
