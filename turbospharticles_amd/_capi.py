@@ -12,7 +12,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "lib")
-HIP_LIB_PATH = os.path.join(LIB_DIR, "libtsp_hip.so")
+# TSP_HIP_LIB names another build of libtsp_hip.so (the variants the tuner measures)
+HIP_LIB_PATH = os.environ.get("TSP_HIP_LIB") or os.path.join(LIB_DIR, "libtsp_hip.so")
 HOST_LIB_PATH = os.path.join(LIB_DIR, "libtsp_host.so")
 
 F_FIELDS = ["x", "y", "z", "vx", "vy", "vz", "vpx", "vpy", "vpz", "apx", "apy", "apz",

@@ -27,6 +27,9 @@ HIPCC_FLAGS = [
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
+    # calls inside the library bind to its own definitions (another build loaded next to it
+    # cannot interpose them)
+    "-Wl,-Bsymbolic",
 ]
 
 

@@ -29,6 +29,10 @@
 //   TSP_TUNE_GRAPHS        each group of solver sweeps runs as one instantiated HIP graph
 //   TSP_TUNE_PA_WAVES, TSP_TUNE_JACOBI_WAVES   waves per SIMD the compiler must leave room for
 //                          (register budget of the sweep kernels); 0 leaves the choice to it
+// the settings chosen by turbospharticles_amd/tune.py, when present, come first
+#if __has_include("tsp_tuned.h")
+#include "tsp_tuned.h"
+#endif
 #ifndef TSP_TUNE_PREFETCH
 #define TSP_TUNE_PREFETCH 1
 #endif
