@@ -170,6 +170,25 @@ int tsp_profile_read(tsp_ctx *ctx, char *names, uint64_t *launches, double *ms, 
 /* waits for all queued work of the context */
 int tsp_synchronize(tsp_ctx *ctx);
 
+/* Statistics of a scene after its last step, computed on the device: the StatMetric values of
+   SimBackend_GPU::FillStatisticsRecord (GPU/backend/gpu_backend.cpp:1067-1208; names in
+   eval/stats/metrics.h:23-88). Neighbor counts are per source particle, relations FF, FB, FR, RF,
+   RB, RR; p99 is the count at sorted position (99 (n - 1)) / 100. */
+typedef struct tsp_statistics {
+    uint32_t n_fluid, n_boundary, n_rigid;
+    uint32_t solver_iters;           /* NumSolverIters */
+    double nbr_avg[6];               /* NumNbr*_Avg */
+    uint32_t nbr_min[6];             /* NumNbr*_Min */
+    uint32_t nbr_max[6];             /* NumNbr*_Max */
+    uint32_t nbr_p99[6];             /* NumNbr*_P99 */
+    uint32_t pad;
+    double dens_rel_err_avg[3];      /* DensRelError_{Raw,Pos,Abs}_Avg: averages of rho/rho0 - 1 */
+    double cfl;                      /* CFL_Number: max |v| dt / h */
+    double vel_avg, vel_min, vel_max;/* VelMag_{Avg,Min,Max} */
+} tsp_statistics;
+/* I_SimBackend::FillStatisticsRecord (core/backend_sim.h:243) for the first scene */
+int tsp_get_statistics(tsp_ctx *ctx, tsp_statistics *out);
+
 /*
  * Several independent scenes in one context. The reference runs one scene per backend
  * (core/simulation.h:179, one SimBackend per Simulation); these calls let one context hold many
@@ -199,6 +218,8 @@ int tsp_batch_scene_counts(tsp_ctx *ctx, uint32_t scene_id, uint32_t counts[4]);
 int tsp_batch_push_scene(tsp_ctx *ctx, uint32_t scene_id, tsp_state_view *dst);
 /* tsp_get_step_info of one scene (the neighbor-list totals count every scene) */
 int tsp_batch_scene_info(tsp_ctx *ctx, uint32_t scene_id, tsp_step_info *info);
+/* tsp_get_statistics of one scene */
+int tsp_batch_scene_statistics(tsp_ctx *ctx, uint32_t scene_id, tsp_statistics *out);
 /* advances every scene n_steps IISPH steps (ExecuteFullSimStep of each scene's backend) */
 int tsp_batch_step(tsp_ctx *ctx, uint32_t n_steps);
 /* device bytes a scene of these counts holds while resident (an upper estimate at the initial

@@ -49,6 +49,8 @@ int main(void) {
   O(tsp_state_view, body) O(tsp_state_view, step) O(tsp_state_view, time)
   S(tsp_step_info) O(tsp_step_info, solver_iters) O(tsp_step_info, err_sum_raw) O(tsp_step_info, frame_dim)
   O(tsp_step_info, n_ff) O(tsp_step_info, list_bytes) O(tsp_step_info, rigid_force_torque)
+  S(tsp_statistics) O(tsp_statistics, nbr_avg) O(tsp_statistics, nbr_p99) O(tsp_statistics, dens_rel_err_avg)
+  O(tsp_statistics, vel_max)
   S(tsp_scene_spec) O(tsp_scene_spec, fluid_counts) O(tsp_scene_spec, jitter_seed) O(tsp_scene_spec, rigid_quat)
   O(tsp_scene_spec, rigid_omega)
   return 0;
@@ -63,7 +65,8 @@ def test_struct_layouts_match_the_headers(tmp_path):
     subprocess.run(["gcc", "-std=c11", "-I", INCLUDE, str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
     mirrors = {"tsp_params": _capi.TspParams, "tsp_state_view": _capi.TspStateView,
-               "tsp_step_info": _capi.TspStepInfo, "tsp_scene_spec": _capi.TspSceneSpec}
+               "tsp_step_info": _capi.TspStepInfo, "tsp_scene_spec": _capi.TspSceneSpec,
+               "tsp_statistics": _capi.TspStatistics}
     for line in out.strip().splitlines():
         parts = line.split()
         T, cls = parts[0], mirrors[parts[0]]

@@ -144,3 +144,37 @@ def test_cpp_backend_matches_the_python_binding(tmp_path):
     from turbospharticles_amd import _capi
     for k, f in enumerate(_capi.F_FIELDS):
         assert np.array_equal(fields[k], sg.fluid[f]), f
+
+
+def test_statistics_on_the_device_match_the_state():
+    # FillStatisticsRecord's values computed on the device, against the same values computed on
+    # the host from the pushed state and the decoded neighbor counts
+    print("case: device statistics, rigid cube scene, 3 steps", flush=True)
+    st = scenes.build_scene(scenes.rigid_cube_spec(16, H, side=0.06, center=(0.25, 0.15, 0.15), tank=(0.4, 0.3, 0.3)))
+    p = scenes.make_params(H, termination_cond=scenes.TERM_POS)
+    g = HipBackend(0)
+    g.set_params(p)
+    g.pull(st)
+    g.step(3)
+    s = g.statistics()
+    sg = g.push()
+    assert (s.n_fluid, s.n_boundary, s.n_rigid) == st.n[:3]
+    assert s.solver_iters == g.info().solver_iters
+    for rel in range(6):
+        counts, _ = g.neighbors(rel)
+        c = np.sort(counts.astype(np.int64))
+        n = len(c)
+        assert s.nbr_min[rel] == (c[0] if n else 0), rel
+        assert s.nbr_max[rel] == (c[-1] if n else 0), rel
+        assert s.nbr_p99[rel] == (c[(99 * (n - 1)) // 100] if n else 0), rel
+        assert s.nbr_avg[rel] == pytest.approx(c.mean() if n else 0.0, rel=1e-12), rel
+    e = sg.fluid["density"].astype(np.float32) / np.float32(p.rest_density) - np.float32(1.0)
+    e = e.astype(np.float64)
+    assert s.dens_rel_err_avg[0] == pytest.approx(e.mean(), rel=1e-9, abs=1e-12)
+    assert s.dens_rel_err_avg[1] == pytest.approx(np.maximum(e, 0).mean(), rel=1e-9, abs=1e-12)
+    assert s.dens_rel_err_avg[2] == pytest.approx(np.abs(e).mean(), rel=1e-9, abs=1e-12)
+    v = np.sqrt((sg.fluid["vx"].astype(np.float64) ** 2 + sg.fluid["vy"] ** 2.0 + sg.fluid["vz"] ** 2.0))
+    assert s.vel_avg == pytest.approx(v.mean(), rel=1e-6)
+    assert s.vel_max == pytest.approx(v.max(), rel=1e-6)
+    assert s.vel_min == pytest.approx(v.min(), rel=1e-6, abs=1e-9)
+    assert s.cfl == pytest.approx(s.vel_max * p.time_step / p.particle_size, rel=1e-12)

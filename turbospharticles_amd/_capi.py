@@ -86,6 +86,19 @@ class TspStepInfo(ctypes.Structure):
     ]
 
 
+class TspStatistics(ctypes.Structure):
+    _fields_ = [
+        ("n_fluid", ctypes.c_uint32), ("n_boundary", ctypes.c_uint32), ("n_rigid", ctypes.c_uint32),
+        ("solver_iters", ctypes.c_uint32),
+        ("nbr_avg", ctypes.c_double * 6),
+        ("nbr_min", ctypes.c_uint32 * 6), ("nbr_max", ctypes.c_uint32 * 6), ("nbr_p99", ctypes.c_uint32 * 6),
+        ("pad", ctypes.c_uint32),
+        ("dens_rel_err_avg", ctypes.c_double * 3),
+        ("cfl", ctypes.c_double), ("vel_avg", ctypes.c_double), ("vel_min", ctypes.c_double),
+        ("vel_max", ctypes.c_double),
+    ]
+
+
 class TspSceneSpec(ctypes.Structure):
     _fields_ = [
         ("particle_size", ctypes.c_float),
@@ -133,6 +146,8 @@ def _declare_hip(lib):
         "tsp_profile_enable": (i, [vp, i]),
         "tsp_profile_read": (i, [vp, ctypes.c_char_p, _u64p, ctypes.POINTER(ctypes.c_double), i]),
         "tsp_synchronize": (i, [vp]),
+        "tsp_get_statistics": (i, [vp, ctypes.POINTER(TspStatistics)]),
+        "tsp_batch_scene_statistics": (i, [vp, ctypes.c_uint32, ctypes.POINTER(TspStatistics)]),
         "tsp_batch_update": (i, [vp, ctypes.c_uint32, _u32p, ctypes.c_uint32, ctypes.POINTER(TspParams),
                                  ctypes.POINTER(ctypes.POINTER(TspStateView)), _fp, _u32p]),
         "tsp_batch_add_scene": (i, [vp, ctypes.POINTER(TspParams), ctypes.POINTER(TspStateView), _fp, _u32p]),

@@ -136,6 +136,12 @@ class HipBackend:
     def synchronize(self):
         self._check(self.lib.tsp_synchronize(self.h), "tsp_synchronize")
 
+    # I_SimBackend::FillStatisticsRecord
+    def statistics(self):
+        st = _capi.TspStatistics()
+        self._check(self.lib.tsp_get_statistics(self.h, ctypes.byref(st)), "tsp_get_statistics")
+        return st
+
 
 class HipBatch:
     """Many independent scenes in one context, advanced together (tsp_batch_* of libtsp_hip.so).
@@ -229,6 +235,11 @@ class HipBatch:
 
     def synchronize(self):
         self._check(self.lib.tsp_synchronize(self.h), "tsp_synchronize")
+
+    def statistics(self, scene_id):
+        st = _capi.TspStatistics()
+        self._check(self.lib.tsp_batch_scene_statistics(self.h, scene_id, ctypes.byref(st)), "tsp_batch_scene_statistics")
+        return st
 
     def profile_enable(self, on=True):
         self._check(self.lib.tsp_profile_enable(self.h, 1 if on else 0), "tsp_profile_enable")

@@ -53,6 +53,8 @@ namespace tsp
     void launchRigidReduce(hipStream_t, const Dev&);
     void launchRigidPredict(hipStream_t, const Dev&);
     void launchRigidIntegrate(hipStream_t, const Dev&);
+    void launchStatsCounts(hipStream_t, const uint16_t*, const uint32_t*, uint32_t, uint32_t, uint32_t*, uint64_t*);
+    void launchStatsFluid(hipStream_t, const float4*, uint32_t, uint32_t, float, double*, double*);
 }
 
 using namespace tsp;
@@ -281,6 +283,7 @@ struct tsp_ctx
     ~tsp_ctx()
     {
         dropGraphs();
+        releaseList(statAllocs);
         releaseList(poolAllocs);
         releaseList(frameAllocs);
         releaseList(work);
@@ -641,6 +644,9 @@ struct tsp_ctx
         d.rfCnt = alloc<uint32_t>(nR, work);
         d.rbCnt = alloc<uint32_t>(nR, work);
         d.rrCnt = alloc<uint32_t>(nR, work);
+        zero(d.rfCnt, sizeof(uint32_t) * nR);
+        zero(d.rbCnt, sizeof(uint32_t) * nR);
+        zero(d.rrCnt, sizeof(uint32_t) * nR);
         d.rfOff = alloc<uint32_t>(nR + 1ull, work);
         d.errPartial = alloc<double>(3ull * (nF / kBlock), work);
         {
@@ -1286,6 +1292,60 @@ struct tsp_ctx
         }
     }
 
+    // --- statistics of one scene on the device (FillStatisticsRecord, gpu_backend.cpp:1067-1208) ---
+    std::vector<void*> statAllocs;
+    uint32_t* statHist = nullptr;
+    uint64_t* statCounts = nullptr;   // 6 relations x (min, max, p99, sum)
+    double* statFluid = nullptr;      // sums of raw, positive, absolute error and |v|, min and max |v|
+    double* statPartial = nullptr;
+    uint64_t statPartialCap = 0;
+
+    void statistics(uint32_t si, tsp_statistics* out)
+    {
+        const SceneHost& h = scenes[si];
+        const uint64_t nb = (static_cast<uint64_t>(h.nF) + kBlock - 1) / kBlock;
+        if (!statHist || statPartialCap < nb)
+        {
+            sync();
+            releaseList(statAllocs);
+            statHist = alloc<uint32_t>(65536, statAllocs);
+            statCounts = alloc<uint64_t>(24, statAllocs);
+            statFluid = alloc<double>(6, statAllocs);
+            statPartialCap = std::max<uint64_t>(nb, 1024);
+            statPartial = alloc<double>(6 * statPartialCap, statAllocs);
+        }
+        const uint16_t* c16[3] = { d.cnt[REL_FF] + h.offF, d.cnt[REL_FB] + h.offF, d.cnt[REL_FR] + h.offF };
+        for (int r = 0; r < 3; r++)
+        {
+            launchStatsCounts(stream, c16[r], nullptr, r == REL_FF ? 0x7fffu : 0xffffu, h.nF, statHist, statCounts + 4 * r);
+        }
+        const uint32_t* c32[3] = { d.rfCnt + h.offR, d.rbCnt + h.offR, d.rrCnt + h.offR };
+        for (int r = 0; r < 3; r++) { launchStatsCounts(stream, nullptr, c32[r], 0u, h.nR, statHist, statCounts + 4 * (3 + r)); }
+        launchStatsFluid(stream, d.vel, h.offF, h.nF, h.params.rest_density, statPartial, statFluid);
+        uint64_t cnt[24];
+        double fl[6];
+        hipCheck(hipMemcpyAsync(cnt, statCounts, sizeof(cnt), hipMemcpyDeviceToHost, stream), "D2H");
+        hipCheck(hipMemcpyAsync(fl, statFluid, sizeof(fl), hipMemcpyDeviceToHost, stream), "D2H");
+        sync();
+        std::memset(out, 0, sizeof(*out));
+        out->n_fluid = h.nF; out->n_boundary = h.nB; out->n_rigid = h.nR;
+        out->solver_iters = h.nF ? h.last.iters : 0;
+        for (int r = 0; r < 6; r++)
+        {
+            const uint32_t n = r < 3 ? h.nF : h.nR;
+            out->nbr_min[r] = static_cast<uint32_t>(cnt[4 * r]);
+            out->nbr_max[r] = static_cast<uint32_t>(cnt[4 * r + 1]);
+            out->nbr_p99[r] = static_cast<uint32_t>(cnt[4 * r + 2]);
+            out->nbr_avg[r] = n ? static_cast<double>(cnt[4 * r + 3]) / n : 0.0;
+        }
+        const double inv = h.nF ? 1.0 / h.nF : 0.0;
+        for (int q = 0; q < 3; q++) { out->dens_rel_err_avg[q] = fl[q] * inv; }
+        out->vel_avg = fl[3] * inv;
+        out->vel_min = h.nF ? fl[4] : 0.0;
+        out->vel_max = h.nF ? fl[5] : 0.0;
+        out->cfl = out->vel_max * h.params.time_step / h.params.particle_size;
+    }
+
     // one relayout: the listed scenes leave, new scenes join at the end
     void update(uint32_t nRemove, const uint32_t* removeIds, uint32_t nAdd, const tsp_params* params,
                 const tsp_state_view* const* views, const float* domains, uint32_t* outIds)
@@ -1699,6 +1759,25 @@ int tsp_profile_read(tsp_ctx* ctx, char* names, uint64_t* launches, double* ms, 
         q++;
     }
     return q;
+}
+
+int tsp_get_statistics(tsp_ctx* ctx, tsp_statistics* out)
+{
+    return guarded(ctx, [&]() {
+        if (!out) { throw Error{ TSP_ERR_INVALID, "null statistics" }; }
+        ctx->requireScene();
+        ctx->statistics(0, out);
+        return TSP_OK;
+    });
+}
+
+int tsp_batch_scene_statistics(tsp_ctx* ctx, uint32_t scene_id, tsp_statistics* out)
+{
+    return guarded(ctx, [&]() {
+        if (!out) { throw Error{ TSP_ERR_INVALID, "null statistics" }; }
+        ctx->statistics(ctx->sceneAt(scene_id), out);
+        return TSP_OK;
+    });
 }
 
 int tsp_synchronize(tsp_ctx* ctx)
