@@ -118,6 +118,7 @@ namespace tsp
         __syncthreads();
     }
 
+    template <bool kFast>
     __global__ void __launch_bounds__(kBlock) kDensity(Dev d)
     {
         __shared__ float4 hp[kHaloMax];
@@ -132,7 +133,7 @@ namespace tsp
         const float xi = pi.x, yi = pi.y, zi = pi.z;
         const List ff = listOf(d, REL_FF, i);
         float wsum = 0.0f;
-        const auto ffTerm = [&](const float4& pj) { wsum += kwUnscaled(dist2(xi - pj.x, yi - pj.y, zi - pj.z), k); };
+        const auto ffTerm = [&](const float4& pj) { wsum += kwUnscaled<kFast>(dist2(xi - pj.x, yi - pj.y, zi - pj.z), k); };
         visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(hp[q]); }, [&](uint32_t j, uint32_t) { ffTerm(d.pos[j]); });
         const float dF = wsum * k.weightScale;
         float dB = 0.0f, dR = 0.0f;
@@ -219,6 +220,7 @@ namespace tsp
         d.vp[i] = make_float4(__builtin_fmaf(ax, k.dt, vxi), __builtin_fmaf(ay, k.dt, vyi), __builtin_fmaf(az, k.dt, vzi), 0.0f);
     }
 
+    template <bool kFast>
     __global__ void __launch_bounds__(kBlock) kSourceDiag(Dev d)
     {
         __shared__ float4 hp[kHaloMax], hv[kHaloMax];
@@ -238,7 +240,7 @@ namespace tsp
             const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
             const float dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
             const float d2 = dist2(dx, dy, dz);
-            const float kg = kgFactor(d2, k);
+            const float kg = kgFactor<kFast>(d2, k);
 #if TSP_TUNE_KG_STORE
             // the sweeps of this step read the pair's gradient factor instead of recomputing it
             d.kgFF[e] = kg;
@@ -859,11 +861,20 @@ namespace tsp
 
     static inline uint32_t blocksOf(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
 
-    void launchDensity(hipStream_t s, const Dev& d) { if (d.nF) { kDensity<<<blocksOf(d.nF), kBlock, 0, s>>>(d); } }
+    void launchDensity(hipStream_t s, const Dev& d)
+    {
+        if (!d.nF) { return; }
+        if (d.fastFF) { kDensity<true><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
+        else { kDensity<false><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
+    }
     void launchNonPressure(hipStream_t s, const Dev& d) { if (d.nF) { kNonPressure<<<blocksOf(d.nF), kBlock, 0, s>>>(d); } }
     void launchSourceDiag(hipStream_t s, const Dev& d)
     {
-        if (d.nF) { kSourceDiag<<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
+        if (d.nF)
+        {
+            if (d.fastFF) { kSourceDiag<true><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
+            else { kSourceDiag<false><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
+        }
         if (d.nScenes) { kSolverInit<<<1, kBlock, 0, s>>>(d); }
     }
     void launchPressureAcc(hipStream_t s, const Dev& d, bool sweep)

@@ -350,9 +350,10 @@ namespace tsp
     }
 
     // KernelWeightUnscaled_CubicSpline_SIMD, CPU/simd.h:392-404
+    template <bool kFast = false>
     __device__ __forceinline__ float kwUnscaled(float d2, const Consts& k)
     {
-        const float dist = __builtin_sqrtf(d2);
+        const float dist = kFast ? sqrtCR(d2) : __builtin_sqrtf(d2);
         const float t1 = vmax(k.ksize - dist, 0.0f);
         const float t2 = vmax(k.ktwo - dist, 0.0f);
         return __builtin_fmaf(-4.0f, t1 * t1 * t1, t2 * t2 * t2);
