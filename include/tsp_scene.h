@@ -51,6 +51,22 @@ void tsp_scene_free(tsp_scene *s);
 /* uniform random unit quaternion from a seed (w, x, y, z) */
 void tsp_scene_random_quat(uint64_t seed, float q[4]);
 
+/* The reference's simulation state file, TSPSIM2 (eval/serial/state_format.h:10-120,
+   serialization.cpp:164-244, 860-935: SaveUniversalSimState / LoadUniversalSimState): a 64-byte
+   header, then the SoA arrays of the fluid, boundary, rigid and body populations in the field
+   order of tsp_state_view. */
+/* counts (fluid, boundary, rigid, bodies), step and time of a state file; checks the header and
+   the file length as the reference's reader does */
+int tsp_state_file_info(const char *path, uint32_t counts[4], uint64_t *step, double *time);
+/* reads a state file into caller-owned arrays sized by tsp_state_file_info (null arrays are
+   skipped); sets the view's step and time */
+int tsp_state_file_read(const char *path, tsp_state_view *dst);
+/* writes a state file from a view (null arrays are written as zeros) through a ".tmp" file
+   renamed over the destination; a view with the invalid id (UINT64_MAX), a rigid particle of a
+   missing body or body counts that disagree with the memberships is refused and nothing is
+   written. tsp_state_file_read refuses the same contents. */
+int tsp_state_file_write(const char *path, const tsp_state_view *src);
+
 #ifdef __cplusplus
 }
 #endif

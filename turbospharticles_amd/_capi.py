@@ -177,6 +177,9 @@ def _declare_host(lib):
         "tsp_scene_export": (i, [vp, ctypes.POINTER(TspStateView)]),
         "tsp_scene_free": (None, [vp]),
         "tsp_scene_random_quat": (None, [ctypes.c_uint64, _fp]),
+        "tsp_state_file_info": (i, [ctypes.c_char_p, _u32p, _u64p, ctypes.POINTER(ctypes.c_double)]),
+        "tsp_state_file_read": (i, [ctypes.c_char_p, ctypes.POINTER(TspStateView)]),
+        "tsp_state_file_write": (i, [ctypes.c_char_p, ctypes.POINTER(TspStateView)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -272,3 +275,27 @@ class HostState:
         v.step = self.step
         v.time = self.time
         return v
+
+
+def save_state(path, state):
+    """Writes a HostState as the reference's TSPSIM2 simulation state file."""
+    v = state.view()
+    rc = host_lib().tsp_state_file_write(os.fsencode(path), ctypes.byref(v))
+    if rc != TSP_OK:
+        raise RuntimeError(f"tsp_state_file_write failed ({rc}): {path}")
+
+
+def load_state(path):
+    """Reads a TSPSIM2 simulation state file into a new HostState (no domain)."""
+    lib = host_lib()
+    counts = (ctypes.c_uint32 * 4)()
+    rc = lib.tsp_state_file_info(os.fsencode(path), counts, None, None)
+    if rc != TSP_OK:
+        raise RuntimeError(f"tsp_state_file_info failed ({rc}): {path}")
+    st = HostState(*counts)
+    v = st.view()
+    rc = lib.tsp_state_file_read(os.fsencode(path), ctypes.byref(v))
+    if rc != TSP_OK:
+        raise RuntimeError(f"tsp_state_file_read failed ({rc}): {path}")
+    st.step, st.time = int(v.step), float(v.time)
+    return st

@@ -108,6 +108,18 @@ def build_scene(spec):
     return st
 
 
+def save_state(path, state):
+    """Writes a state as the reference's TSPSIM2 restart file (SaveUniversalSimState,
+    eval/serial/serialization.cpp:830-895)."""
+    _capi.save_state(path, state)
+
+
+def load_state(path):
+    """Reads a TSPSIM2 restart file (LoadUniversalSimState, serialization.cpp:897-935); the
+    returned state has no domain (the file holds none)."""
+    return _capi.load_state(path)
+
+
 # --- mixed-size scene sets: each scene's tank scaled to its own fluid block ---------------------
 
 TANK_RATIO = (2.5, 1.25, 1.25)   # tank / fluid block edge, as the 0.4 m block in the 1.0 x 0.5 x 0.5 tank
