@@ -54,6 +54,8 @@ typedef struct tsp_params {
     uint32_t min_solver_iters;
     uint32_t max_solver_iters;
     uint32_t termination_cond; /* Base::SolverTerminationCond: 0 FixedIters, 1 Raw, 2 Pos, 3 Abs */
+    float rigid_contact_density_dev_target;   /* rigidContactDensityDevTarget (default 0.001) */
+    uint32_t rigid_contact_max_solver_iters;  /* rigidContactMaxSolverIters (default 100) */
 } tsp_params;
 
 /* SoA field order of sync/state_sim.h: FluidParticles :101-131, BoundaryParticles :272-285,

@@ -31,8 +31,11 @@
  * Differences from the reference binary that no test can pin here (the reference cannot be
  * built in this container): TBB's parallel_reduce join order for the FP64 Jacobi error sums
  * and for the rigid force sums is run-dependent in the reference; the oracle sums serially
- * in particle order. The rigid-body contact solver (rigid_solver.cpp:602) is not restated;
- * orc_info.contact_pairs reports when it would have been active.
+ * in particle order (the rigid force sums in FP64 unless set_rigid_sums_f64(0)). The same
+ * holds for the rigid contact solver's FP64 error sums and its contact force sums.
+ *   - source/simulation/CPU/backend/rigid_solver.cpp:23-799 (rigid-boundary and rigid-rigid
+ *     contact IISPH: contact lists, geometry, density and diagonal, relaxation, pressure
+ *     forces, velocities, density error, relaxed Jacobi, SolveRigidContacts)
  */
 #ifndef TSP_IISPH_ORACLE_H
 #define TSP_IISPH_ORACLE_H
@@ -63,6 +66,8 @@ typedef struct orc_params {
     uint32_t min_solver_iters;
     uint32_t max_solver_iters;
     uint32_t termination_cond;  /* 0 FixedIters, 1 Raw, 2 Pos, 3 Abs (base/enum_hub.h:212-218) */
+    float rigid_contact_density_dev_target;   /* rigidContactDensityDevTarget (0.001) */
+    uint32_t rigid_contact_max_solver_iters;  /* rigidContactMaxSolverIters (100) */
 } orc_params;
 
 /* field order of the SoA structs in sync/state_sim.h:101-131, :272-285, :382-396, :497-522 */
@@ -100,7 +105,13 @@ typedef struct orc_info {
     uint32_t frame_builds;
     uint32_t n_dict_chunks, n_chunks;
     uint64_t n_ff, n_fb, n_fr, n_rf, n_rb, n_rr;  /* total list entries per relation */
-    uint32_t contact_pairs;         /* RB + RR entries: the contact solver would run */
+    uint32_t contact_pairs;         /* RB + RR entries of the last search */
+    /* RigidContactReport of the last step (rigid_solver.h) */
+    uint32_t contact_iters;         /* N_solverIters */
+    float contact_avg_err;          /* avgDensityError */
+    uint32_t contact_converged;
+    uint32_t contact_boundary;      /* boundary particles in contact (localIdxsB.size()) */
+    uint64_t contacts;              /* N_contacts of the relaxation */
 } orc_info;
 
 /* relations for orc*_get_list */
@@ -129,6 +140,15 @@ enum { ORC_FF, ORC_FB, ORC_FR, ORC_RF, ORC_RB, ORC_RR };
                        uint32_t *order_b, uint64_t *keys_b);                             \
     void P##_get_rigid_world(P##_ctx *c, float *xyz, float *vel);                        \
     void P##_get_body_force(P##_ctx *c, float *ft);                                      \
+    /* the contact solver's pieces (tests): after neighbor_search, sets up the contact      \
+       problem and, with the given contact pressures (null: zero), evaluates the body loads \
+       and the residuals once. Returns the count of contact boundary particles, or < 0.     \
+       Every output may be null; rigid arrays have n_rigid entries, boundary arrays the     \
+       returned count, loads 6 per body. */                                                 \
+    int64_t P##_contact_probe(P##_ctx *c, const float *p_rigid, const float *p_boundary,   \
+                              float *rho_r, float *diag_r, float *omega_r, float *err_r,    \
+                              float *rho_b, float *diag_b, float *omega_b, float *err_b,    \
+                              uint32_t *local_b, float *loads);                             \
     /* single kernels over explicit per-particle lists (tests of closed forms) */        \
     void P##_kernel_weight(const orc_params *p, const float *dxyz, uint32_t n,           \
                            float *w_out, float *grad_out, float *w_unscaled_out);        \

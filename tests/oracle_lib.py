@@ -43,6 +43,9 @@ class OrcInfo(ctypes.Structure):
         ("n_ff", ctypes.c_uint64), ("n_fb", ctypes.c_uint64), ("n_fr", ctypes.c_uint64),
         ("n_rf", ctypes.c_uint64), ("n_rb", ctypes.c_uint64), ("n_rr", ctypes.c_uint64),
         ("contact_pairs", ctypes.c_uint32),
+        ("contact_iters", ctypes.c_uint32), ("contact_avg_err", ctypes.c_float),
+        ("contact_converged", ctypes.c_uint32), ("contact_boundary", ctypes.c_uint32),
+        ("contacts", ctypes.c_uint64),
     ]
 
 
@@ -82,6 +85,8 @@ def lib():
                 "kernel_weight": (None, [ctypes.POINTER(OrcParams), _fp, ctypes.c_uint32, _fp, _fp, _fp]),
                 "kernels_from_lists": (ctypes.c_int, [vp, _u32p, _u32p, _u8p, _u32p, _u32p, _u32p, _u32p,
                                                       ctypes.c_int]),
+                "contact_probe": (ctypes.c_int64, [vp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp,
+                                                   _u32p, _fp]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, f"{p}_{name}")
@@ -195,6 +200,30 @@ class Oracle:
         ft = np.zeros(6 * self.n[3], np.float32)
         self.fn("get_body_force")(self.h, _capi.fptr(ft))
         return ft.reshape(-1, 6)
+
+    def contact_probe(self, p_rigid=None, p_boundary=None):
+        """The contact problem after the last neighbor search, evaluated once with the given
+        contact pressures: a dict of per-particle arrays (rigid *_r, contact boundary *_b), the
+        boundary particles' local indices and the contact loads per body (force, torque)."""
+        f = self.fn("contact_probe")
+        nCB = f(self.h, None, None, *([None] * 10))
+        assert nCB >= 0, nCB
+        nR, nRB = self.n[2], self.n[3]
+        out = {k: np.zeros(nR, np.float32) for k in ("rho_r", "diag_r", "omega_r", "err_r")}
+        out.update({k: np.zeros(max(nCB, 1), np.float32) for k in ("rho_b", "diag_b", "omega_b", "err_b")})
+        out["local_b"] = np.zeros(max(nCB, 1), np.uint32)
+        out["loads"] = np.zeros(6 * max(nRB, 1), np.float32)
+        pr = None if p_rigid is None else np.ascontiguousarray(p_rigid, np.float32)
+        pb = None if p_boundary is None else np.ascontiguousarray(p_boundary, np.float32)
+        fp = _capi.fptr
+        rc = f(self.h, fp(pr), fp(pb), fp(out["rho_r"]), fp(out["diag_r"]), fp(out["omega_r"]), fp(out["err_r"]),
+               fp(out["rho_b"]), fp(out["diag_b"]), fp(out["omega_b"]), fp(out["err_b"]),
+               _capi.u32ptr(out["local_b"]), fp(out["loads"]))
+        assert rc == nCB, rc
+        for k in ("rho_b", "diag_b", "omega_b", "err_b", "local_b"):
+            out[k] = out[k][:nCB]
+        out["loads"] = out["loads"][:6 * nRB].reshape(-1, 6)
+        return out
 
     def kernels_from_lists(self, ff, fb, fr, ff_dict, stage):
         def csr(lists):

@@ -55,6 +55,8 @@ class TspParams(ctypes.Structure):
         ("min_solver_iters", ctypes.c_uint32),
         ("max_solver_iters", ctypes.c_uint32),
         ("termination_cond", ctypes.c_uint32),
+        ("rigid_contact_density_dev_target", ctypes.c_float),
+        ("rigid_contact_max_solver_iters", ctypes.c_uint32),
     ]
 
 

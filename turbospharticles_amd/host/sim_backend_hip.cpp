@@ -41,6 +41,8 @@ namespace tsp
             c.min_solver_iters = p.minSolverIters;
             c.max_solver_iters = p.maxSolverIters;
             c.termination_cond = static_cast<uint32_t>(p.solverTerminationCond);
+            c.rigid_contact_density_dev_target = p.rigidContactDensityDevTarget;
+            c.rigid_contact_max_solver_iters = p.rigidContactMaxSolverIters;
             return c;
         }
 

@@ -93,6 +93,8 @@ namespace tsp
         uint32_t minSolverIters = 2;
         uint32_t maxSolverIters = 100;
         SolverTerminationCond solverTerminationCond = SolverTerminationCond::RelDensityError_Abs;
+        float rigidContactDensityDevTarget = 0.001f;   // parameters_sim.h:100-101
+        uint32_t rigidContactMaxSolverIters = 100;
 
         // the derived values of the reference's setters (core/parameters_sim.cpp:1478-1483)
         void SetParticleSize(float h);

@@ -21,7 +21,8 @@ def make_params(h, *, time_step=1e-3, rest_density=1000.0, viscosity_fluid=4e-5,
                 viscosity_boundary=8e-5, gravity=(0.0, -9.81, 0.0), press_gamma=0.5,
                 jacobi_omega=0.5, jacobi_warm_start=0.0, density_dev_target=0.001,
                 fixed_iters=5, min_solver_iters=2, max_solver_iters=100,
-                termination_cond=TERM_ABS, grid_cell_size_ratio=1.0):
+                termination_cond=TERM_ABS, grid_cell_size_ratio=1.0,
+                rigid_contact_density_dev_target=0.001, rigid_contact_max_solver_iters=100):
     f32 = np.float32
     h = f32(h)
     rho0 = f32(rest_density)
@@ -45,6 +46,8 @@ def make_params(h, *, time_step=1e-3, rest_density=1000.0, viscosity_fluid=4e-5,
     p.min_solver_iters = min_solver_iters
     p.max_solver_iters = max_solver_iters
     p.termination_cond = termination_cond
+    p.rigid_contact_density_dev_target = f32(rigid_contact_density_dev_target)
+    p.rigid_contact_max_solver_iters = rigid_contact_max_solver_iters
     return p
 
 
