@@ -94,8 +94,8 @@ typedef struct tsp_state_view {
 enum {
     TSP_FLAG_MAX_ITERS = 1u << 0,      /* the last solve stopped at max_solver_iters unconverged */
     TSP_FLAG_FRAME_REBUILT = 1u << 1,  /* the cell frame was rebuilt during the call */
-    TSP_FLAG_RIGID_CONTACT = 1u << 2,  /* rigid particles have boundary/rigid neighbors: the
-                                          reference CPU would run its contact solver here */
+    TSP_FLAG_RIGID_CONTACT = 1u << 2,  /* rigid particles had boundary neighbors or neighbors of
+                                          other bodies: the rigid contact solve ran */
     TSP_FLAG_LIST_REGROWN = 1u << 3    /* a neighbor list pool was regrown and the query re-run */
 };
 
@@ -113,6 +113,12 @@ typedef struct tsp_step_info {
     uint64_t n_ff, n_fb, n_fr, n_rf, n_rb, n_rr; /* neighbor list entries of the last search */
     uint64_t list_bytes;          /* bytes of the SELL pools as stored */
     float rigid_force_torque[6];  /* body 0: force and torque of the last step */
+    /* rigid contact solve of the last step (RigidContactReport, CPU/backend/rigid_solver.h) */
+    uint32_t contact_iters;       /* N_solverIters */
+    float contact_avg_err;        /* avgDensityError */
+    uint32_t contact_converged;
+    uint32_t contact_boundary;    /* boundary particles in contact */
+    uint64_t contacts;            /* N_contacts of the relaxation */
 } tsp_step_info;
 
 typedef struct tsp_ctx tsp_ctx;

@@ -85,6 +85,9 @@ class TspStepInfo(ctypes.Structure):
         ("n_rf", ctypes.c_uint64), ("n_rb", ctypes.c_uint64), ("n_rr", ctypes.c_uint64),
         ("list_bytes", ctypes.c_uint64),
         ("rigid_force_torque", ctypes.c_float * 6),
+        ("contact_iters", ctypes.c_uint32), ("contact_avg_err", ctypes.c_float),
+        ("contact_converged", ctypes.c_uint32), ("contact_boundary", ctypes.c_uint32),
+        ("contacts", ctypes.c_uint64),
     ]
 
 

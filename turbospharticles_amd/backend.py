@@ -93,8 +93,6 @@ class HipBackend:
         total = ctypes.c_uint64()
         self._check(self.lib.tsp_get_neighbors(self.h, rel, _capi.u32ptr(counts), None, 0, ctypes.byref(total)),
                     "tsp_get_neighbors")
-        if rel in (4, 5):
-            return counts, None
         idx = np.zeros(max(total.value, 1), np.uint32)
         self._check(self.lib.tsp_get_neighbors(self.h, rel, _capi.u32ptr(counts), _capi.u32ptr(idx), total.value,
                                                ctypes.byref(total)), "tsp_get_neighbors")

@@ -105,7 +105,13 @@ namespace tsp
         constexpr uint32_t words = static_cast<uint32_t>(sizeof(Status) / sizeof(uint32_t));
         static_assert(words <= kBlock, "the reset covers one block");
         if (threadIdx.x < words) { reinterpret_cast<uint32_t*>(d.status)[threadIdx.x] = 0u; }
-        for (uint32_t s = threadIdx.x; s < d.nScenes; s += kBlock) { d.solve[s].abort = 0u; }
+        for (uint32_t s = threadIdx.x; s < d.nScenes; s += kBlock)
+        {
+            Solve& so = d.solve[s];
+            so.abort = 0u;
+            so.contactActive = 0u; so.contactDone = 0u; so.contactIters = 0u; so.contactConverged = 0u;
+            so.contactAvg = 0.0f; so.contactBoundary = 0u; so.contactBad = 0u; so.contacts = 0ull;
+        }
     }
 
     // ----------------------------------------------------------------------------------------
@@ -897,10 +903,13 @@ namespace tsp
     }
 
     // ----------------------------------------------------------------------------------------
-    // rigid-source lists (RF CSR, RB/RR counts), one thread per rigid particle
+    // rigid-source lists, one thread per rigid particle: mode 0 counts RF, RB and RR; mode 1
+    // fills the RF CSR; mode 2 fills RF and the RB and RR lists of the contact solver
 
-    __global__ void __launch_bounds__(kBlock) kQueryRigid(Dev d, bool fill)
+    __global__ void __launch_bounds__(kBlock) kQueryRigid(Dev d, int mode)
     {
+        const bool fill = mode != 0;
+        const bool contact = mode == 2;
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= d.nR) { return; }
         const uint32_t sc = d.bodyScene[d.rbody[i]];
@@ -913,6 +922,7 @@ namespace tsp
         const int cz = floorToInt(pi.z * f.cellSizeInv) - f.orig[2];
         uint32_t nF = 0, nB = 0, nRR = 0;
         uint32_t w = fill ? d.rfOff[i] : 0u;
+        uint32_t wb = contact ? d.rbOff[i] : 0u, wr = contact ? d.rrOff[i] : 0u;
         const bool inside = cx >= 0 && cx < static_cast<int>(f.dim[0]) && cy >= 0 && cy < static_cast<int>(f.dim[1])
                          && cz >= 0 && cz < static_cast<int>(f.dim[2]);
         if (inside)
@@ -936,18 +946,24 @@ namespace tsp
                                 if (fill) { d.rfIdx[w++] = j; } else { nF++; }
                             }
                         }
-                        if (fill) { continue; }
+                        if (fill && !contact) { continue; }
                         const uint2 rb = d.cellB[lin];
                         for (uint32_t j = rb.x; j < rb.y; j++)
                         {
-                            if (within(pi.x, pi.y, pi.z, d.bpos[j], r2)) { nB++; }
+                            if (within(pi.x, pi.y, pi.z, d.bpos[j], r2))
+                            {
+                                if (contact) { d.rbIdx[wb++] = j; } else { nB++; }
+                            }
                         }
                         const uint2 rr = d.cellR[lin];
                         for (uint32_t t = rr.x; t < rr.y; t++)
                         {
                             const uint32_t j = d.idxR[t];
                             if (j == i || d.rbody[j] == body) { continue; }
-                            if (within(pi.x, pi.y, pi.z, d.rpos[j], r2)) { nRR++; }
+                            if (within(pi.x, pi.y, pi.z, d.rpos[j], r2))
+                            {
+                                if (contact) { d.rrIdx[wr++] = j; } else { nRR++; }
+                            }
                         }
                     }
                 }
@@ -963,11 +979,17 @@ namespace tsp
         }
     }
 
-    // totals of the rigid-source relations and the RF capacity check (after the RF offset scan)
+    // totals of the rigid-source relations and the RF capacity check (after the RF offset scan);
+    // a scene whose rigid particles have RB or RR entries runs the contact solve
+    // (BuildRigidBoundaryNeighbors, rigid_solver.cpp:90-149)
     __global__ void __launch_bounds__(kBlock) kRigidTotals(Dev d)
     {
         __shared__ uint32_t red[2][kBlock];
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+        if (i < d.nR && (d.rbCnt[i] | d.rrCnt[i]) != 0u)
+        {
+            d.solve[d.bodyScene[d.rbody[i]]].contactActive = 1u;
+        }
         red[0][threadIdx.x] = i < d.nR ? d.rbCnt[i] : 0u;
         red[1][threadIdx.x] = i < d.nR ? d.rrCnt[i] : 0u;
         __syncthreads();
@@ -1156,11 +1178,11 @@ namespace tsp
         if (d.nF) { kHaloBuild<<<blocksFor(d.nF), kBlock, 0, s>>>(d); }
     }
 
-    void launchQueryRigid(hipStream_t s, const Dev& d, bool fill, uint32_t* tmp)
+    void launchQueryRigid(hipStream_t s, const Dev& d, int mode, uint32_t* tmp)
     {
         if (!d.nR) { return; }
-        kQueryRigid<<<blocksFor(d.nR), kBlock, 0, s>>>(d, fill);
-        if (!fill)
+        kQueryRigid<<<blocksFor(d.nR), kBlock, 0, s>>>(d, mode);
+        if (mode == 0)
         {
             launchScan(s, d.rfOff, d.nR + 1, tmp);
             kRigidTotals<<<blocksFor(d.nR), kBlock, 0, s>>>(d);

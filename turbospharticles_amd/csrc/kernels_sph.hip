@@ -637,14 +637,7 @@ namespace tsp
     }
 
     // ----------------------------------------------------------------------------------------
-    // rigid bodies; the body state is field-major: rb[field * nRB + body]
-
-    enum
-    {
-        RB_XCM = 0, RB_VX = 3, RB_WX = 6, RB_LX = 9, RB_A00 = 12, RB_MASS = 21, RB_MASS_INV = 22, RB_I00 = 23
-    };
-
-    __device__ __forceinline__ float rbv(const Dev& d, int f, uint32_t b) { return d.rb[f * d.nRB + b]; }
+    // rigid bodies; the body state is field-major: rb[field * nRB + body] (rbv)
 
     // UpdateRigidParticles(predicted), cpu_backend.cpp:2009-2063
     __global__ void __launch_bounds__(kBlock) kRigidUpdate(Dev d, bool predicted)

@@ -37,7 +37,12 @@ namespace tsp
     void launchQueryCount(hipStream_t, const Dev&);
     void launchScanSlices(hipStream_t, const Dev&, uint32_t*, uint32_t*);
     void launchQueryFill(hipStream_t, const Dev&);
-    void launchQueryRigid(hipStream_t, const Dev&, bool, uint32_t*);
+    void launchQueryRigid(hipStream_t, const Dev&, int, uint32_t*);
+    void launchContactOffsets(hipStream_t, const Dev&, uint32_t*);
+    void launchContactSetup(hipStream_t, const Dev&, uint32_t, uint32_t*);
+    void launchContactSource(hipStream_t, const Dev&, uint32_t);
+    void launchContactIteration(hipStream_t, const Dev&, uint32_t);
+    void launchContactApply(hipStream_t, const Dev&);
     void launchHaloBuild(hipStream_t, const Dev&);
     void launchAabb(hipStream_t, const Dev&, int, const float4*, uint32_t, uint32_t*);
     void launchDensity(hipStream_t, const Dev&);
@@ -179,6 +184,8 @@ namespace
         k.fixedIters = p.fixed_iters;
         k.termCond = p.termination_cond;
         k.nF = nF;
+        k.contactDevTarget = p.rigid_contact_density_dev_target;
+        k.contactMaxIters = p.rigid_contact_max_solver_iters;
         return k;
     }
 
@@ -189,6 +196,17 @@ namespace
             || p.max_solver_iters < 1)
         {
             throw Error{ TSP_ERR_INVALID, "invalid solver parameters" };
+        }
+    }
+
+    // SolveRigidContacts' check (rigid_solver.cpp:604-610), made before every step of a scene
+    // with bodies as in the reference
+    void validateContactParams(const tsp_params& p)
+    {
+        if (!std::isfinite(p.rigid_contact_density_dev_target) || !(p.rigid_contact_density_dev_target > 0.0f)
+            || p.rigid_contact_max_solver_iters == 0)
+        {
+            throw Error{ TSP_ERR_INVALID, "Invalid rigid contact solver tolerance or iteration limit." };
         }
     }
 
@@ -254,7 +272,14 @@ struct tsp_ctx
     std::vector<void*> work;        // search and scene-table buffers sized by the layout
     std::vector<void*> frameAllocs; // cell tables
     std::vector<void*> poolAllocs;  // SELL-32 pools and the RF list
+    std::vector<void*> contactAllocs; // RB/RR lists and the contact boundary arrays (grown on demand)
     uint64_t cellCap = 0;
+
+    // rigid contacts of the last search
+    bool contactLists = false;      // RB or RR entries exist: the lists are stored
+    uint64_t contactRB = 0;         // RB entries (bounds the contact boundary particles)
+    uint32_t* contactScanTmp = nullptr;
+    uint32_t lastContactEvals = 1;  // evaluations of the last contact solve (the first group)
 
     bool layoutDirty = true;        // key bases and cell offsets must be assigned again
     bool cellsClean = false;        // fluid and rigid cell tables hold no runs
@@ -285,6 +310,7 @@ struct tsp_ctx
         dropGraphs();
         releaseList(statAllocs);
         releaseList(poolAllocs);
+        releaseList(contactAllocs);
         releaseList(frameAllocs);
         releaseList(work);
         releaseList(arena);
@@ -374,6 +400,78 @@ struct tsp_ctx
 #else
         d.kgFF = nullptr;
 #endif
+    }
+
+    // --- storage of the contact solve sized by the RB and RR entries of a search ---
+    void ensureContactCapacity(uint64_t rbTotal, uint64_t rrTotal)
+    {
+        if (rbTotal <= d.rbCap && rrTotal <= d.rrCap && !contactAllocs.empty()) { return; }
+        if (rbTotal > 0x7fffffffull || rrTotal > 0x7fffffffull)
+        {
+            throw Error{ TSP_ERR_CAPACITY, "rigid contact lists exceed 2^31 entries" };
+        }
+        const uint64_t rbCap = std::max<uint64_t>(d.rbCap, rbTotal + rbTotal / 4 + 256);
+        const uint64_t rrCap = std::max<uint64_t>(d.rrCap, rrTotal + rrTotal / 4 + 256);
+        sync();
+        releaseList(contactAllocs);
+        d.rbCap = rbCap;
+        d.rrCap = rrCap;
+        d.rbIdx = alloc<uint32_t>(rbCap, contactAllocs);
+        d.rrIdx = alloc<uint32_t>(rrCap, contactAllocs);
+        d.cbLocal = alloc<uint32_t>(rbCap, contactAllocs);
+        d.brOff = alloc<uint32_t>(rbCap + 1, contactAllocs);
+        d.brPos = alloc<uint32_t>(rbCap + 1, contactAllocs);
+        d.brIdx = alloc<uint32_t>(rbCap, contactAllocs);
+        d.cB = alloc<float>(static_cast<uint64_t>(kCB) * rbCap, contactAllocs);
+        d.cErrB = alloc<double>(rbCap, contactAllocs);
+        // block sums of the contact scans (boundary flags, BR offsets)
+        const uint64_t m = std::max<uint64_t>(d.nB + 1ull, rbCap + 1);
+        contactScanTmp = alloc<uint32_t>((m + 2047ull) / 2048ull + 1ull, contactAllocs);
+    }
+
+    // SolveRigidContacts (CPU/backend/rigid_solver.cpp:602-799) for every scene whose rigid
+    // particles have RB or RR entries: set-up, source terms from the predicted body motion, then
+    // groups of evaluations (each scene stops on its own), then the loads added to the bodies
+    void solveContacts()
+    {
+        uint32_t maxEvals = 0;
+        for (uint32_t s = 0; s < nScenes(); s++)
+        {
+            if (!hostSolve[s].contactActive) { continue; }
+            const tsp_params& p = scenes[s].params;
+            validateContactParams(p);
+            maxEvals = std::max(maxEvals, p.rigid_contact_max_solver_iters + 1u);
+        }
+        if (maxEvals == 0) { return; }
+        Scope sc(prof, stream, "contact");
+        const uint32_t nCBMax = static_cast<uint32_t>(contactRB);
+        launchContactSetup(stream, d, nCBMax, contactScanTmp);
+        launchRigidPredict(stream, d);
+        launchRigidUpdate(stream, d, true);
+        launchContactSource(stream, d, nCBMax);
+        uint32_t group = std::max(1u, std::min(lastContactEvals, maxEvals));
+        uint32_t queued = 0, evals = 0;
+        for (;;)
+        {
+            const uint32_t n = std::min(group, maxEvals - queued);
+            for (uint32_t q = 0; q < n; q++) { launchContactIteration(stream, d, nCBMax); }
+            queued += n;
+            readStatus();
+            bool all = true;
+            evals = 0;
+            for (uint32_t s = 0; s < nScenes(); s++)
+            {
+                const Solve& so = hostSolve[s];
+                if (!so.contactActive) { continue; }
+                all = all && so.contactDone;
+                evals = std::max(evals, so.contactIters + 1u);
+            }
+            if (all) { break; }
+            if (queued >= maxEvals) { throw Error{ TSP_ERR_STATE, "contact solve did not stop at its iteration limit" }; }
+            group = std::min(2u * group, 16u);
+        }
+        lastContactEvals = evals;
+        launchContactApply(stream, d);
     }
 
     // --- the arena: the particles and bodies of every scene, laid out anew. Resident scenes
@@ -648,6 +746,21 @@ struct tsp_ctx
         zero(d.rbCnt, sizeof(uint32_t) * nR);
         zero(d.rrCnt, sizeof(uint32_t) * nR);
         d.rfOff = alloc<uint32_t>(nR + 1ull, work);
+        d.rbOff = alloc<uint32_t>(nR + 1ull, work);
+        d.rrOff = alloc<uint32_t>(nR + 1ull, work);
+        d.cbScan = alloc<uint32_t>(nB + 1ull, work);
+        zero(d.cbScan, sizeof(uint32_t) * (nB + 1ull));
+        d.cRw = alloc<float>(3ull * nR, work);
+        d.cR = alloc<float>(static_cast<uint64_t>(kCR) * nR, work);
+        d.cErrR = alloc<double>(nR, work);
+        d.cIw = alloc<float>(9ull * std::max<uint32_t>(nRB, 1), work);
+        d.cLoad = alloc<float>(6ull * std::max<uint32_t>(nRB, 1), work);
+        d.cBodyB = alloc<uint32_t>(std::max<uint32_t>(nRB, 1), work);
+        // the contact boundary arrays and their scan buffer depend on the boundary count
+        releaseList(contactAllocs);
+        d.rbCap = d.rrCap = 0;
+        contactLists = false;
+        contactRB = 0;
         d.errPartial = alloc<double>(3ull * (nF / kBlock), work);
         {
             static_assert(sizeof(Status) <= kSolveOff, "status block layout");
@@ -960,7 +1073,7 @@ struct tsp_ctx
             {
                 Scope sc(prof, stream, "query_count");
                 launchQueryCount(stream, d);
-                launchQueryRigid(stream, d, false, d.scanTmp);
+                launchQueryRigid(stream, d, 0, d.scanTmp);
             }
             {
                 Scope sc(prof, stream, "scan");
@@ -994,18 +1107,22 @@ struct tsp_ctx
                 allocPools(cap, rfCap);
                 for (SceneHost& h : scenes) { h.flags |= TSP_FLAG_LIST_REGROWN; }
             }
+            // the contact solve's RB and RR lists are stored when some scene has entries
+            contactLists = last.rbTotal + last.rrTotal > 0;
+            contactRB = last.rbTotal;
+            if (contactLists)
+            {
+                ensureContactCapacity(last.rbTotal, last.rrTotal);
+                launchContactOffsets(stream, d, d.scanTmp);
+            }
             {
                 Scope sc(prof, stream, "query_fill");
                 launchQueryFill(stream, d);
-                launchQueryRigid(stream, d, true, d.scanTmp);
+                launchQueryRigid(stream, d, contactLists ? 2 : 1, d.scanTmp);
             }
             {
                 Scope sc(prof, stream, "halo");
                 launchHaloBuild(stream, d);
-            }
-            if (last.rbTotal + last.rrTotal > 0)
-            {
-                for (SceneHost& h : scenes) { if (h.nR) { h.flags |= TSP_FLAG_RIGID_CONTACT; } }
             }
             return;
         }
@@ -1132,6 +1249,7 @@ struct tsp_ctx
             Scope sc(prof, stream, "integration");
             launchIntegration(stream, d);
         }
+        if (d.nRB && contactLists) { solveContacts(); }
         if (d.nRB)
         {
             Scope sc(prof, stream, "rigid");
@@ -1144,6 +1262,7 @@ struct tsp_ctx
             SceneHost& h = scenes[s];
             const Solve& so = hostSolve[s];
             h.last = so;
+            if (so.contactActive) { h.flags |= TSP_FLAG_RIGID_CONTACT; }
             h.lastSweeps = h.nF ? so.sweeps : 0;
             h.sweepsTotal += h.lastSweeps;
             const tsp_params& p = h.params;
@@ -1160,6 +1279,7 @@ struct tsp_ctx
     void steps(uint32_t n)
     {
         for (SceneHost& h : scenes) { h.flags = 0; }
+        for (const SceneHost& h : scenes) { if (h.nRB) { validateContactParams(h.params); } }
         bool work = false;
         for (const SceneHost& h : scenes) { work = work || h.nF > 0 || h.nRB > 0; }
         for (uint32_t s = 0; s < n; s++)
@@ -1175,6 +1295,14 @@ struct tsp_ctx
         for (uint32_t s = 0; s < nScenes(); s++)
         {
             if (hostSolve[s].rigidDegenerate) { throw Error{ TSP_ERR_NUMERIC, "Degenerate rigid body orientation." }; }
+            const uint32_t bad = hostSolve[s].contactBad;
+            if (bad & kContactManyBodies)
+            {
+                throw Error{ TSP_ERR_CAPACITY, "more than 8 bodies around one rigid contact particle" };
+            }
+            if (bad & kContactBadDiag) { throw Error{ TSP_ERR_NUMERIC, "Invalid rigid contact diagonal." }; }
+            if (bad & kContactBadResidual) { throw Error{ TSP_ERR_NUMERIC, "Non-finite rigid contact density error." }; }
+            if (bad & kContactBadPressure) { throw Error{ TSP_ERR_NUMERIC, "Non-finite rigid contact pressure." }; }
         }
     }
 
@@ -1280,6 +1408,11 @@ struct tsp_ctx
         info->n_ff = last.entries[0]; info->n_fb = last.entries[1]; info->n_fr = last.entries[2];
         info->n_rf = last.rfTotal; info->n_rb = last.rbTotal; info->n_rr = last.rrTotal;
         info->list_bytes = 2ull * (last.poolUnits[0] + last.poolUnits[1] + last.poolUnits[2]);
+        info->contact_iters = s.contactIters;
+        info->contact_avg_err = s.contactAvg;
+        info->contact_converged = s.contactConverged;
+        info->contact_boundary = s.contactBoundary;
+        info->contacts = s.contacts;
         if (h.nRB)
         {
             sync();
@@ -1600,7 +1733,8 @@ int tsp_get_neighbors(tsp_ctx* ctx, int rel, uint32_t* counts, uint32_t* idx, ui
         ctx->sync();
         if (rel >= 3)
         {
-            // RF is stored as CSR; RB and RR are only counted (they drive the contact flag)
+            // RF is stored as CSR; RB and RR are stored when some scene has entries (the contact
+            // solve reads them), which is whenever this scene has any
             const uint32_t n = h.nR;
             std::vector<uint32_t> cnt(n);
             const uint32_t* src = rel == 3 ? d.rfCnt : (rel == 4 ? d.rbCnt : d.rrCnt);
@@ -1608,14 +1742,17 @@ int tsp_get_neighbors(tsp_ctx* ctx, int rel, uint32_t* counts, uint32_t* idx, ui
             uint64_t tot = 0;
             for (uint32_t i = 0; i < n; i++) { tot += cnt[i]; if (counts) { counts[i] = cnt[i]; } }
             if (total) { *total = tot; }
-            if (rel == 3 && idx && cap >= tot && tot)
+            if (idx && cap >= tot && tot)
             {
+                const uint32_t* offs = rel == 3 ? d.rfOff : (rel == 4 ? d.rbOff : d.rrOff);
+                const uint32_t* list = rel == 3 ? d.rfIdx : (rel == 4 ? d.rbIdx : d.rrIdx);
+                if (rel != 3 && !ctx->contactLists) { throw Error{ TSP_ERR_STATE, "RB/RR lists were not stored" }; }
                 uint32_t first = 0;
-                hipCheck(hipMemcpy(&first, d.rfOff + h.offR, sizeof(uint32_t), hipMemcpyDeviceToHost), "D2H");
-                hipCheck(hipMemcpy(idx, d.rfIdx + first, sizeof(uint32_t) * tot, hipMemcpyDeviceToHost), "D2H");
-                for (uint64_t q = 0; q < tot; q++) { idx[q] -= h.offF; }
+                hipCheck(hipMemcpy(&first, offs + h.offR, sizeof(uint32_t), hipMemcpyDeviceToHost), "D2H");
+                hipCheck(hipMemcpy(idx, list + first, sizeof(uint32_t) * tot, hipMemcpyDeviceToHost), "D2H");
+                const uint32_t base = rel == 3 ? h.offF : (rel == 4 ? h.offB : h.offR);
+                for (uint64_t q = 0; q < tot; q++) { idx[q] -= base; }
             }
-            else if (rel != 3 && idx && tot) { throw Error{ TSP_ERR_INVALID, "RB/RR lists are counted, not stored" }; }
             return TSP_OK;
         }
         // the scene's slices (its fluid segment starts at a multiple of the slice height)

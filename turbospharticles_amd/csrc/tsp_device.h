@@ -116,8 +116,21 @@ namespace tsp
         double sumRaw, sumPos, sumAbs;
         uint32_t abort;          // a particle of this scene left the inner frame
         uint32_t rigidDegenerate;
-        uint32_t pad[2];
+        // rigid contact solve of the step (RigidContactReport, rigid_solver.h)
+        uint32_t contactActive;  // the scene has RB or RR entries: its contact solve runs
+        uint32_t contactDone;
+        uint32_t contactIters;
+        uint32_t contactConverged;
+        float contactAvg;
+        uint32_t contactBoundary;   // boundary particles in contact
+        uint32_t contactBad;        // kContactBad* flags
+        uint32_t pad1;
+        uint64_t contacts;          // N_contacts of the relaxation
     };
+
+    // Solve::contactBad
+    constexpr uint32_t kContactBadDiag = 1u, kContactBadResidual = 2u, kContactBadPressure = 4u,
+                       kContactManyBodies = 8u;
 
     // where a scene's particles live in the arrays of a context. Fluid segments start at a
     // multiple of kBlock and are padded to one, so a thread block never spans two scenes;
@@ -160,6 +173,8 @@ namespace tsp
         float densityDevTarget;
         uint32_t minIters, maxIters, fixedIters, termCond;
         uint32_t nF;           // active fluid particles of the scene (the error averages divide by it)
+        float contactDevTarget;     // rigidContactDensityDevTarget
+        uint32_t contactMaxIters;   // rigidContactMaxSolverIters
     };
 
     // records gathered by one permutation launch
@@ -175,6 +190,16 @@ namespace tsp
         uint64_t* idD;
         uint32_t n;
     };
+
+    // fields of the body state Dev::rb (field-major: rb[field * nRB + body], TSP_RB_* order)
+    enum
+    {
+        RB_XCM = 0, RB_VX = 3, RB_WX = 6, RB_LX = 9, RB_A00 = 12, RB_MASS = 21, RB_MASS_INV = 22, RB_I00 = 23
+    };
+
+    // fields of Dev::cR (per rigid particle) and Dev::cB (per contact boundary particle)
+    enum { CR_RHO, CR_DIAG, CR_P, CR_SRC, CR_ERR, CR_OM, CR_VX, CR_VY, CR_VZ, CR_FX, CR_FY, CR_FZ, kCR };
+    enum { CB_RHO, CB_DIAG, CB_P, CB_SRC, CB_ERR, CB_OM, kCB };
 
     // device pointers of one context (passed by value to every kernel)
     struct Dev
@@ -225,11 +250,28 @@ namespace tsp
         uint32_t *rfCnt, *rfOff, *rfIdx;
         uint32_t *rbCnt, *rrCnt;
         uint64_t rfCap;
+        // rigid contacts: RB and RR lists (filled when some scene has contacts), the boundary
+        // particles in contact, and the state of the contact solve
+        uint32_t *rbOff, *rrOff;     // nR + 1
+        uint32_t *rbIdx, *rrIdx;     // boundary / rigid indices
+        uint64_t rbCap, rrCap;
+        uint32_t *cbScan;            // nB + 1: contact flags of the boundary particles, then their exclusive scan
+        uint32_t *cbLocal;           // per contact boundary particle: its boundary index (rbCap)
+        uint32_t *brOff, *brPos, *brIdx;   // per contact boundary particle: rigid neighbors, ascending
+        float *cRw;                  // 3 x nR: world offsets A r_rel
+        float *cR;                   // kCR x nR: contact state of the rigid particles
+        float *cB;                   // kCB x rbCap: contact state of the contact boundary particles
+        double *cErrR, *cErrB;       // FP64 error terms of the last evaluation
+        float *cIw;                  // 9 x nRB: world inverse inertia
+        float *cLoad;                // 6 x nRB: contact force and torque
+        uint32_t *cBodyB;            // nRB: boundary contacts of each movable body
         // reductions and scans
         double *errPartial;      // Jacobi error partials, [3][blocks of the fluid grid]
         uint32_t *scanTmp;       // block sums of the multi-block scans
         Status *status;
     };
+
+    __device__ __forceinline__ float rbv(const Dev& d, int f, uint32_t b) { return d.rb[f * d.nRB + b]; }
 
     // scene of a key: the last scene whose key base is not above it
     __device__ __forceinline__ uint32_t sceneOfKey(const Dev& d, uint32_t key)
