@@ -197,6 +197,36 @@ typedef struct tsp_statistics {
 /* I_SimBackend::FillStatisticsRecord (core/backend_sim.h:243) for the first scene */
 int tsp_get_statistics(tsp_ctx *ctx, tsp_statistics *out);
 
+/* Sync::ExportSnapshot (sync/snapshots.h:37-66) as the caller's arrays. The fluid arrays hold
+   the exported particles: those near the surface (IsNearSurfaceFluidParticle, core/backend_sim.h:
+   109-117: a boundary or rigid neighbor, or fewer FF neighbors than the threshold) and their FF
+   neighbors, in the backend's particle order, with their position triples and speed. */
+typedef struct tsp_export_view {
+    uint64_t step;                /* written: simStep */
+    double time;                  /* written: simTime */
+    uint32_t cap_fluid;           /* capacity of the fluid arrays, in particles */
+    uint32_t n_fluid;             /* written: exported fluid particles */
+    uint32_t n_boundary;          /* written: the scene's boundary count when boundary_xyz is set, else 0 */
+    uint32_t n_rigid;             /* written */
+    float *fluid_xyz;             /* 3 per exported fluid particle (x, y, z triples) */
+    float *fluid_vel_mag;         /* |v| */
+    float *boundary_xyz;          /* null: no boundary export (includeB false) */
+    float *boundary_vol;
+    uint8_t *boundary_is_mesh;    /* id >= mesh_id_begin */
+    float *rigid_xyz;             /* world positions, 3 per rigid particle */
+    float *rigid_vol;
+    uint32_t *rigid_body;         /* the particle's body index */
+} tsp_export_view;
+
+/* I_SimBackend::PackExportSnapshot (core/backend_sim.h:215; gpu_backend.cpp:346-551) for the first
+   scene: the surface classification and compaction run on the device against the lists of the
+   last neighbor search (TSP_ERR_STATE when none ran since the last pull), and only the exported
+   particles are copied out. surface_nbr_count_threshold is SimParams::surfaceNbrCountThreshold
+   (27 by default); mesh_id_begin the first boundary mesh id (UINT64_MAX: none). Returns
+   TSP_ERR_CAPACITY with n_fluid set when cap_fluid is too small. Null arrays are skipped. */
+int tsp_export_snapshot(tsp_ctx *ctx, uint32_t surface_nbr_count_threshold, uint64_t mesh_id_begin,
+                        tsp_export_view *out);
+
 /*
  * Several independent scenes in one context. The reference runs one scene per backend
  * (core/simulation.h:179, one SimBackend per Simulation); these calls let one context hold many
@@ -228,6 +258,9 @@ int tsp_batch_push_scene(tsp_ctx *ctx, uint32_t scene_id, tsp_state_view *dst);
 int tsp_batch_scene_info(tsp_ctx *ctx, uint32_t scene_id, tsp_step_info *info);
 /* tsp_get_statistics of one scene */
 int tsp_batch_scene_statistics(tsp_ctx *ctx, uint32_t scene_id, tsp_statistics *out);
+/* tsp_export_snapshot of one scene */
+int tsp_batch_export_snapshot(tsp_ctx *ctx, uint32_t scene_id, uint32_t surface_nbr_count_threshold,
+                              uint64_t mesh_id_begin, tsp_export_view *out);
 /* advances every scene n_steps IISPH steps (ExecuteFullSimStep of each scene's backend) */
 int tsp_batch_step(tsp_ctx *ctx, uint32_t n_steps);
 /* device bytes a scene of these counts holds while resident (an upper estimate at the initial

@@ -145,6 +145,10 @@ enum { ORC_FF, ORC_FB, ORC_FR, ORC_RF, ORC_RB, ORC_RR };
        and the residuals once. Returns the count of contact boundary particles, or < 0.     \
        Every output may be null; rigid arrays have n_rigid entries, boundary arrays the     \
        returned count, loads 6 per body. */                                                 \
+    /* the fluid particles the export snapshot carries (FillFluidParticleNearSurfaceFlags and \
+       ClassifyExportSurfaceParticles, cpu_backend.cpp:910-981): flags[i] = 1 in the current   \
+       particle order, from the lists of the last search */                                   \
+    void P##_export_surface(P##_ctx *c, uint32_t threshold, uint8_t *flags);               \
     int64_t P##_contact_probe(P##_ctx *c, const float *p_rigid, const float *p_boundary,   \
                               float *rho_r, float *diag_r, float *omega_r, float *err_r,    \
                               float *rho_b, float *diag_b, float *omega_b, float *err_b,    \

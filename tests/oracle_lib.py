@@ -85,6 +85,7 @@ def lib():
                 "kernel_weight": (None, [ctypes.POINTER(OrcParams), _fp, ctypes.c_uint32, _fp, _fp, _fp]),
                 "kernels_from_lists": (ctypes.c_int, [vp, _u32p, _u32p, _u8p, _u32p, _u32p, _u32p, _u32p,
                                                       ctypes.c_int]),
+                "export_surface": (None, [vp, ctypes.c_uint32, _u8p]),
                 "contact_probe": (ctypes.c_int64, [vp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp,
                                                    _u32p, _fp]),
             }
@@ -200,6 +201,12 @@ class Oracle:
         ft = np.zeros(6 * self.n[3], np.float32)
         self.fn("get_body_force")(self.h, _capi.fptr(ft))
         return ft.reshape(-1, 6)
+
+    def export_surface(self, threshold=27):
+        """flags (uint8, current particle order) of the fluid particles an export carries"""
+        flags = np.zeros(max(self.n[0], 1), np.uint8)
+        self.fn("export_surface")(self.h, threshold, flags.ctypes.data_as(_u8p))
+        return flags[:self.n[0]]
 
     def contact_probe(self, p_rigid=None, p_boundary=None):
         """The contact problem after the last neighbor search, evaluated once with the given

@@ -25,6 +25,7 @@ RB_FIELDS = ["xcm", "ycm", "zcm", "vx", "vy", "vz", "wx", "wy", "wz", "lx", "ly"
              "i00", "i01", "i02", "i10", "i11", "i12", "i20", "i21", "i22"]
 
 TSP_OK = 0
+TSP_ERR_CAPACITY = -4
 FLAG_MAX_ITERS = 1
 FLAG_FRAME_REBUILT = 2
 FLAG_RIGID_CONTACT = 4
@@ -91,6 +92,17 @@ class TspStepInfo(ctypes.Structure):
     ]
 
 
+class TspExportView(ctypes.Structure):
+    _fields_ = [
+        ("step", ctypes.c_uint64), ("time", ctypes.c_double),
+        ("cap_fluid", ctypes.c_uint32), ("n_fluid", ctypes.c_uint32),
+        ("n_boundary", ctypes.c_uint32), ("n_rigid", ctypes.c_uint32),
+        ("fluid_xyz", _fp), ("fluid_vel_mag", _fp),
+        ("boundary_xyz", _fp), ("boundary_vol", _fp), ("boundary_is_mesh", ctypes.POINTER(ctypes.c_uint8)),
+        ("rigid_xyz", _fp), ("rigid_vol", _fp), ("rigid_body", _u32p),
+    ]
+
+
 class TspStatistics(ctypes.Structure):
     _fields_ = [
         ("n_fluid", ctypes.c_uint32), ("n_boundary", ctypes.c_uint32), ("n_rigid", ctypes.c_uint32),
@@ -153,6 +165,9 @@ def _declare_hip(lib):
         "tsp_synchronize": (i, [vp]),
         "tsp_get_statistics": (i, [vp, ctypes.POINTER(TspStatistics)]),
         "tsp_batch_scene_statistics": (i, [vp, ctypes.c_uint32, ctypes.POINTER(TspStatistics)]),
+        "tsp_export_snapshot": (i, [vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(TspExportView)]),
+        "tsp_batch_export_snapshot": (i, [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                          ctypes.POINTER(TspExportView)]),
         "tsp_batch_update": (i, [vp, ctypes.c_uint32, _u32p, ctypes.c_uint32, ctypes.POINTER(TspParams),
                                  ctypes.POINTER(ctypes.POINTER(TspStateView)), _fp, _u32p]),
         "tsp_batch_add_scene": (i, [vp, ctypes.POINTER(TspParams), ctypes.POINTER(TspStateView), _fp, _u32p]),
@@ -185,6 +200,8 @@ def _declare_host(lib):
         "tsp_state_file_info": (i, [ctypes.c_char_p, _u32p, _u64p, ctypes.POINTER(ctypes.c_double)]),
         "tsp_state_file_read": (i, [ctypes.c_char_p, ctypes.POINTER(TspStateView)]),
         "tsp_state_file_write": (i, [ctypes.c_char_p, ctypes.POINTER(TspStateView)]),
+        "tsp_snapshot_file_write": (i, [ctypes.c_char_p, ctypes.POINTER(TspExportView)]),
+        "tsp_boundary_file_write": (i, [ctypes.c_char_p, ctypes.POINTER(TspExportView)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -304,3 +321,79 @@ def load_state(path):
         raise RuntimeError(f"tsp_state_file_read failed ({rc}): {path}")
     st.step, st.time = int(v.step), float(v.time)
     return st
+
+
+class ExportSnapshot:
+    """Sync::ExportSnapshot as numpy arrays (fluid_xyz is n x 3 in the file's triple layout)."""
+
+    def __init__(self, step=0, time=0.0, fluid_xyz=None, fluid_vel_mag=None, boundary_xyz=None,
+                 boundary_vol=None, boundary_is_mesh=None, rigid_xyz=None, rigid_vol=None, rigid_body=None):
+        f32 = np.float32
+        self.step, self.time = int(step), float(time)
+        self.fluid_xyz = np.zeros((0, 3), f32) if fluid_xyz is None else np.ascontiguousarray(fluid_xyz, f32)
+        self.fluid_vel_mag = np.zeros(0, f32) if fluid_vel_mag is None else np.ascontiguousarray(fluid_vel_mag, f32)
+        self.boundary_xyz = None if boundary_xyz is None else np.ascontiguousarray(boundary_xyz, f32)
+        self.boundary_vol = None if boundary_vol is None else np.ascontiguousarray(boundary_vol, f32)
+        self.boundary_is_mesh = None if boundary_is_mesh is None else np.ascontiguousarray(boundary_is_mesh, np.uint8)
+        self.rigid_xyz = np.zeros((0, 3), f32) if rigid_xyz is None else np.ascontiguousarray(rigid_xyz, f32)
+        self.rigid_vol = np.zeros(0, f32) if rigid_vol is None else np.ascontiguousarray(rigid_vol, f32)
+        self.rigid_body = np.zeros(0, np.uint32) if rigid_body is None else np.ascontiguousarray(rigid_body, np.uint32)
+
+    def view(self):
+        v = TspExportView()
+        v.step, v.time = self.step, self.time
+        v.n_fluid = v.cap_fluid = self.fluid_vel_mag.size
+        v.n_rigid = self.rigid_vol.size
+        v.fluid_xyz, v.fluid_vel_mag = fptr(self.fluid_xyz), fptr(self.fluid_vel_mag)
+        v.rigid_xyz, v.rigid_vol, v.rigid_body = fptr(self.rigid_xyz), fptr(self.rigid_vol), u32ptr(self.rigid_body)
+        if self.boundary_xyz is not None:
+            v.n_boundary = self.boundary_xyz.shape[0]
+            v.boundary_xyz, v.boundary_vol = fptr(self.boundary_xyz), fptr(self.boundary_vol)
+            if self.boundary_is_mesh is not None:
+                v.boundary_is_mesh = self.boundary_is_mesh.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        return v
+
+
+def export_from(call, n_fluid_cap, n_boundary, n_rigid, include_boundary):
+    """Runs an export call (call(view) -> status) into new arrays, growing the fluid arrays once
+    if the surface set is larger than n_fluid_cap."""
+    cap = max(int(n_fluid_cap), 1)
+    for _ in range(2):
+        snap = ExportSnapshot(fluid_xyz=np.zeros((cap, 3), np.float32), fluid_vel_mag=np.zeros(cap, np.float32),
+                              rigid_xyz=np.zeros((n_rigid, 3), np.float32), rigid_vol=np.zeros(n_rigid, np.float32),
+                              rigid_body=np.zeros(n_rigid, np.uint32))
+        if include_boundary:
+            snap.boundary_xyz = np.zeros((n_boundary, 3), np.float32)
+            snap.boundary_vol = np.zeros(n_boundary, np.float32)
+            snap.boundary_is_mesh = np.zeros(n_boundary, np.uint8)
+        v = snap.view()
+        v.cap_fluid = cap
+        if not include_boundary:
+            v.boundary_xyz = None
+        rc = call(v)
+        if rc == TSP_ERR_CAPACITY and v.n_fluid > cap:
+            cap = v.n_fluid
+            continue
+        if rc != TSP_OK:
+            return rc, None
+        snap.step, snap.time = int(v.step), float(v.time)
+        snap.fluid_xyz = snap.fluid_xyz[:v.n_fluid]
+        snap.fluid_vel_mag = snap.fluid_vel_mag[:v.n_fluid]
+        return rc, snap
+    return TSP_ERR_CAPACITY, None
+
+
+def save_snapshot(path, snap):
+    """Writes the reference's particle data file (TSPSNP4) of an export snapshot."""
+    v = snap.view()
+    rc = host_lib().tsp_snapshot_file_write(os.fsencode(path), ctypes.byref(v))
+    if rc != TSP_OK:
+        raise RuntimeError(f"tsp_snapshot_file_write failed ({rc}): {path}")
+
+
+def save_boundary_file(path, snap):
+    """Writes the reference's static boundary file (TSPSNP4) of an export snapshot."""
+    v = snap.view()
+    rc = host_lib().tsp_boundary_file_write(os.fsencode(path), ctypes.byref(v))
+    if rc != TSP_OK:
+        raise RuntimeError(f"tsp_boundary_file_write failed ({rc}): {path}")

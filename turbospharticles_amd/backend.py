@@ -80,6 +80,18 @@ class HipBackend:
         self._check(self.lib.tsp_get_step_info(self.h, ctypes.byref(i)), "tsp_get_step_info")
         return i
 
+    # I_SimBackend::PackExportSnapshot
+    def export_snapshot(self, surface_nbr_count_threshold=27, include_boundary=False, mesh_id_begin=None):
+        """The surface fluid particles (and rigid, optionally boundary, particles) as an
+        ExportSnapshot, classified and compacted on the device."""
+        mesh = 0xFFFFFFFFFFFFFFFF if mesh_id_begin is None else int(mesh_id_begin)
+        nF, nB, nR, _ = self.n
+        rc, snap = _capi.export_from(
+            lambda v: self.lib.tsp_export_snapshot(self.h, surface_nbr_count_threshold, mesh, ctypes.byref(v)),
+            max(nF // 4, 1024), nB, nR, include_boundary)
+        self._check(rc, "tsp_export_snapshot")
+        return snap
+
     # I_SimBackend::GetNumFluidParticles / Boundary / Rigid (+ bodies)
     def counts(self):
         c = (ctypes.c_uint32 * 4)()
@@ -233,6 +245,17 @@ class HipBatch:
 
     def synchronize(self):
         self._check(self.lib.tsp_synchronize(self.h), "tsp_synchronize")
+
+    def export_snapshot(self, scene_id, surface_nbr_count_threshold=27, include_boundary=False, mesh_id_begin=None):
+        """HipBackend.export_snapshot of one scene."""
+        mesh = 0xFFFFFFFFFFFFFFFF if mesh_id_begin is None else int(mesh_id_begin)
+        nF, nB, nR, _ = self.counts[scene_id]
+        rc, snap = _capi.export_from(
+            lambda v: self.lib.tsp_batch_export_snapshot(self.h, scene_id, surface_nbr_count_threshold, mesh,
+                                                         ctypes.byref(v)),
+            max(nF // 4, 1024), nB, nR, include_boundary)
+        self._check(rc, "tsp_batch_export_snapshot")
+        return snap
 
     def statistics(self, scene_id):
         st = _capi.TspStatistics()

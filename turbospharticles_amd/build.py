@@ -19,7 +19,7 @@ INC = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "lib")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HIP_SOURCES = ["kernels_search.hip", "kernels_sph.hip", "kernels_stats.hip", "kernels_contact.hip", "tsp_capi.hip"]
+HIP_SOURCES = ["kernels_search.hip", "kernels_sph.hip", "kernels_stats.hip", "kernels_contact.hip", "kernels_export.hip", "tsp_capi.hip"]
 HOST_SOURCES = ["scene_builder.cpp", "sim_backend_hip.cpp", "state_io.cpp"]
 
 HIPCC_FLAGS = [

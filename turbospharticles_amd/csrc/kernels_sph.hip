@@ -20,91 +20,6 @@
 
 namespace tsp
 {
-    struct List
-    {
-        const uint16_t* pool;
-        SliceMeta m;
-        uint32_t lane, n;
-        __device__ __forceinline__ uint32_t operator[](uint32_t k) const { return sellRead(pool, m, lane, k); }
-    };
-
-    __device__ __forceinline__ List listOf(const Dev& d, int rel, uint32_t i)
-    {
-        List l;
-        l.pool = d.pool[rel];
-        l.m = d.meta[rel][i / kSlice];
-        l.lane = i & (kSlice - 1);
-        l.n = d.cnt[rel][i] & (rel == REL_FF ? 0x7fffu : 0xffffu);
-        return l;
-    }
-
-    // visits the entries of one list in storage order, as fn(j, e): j the neighbor (or its halo
-    // slot when kSlots: the stored value without the slice base) and e the entry's position in
-    // u16 units of the pool, which also indexes the per-entry gradient factors. The index loads of
-    // four entries are issued together so that their gathers overlap; the sums still run entry
-    // by entry.
-    template <bool kSlots, typename Fn>
-    __device__ __forceinline__ void visitEntries(const List& l, Fn&& fn)
-    {
-        uint32_t q = 0;
-        if (l.m.format == kFmtDelta16)
-        {
-            const uint32_t e0 = l.m.offset + l.lane;
-            const uint16_t* p = l.pool + e0;
-            const uint32_t base = kSlots ? 0u : l.m.base;
-#if TSP_TUNE_PREFETCH
-            // the next four indices load while the current four are summed
-            uint32_t j0 = 0, j1 = 0, j2 = 0, j3 = 0;
-            if (l.n >= 4) { j0 = p[0]; j1 = p[kSlice]; j2 = p[2 * kSlice]; j3 = p[3 * kSlice]; }
-            for (; q + 4 <= l.n; q += 4)
-            {
-                uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
-                if (q + 8 <= l.n)
-                {
-                    n0 = p[(q + 4) * kSlice]; n1 = p[(q + 5) * kSlice];
-                    n2 = p[(q + 6) * kSlice]; n3 = p[(q + 7) * kSlice];
-                }
-                fn(base + j0, e0 + (q + 0) * kSlice); fn(base + j1, e0 + (q + 1) * kSlice);
-                fn(base + j2, e0 + (q + 2) * kSlice); fn(base + j3, e0 + (q + 3) * kSlice);
-                j0 = n0; j1 = n1; j2 = n2; j3 = n3;
-            }
-#else
-            for (; q + 4 <= l.n; q += 4)
-            {
-                const uint32_t j0 = p[(q + 0) * kSlice], j1 = p[(q + 1) * kSlice];
-                const uint32_t j2 = p[(q + 2) * kSlice], j3 = p[(q + 3) * kSlice];
-                fn(base + j0, e0 + (q + 0) * kSlice); fn(base + j1, e0 + (q + 1) * kSlice);
-                fn(base + j2, e0 + (q + 2) * kSlice); fn(base + j3, e0 + (q + 3) * kSlice);
-            }
-#endif
-            for (; q < l.n; q++) { fn(base + p[q * kSlice], e0 + q * kSlice); }
-        }
-        else
-        {
-            const uint32_t e0 = l.m.offset + 2u * l.lane;
-            const uint32_t* p = reinterpret_cast<const uint32_t*>(l.pool + l.m.offset) + l.lane;
-            for (; q + 4 <= l.n; q += 4)
-            {
-                const uint32_t j0 = p[(q + 0) * kSlice], j1 = p[(q + 1) * kSlice];
-                const uint32_t j2 = p[(q + 2) * kSlice], j3 = p[(q + 3) * kSlice];
-                fn(j0, e0 + 2u * (q + 0) * kSlice); fn(j1, e0 + 2u * (q + 1) * kSlice);
-                fn(j2, e0 + 2u * (q + 2) * kSlice); fn(j3, e0 + 2u * (q + 3) * kSlice);
-            }
-            for (; q < l.n; q++) { fn(p[q * kSlice], e0 + 2u * q * kSlice); }
-        }
-    }
-
-    template <typename Fn>
-    __device__ __forceinline__ void visit(const List& l, Fn&& fn) { visitEntries<false>(l, fn); }
-
-    // FF neighbors of a particle: from the block's LDS halo when it has one, else from HBM
-    template <typename FromHalo, typename FromGlobal>
-    __device__ __forceinline__ void visitFF(const List& l, bool halo, FromHalo&& fh, FromGlobal&& fg)
-    {
-        if (halo) { visitEntries<true>(l, fh); }
-        else { visitEntries<false>(l, fg); }
-    }
-
     // copies the FF halo records of a block into LDS (load(slot, particle index)); every thread
     // of the block calls it. A count above kHaloMax (kNoHalo) means the block has no halo.
     template <typename Load>

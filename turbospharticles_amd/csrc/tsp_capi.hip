@@ -43,6 +43,7 @@ namespace tsp
     void launchContactSource(hipStream_t, const Dev&, uint32_t);
     void launchContactIteration(hipStream_t, const Dev&, uint32_t);
     void launchContactApply(hipStream_t, const Dev&);
+    void launchExportSurface(hipStream_t, const Dev&, uint32_t, uint32_t, uint32_t, uint32_t*, uint32_t*, float*, float*);
     void launchHaloBuild(hipStream_t, const Dev&);
     void launchAabb(hipStream_t, const Dev&, int, const float4*, uint32_t, uint32_t*);
     void launchDensity(hipStream_t, const Dev&);
@@ -282,6 +283,7 @@ struct tsp_ctx
     uint32_t lastContactEvals = 1;  // evaluations of the last contact solve (the first group)
 
     bool layoutDirty = true;        // key bases and cell offsets must be assigned again
+    bool listsValid = false;        // a neighbor search ran since the last layout
     bool cellsClean = false;        // fluid and rigid cell tables hold no runs
     uint32_t keyBits = 1;
 
@@ -311,6 +313,7 @@ struct tsp_ctx
         releaseList(statAllocs);
         releaseList(poolAllocs);
         releaseList(contactAllocs);
+        releaseList(expAllocs);
         releaseList(frameAllocs);
         releaseList(work);
         releaseList(arena);
@@ -334,6 +337,12 @@ struct tsp_ctx
     }
 
     void sync() { hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+
+    // a synchronous device-to-host copy on the context stream
+    void d2h(void* dst, const void* src, size_t bytes)
+    {
+        if (bytes) { hipCheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream), "D2H"); sync(); }
+    }
 
     void h2d(void* dst, const void* src, size_t bytes)
     {
@@ -702,6 +711,7 @@ struct tsp_ctx
         bpos2 = nbpos2; bvol2 = nbvol2; idB = nidB; idB2 = nidB2; idR = nidR;
         scenes = std::move(ns);
         layoutDirty = true;
+        listsValid = false;
         cellsClean = false;
 
         // buffers sized by the layout
@@ -1120,6 +1130,7 @@ struct tsp_ctx
                 launchQueryFill(stream, d);
                 launchQueryRigid(stream, d, contactLists ? 2 : 1, d.scanTmp);
             }
+            listsValid = true;
             {
                 Scope sc(prof, stream, "halo");
                 launchHaloBuild(stream, d);
@@ -1477,6 +1488,81 @@ struct tsp_ctx
         out->vel_min = h.nF ? fl[4] : 0.0;
         out->vel_max = h.nF ? fl[5] : 0.0;
         out->cfl = out->vel_max * h.params.time_step / h.params.particle_size;
+    }
+
+    // --- export snapshot of one scene (SimBackend_GPU::PackExportSnapshot, gpu_backend.cpp:346-551) ---
+    std::vector<void*> expAllocs;
+    uint32_t* expFlags = nullptr;     // per fluid particle of the scene, then their scan
+    uint32_t* expTmp = nullptr;
+    float* expXyz = nullptr;
+    float* expSpeed = nullptr;
+    uint64_t expCap = 0;
+
+    void exportSnapshot(uint32_t si, uint32_t threshold, uint64_t meshIdBegin, tsp_export_view* v)
+    {
+        if (!listsValid) { throw Error{ TSP_ERR_STATE, "export needs a neighbor search since the last layout" }; }
+        const SceneHost& h = scenes[si];
+        v->step = h.step;
+        v->time = h.time;
+        v->n_fluid = 0;
+        v->n_boundary = v->boundary_xyz ? h.nB : 0;
+        v->n_rigid = h.nR;
+        if (expCap < h.nF + 1ull)
+        {
+            sync();
+            releaseList(expAllocs);
+            expCap = std::max<uint64_t>(h.nF + 1ull, 1024);
+            expFlags = alloc<uint32_t>(expCap, expAllocs);
+            expTmp = alloc<uint32_t>((expCap + 2047) / 2048 + 1, expAllocs);
+            expXyz = alloc<float>(3 * expCap, expAllocs);
+            expSpeed = alloc<float>(expCap, expAllocs);
+        }
+        launchExportSurface(stream, d, h.offF, h.nF, threshold, expFlags, expTmp, expXyz, expSpeed);
+        uint32_t count = 0;
+        hipCheck(hipMemcpyAsync(&count, expFlags + h.nF, sizeof(uint32_t), hipMemcpyDeviceToHost, stream), "D2H");
+        sync();
+        v->n_fluid = count;
+        if (count > v->cap_fluid) { throw Error{ TSP_ERR_CAPACITY, "the fluid export arrays hold fewer particles than the surface set" }; }
+        if (count && v->fluid_xyz) { d2h(v->fluid_xyz, expXyz, sizeof(float) * 3ull * count); }
+        if (count && v->fluid_vel_mag) { d2h(v->fluid_vel_mag, expSpeed, sizeof(float) * count); }
+        if (v->boundary_xyz && h.nB)
+        {
+            std::vector<float4> b(h.nB);
+            d2h(b.data(), d.bpos + h.offB, sizeof(float4) * h.nB);
+            for (uint32_t i = 0; i < h.nB; i++)
+            {
+                v->boundary_xyz[3ull * i] = b[i].x; v->boundary_xyz[3ull * i + 1] = b[i].y; v->boundary_xyz[3ull * i + 2] = b[i].z;
+            }
+            if (v->boundary_vol) { d2h(v->boundary_vol, d.bvol + h.offB, sizeof(float) * h.nB); }
+            if (v->boundary_is_mesh)
+            {
+                std::vector<uint64_t> ids(h.nB);
+                d2h(ids.data(), idB + h.offB, sizeof(uint64_t) * h.nB);
+                for (uint32_t i = 0; i < h.nB; i++) { v->boundary_is_mesh[i] = ids[i] >= meshIdBegin ? 1u : 0u; }
+            }
+        }
+        if (h.nR)
+        {
+            std::vector<float4> r(h.nR);
+            if (v->rigid_xyz)
+            {
+                d2h(r.data(), d.rpos + h.offR, sizeof(float4) * h.nR);
+                for (uint32_t i = 0; i < h.nR; i++)
+                {
+                    v->rigid_xyz[3ull * i] = r[i].x; v->rigid_xyz[3ull * i + 1] = r[i].y; v->rigid_xyz[3ull * i + 2] = r[i].z;
+                }
+            }
+            if (v->rigid_vol)
+            {
+                d2h(r.data(), d.rvel + h.offR, sizeof(float4) * h.nR);
+                for (uint32_t i = 0; i < h.nR; i++) { v->rigid_vol[i] = r[i].w; }
+            }
+            if (v->rigid_body)
+            {
+                d2h(v->rigid_body, d.rbody + h.offR, sizeof(uint32_t) * h.nR);
+                for (uint32_t i = 0; i < h.nR; i++) { v->rigid_body[i] -= h.offRB; }
+            }
+        }
     }
 
     // one relayout: the listed scenes leave, new scenes join at the end
@@ -1913,6 +1999,26 @@ int tsp_batch_scene_statistics(tsp_ctx* ctx, uint32_t scene_id, tsp_statistics* 
     return guarded(ctx, [&]() {
         if (!out) { throw Error{ TSP_ERR_INVALID, "null statistics" }; }
         ctx->statistics(ctx->sceneAt(scene_id), out);
+        return TSP_OK;
+    });
+}
+
+int tsp_export_snapshot(tsp_ctx* ctx, uint32_t surface_nbr_count_threshold, uint64_t mesh_id_begin, tsp_export_view* out)
+{
+    return guarded(ctx, [&]() {
+        if (!out) { throw Error{ TSP_ERR_INVALID, "null export view" }; }
+        ctx->requireScene();
+        ctx->exportSnapshot(0, surface_nbr_count_threshold, mesh_id_begin, out);
+        return TSP_OK;
+    });
+}
+
+int tsp_batch_export_snapshot(tsp_ctx* ctx, uint32_t scene_id, uint32_t surface_nbr_count_threshold, uint64_t mesh_id_begin,
+                              tsp_export_view* out)
+{
+    return guarded(ctx, [&]() {
+        if (!out) { throw Error{ TSP_ERR_INVALID, "null export view" }; }
+        ctx->exportSnapshot(ctx->sceneAt(scene_id), surface_nbr_count_threshold, mesh_id_begin, out);
         return TSP_OK;
     });
 }
