@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "lib")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIP_SOURCES = ["kernels_search.hip", "kernels_sph.hip", "kernels_stats.hip", "kernels_contact.hip", "kernels_export.hip", "tsp_capi.hip"]
-HOST_SOURCES = ["scene_builder.cpp", "sim_backend_hip.cpp", "state_io.cpp"]
+HOST_SOURCES = ["scene_plan.cpp", "scene_builder.cpp", "sim_backend_hip.cpp", "state_io.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
