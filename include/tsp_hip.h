@@ -256,6 +256,17 @@ int tsp_batch_scene_counts(tsp_ctx *ctx, uint32_t scene_id, uint32_t counts[4]);
 int tsp_batch_push_scene(tsp_ctx *ctx, uint32_t scene_id, tsp_state_view *dst);
 /* tsp_get_step_info of one scene (the neighbor-list totals count every scene) */
 int tsp_batch_scene_info(tsp_ctx *ctx, uint32_t scene_id, tsp_step_info *info);
+/* A scene described by a tsp_scene_spec (include/tsp_scene.h) generated on the device: the fluid
+   block and the container's boundary layers with their psi / V sums are written straight into
+   the context's arrays, bit for bit as tsp_scene_build followed by tsp_pull, without host
+   arrays (the reference's scenario builders, scene/builder.cpp:174-642, :925-1087; the rigid body
+   is generated on the host). tsp_pull_spec replaces the context's scenes with the spec's and sets
+   the domain to the spec's (ComputeSimDomainBounds, scene/scenario_sim.cpp:123-148);
+   tsp_batch_add_scene_spec adds a scene with its own params. */
+typedef struct tsp_scene_spec tsp_scene_spec;
+int tsp_pull_spec(tsp_ctx *ctx, const tsp_scene_spec *spec);
+int tsp_batch_add_scene_spec(tsp_ctx *ctx, const tsp_params *params, const tsp_scene_spec *spec, uint32_t *scene_id);
+
 /* tsp_get_statistics of one scene */
 int tsp_batch_scene_statistics(tsp_ctx *ctx, uint32_t scene_id, tsp_statistics *out);
 /* tsp_export_snapshot of one scene */

@@ -49,6 +49,10 @@ int main(void) {
   O(tsp_state_view, body) O(tsp_state_view, step) O(tsp_state_view, time)
   S(tsp_step_info) O(tsp_step_info, solver_iters) O(tsp_step_info, err_sum_raw) O(tsp_step_info, frame_dim)
   O(tsp_step_info, n_ff) O(tsp_step_info, list_bytes) O(tsp_step_info, rigid_force_torque)
+  O(tsp_step_info, contact_iters) O(tsp_step_info, contacts)
+  O(tsp_params, rigid_contact_density_dev_target) O(tsp_params, rigid_contact_max_solver_iters)
+  S(tsp_export_view) O(tsp_export_view, cap_fluid) O(tsp_export_view, fluid_xyz) O(tsp_export_view, boundary_is_mesh)
+  O(tsp_export_view, rigid_body)
   S(tsp_statistics) O(tsp_statistics, nbr_avg) O(tsp_statistics, nbr_p99) O(tsp_statistics, dens_rel_err_avg)
   O(tsp_statistics, vel_max)
   S(tsp_scene_spec) O(tsp_scene_spec, fluid_counts) O(tsp_scene_spec, jitter_seed) O(tsp_scene_spec, rigid_quat)
@@ -66,7 +70,7 @@ def test_struct_layouts_match_the_headers(tmp_path):
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
     mirrors = {"tsp_params": _capi.TspParams, "tsp_state_view": _capi.TspStateView,
                "tsp_step_info": _capi.TspStepInfo, "tsp_scene_spec": _capi.TspSceneSpec,
-               "tsp_statistics": _capi.TspStatistics}
+               "tsp_statistics": _capi.TspStatistics, "tsp_export_view": _capi.TspExportView}
     for line in out.strip().splitlines():
         parts = line.split()
         T, cls = parts[0], mirrors[parts[0]]

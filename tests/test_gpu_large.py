@@ -7,6 +7,7 @@ for density, pressure and velocity (normwise, max |a - b| / max |b|; printed, an
 1e-3 as a sanity check -- the FP32 path itself differs from FP64 by rounding only).
 """
 import os
+import time
 
 import numpy as np
 import pytest
@@ -59,3 +60,32 @@ def test_8m_scene_one_step_matches_the_oracle():
     print(f"  error against the FP64 oracle after one step: {errs}", flush=True)
     for f, e in errs.items():
         assert e <= 1e-3, (f, e)
+
+
+@pytest.mark.timeout(600)
+def test_8m_scene_generated_on_the_device_equals_the_host_build():
+    print("case: 8M dam break generated on the device", flush=True)
+    spec = scenes.dam_break_spec(200, 0.002)
+    p = scenes.make_params(0.002, termination_cond=scenes.TERM_POS)
+    t0 = time.perf_counter()
+    st = scenes.build_scene(spec)
+    a = HipBackend(0)
+    a.set_params(p)
+    a.pull(st)
+    a.synchronize()
+    t1 = time.perf_counter()
+    b = HipBackend(0)
+    b.set_params(p)
+    b.pull_spec(spec)
+    b.synchronize()
+    t2 = time.perf_counter()
+    print(f"  host build + pull {t1 - t0:.2f} s, device build {t2 - t1:.2f} s", flush=True)
+    del st
+    sa, sb = a.push(), b.push()
+    a.close()
+    b.close()
+    assert sa.n == sb.n == (8_000_000, 1_262_036, 0, 0)
+    assert np.array_equal(sa.fluid_id, sb.fluid_id) and np.array_equal(sa.boundary_id, sb.boundary_id)
+    for g in ("fluid", "boundary"):
+        for k in getattr(sa, g):
+            assert np.array_equal(getattr(sa, g)[k].view(np.uint32), getattr(sb, g)[k].view(np.uint32)), (g, k)

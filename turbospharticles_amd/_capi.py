@@ -171,6 +171,8 @@ def _declare_hip(lib):
         "tsp_batch_update": (i, [vp, ctypes.c_uint32, _u32p, ctypes.c_uint32, ctypes.POINTER(TspParams),
                                  ctypes.POINTER(ctypes.POINTER(TspStateView)), _fp, _u32p]),
         "tsp_batch_add_scene": (i, [vp, ctypes.POINTER(TspParams), ctypes.POINTER(TspStateView), _fp, _u32p]),
+        "tsp_pull_spec": (i, [vp, ctypes.POINTER(TspSceneSpec)]),
+        "tsp_batch_add_scene_spec": (i, [vp, ctypes.POINTER(TspParams), ctypes.POINTER(TspSceneSpec), _u32p]),
         "tsp_batch_remove_scene": (i, [vp, ctypes.c_uint32]),
         "tsp_batch_scene_ids": (i, [vp, _u32p, ctypes.c_uint32, _u32p]),
         "tsp_batch_scene_counts": (i, [vp, ctypes.c_uint32, _u32p]),

@@ -59,6 +59,13 @@ class HipBackend:
         self._check(self.lib.tsp_pull(self.h, ctypes.byref(v)), "tsp_pull")
         self.n = state.n
 
+    def pull_spec(self, spec):
+        """The scene of a tsp_scene_spec generated on the device (as scenes.build_scene + pull)"""
+        self._check(self.lib.tsp_pull_spec(self.h, ctypes.byref(spec)), "tsp_pull_spec")
+        c = (ctypes.c_uint32 * 4)()
+        self._check(self.lib.tsp_get_counts(self.h, c), "tsp_get_counts")
+        self.n = tuple(c)
+
     # I_SimBackend::PushToUniversalSimState
     def push(self):
         st = _capi.HostState(*self.n)
@@ -216,6 +223,16 @@ class HipBatch:
 
     def add(self, params, state):
         return self.update(add=[(params, state)])[0]
+
+    def add_spec(self, params, spec):
+        """a scene of a tsp_scene_spec generated on the device; returns its id"""
+        sid = ctypes.c_uint32()
+        self._check(self.lib.tsp_batch_add_scene_spec(self.h, ctypes.byref(params), ctypes.byref(spec), ctypes.byref(sid)),
+                    "tsp_batch_add_scene_spec")
+        c = (ctypes.c_uint32 * 4)()
+        self._check(self.lib.tsp_batch_scene_counts(self.h, sid.value, c), "tsp_batch_scene_counts")
+        self.counts[sid.value] = tuple(c)
+        return sid.value
 
     def remove(self, scene_id):
         self.update(remove=[scene_id])

@@ -19,7 +19,7 @@ INC = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "lib")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HIP_SOURCES = ["kernels_search.hip", "kernels_sph.hip", "kernels_stats.hip", "kernels_contact.hip", "kernels_export.hip", "tsp_capi.hip"]
+HIP_SOURCES = ["kernels_search.hip", "kernels_sph.hip", "kernels_stats.hip", "kernels_contact.hip", "kernels_export.hip", "kernels_build.hip", "tsp_capi.hip"]
 HOST_SOURCES = ["scene_plan.cpp", "scene_builder.cpp", "sim_backend_hip.cpp", "state_io.cpp"]
 
 HIPCC_FLAGS = [
@@ -44,8 +44,10 @@ def build_hip(verbose=False):
     os.makedirs(LIB, exist_ok=True)
     hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
     out = os.path.join(LIB, "libtsp_hip.so")
-    srcs = [os.path.join(HERE, "csrc", s) for s in HIP_SOURCES]
-    cmd = [hipcc] + HIPCC_FLAGS + ["-I" + INC, "-I" + os.path.join(HERE, "csrc"), "-shared", "-o", out + ".tmp"] + srcs
+    # the scene plan is shared with the host builder (plain C++, compiled here too)
+    srcs = [os.path.join(HERE, "csrc", s) for s in HIP_SOURCES] + [os.path.join(HERE, "host", "scene_plan.cpp")]
+    cmd = [hipcc] + HIPCC_FLAGS + ["-I" + INC, "-I" + os.path.join(HERE, "csrc"), "-I" + os.path.join(HERE, "host"),
+                                   "-shared", "-o", out + ".tmp"] + srcs
     r = _run(cmd)
     os.replace(out + ".tmp", out)
     if verbose and r.stderr:

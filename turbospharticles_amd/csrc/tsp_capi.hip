@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "scene_plan.h"
 #include "tsp_device.h"
 #include "tsp_hip.h"
 
@@ -44,6 +45,9 @@ namespace tsp
     void launchContactIteration(hipStream_t, const Dev&, uint32_t);
     void launchContactApply(hipStream_t, const Dev&);
     void launchExportSurface(hipStream_t, const Dev&, uint32_t, uint32_t, uint32_t, uint32_t*, uint32_t*, float*, float*);
+    void launchBuildFluid(hipStream_t, const Dev&, uint32_t, uint32_t, const BuildFluid&);
+    void launchBuildBoundary(hipStream_t, const BuildBoundary&, uint32_t, float4*, float*, uint64_t*, float*, uint32_t*,
+                             uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t*);
     void launchHaloBuild(hipStream_t, const Dev&);
     void launchAabb(hipStream_t, const Dev&, int, const float4*, uint32_t, uint32_t*);
     void launchDensity(hipStream_t, const Dev&);
@@ -241,8 +245,30 @@ namespace
     {
         int keep = -1;
         const tsp_state_view* view = nullptr;
+        const tsp::scene::Plan* plan = nullptr;   // generated on the device (the view holds its rigid body)
         SceneHost host;
     };
+
+    BuildVec vec(const tsp::scene::V3& v) { return BuildVec{ v.x, v.y, v.z }; }
+
+    // the view of a plan: counts, the rigid particles and bodies; fluid and boundary come from the
+    // device builder
+    tsp_state_view planView(const tsp::scene::Plan& plan)
+    {
+        tsp_state_view v{};
+        v.n_fluid = static_cast<uint32_t>(plan.nF);
+        v.n_boundary = static_cast<uint32_t>(plan.nB);
+        v.n_rigid = static_cast<uint32_t>(plan.idR.size());
+        v.n_bodies = static_cast<uint32_t>(plan.idRB.size());
+        auto& P = const_cast<tsp::scene::Plan&>(plan);
+        v.rigid_id = P.idR.empty() ? nullptr : P.idR.data();
+        v.rigid_body = P.bodyR.empty() ? nullptr : P.bodyR.data();
+        for (int f = 0; f < TSP_R_COUNT; f++) { v.rigid[f] = P.R[f].empty() ? nullptr : P.R[f].data(); }
+        v.body_id = P.idRB.empty() ? nullptr : P.idRB.data();
+        v.body_count = P.countRB.empty() ? nullptr : P.countRB.data();
+        for (int f = 0; f < TSP_RB_COUNT; f++) { v.body[f] = P.RB[f].empty() ? nullptr : P.RB[f].data(); }
+        return v;
+    }
 }
 
 struct tsp_ctx
@@ -409,6 +435,87 @@ struct tsp_ctx
 #else
         d.kgFF = nullptr;
 #endif
+    }
+
+    // --- a planned scene's fluid and boundary generated in the new arena (n) at h's offsets ---
+    void buildOnDevice(const Dev& n, const SceneHost& h, const tsp::scene::Plan& plan, uint64_t* nidB)
+    {
+        const tsp::scene::Params& p = plan.p;
+        const tsp::scene::FluidBlock& fb = plan.fluid;
+        BuildFluid bf{};
+        bf.origin = vec(fb.blk.origin); bf.u = vec(fb.blk.u); bf.v = vec(fb.blk.v); bf.w = vec(fb.blk.w);
+        bf.uLen = fb.blk.uLen; bf.vLen = fb.blk.vLen; bf.wLen = fb.blk.wLen;
+        bf.uOff = fb.uOff; bf.vOff = fb.vOff; bf.wOff = fb.wOff;
+        bf.radius = fb.radius; bf.jitAmp = fb.jitAmp; bf.size = p.size; bf.rho0 = p.rho0;
+        bf.seed = fb.seed;
+        bf.nu = fb.nu; bf.nv = fb.nv; bf.nw = fb.nw;
+        launchBuildFluid(stream, n, h.offF, h.nF, bf);
+        if (!h.nB) { return; }
+        BuildBoundary bb{};
+        bb.nFaces = plan.nFaces;
+        bb.layers = p.layers;
+        bb.size = p.size; bb.sizeInv = p.sizeInv; bb.radiusSqu = p.radiusSqu; bb.alpha = p.alpha;
+        bb.rho0 = p.rho0; bb.massB = p.massB;
+        bb.cellSizeInv = 1.0f / p.radius;
+        // cell bounds of the layers: each face's positions are monotone in i, j and the layer, so
+        // its corners bound it
+        float lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+        for (uint32_t q = 0; q < plan.nFaces; q++)
+        {
+            const tsp::scene::BoundaryFace& f = plan.faces[q];
+            BuildFace& g = bb.faces[q];
+            g.origin = vec(f.f.origin); g.u = vec(f.f.u); g.v = vec(f.f.v); g.n = vec(f.f.n);
+            g.uLen = f.f.uLen; g.vLen = f.f.vLen;
+            g.nu = f.nu; g.nv = f.nv; g.first = f.first;
+            for (uint32_t layer : { 0u, p.layers - 1u })
+            {
+                for (uint32_t i : { 0u, f.nu - 1u })
+                {
+                    for (uint32_t j : { 0u, f.nv - 1u })
+                    {
+                        const tsp::scene::V3 x = tsp::scene::BoundaryPosition(f.f, p.size, layer, i, j);
+                        const float c[3] = { x.x, x.y, x.z };
+                        for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], c[a]); hi[a] = std::max(hi[a], c[a]); }
+                    }
+                }
+            }
+        }
+        uint32_t totalBits = 0;
+        for (int a = 0; a < 3; a++)
+        {
+            bb.cellMin[a] = static_cast<int32_t>(std::floor(lo[a] * bb.cellSizeInv)) - 1;
+            bb.cellMax[a] = static_cast<int32_t>(std::floor(hi[a] * bb.cellSizeInv)) + 1;
+            const uint64_t range = static_cast<uint64_t>(static_cast<int64_t>(bb.cellMax[a]) - bb.cellMin[a]) + 1;
+            uint32_t bits = 1;
+            while ((1ull << bits) < range) { bits++; }
+            bb.bits[a] = bits;
+            totalBits += bits;
+        }
+        if (totalBits > 32) { throw Error{ TSP_ERR_CAPACITY, "container too large for the device builder's 32-bit cell keys" }; }
+        const uint32_t nB = h.nB;
+        const uint64_t nTiles = (nB + 2047ull) / 2048ull;
+        std::vector<void*> tmp;
+        try
+        {
+            float* xyz = alloc<float>(3ull * nB, tmp);
+            uint32_t* key = alloc<uint32_t>(nB, tmp);
+            uint32_t* idx = alloc<uint32_t>(nB, tmp);
+            uint32_t* key2 = alloc<uint32_t>(nB, tmp);
+            uint32_t* idx2 = alloc<uint32_t>(nB, tmp);
+            uint32_t* hist = alloc<uint32_t>(256ull * nTiles, tmp);
+            uint32_t* sums = alloc<uint32_t>((256ull * nTiles + 2047ull) / 2048ull + 1ull, tmp);
+            launchBuildBoundary(stream, bb, nB, n.bpos + h.offB, n.bvol + h.offB, nidB + h.offB, xyz, key, idx, key2, idx2,
+                                hist, sums);
+            hipCheck(hipGetLastError(), "device builder launch");
+            sync();
+        }
+        catch (...)
+        {
+            (void)hipStreamSynchronize(stream);
+            releaseList(tmp);
+            throw;
+        }
+        releaseList(tmp);
     }
 
     // --- storage of the contact solve sized by the RB and RR entries of a search ---
@@ -598,7 +705,9 @@ struct tsp_ctx
                     continue;
                 }
                 const tsp_state_view* s = src[q].view;
+                if (src[q].plan) { buildOnDevice(n, h, *src[q].plan, nidB); }
                 // fluid: the SoA fields of the view packed into the float4 records
+                if (!src[q].plan)
                 {
                     std::vector<float4> r[4];
                     for (auto& v : r) { v.resize(h.nF); }
@@ -622,6 +731,7 @@ struct tsp_ctx
                     h2d(n.id + h.offF, ids(s->fluid_id, h.nF), sizeof(uint64_t) * h.nF);
                 }
                 // boundary (ensureBoundaryCoefficientsPresent defaults: psi = massB, V = psi / rho0)
+                if (!src[q].plan)
                 {
                     const float hh = h.params.particle_size, rho0 = h.params.rest_density;
                     const float massB = rho0 * hh * hh * hh;
@@ -1567,7 +1677,8 @@ struct tsp_ctx
 
     // one relayout: the listed scenes leave, new scenes join at the end
     void update(uint32_t nRemove, const uint32_t* removeIds, uint32_t nAdd, const tsp_params* params,
-                const tsp_state_view* const* views, const float* domains, uint32_t* outIds)
+                const tsp_state_view* const* views, const float* domains, uint32_t* outIds,
+                const tsp::scene::Plan* const* plans = nullptr)
     {
         if ((nRemove && !removeIds) || (nAdd && (!params || !views))) { throw Error{ TSP_ERR_INVALID, "null scene arrays" }; }
         std::vector<bool> drop(nScenes(), false);
@@ -1590,6 +1701,7 @@ struct tsp_ctx
             if (!views[q]) { throw Error{ TSP_ERR_INVALID, "null state" }; }
             Source x;
             x.view = views[q];
+            x.plan = plans ? plans[q] : nullptr;
             x.host = newScene(params[q], views[q], domains ? domains + 6 * q : nullptr, domains ? domains + 6 * q + 3 : nullptr);
             src.push_back(std::move(x));
         }
@@ -1752,6 +1864,30 @@ int tsp_pull(tsp_ctx* ctx, const tsp_state_view* s)
         float dom[6];
         for (int a = 0; a < 3; a++) { dom[a] = ctx->domMin[a]; dom[3 + a] = ctx->domMax[a]; }
         ctx->update(static_cast<uint32_t>(all.size()), all.data(), 1, &ctx->defaults, &s, ctx->domainSet ? dom : nullptr, nullptr);
+        return TSP_OK;
+    });
+}
+
+int tsp_pull_spec(tsp_ctx* ctx, const tsp_scene_spec* spec)
+{
+    return guarded(ctx, [&]() {
+        if (!spec) { throw Error{ TSP_ERR_INVALID, "null scene spec" }; }
+        if (!ctx->paramsSet) { throw Error{ TSP_ERR_STATE, "pull before params" }; }
+        tsp::scene::Plan plan;
+        const int rc = tsp::scene::MakePlan(*spec, plan);
+        if (rc != TSP_OK) { throw Error{ rc, "invalid scene spec" }; }
+        if (plan.nF > 0x7fffffffull || plan.nB > 0x7fffffffull) { throw Error{ TSP_ERR_CAPACITY, "particle count exceeds the local particle limit" }; }
+        const tsp_state_view v = planView(plan);
+        const tsp_state_view* vp = &v;
+        const tsp::scene::Plan* pp = &plan;
+        // the spec's domain becomes the context's, as SetSimDomainBounds with the scene's bounds
+        for (int a = 0; a < 3; a++) { ctx->domMin[a] = plan.domMin[a]; ctx->domMax[a] = plan.domMax[a]; }
+        ctx->domainSet = true;
+        std::vector<uint32_t> all;
+        for (const SceneHost& h : ctx->scenes) { all.push_back(h.id); }
+        float dom[6];
+        for (int a = 0; a < 3; a++) { dom[a] = plan.domMin[a]; dom[3 + a] = plan.domMax[a]; }
+        ctx->update(static_cast<uint32_t>(all.size()), all.data(), 1, &ctx->defaults, &vp, dom, nullptr, &pp);
         return TSP_OK;
     });
 }
@@ -2047,6 +2183,25 @@ int tsp_batch_add_scene(tsp_ctx* ctx, const tsp_params* params, const tsp_state_
                         uint32_t* scene_id)
 {
     return tsp_batch_update(ctx, 0, nullptr, 1, params, &state, domain, scene_id);
+}
+
+int tsp_batch_add_scene_spec(tsp_ctx* ctx, const tsp_params* params, const tsp_scene_spec* spec, uint32_t* scene_id)
+{
+    return guarded(ctx, [&]() {
+        if (!params || !spec) { throw Error{ TSP_ERR_INVALID, "null params or scene spec" }; }
+        tsp::scene::Plan plan;
+        const int rc = tsp::scene::MakePlan(*spec, plan);
+        if (rc != TSP_OK) { throw Error{ rc, "invalid scene spec" }; }
+        if (plan.nF > 0x7fffffffull || plan.nB > 0x7fffffffull) { throw Error{ TSP_ERR_CAPACITY, "particle count exceeds the local particle limit" }; }
+        const tsp_state_view v = planView(plan);
+        const tsp_state_view* vp = &v;
+        const tsp::scene::Plan* pp = &plan;
+        float dom[6];
+        for (int a = 0; a < 3; a++) { dom[a] = plan.domMin[a]; dom[3 + a] = plan.domMax[a]; }
+        if (!ctx->paramsSet) { ctx->defaults = *params; ctx->paramsSet = true; }
+        ctx->update(0, nullptr, 1, params, &vp, dom, scene_id, &pp);
+        return TSP_OK;
+    });
 }
 
 int tsp_batch_remove_scene(tsp_ctx* ctx, uint32_t scene_id)

@@ -191,6 +191,42 @@ namespace tsp
         uint32_t n;
     };
 
+    // --- device scene builder (kernels_build.hip): the parts of a scene plan its kernels read ---
+    struct BuildVec
+    {
+        float x, y, z;
+    };
+
+    // the fluid block: nu x nv x nw lattice, index = (kw nv + jv) nu + iu
+    struct BuildFluid
+    {
+        BuildVec origin, u, v, w;
+        float uLen, vLen, wLen;
+        float uOff, vOff, wOff, radius, jitAmp, size, rho0;
+        uint64_t seed;
+        uint32_t nu, nv, nw;
+    };
+
+    // one container face: layers x nu x nv particles from index `first`
+    struct BuildFace
+    {
+        BuildVec origin, u, v, n;
+        float uLen, vLen;
+        uint32_t nu, nv;
+        uint64_t first;
+    };
+
+    // the boundary layers and their psi / V sums (cells of size 2h; keys pack the cell offsets
+    // from cellMin as z | y | x with bits[] bits each)
+    struct BuildBoundary
+    {
+        BuildFace faces[6];
+        uint32_t nFaces, layers;
+        float size, sizeInv, radiusSqu, alpha, rho0, massB, cellSizeInv;
+        int32_t cellMin[3], cellMax[3];
+        uint32_t bits[3];
+    };
+
     // fields of the body state Dev::rb (field-major: rb[field * nRB + body], TSP_RB_* order)
     enum
     {
