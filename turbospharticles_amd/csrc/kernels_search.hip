@@ -903,15 +903,18 @@ namespace tsp
     }
 
     // ----------------------------------------------------------------------------------------
-    // rigid-source lists, one thread per rigid particle: mode 0 counts RF, RB and RR; mode 1
-    // fills the RF CSR; mode 2 fills RF and the RB and RR lists of the contact solver
+    // rigid-source lists, one wave per rigid particle: mode 0 counts RF, RB and RR; mode 1 fills
+    // the RF CSR; mode 2 fills RF and the RB and RR lists of the contact solver. The candidates of
+    // a relation -- cell by cell in z-y-x order, by index inside a cell -- are tested 64 at a
+    // time, and ballots keep that order in the lists.
 
     __global__ void __launch_bounds__(kBlock) kQueryRigid(Dev d, int mode)
     {
+        const uint32_t i = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+        const uint32_t lane = threadIdx.x & 63u;
+        if (i >= d.nR) { return; }
         const bool fill = mode != 0;
         const bool contact = mode == 2;
-        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-        if (i >= d.nR) { return; }
         const uint32_t sc = d.bodyScene[d.rbody[i]];
         const Frame& f = d.frames[sc];
         const float r2 = d.consts[sc].r2;
@@ -920,9 +923,8 @@ namespace tsp
         const int cx = floorToInt(pi.x * f.cellSizeInv) - f.orig[0];
         const int cy = floorToInt(pi.y * f.cellSizeInv) - f.orig[1];
         const int cz = floorToInt(pi.z * f.cellSizeInv) - f.orig[2];
-        uint32_t nF = 0, nB = 0, nRR = 0;
-        uint32_t w = fill ? d.rfOff[i] : 0u;
-        uint32_t wb = contact ? d.rbOff[i] : 0u, wr = contact ? d.rrOff[i] : 0u;
+        uint32_t cnt[3] = { 0, 0, 0 };
+        uint32_t w[3] = { fill ? d.rfOff[i] : 0u, contact ? d.rbOff[i] : 0u, contact ? d.rrOff[i] : 0u };
         const bool inside = cx >= 0 && cx < static_cast<int>(f.dim[0]) && cy >= 0 && cy < static_cast<int>(f.dim[1])
                          && cz >= 0 && cz < static_cast<int>(f.dim[2]);
         if (inside)
@@ -930,52 +932,84 @@ namespace tsp
             const int lo[3] = { max(cx - reach, 0), max(cy - reach, 0), max(cz - reach, 0) };
             const int hi[3] = { min(cx + reach, static_cast<int>(f.dim[0]) - 1), min(cy + reach, static_cast<int>(f.dim[1]) - 1),
                                 min(cz + reach, static_cast<int>(f.dim[2]) - 1) };
+            const uint32_t nx = static_cast<uint32_t>(hi[0] - lo[0] + 1), ny = static_cast<uint32_t>(hi[1] - lo[1] + 1);
+            const uint32_t nCells = nx * ny * static_cast<uint32_t>(hi[2] - lo[2] + 1);
             const uint32_t body = d.rbody[i];
-            for (int z = lo[2]; z <= hi[2]; z++)
+            const uint64_t below = (1ull << lane) - 1ull;
+            const int nRel = (fill && !contact) ? 1 : 3;
+            for (uint32_t c0 = 0; c0 < nCells; c0 += 64)
             {
-                for (int y = lo[1]; y <= hi[1]; y++)
+                // this lane's cell of the batch and its three ranges
+                const uint32_t c = c0 + lane;
+                uint2 rng[3] = { make_uint2(0u, 0u), make_uint2(0u, 0u), make_uint2(0u, 0u) };
+                if (c < nCells)
                 {
-                    for (int x = lo[0]; x <= hi[0]; x++)
+                    const uint32_t lin = linearCell(f, lo[0] + static_cast<int>(c % nx), lo[1] + static_cast<int>((c / nx) % ny),
+                                                    lo[2] + static_cast<int>(c / (nx * ny)));
+                    rng[0] = d.cellF[lin];
+                    rng[1] = d.cellB[lin];
+                    if (d.nR) { rng[2] = d.cellR[lin]; }
+                }
+                for (int rel = 0; rel < nRel; rel++)
+                {
+                    // inclusive prefix of the cells' candidate counts over the lanes
+                    const uint32_t size = rng[rel].y - rng[rel].x;
+                    uint32_t incl = size;
+                    for (int o = 1; o < 64; o <<= 1)
                     {
-                        const uint32_t lin = linearCell(f, x, y, z);
-                        const uint2 rf = d.cellF[lin];
-                        for (uint32_t j = rf.x; j < rf.y; j++)
+                        const uint32_t v = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), o, 64));
+                        if (lane >= static_cast<uint32_t>(o)) { incl += v; }
+                    }
+                    const uint32_t total = static_cast<uint32_t>(__shfl(static_cast<int>(incl), 63, 64));
+                    for (uint32_t f0 = 0; f0 < total; f0 += 64)
+                    {
+                        const uint32_t fi = f0 + lane;
+                        bool hit = false;
+                        uint32_t j = 0;
+                        // the lane (cell) whose prefix range holds fi: the first with incl > fi
+                        // (six halvings of [0, 63], taken by every lane together)
+                        uint32_t a = 0, b = 63;
+                        for (int step = 0; step < 6; step++)
                         {
-                            if (within(pi.x, pi.y, pi.z, d.pos[j], r2))
+                            const uint32_t m = (a + b) >> 1;
+                            const uint32_t im = static_cast<uint32_t>(__shfl(static_cast<int>(incl), static_cast<int>(m), 64));
+                            if (im > fi) { b = m; } else { a = m + 1; }
+                        }
+                        const uint32_t inclA = static_cast<uint32_t>(__shfl(static_cast<int>(incl), static_cast<int>(a), 64));
+                        const uint32_t sizeA = static_cast<uint32_t>(__shfl(static_cast<int>(size), static_cast<int>(a), 64));
+                        const uint32_t startA = static_cast<uint32_t>(__shfl(static_cast<int>(rng[rel].x), static_cast<int>(a), 64));
+                        if (fi < total)
+                        {
+                            const uint32_t t = startA + (fi - (inclA - sizeA));
+                            if (rel == 0) { j = t; hit = within(pi.x, pi.y, pi.z, d.pos[j], r2); }
+                            else if (rel == 1) { j = t; hit = within(pi.x, pi.y, pi.z, d.bpos[j], r2); }
+                            else
                             {
-                                if (fill) { d.rfIdx[w++] = j; } else { nF++; }
+                                j = d.idxR[t];
+                                hit = j != i && d.rbody[j] != body && within(pi.x, pi.y, pi.z, d.rpos[j], r2);
                             }
                         }
-                        if (fill && !contact) { continue; }
-                        const uint2 rb = d.cellB[lin];
-                        for (uint32_t j = rb.x; j < rb.y; j++)
+                        const uint64_t bal = __ballot(hit);
+                        const bool store = rel == 0 ? fill : contact;
+                        if (store && hit)
                         {
-                            if (within(pi.x, pi.y, pi.z, d.bpos[j], r2))
-                            {
-                                if (contact) { d.rbIdx[wb++] = j; } else { nB++; }
-                            }
+                            uint32_t* dst = rel == 0 ? d.rfIdx : (rel == 1 ? d.rbIdx : d.rrIdx);
+                            dst[w[rel] + static_cast<uint32_t>(__popcll(bal & below))] = j;
                         }
-                        const uint2 rr = d.cellR[lin];
-                        for (uint32_t t = rr.x; t < rr.y; t++)
-                        {
-                            const uint32_t j = d.idxR[t];
-                            if (j == i || d.rbody[j] == body) { continue; }
-                            if (within(pi.x, pi.y, pi.z, d.rpos[j], r2))
-                            {
-                                if (contact) { d.rrIdx[wr++] = j; } else { nRR++; }
-                            }
-                        }
+                        const uint32_t n = static_cast<uint32_t>(__popcll(bal));
+                        w[rel] += n;
+                        cnt[rel] += n;
                     }
                 }
             }
         }
-        if (!fill)
+        if (!fill && lane == 0)
         {
             if (i == 0) { d.rfOff[d.nR] = 0u; }
-            d.rfCnt[i] = nF;
-            d.rfOff[i] = nF;
-            d.rbCnt[i] = nB;
-            d.rrCnt[i] = nRR;
+            d.rfCnt[i] = cnt[0];
+            d.rfOff[i] = cnt[0];
+            d.rbCnt[i] = cnt[1];
+            d.rrCnt[i] = cnt[2];
         }
     }
 
@@ -1181,7 +1215,7 @@ namespace tsp
     void launchQueryRigid(hipStream_t s, const Dev& d, int mode, uint32_t* tmp)
     {
         if (!d.nR) { return; }
-        kQueryRigid<<<blocksFor(d.nR), kBlock, 0, s>>>(d, mode);
+        kQueryRigid<<<blocksFor(64ull * d.nR), kBlock, 0, s>>>(d, mode);
         if (mode == 0)
         {
             launchScan(s, d.rfOff, d.nR + 1, tmp);

@@ -599,37 +599,50 @@ namespace tsp
     // :1717-1778): reaction of the fluid neighbors of each rigid particle
     __global__ void __launch_bounds__(kBlock) kRigidForces(Dev d, bool pressure)
     {
-        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+        // one wave per rigid particle: the lanes compute 64 pairs' coefficients and gradients at
+        // once, then every lane folds them in list order (the single-thread sum, term by term)
+        const uint32_t i = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+        const uint32_t lane = threadIdx.x & 63u;
         if (i >= d.nR) { return; }
         const Consts& k = d.consts[d.bodyScene[d.rbody[i]]];
         const float4 ri = d.rpos[i], ui = d.rvel[i];
         const float xi = ri.x, yi = ri.y, zi = ri.z;
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        for (uint32_t q = d.rfOff[i]; q < d.rfOff[i + 1]; q++)
+        const uint32_t q0 = d.rfOff[i], q1 = d.rfOff[i + 1];
+        for (uint32_t base = q0; base < q1; base += 64)
         {
-            const uint32_t j = d.rfIdx[q];
-            const float4 pj = d.pos[j];
-            const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
-            float gx, gy, gz;
-            kgScalar(dx, dy, dz, k, gx, gy, gz);
-            float fc;
-            if (pressure)
+            const uint32_t q = base + lane;
+            float fc = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+            if (q < q1)
             {
-                fc = k.mass * k.presOuterB * ri.w * pj.w;
+                const uint32_t j = d.rfIdx[q];
+                const float4 pj = d.pos[j];
+                const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+                kgScalar(dx, dy, dz, k, gx, gy, gz);
+                if (pressure)
+                {
+                    fc = k.mass * k.presOuterB * ri.w * pj.w;
+                }
+                else
+                {
+                    const float4 vj = d.vel[j];
+                    const float dvx = vj.x - ui.x, dvy = vj.y - ui.y, dvz = vj.z - ui.z;
+                    const float dot = __builtin_fmaf(dvx, dx, __builtin_fmaf(dvy, dy, dvz * dz));
+                    const float d2 = dist2(dx, dy, dz);
+                    const float coeff = (ui.w * dot) / (d2 + k.hsqEps);
+                    fc = -k.mass * k.visB * coeff;
+                }
             }
-            else
+            const uint32_t n = min(64u, q1 - base);
+            for (uint32_t l = 0; l < n; l++)
             {
-                const float4 vj = d.vel[j];
-                const float dvx = vj.x - ui.x, dvy = vj.y - ui.y, dvz = vj.z - ui.z;
-                const float dot = __builtin_fmaf(dvx, dx, __builtin_fmaf(dvy, dy, dvz * dz));
-                const float d2 = dist2(dx, dy, dz);
-                const float coeff = (ui.w * dot) / (d2 + k.hsqEps);
-                fc = -k.mass * k.visB * coeff;
+                const float c = __shfl(fc, static_cast<int>(l), 64);
+                fx = __builtin_fmaf(c, __shfl(gx, static_cast<int>(l), 64), fx);
+                fy = __builtin_fmaf(c, __shfl(gy, static_cast<int>(l), 64), fy);
+                fz = __builtin_fmaf(c, __shfl(gz, static_cast<int>(l), 64), fz);
             }
-            fx = __builtin_fmaf(fc, gx, fx);
-            fy = __builtin_fmaf(fc, gy, fy);
-            fz = __builtin_fmaf(fc, gz, fz);
         }
+        if (lane != 0) { return; }
         if (pressure) { d.rfx[i] += fx; d.rfy[i] += fy; d.rfz[i] += fz; }
         else { d.rfx[i] = fx; d.rfy[i] = fy; d.rfz[i] = fz; }
     }
@@ -805,7 +818,10 @@ namespace tsp
 
     void launchRigidUpdate(hipStream_t s, const Dev& d, bool predicted) { if (d.nR) { kRigidUpdate<<<blocksOf(d.nR), kBlock, 0, s>>>(d, predicted); } }
     void launchRigidClearGravity(hipStream_t s, const Dev& d) { if (d.nRB) { kRigidClearGravity<<<blocksOf(d.nRB), kBlock, 0, s>>>(d); } }
-    void launchRigidForces(hipStream_t s, const Dev& d, bool pressure) { if (d.nR) { kRigidForces<<<blocksOf(d.nR), kBlock, 0, s>>>(d, pressure); } }
+    void launchRigidForces(hipStream_t s, const Dev& d, bool pressure)
+    {
+        if (d.nR) { kRigidForces<<<blocksOf(64ull * d.nR), kBlock, 0, s>>>(d, pressure); }
+    }
     void launchRigidReduce(hipStream_t s, const Dev& d) { if (d.nRB) { kRigidReduce<<<d.nRB, kBlock, 0, s>>>(d); } }
     void launchRigidPredict(hipStream_t s, const Dev& d) { if (d.nRB) { kRigidPredict<<<blocksOf(d.nRB), kBlock, 0, s>>>(d); } }
     void launchRigidIntegrate(hipStream_t s, const Dev& d) { if (d.nRB) { kRigidIntegrate<<<blocksOf(d.nRB), kBlock, 0, s>>>(d); } }
