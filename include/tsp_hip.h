@@ -266,6 +266,10 @@ int tsp_batch_scene_info(tsp_ctx *ctx, uint32_t scene_id, tsp_step_info *info);
 typedef struct tsp_scene_spec tsp_scene_spec;
 int tsp_pull_spec(tsp_ctx *ctx, const tsp_scene_spec *spec);
 int tsp_batch_add_scene_spec(tsp_ctx *ctx, const tsp_params *params, const tsp_scene_spec *spec, uint32_t *scene_id);
+/* tsp_batch_update with the new scenes given as specs and generated on the device (one relayout
+   for the removals and all the additions) */
+int tsp_batch_update_specs(tsp_ctx *ctx, uint32_t n_remove, const uint32_t *remove_ids, uint32_t n_add,
+                           const tsp_params *params, const tsp_scene_spec *const *specs, uint32_t *out_ids);
 
 /* tsp_get_statistics of one scene */
 int tsp_batch_scene_statistics(tsp_ctx *ctx, uint32_t scene_id, tsp_statistics *out);

@@ -40,6 +40,8 @@ typedef struct tsp_scene tsp_scene;
 
 /* builds a scene; returns 0 or a negative tsp_status */
 int tsp_scene_build(const tsp_scene_spec *spec, tsp_scene **out);
+/* the counts a spec builds (fluid, boundary, rigid, bodies) without building it */
+int tsp_scene_spec_counts(const tsp_scene_spec *spec, uint32_t counts[4]);
 /* particle counts of a built scene (fluid, boundary, rigid, bodies) */
 void tsp_scene_counts(const tsp_scene *s, uint32_t counts[4]);
 /* simulation domain of the scene (ComputeSimDomainBounds, scene/scenario_sim.cpp:123-148) */

@@ -138,3 +138,25 @@ def test_served_stream_with_a_tight_budget_matches_the_plain_form():
     for a, c in zip(rb, rp):
         _same_state(a.result, c.result, f"request {a.rid}")
         assert a.result.step == a.steps
+
+
+def test_served_stream_of_scene_specs_matches_host_built_states():
+    # the same five requests, every other one carried as a scene spec (generated on the device
+    # at admission); each result must equal the plain run from host-built states
+    from turbospharticles_amd import serving
+    print("case: served stream, specs and states mixed, budget for 2", flush=True)
+    p = scenes.make_params(0.01, termination_cond=scenes.TERM_POS)
+    specs = [(scenes.scaled_cube_drop_spec(10, 0.01), 12), (scenes.scaled_dam_break_spec(8, 0.01), 7),
+             (scenes.scaled_dam_break_spec(12, 0.01, jitter_seed=3), 5), (scenes.scaled_dam_break_spec(9, 0.01), 9),
+             (scenes.scaled_cube_drop_spec(8, 0.01, rotation_seed=2), 4)]
+    rb = [serving.Request(q, 0.0, p, None, n, spec=s) if q % 2 == 0 else
+          serving.Request(q, 0.0, p, scenes.build_scene(s), n) for q, (s, n) in enumerate(specs)]
+    budget = 2.5 * max(r.footprint for r in rb)
+    serving.serve_batched(rb, budget, chunk_steps=3)
+    rp = [serving.Request(q, 0.0, p, scenes.build_scene(s), n) for q, (s, n) in enumerate(specs)]
+    serving.serve_plain(rp)
+    rs = [serving.Request(q, 0.0, p, None, n, spec=s) for q, (s, n) in enumerate(specs)]
+    serving.serve_plain(rs)
+    for a, c, e in zip(rb, rp, rs):
+        _same_state(a.result, c.result, f"request {a.rid}")
+        _same_state(e.result, c.result, f"request {e.rid} (plain, from its spec)")

@@ -173,6 +173,8 @@ def _declare_hip(lib):
         "tsp_batch_add_scene": (i, [vp, ctypes.POINTER(TspParams), ctypes.POINTER(TspStateView), _fp, _u32p]),
         "tsp_pull_spec": (i, [vp, ctypes.POINTER(TspSceneSpec)]),
         "tsp_batch_add_scene_spec": (i, [vp, ctypes.POINTER(TspParams), ctypes.POINTER(TspSceneSpec), _u32p]),
+        "tsp_batch_update_specs": (i, [vp, ctypes.c_uint32, _u32p, ctypes.c_uint32, ctypes.POINTER(TspParams),
+                                       ctypes.POINTER(ctypes.POINTER(TspSceneSpec)), _u32p]),
         "tsp_batch_remove_scene": (i, [vp, ctypes.c_uint32]),
         "tsp_batch_scene_ids": (i, [vp, _u32p, ctypes.c_uint32, _u32p]),
         "tsp_batch_scene_counts": (i, [vp, ctypes.c_uint32, _u32p]),
@@ -199,6 +201,7 @@ def _declare_host(lib):
         "tsp_scene_export": (i, [vp, ctypes.POINTER(TspStateView)]),
         "tsp_scene_free": (None, [vp]),
         "tsp_scene_random_quat": (None, [ctypes.c_uint64, _fp]),
+        "tsp_scene_spec_counts": (i, [ctypes.POINTER(TspSceneSpec), _u32p]),
         "tsp_state_file_info": (i, [ctypes.c_char_p, _u32p, _u64p, ctypes.POINTER(ctypes.c_double)]),
         "tsp_state_file_read": (i, [ctypes.c_char_p, ctypes.POINTER(TspStateView)]),
         "tsp_state_file_write": (i, [ctypes.c_char_p, ctypes.POINTER(TspStateView)]),

@@ -224,6 +224,27 @@ class HipBatch:
     def add(self, params, state):
         return self.update(add=[(params, state)])[0]
 
+    def update_specs(self, remove=(), add=()):
+        """One relayout: scenes `remove` leave, scenes `add` = [(params, tsp_scene_spec)] are
+        generated on the device and join; returns the new ids."""
+        rm = np.array([int(v) for v in remove], np.uint32)
+        n = len(add)
+        params = (_capi.TspParams * max(n, 1))()
+        specs = [sp for _, sp in add]
+        sptrs = (ctypes.POINTER(_capi.TspSceneSpec) * max(n, 1))()
+        for q, (p, sp) in enumerate(add):
+            params[q] = p
+            sptrs[q] = ctypes.pointer(sp)
+        ids = np.zeros(max(n, 1), np.uint32)
+        self._check(self.lib.tsp_batch_update_specs(self.h, len(rm), _capi.u32ptr(rm) if len(rm) else None, n, params,
+                                                    sptrs, _capi.u32ptr(ids)), "tsp_batch_update_specs")
+        for r in rm:
+            self.counts.pop(int(r), None)
+        out = [int(v) for v in ids[:n]]
+        for sid, sp in zip(out, specs):
+            self.counts[sid] = spec_counts(sp)
+        return out
+
     def add_spec(self, params, spec):
         """a scene of a tsp_scene_spec generated on the device; returns its id"""
         sid = ctypes.c_uint32()
@@ -286,3 +307,12 @@ class HipBatch:
     def footprint(n):
         """device bytes a scene with counts n = (fluid, boundary, rigid, bodies) holds while resident"""
         return int(_capi.hip_lib().tsp_batch_footprint(*[int(v) for v in n]))
+
+
+def spec_counts(spec):
+    """(fluid, boundary, rigid, bodies) that a tsp_scene_spec builds"""
+    c = (ctypes.c_uint32 * 4)()
+    rc = _capi.host_lib().tsp_scene_spec_counts(ctypes.byref(spec), c)
+    if rc != 0:
+        raise RuntimeError(f"tsp_scene_spec_counts failed ({rc})")
+    return tuple(int(v) for v in c)

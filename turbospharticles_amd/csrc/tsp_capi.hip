@@ -2204,6 +2204,36 @@ int tsp_batch_add_scene_spec(tsp_ctx* ctx, const tsp_params* params, const tsp_s
     });
 }
 
+int tsp_batch_update_specs(tsp_ctx* ctx, uint32_t n_remove, const uint32_t* remove_ids, uint32_t n_add,
+                           const tsp_params* params, const tsp_scene_spec* const* specs, uint32_t* out_ids)
+{
+    return guarded(ctx, [&]() {
+        if (n_add && (!params || !specs)) { throw Error{ TSP_ERR_INVALID, "null params or scene specs" }; }
+        std::vector<tsp::scene::Plan> plans(n_add);
+        std::vector<tsp_state_view> views(n_add);
+        std::vector<const tsp_state_view*> vptr(n_add);
+        std::vector<const tsp::scene::Plan*> pptr(n_add);
+        std::vector<float> dom(6ull * n_add);
+        for (uint32_t q = 0; q < n_add; q++)
+        {
+            if (!specs[q]) { throw Error{ TSP_ERR_INVALID, "null scene spec" }; }
+            const int rc = tsp::scene::MakePlan(*specs[q], plans[q]);
+            if (rc != TSP_OK) { throw Error{ rc, "invalid scene spec" }; }
+            if (plans[q].nF > 0x7fffffffull || plans[q].nB > 0x7fffffffull)
+            {
+                throw Error{ TSP_ERR_CAPACITY, "particle count exceeds the local particle limit" };
+            }
+            views[q] = planView(plans[q]);
+            vptr[q] = &views[q];
+            pptr[q] = &plans[q];
+            for (int a = 0; a < 3; a++) { dom[6 * q + a] = plans[q].domMin[a]; dom[6 * q + 3 + a] = plans[q].domMax[a]; }
+        }
+        if (!ctx->paramsSet && n_add) { ctx->defaults = params[0]; ctx->paramsSet = true; }
+        ctx->update(n_remove, remove_ids, n_add, params, vptr.data(), dom.data(), out_ids, pptr.data());
+        return TSP_OK;
+    });
+}
+
 int tsp_batch_remove_scene(tsp_ctx* ctx, uint32_t scene_id)
 {
     return tsp_batch_update(ctx, 1, &scene_id, 0, nullptr, nullptr, nullptr, nullptr);

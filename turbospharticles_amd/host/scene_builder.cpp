@@ -113,6 +113,20 @@ extern "C" int tsp_scene_build(const tsp_scene_spec* spec, tsp_scene** out)
     return TSP_OK;
 }
 
+extern "C" int tsp_scene_spec_counts(const tsp_scene_spec* spec, uint32_t counts[4])
+{
+    if (!spec || !counts) { return TSP_ERR_INVALID; }
+    Plan plan;
+    const int rc = MakePlan(*spec, plan);
+    if (rc != TSP_OK) { return rc; }
+    if (plan.nF > 0x7fffffffull || plan.nB > 0x7fffffffull) { return TSP_ERR_CAPACITY; }
+    counts[0] = static_cast<uint32_t>(plan.nF);
+    counts[1] = static_cast<uint32_t>(plan.nB);
+    counts[2] = static_cast<uint32_t>(plan.idR.size());
+    counts[3] = static_cast<uint32_t>(plan.idRB.size());
+    return TSP_OK;
+}
+
 extern "C" void tsp_scene_counts(const tsp_scene* s, uint32_t counts[4])
 {
     counts[0] = static_cast<uint32_t>(s->idF.size());

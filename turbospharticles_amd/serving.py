@@ -10,21 +10,31 @@ takes exactly its own number of steps, and its result is bit-identical to the sc
 
 The plain form (`serve_plain`) runs the same stream one scene at a time, one context per scene,
 in arrival order: the reference's one-simulation-per-backend model.
+
+A request's scene is either a host state or a scene spec; a spec is generated on the device when
+the request is admitted (tsp_batch_update_specs / tsp_pull_spec), so no host state is built or
+copied for it.
 """
 import collections
 import time
 
-from .backend import HipBackend, HipBatch
+from .backend import HipBackend, HipBatch, spec_counts
 
 
 class Request:
-    def __init__(self, rid, arrival_s, params, state, steps):
+    """One scene and its step count; `state` is a HostState, or None with `spec` a tsp_scene_spec."""
+
+    def __init__(self, rid, arrival_s, params, state, steps, spec=None):
+        if (state is None) == (spec is None):
+            raise ValueError("a request carries either a state or a scene spec")
         self.rid = rid
         self.arrival_s = float(arrival_s)
         self.params = params
         self.state = state
+        self.spec = spec
         self.steps = int(steps)
-        self.footprint = HipBatch.footprint(state.n)
+        self.n = state.n if state is not None else spec_counts(spec)
+        self.footprint = HipBatch.footprint(self.n)
         # filled in by the server
         self.admitted_s = None
         self.done_s = None
@@ -32,7 +42,7 @@ class Request:
 
     @property
     def particle_steps(self):
-        return self.state.n[0] * self.steps
+        return self.n[0] * self.steps
 
     @property
     def latency_s(self):
@@ -68,7 +78,11 @@ def serve_batched(requests, hbm_budget_bytes, chunk_steps=10, device=0, keep_res
                 r = waiting[0]
                 raise RuntimeError(f"request {r.rid} needs {r.footprint} B, more than the {hbm_budget_bytes} B budget")
             if admit:
-                ids = batch.update(add=[(r.params, r.state) for r in admit])
+                hosted = [r for r in admit if r.state is not None]
+                specced = [r for r in admit if r.state is None]
+                ids = batch.update(add=[(r.params, r.state) for r in hosted]) if hosted else []
+                ids += batch.update_specs(add=[(r.params, r.spec) for r in specced]) if specced else []
+                admit = hosted + specced
                 now = time.perf_counter() - t0
                 for sid, r in zip(ids, admit):
                     r.admitted_s = now
@@ -109,7 +123,10 @@ def serve_plain(requests, device=0, keep_results=True):
         g = HipBackend(device)
         try:
             g.set_params(r.params)
-            g.pull(r.state)
+            if r.state is not None:
+                g.pull(r.state)
+            else:
+                g.pull_spec(r.spec)
             g.step(r.steps)
             out = g.push()
             if keep_results:
