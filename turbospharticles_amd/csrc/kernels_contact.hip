@@ -7,8 +7,11 @@
 // diagonals from the body-level gradient sums, relaxation factors shared by each body's
 // contacts, and relaxed Jacobi sweeps on non-negative contact pressures whose forces move the
 // bodies (inverse mass, world inverse inertia) until the average density error meets the
-// scene's target. Every scene decides on its own; the per-scene FP64 sums run in one block per
-// scene in a fixed order, so a scene's result does not depend on the other scenes of the batch.
+// scene's target. The contact lists are built for all scenes by a few kernels; then one block
+// per scene runs the whole solve -- set-up, every sweep and the final loads -- with block
+// barriers between its phases: contact problems are small, and one launch with no host round
+// trip beats a launch per phase and sweep. Every FP64 sum runs in a fixed order inside its
+// scene's block, so a scene's result does not depend on the other scenes of the batch.
 //
 // Layout: the RB lists hold global boundary indices; the boundary particles in contact are
 // numbered by the exclusive scan of their flags (cbScan), in ascending boundary order, which
@@ -24,14 +27,6 @@ namespace tsp
     constexpr int kMaxBodies = 8;            // distinct bodies around one contact particle
 
     static inline uint32_t blocksN(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
-
-    __device__ __forceinline__ uint32_t sceneOfRigid(const Dev& d, uint32_t i) { return d.bodyScene[d.rbody[i]]; }
-
-    __device__ __forceinline__ bool contactRuns(const Dev& d, uint32_t sc)
-    {
-        const Solve& so = d.solve[sc];
-        return so.contactActive != 0u && so.contactDone == 0u;
-    }
 
     __device__ __forceinline__ uint32_t contactCount(const Dev& d) { return d.cbScan[d.nB]; }
 
@@ -115,49 +110,9 @@ namespace tsp
     }
 
     // ----------------------------------------------------------------------------------------
-    // RigidContactGeometry, rigid_solver.cpp:151-227, and the per-scene set-up
-
-    __global__ void __launch_bounds__(kBlock) kContactGeometry(Dev d)
-    {
-        const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-        if (t < d.nRB)
-        {
-            const uint32_t b = t;
-            float A[9], B[9], T[9];
-            for (int q = 0; q < 9; q++) { A[q] = rbv(d, RB_A00 + q, b); B[q] = rbv(d, RB_I00 + q, b); }
-            for (int r = 0; r < 3; r++)
-            {
-                for (int c = 0; c < 3; c++) { T[3 * r + c] = A[3 * r] * B[c] + A[3 * r + 1] * B[3 + c] + A[3 * r + 2] * B[6 + c]; }
-            }
-            for (int r = 0; r < 3; r++)
-            {
-                for (int c = 0; c < 3; c++)
-                {
-                    d.cIw[(3 * r + c) * d.nRB + b] = T[3 * r] * A[3 * c] + T[3 * r + 1] * A[3 * c + 1] + T[3 * r + 2] * A[3 * c + 2];
-                }
-            }
-            d.cBodyB[b] = 0u;
-        }
-        if (t < d.nR)
-        {
-            const uint32_t i = t;
-            const uint32_t b = d.rbody[i];
-            const float lx = d.rxrel[i], ly = d.ryrel[i], lz = d.rzrel[i];
-            d.cRw[i] = __builtin_fmaf(rbv(d, RB_A00 + 0, b), lx, __builtin_fmaf(rbv(d, RB_A00 + 1, b), ly, rbv(d, RB_A00 + 2, b) * lz));
-            d.cRw[d.nR + i] = __builtin_fmaf(rbv(d, RB_A00 + 3, b), lx, __builtin_fmaf(rbv(d, RB_A00 + 4, b), ly, rbv(d, RB_A00 + 5, b) * lz));
-            d.cRw[2 * d.nR + i] = __builtin_fmaf(rbv(d, RB_A00 + 6, b), lx, __builtin_fmaf(rbv(d, RB_A00 + 7, b), ly, rbv(d, RB_A00 + 8, b) * lz));
-        }
-        if (t < d.nScenes)
-        {
-            const Scene& s = d.scenes[t];
-            d.solve[t].contactBoundary = d.cbScan[s.offB + s.nB] - d.cbScan[s.offB];
-        }
-    }
-
-    // ----------------------------------------------------------------------------------------
-    // the contact neighbors of one particle (ForEachContactNeighbor, rigid_solver.cpp:23-49):
-    // of a rigid particle its RR entries, then its RB entries; of a contact boundary particle its
-    // BR entries. fn(j, jBoundary, xj, yj, zj, Vj) with j a rigid index or a contact index.
+    // one contact neighbor set (ForEachContactNeighbor, rigid_solver.cpp:23-49): of a rigid
+    // particle its RR entries, then its RB entries; of a contact boundary particle its BR entries.
+    // fn(j, jBoundary, xj, yj, zj, Vj) with j a rigid index or a contact index.
 
     template <typename Fn>
     __device__ __forceinline__ void forEachContact(const Dev& d, bool boundary, uint32_t i, Fn&& fn)
@@ -186,40 +141,23 @@ namespace tsp
         }
     }
 
-    // one thread per rigid particle (t < nR) or contact boundary particle (t - nR); the scene
-    // and position of the particle; false when the thread has no particle or its scene's solve
-    // does not run
+    // the position and volume of a rigid particle (boundary = false) or contact boundary particle
     struct ContactSelf
     {
         bool boundary;
-        uint32_t i, sc;
+        uint32_t i;
         float x, y, z, vol;
     };
 
-    __device__ __forceinline__ bool contactSelf(const Dev& d, uint32_t t, ContactSelf& s)
+    __device__ __forceinline__ ContactSelf contactSelf(const Dev& d, bool boundary, uint32_t i)
     {
-        if (t < d.nR)
-        {
-            s.boundary = false;
-            s.i = t;
-            s.sc = sceneOfRigid(d, t);
-            const float4 p = d.rpos[t];
-            s.x = p.x; s.y = p.y; s.z = p.z;
-            s.vol = d.rvel[t].w;
-        }
-        else
-        {
-            const uint32_t c = t - d.nR;
-            if (c >= contactCount(d)) { return false; }
-            const uint32_t lb = d.cbLocal[c];
-            s.boundary = true;
-            s.i = c;
-            s.sc = d.sceneB[lb];
-            const float4 p = d.bpos[lb];
-            s.x = p.x; s.y = p.y; s.z = p.z;
-            s.vol = d.bvol[lb];
-        }
-        return contactRuns(d, s.sc);
+        ContactSelf s;
+        s.boundary = boundary;
+        s.i = i;
+        const float4 p = boundary ? d.bpos[d.cbLocal[i]] : d.rpos[i];
+        s.x = p.x; s.y = p.y; s.z = p.z;
+        s.vol = boundary ? d.bvol[d.cbLocal[i]] : d.rvel[i].w;
+        return s;
     }
 
     // BodyGradient / AddBodyGradient, rigid_solver.cpp:65-86
@@ -247,12 +185,39 @@ namespace tsp
         }
     };
 
-    // RigidContactDensityAndDiagonal, rigid_solver.cpp:229-344
-    __global__ void __launch_bounds__(kBlock) kContactDensity(Dev d)
+    // RigidContactGeometry, rigid_solver.cpp:151-227: the world inverse inertia of a body
+    __device__ __forceinline__ void geometryBody(const Dev& d, uint32_t b)
     {
-        ContactSelf s;
-        if (!contactSelf(d, blockIdx.x * kBlock + threadIdx.x, s)) { return; }
-        const Consts& k = d.consts[s.sc];
+        float A[9], B[9], T[9];
+        for (int q = 0; q < 9; q++) { A[q] = rbv(d, RB_A00 + q, b); B[q] = rbv(d, RB_I00 + q, b); }
+        for (int r = 0; r < 3; r++)
+        {
+            for (int c = 0; c < 3; c++) { T[3 * r + c] = A[3 * r] * B[c] + A[3 * r + 1] * B[3 + c] + A[3 * r + 2] * B[6 + c]; }
+        }
+        for (int r = 0; r < 3; r++)
+        {
+            for (int c = 0; c < 3; c++)
+            {
+                d.cIw[(3 * r + c) * d.nRB + b] = T[3 * r] * A[3 * c] + T[3 * r + 1] * A[3 * c + 1] + T[3 * r + 2] * A[3 * c + 2];
+            }
+        }
+        d.cBodyB[b] = 0u;
+    }
+
+    // ... and the world offset A r_rel of a rigid particle
+    __device__ __forceinline__ void geometryParticle(const Dev& d, uint32_t i)
+    {
+        const uint32_t b = d.rbody[i];
+        const float lx = d.rxrel[i], ly = d.ryrel[i], lz = d.rzrel[i];
+        d.cRw[i] = __builtin_fmaf(rbv(d, RB_A00 + 0, b), lx, __builtin_fmaf(rbv(d, RB_A00 + 1, b), ly, rbv(d, RB_A00 + 2, b) * lz));
+        d.cRw[d.nR + i] = __builtin_fmaf(rbv(d, RB_A00 + 3, b), lx, __builtin_fmaf(rbv(d, RB_A00 + 4, b), ly, rbv(d, RB_A00 + 5, b) * lz));
+        d.cRw[2 * d.nR + i] = __builtin_fmaf(rbv(d, RB_A00 + 6, b), lx, __builtin_fmaf(rbv(d, RB_A00 + 7, b), ly, rbv(d, RB_A00 + 8, b) * lz));
+    }
+
+    // RigidContactDensityAndDiagonal, rigid_solver.cpp:229-344, for one particle; returns the
+    // kContactBad* flags it raises
+    __device__ uint32_t densityOne(const Dev& d, const ContactSelf& s, const Consts& k)
+    {
         const float volRest = kContactVolume * s.vol;
         float rho = kContactVolume;
         bool has = false;
@@ -272,13 +237,13 @@ namespace tsp
             if (!s.boundary) { g.add(ownBody, gx, gy, gz, rxi, ryi, rzi); }
             if (!jb) { g.add(d.rbody[j], -gx, -gy, -gz, d.cRw[j], d.cRw[d.nR + j], d.cRw[2 * d.nR + j]); }
         });
-        if (g.overflow) { atomicOr(&d.solve[s.sc].contactBad, kContactManyBodies); return; }
+        if (g.overflow) { return kContactManyBodies; }
         double diag = 0.0;
+        const float* iw = d.cIw;
+        const uint32_t nb = d.nRB;
         for (int e = 0; e < g.n; e++)
         {
             const uint32_t b = g.body[e];
-            const float* iw = d.cIw;
-            const uint32_t nb = d.nRB;
             const float wx = iw[0 * nb + b] * g.tx[e] + iw[1 * nb + b] * g.ty[e] + iw[2 * nb + b] * g.tz[e];
             const float wy = iw[3 * nb + b] * g.tx[e] + iw[4 * nb + b] * g.ty[e] + iw[5 * nb + b] * g.tz[e];
             const float wz = iw[6 * nb + b] * g.tx[e] + iw[7 * nb + b] * g.ty[e] + iw[8 * nb + b] * g.tz[e];
@@ -288,7 +253,6 @@ namespace tsp
         }
         const float dg = static_cast<float>(static_cast<double>(k.dt) * static_cast<double>(k.dt) * static_cast<double>(volRest)
                                             / static_cast<double>(rho * rho) * diag);
-        if (!__builtin_isfinite(dg) || dg < 0.0f) { atomicOr(&d.solve[s.sc].contactBad, kContactBadDiag); }
         const float om = has ? 0.5f : 0.0f;
         if (s.boundary)
         {
@@ -300,39 +264,7 @@ namespace tsp
             cr(d, CR_RHO, s.i) = rho; cr(d, CR_DIAG, s.i) = dg; cr(d, CR_P, s.i) = 0.0f;
             cr(d, CR_SRC, s.i) = rho - 1.0f; cr(d, CR_OM, s.i) = om;
         }
-    }
-
-    // ----------------------------------------------------------------------------------------
-    // RigidContactRelaxation, rigid_solver.cpp:346-407
-
-    // one block per body: the body's contacting particles share the factor
-    __global__ void __launch_bounds__(kBlock) kContactRelaxRigid(Dev d)
-    {
-        __shared__ uint32_t red[kBlock];
-        const uint32_t b = blockIdx.x;
-        const uint32_t sc = d.bodyScene[b];
-        if (!contactRuns(d, sc)) { return; }
-        uint32_t n = 0;
-        for (uint32_t q = d.byBodyOff[b] + threadIdx.x; q < d.byBodyOff[b + 1]; q += kBlock)
-        {
-            n += cr(d, CR_OM, d.byBodyIdx[q]) > 0.0f ? 1u : 0u;
-        }
-        red[threadIdx.x] = n;
-        __syncthreads();
-        for (uint32_t o = kBlock / 2; o > 0; o >>= 1)
-        {
-            if (threadIdx.x < o) { red[threadIdx.x] += red[threadIdx.x + o]; }
-            __syncthreads();
-        }
-        const uint32_t count = red[0];
-        if (count == 0u) { return; }
-        const float fc = static_cast<float>(count);
-        for (uint32_t q = d.byBodyOff[b] + threadIdx.x; q < d.byBodyOff[b + 1]; q += kBlock)
-        {
-            float& om = cr(d, CR_OM, d.byBodyIdx[q]);
-            om = om / fc;
-        }
-        if (threadIdx.x == 0) { atomicAdd(reinterpret_cast<unsigned long long*>(&d.solve[sc].contacts), static_cast<unsigned long long>(count)); }
+        return (!__builtin_isfinite(dg) || dg < 0.0f) ? kContactBadDiag : 0u;
     }
 
     // the distinct movable bodies among a contact boundary particle's rigid neighbors
@@ -352,47 +284,9 @@ namespace tsp
         return n;
     }
 
-    // boundary contacts per movable body
-    __global__ void __launch_bounds__(kBlock) kContactRelaxBoundaryCount(Dev d)
+    // RigidContactPressureForces, rigid_solver.cpp:409-460, for one rigid particle
+    __device__ void forcesOne(const Dev& d, uint32_t i, const Consts& k)
     {
-        const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-        if (c >= contactCount(d)) { return; }
-        const uint32_t sc = d.sceneB[d.cbLocal[c]];
-        if (!contactRuns(d, sc) || cb(d, CB_OM, c) <= 0.0f) { return; }
-        atomicAdd(reinterpret_cast<unsigned long long*>(&d.solve[sc].contacts), 1ull);
-        if (cb(d, CB_DIAG, c) <= 0.0f) { return; }
-        uint32_t bodies[kMaxBodies];
-        bool overflow = false;
-        const int n = movableBodies(d, c, bodies, overflow);
-        if (overflow) { atomicOr(&d.solve[sc].contactBad, kContactManyBodies); return; }
-        for (int e = 0; e < n; e++) { atomicAdd(&d.cBodyB[bodies[e]], 1u); }
-    }
-
-    // a boundary particle shared by several bodies takes the largest of their contact counts
-    __global__ void __launch_bounds__(kBlock) kContactRelaxBoundary(Dev d)
-    {
-        const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-        if (c >= contactCount(d)) { return; }
-        const uint32_t sc = d.sceneB[d.cbLocal[c]];
-        if (!contactRuns(d, sc) || cb(d, CB_OM, c) <= 0.0f || cb(d, CB_DIAG, c) <= 0.0f) { return; }
-        uint32_t mx = 0;
-        for (uint32_t q = d.brOff[c]; q < d.brOff[c + 1]; q++)
-        {
-            const uint32_t n = d.cBodyB[d.rbody[d.brIdx[q]]];
-            mx = n > mx ? n : mx;
-        }
-        if (mx > 0u) { cb(d, CB_OM, c) = cb(d, CB_OM, c) / static_cast<float>(mx); }
-    }
-
-    // ----------------------------------------------------------------------------------------
-    // RigidContactPressureForces, rigid_solver.cpp:409-460
-    __global__ void __launch_bounds__(kBlock) kContactForces(Dev d)
-    {
-        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-        if (i >= d.nR) { return; }
-        const uint32_t sc = sceneOfRigid(d, i);
-        if (!contactRuns(d, sc)) { return; }
-        const Consts& k = d.consts[sc];
         const float4 pi4 = d.rpos[i];
         const float vi = d.rvel[i].w;
         const float rhoi = cr(d, CR_RHO, i);
@@ -410,52 +304,9 @@ namespace tsp
         cr(d, CR_FX, i) = fx; cr(d, CR_FY, i) = fy; cr(d, CR_FZ, i) = fz;
     }
 
-    // ReduceRigidParticleForcesToBody(Contact), cpu_backend.cpp:2089-2172 with FP64 sums, into
-    // the cleared contact loads: one block per body
-    __global__ void __launch_bounds__(kBlock) kContactLoads(Dev d)
+    // RigidContactParticleVelocities, rigid_solver.cpp:462-506, for one rigid particle
+    __device__ __forceinline__ void velocityOne(const Dev& d, uint32_t i, const Consts& k)
     {
-        __shared__ double red[6][kBlock];
-        const uint32_t b = blockIdx.x;
-        if (!contactRuns(d, d.bodyScene[b])) { return; }
-        const uint32_t nb = d.nRB;
-        const float a[9] = { rbv(d, RB_A00 + 0, b), rbv(d, RB_A00 + 1, b), rbv(d, RB_A00 + 2, b),
-                             rbv(d, RB_A00 + 3, b), rbv(d, RB_A00 + 4, b), rbv(d, RB_A00 + 5, b),
-                             rbv(d, RB_A00 + 6, b), rbv(d, RB_A00 + 7, b), rbv(d, RB_A00 + 8, b) };
-        double acc[6] = { 0, 0, 0, 0, 0, 0 };
-        for (uint32_t q = d.byBodyOff[b] + threadIdx.x; q < d.byBodyOff[b + 1]; q += kBlock)
-        {
-            const uint32_t i = d.byBodyIdx[q];
-            const float lx = d.rxrel[i], ly = d.ryrel[i], lz = d.rzrel[i];
-            const float rx = __builtin_fmaf(a[0], lx, __builtin_fmaf(a[1], ly, a[2] * lz));
-            const float ry = __builtin_fmaf(a[3], lx, __builtin_fmaf(a[4], ly, a[5] * lz));
-            const float rz = __builtin_fmaf(a[6], lx, __builtin_fmaf(a[7], ly, a[8] * lz));
-            const double fx = cr(d, CR_FX, i), fy = cr(d, CR_FY, i), fz = cr(d, CR_FZ, i);
-            acc[0] += fx; acc[1] += fy; acc[2] += fz;
-            acc[3] += static_cast<double>(ry) * fz - static_cast<double>(rz) * fy;
-            acc[4] += static_cast<double>(rz) * fx - static_cast<double>(rx) * fz;
-            acc[5] += static_cast<double>(rx) * fy - static_cast<double>(ry) * fx;
-        }
-        for (int f = 0; f < 6; f++) { red[f][threadIdx.x] = acc[f]; }
-        __syncthreads();
-        for (uint32_t o = kBlock / 2; o > 0; o >>= 1)
-        {
-            if (threadIdx.x < o) { for (int f = 0; f < 6; f++) { red[f][threadIdx.x] += red[f][threadIdx.x + o]; } }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0)
-        {
-            for (int f = 0; f < 6; f++) { d.cLoad[f * nb + b] = 0.0f + static_cast<float>(red[f][0]); }
-        }
-    }
-
-    // RigidContactParticleVelocities, rigid_solver.cpp:462-506
-    __global__ void __launch_bounds__(kBlock) kContactVelocities(Dev d)
-    {
-        const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-        if (i >= d.nR) { return; }
-        const uint32_t sc = sceneOfRigid(d, i);
-        if (!contactRuns(d, sc)) { return; }
-        const Consts& k = d.consts[sc];
         const uint32_t b = d.rbody[i], nb = d.nRB;
         const float* L = d.cLoad;
         const float* iw = d.cIw;
@@ -470,14 +321,13 @@ namespace tsp
         cr(d, CR_VZ, i) = k.dt * (mi * L[2 * nb + b] + wx * ry - wy * rx);
     }
 
-    // RigidContactDensityError, rigid_solver.cpp:508-579. initial: with the predicted rigid
-    // velocities, into the source terms (the swap of SolveRigidContacts); otherwise with the
-    // contact velocities, into the residuals and the FP64 error terms
-    __global__ void __launch_bounds__(kBlock) kContactError(Dev d, bool initial)
+    // RigidContactDensityError, rigid_solver.cpp:508-579, for one particle. initial: with the
+    // predicted rigid velocities, into the source term (the swap of SolveRigidContacts);
+    // otherwise with the contact velocities, into the residual. Returns the FP64 error term
+    // (|residual| where the pressure is positive, max(residual, 0) elsewhere), or NaN when the
+    // residual is not finite.
+    __device__ double errorOne(const Dev& d, const ContactSelf& s, const Consts& k, bool initial)
     {
-        ContactSelf s;
-        if (!contactSelf(d, blockIdx.x * kBlock + threadIdx.x, s)) { return; }
-        const Consts& k = d.consts[s.sc];
         const auto velOf = [&](uint32_t j, float& vx, float& vy, float& vz) {
             if (initial) { const float4 v = d.rvp[j]; vx = v.x; vy = v.y; vz = v.z; }
             else { vx = cr(d, CR_VX, j); vy = cr(d, CR_VY, j); vz = cr(d, CR_VZ, j); }
@@ -495,78 +345,207 @@ namespace tsp
             const float div = k.dt * (dvx * gx + dvy * gy + dvz * gz);
             res -= kContactVolume * vj * div;
         });
-        if (!__builtin_isfinite(res)) { atomicOr(&d.solve[s.sc].contactBad, kContactBadResidual); res = 0.0f; }
+        if (!__builtin_isfinite(res)) { return __builtin_nan(""); }
         if (initial)
         {
             if (s.boundary) { cb(d, CB_SRC, s.i) = res; } else { cr(d, CR_SRC, s.i) = res; }
-            return;
+            return 0.0;
         }
         const float p = s.boundary ? cb(d, CB_P, s.i) : cr(d, CR_P, s.i);
-        const float term = p > 0.0f ? __builtin_fabsf(res) : (res < 0.0f ? 0.0f : res);
-        if (s.boundary) { cb(d, CB_ERR, s.i) = res; d.cErrB[s.i] = term; }
-        else { cr(d, CR_ERR, s.i) = res; d.cErrR[s.i] = term; }
+        if (s.boundary) { cb(d, CB_ERR, s.i) = res; } else { cr(d, CR_ERR, s.i) = res; }
+        return p > 0.0f ? static_cast<double>(__builtin_fabsf(res)) : static_cast<double>(res < 0.0f ? 0.0f : res);
     }
 
-    // one block per scene: the FP64 error sum (rigid particles, then contact boundary particles),
-    // the average over the contacts and the decision of SolveRigidContacts
-    __global__ void __launch_bounds__(kBlock) kContactFinalize(Dev d)
+    // RigidContactPressureRelaxedJacobi, rigid_solver.cpp:581-600, for one particle; false when
+    // the new pressure is not finite
+    __device__ __forceinline__ bool jacobiOne(const Dev& d, bool boundary, uint32_t i)
     {
-        __shared__ double red[kBlock];
-        const uint32_t sc = blockIdx.x;
-        if (!contactRuns(d, sc)) { return; }
-        const Scene& s = d.scenes[sc];
-        double part[2];
-        for (int h = 0; h < 2; h++)
-        {
-            const uint32_t b0 = h ? d.cbScan[s.offB] : s.offR;
-            const uint32_t b1 = h ? d.cbScan[s.offB + s.nB] : s.offR + s.nR;
-            const double* src = h ? d.cErrB : d.cErrR;
-            double a = 0.0;
-            for (uint32_t q = b0 + threadIdx.x; q < b1; q += kBlock) { a += src[q]; }
-            red[threadIdx.x] = a;
-            __syncthreads();
-            for (uint32_t o = kBlock / 2; o > 0; o >>= 1)
-            {
-                if (threadIdx.x < o) { red[threadIdx.x] += red[threadIdx.x + o]; }
-                __syncthreads();
-            }
-            part[h] = red[0];
-            __syncthreads();
-        }
-        if (threadIdx.x != 0) { return; }
-        Solve& so = d.solve[sc];
-        const Consts& k = d.consts[sc];
-        const double sum = part[0] + part[1];
-        so.contactAvg = so.contacts > 0ull ? static_cast<float>(sum / static_cast<double>(so.contacts)) : 0.0f;
-        so.contactConverged = so.contactAvg <= k.contactDevTarget ? 1u : 0u;
-        if (so.contactConverged || so.contactIters == k.contactMaxIters || so.contactBad) { so.contactDone = 1u; }
-        else { so.contactIters++; }
-    }
-
-    // RigidContactPressureRelaxedJacobi, rigid_solver.cpp:581-600, for the scenes not done
-    __global__ void __launch_bounds__(kBlock) kContactJacobi(Dev d)
-    {
-        ContactSelf s;
-        if (!contactSelf(d, blockIdx.x * kBlock + threadIdx.x, s)) { return; }
-        float* P = s.boundary ? &cb(d, CB_P, s.i) : &cr(d, CR_P, s.i);
-        const float dg = s.boundary ? cb(d, CB_DIAG, s.i) : cr(d, CR_DIAG, s.i);
-        if (!(dg > 0.0f)) { return; }
-        const float om = s.boundary ? cb(d, CB_OM, s.i) : cr(d, CR_OM, s.i);
-        const float err = s.boundary ? cb(d, CB_ERR, s.i) : cr(d, CR_ERR, s.i);
+        float* P = boundary ? &cb(d, CB_P, i) : &cr(d, CR_P, i);
+        const float dg = boundary ? cb(d, CB_DIAG, i) : cr(d, CR_DIAG, i);
+        if (!(dg > 0.0f)) { return true; }
+        const float om = boundary ? cb(d, CB_OM, i) : cr(d, CR_OM, i);
+        const float err = boundary ? cb(d, CB_ERR, i) : cr(d, CR_ERR, i);
         const float p = *P + om * err / dg;
-        if (!__builtin_isfinite(p)) { atomicOr(&d.solve[s.sc].contactBad, kContactBadPressure); return; }
+        if (!__builtin_isfinite(p)) { return false; }
         *P = 0.0f < p ? p : 0.0f;
+        return true;
     }
 
-    // the final contact loads added to the body loads of every body of a scene whose solve ran
-    __global__ void __launch_bounds__(kBlock) kContactApply(Dev d)
+    // ----------------------------------------------------------------------------------------
+    // the solve of one scene (SolveRigidContacts, rigid_solver.cpp:602-799), one block
+
+    constexpr uint32_t kSolveThreads = 1024;
+
+    // sum over the block in a fixed order (per-wave xor trees, then the waves in order);
+    // every thread gets the result
+    __device__ double blockSum(double v, double* red)
     {
-        const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
-        if (b >= d.nRB) { return; }
-        const Solve& so = d.solve[d.bodyScene[b]];
-        if (!so.contactActive || so.contactBad) { return; }
-        const uint32_t nb = d.nRB;
-        for (int f = 0; f < 6; f++) { d.rbForce[f * nb + b] += d.cLoad[f * nb + b]; }
+        for (int o = 32; o > 0; o >>= 1) { v += __shfl_xor(v, o, 64); }
+        const uint32_t w = threadIdx.x >> 6;
+        __syncthreads();
+        if ((threadIdx.x & 63u) == 0) { red[w] = v; }
+        __syncthreads();
+        double a = red[0];
+        for (uint32_t q = 1; q < kSolveThreads / 64; q++) { a += red[q]; }
+        return a;
+    }
+
+    __global__ void __launch_bounds__(kSolveThreads) kContactSolve(Dev d)
+    {
+        __shared__ double red[kSolveThreads / 64];
+        __shared__ uint32_t bad;
+        const uint32_t sc = blockIdx.x;
+        Solve& so = d.solve[sc];
+        if (!so.contactActive) { return; }
+        const Scene& S = d.scenes[sc];
+        const Consts& k = d.consts[sc];
+        const uint32_t t = threadIdx.x, T = kSolveThreads;
+        const uint32_t r0 = S.offR, r1 = S.offR + S.nR;
+        const uint32_t c0 = d.cbScan[S.offB], c1 = d.cbScan[S.offB + S.nB];
+        const uint32_t b0 = S.offRB, b1 = S.offRB + S.nRB;
+        if (t == 0) { bad = 0u; }
+        // geometry, densities and diagonals
+        for (uint32_t b = b0 + t; b < b1; b += T) { geometryBody(d, b); }
+        for (uint32_t i = r0 + t; i < r1; i += T) { geometryParticle(d, i); }
+        __syncthreads();
+        uint32_t myBad = 0;
+        for (uint32_t i = r0 + t; i < r1; i += T) { myBad |= densityOne(d, contactSelf(d, false, i), k); }
+        for (uint32_t c = c0 + t; c < c1; c += T) { myBad |= densityOne(d, contactSelf(d, true, c), k); }
+        __syncthreads();
+        // relaxation (RigidContactRelaxation, rigid_solver.cpp:346-407): a body's contacting
+        // particles share the factor; boundary particles take the largest count of their bodies
+        uint64_t contacts = 0;
+        for (uint32_t b = b0; b < b1; b++)
+        {
+            double n = 0.0;
+            for (uint32_t q = d.byBodyOff[b] + t; q < d.byBodyOff[b + 1]; q += T) { n += cr(d, CR_OM, d.byBodyIdx[q]) > 0.0f ? 1.0 : 0.0; }
+            const uint32_t count = static_cast<uint32_t>(blockSum(n, red));
+            if (count > 0u)
+            {
+                const float fc = static_cast<float>(count);
+                for (uint32_t q = d.byBodyOff[b] + t; q < d.byBodyOff[b + 1]; q += T)
+                {
+                    float& om = cr(d, CR_OM, d.byBodyIdx[q]);
+                    om = om / fc;
+                }
+                contacts += count;
+            }
+        }
+        double nb = 0.0;
+        for (uint32_t c = c0 + t; c < c1; c += T)
+        {
+            if (cb(d, CB_OM, c) <= 0.0f) { continue; }
+            nb += 1.0;
+            if (cb(d, CB_DIAG, c) <= 0.0f) { continue; }
+            uint32_t bodies[kMaxBodies];
+            bool overflow = false;
+            const int n = movableBodies(d, c, bodies, overflow);
+            if (overflow) { myBad |= kContactManyBodies; continue; }
+            for (int e = 0; e < n; e++) { atomicAdd(&d.cBodyB[bodies[e]], 1u); }
+        }
+        contacts += static_cast<uint64_t>(blockSum(nb, red));
+        __syncthreads();
+        for (uint32_t c = c0 + t; c < c1; c += T)
+        {
+            if (cb(d, CB_OM, c) <= 0.0f || cb(d, CB_DIAG, c) <= 0.0f) { continue; }
+            uint32_t mx = 0;
+            for (uint32_t q = d.brOff[c]; q < d.brOff[c + 1]; q++)
+            {
+                const uint32_t n = d.cBodyB[d.rbody[d.brIdx[q]]];
+                mx = n > mx ? n : mx;
+            }
+            if (mx > 0u) { cb(d, CB_OM, c) = cb(d, CB_OM, c) / static_cast<float>(mx); }
+        }
+        // the source motion: the body loads so far (external and fluid), no contact loads
+        for (uint32_t b = b0 + t; b < b1; b += T) { rigidPredictOne(d, b); }
+        __syncthreads();
+        for (uint32_t i = r0 + t; i < r1; i += T) { rigidUpdateOne(d, i, true); }
+        __syncthreads();
+        for (uint32_t i = r0 + t; i < r1; i += T) { if (errorOne(d, contactSelf(d, false, i), k, true) != 0.0) { myBad |= kContactBadResidual; } }
+        for (uint32_t c = c0 + t; c < c1; c += T) { if (errorOne(d, contactSelf(d, true, c), k, true) != 0.0) { myBad |= kContactBadResidual; } }
+        if (myBad) { atomicOr(&bad, myBad); }
+        __syncthreads();
+        uint32_t iters = 0;
+        float avg = 0.0f;
+        bool converged = false;
+        while (!bad)
+        {
+            for (uint32_t i = r0 + t; i < r1; i += T) { forcesOne(d, i, k); }
+            __syncthreads();
+            // ReduceRigidParticleForcesToBody(Contact) into the cleared contact loads, FP64 sums
+            for (uint32_t b = b0; b < b1; b++)
+            {
+                const float a[9] = { rbv(d, RB_A00 + 0, b), rbv(d, RB_A00 + 1, b), rbv(d, RB_A00 + 2, b),
+                                     rbv(d, RB_A00 + 3, b), rbv(d, RB_A00 + 4, b), rbv(d, RB_A00 + 5, b),
+                                     rbv(d, RB_A00 + 6, b), rbv(d, RB_A00 + 7, b), rbv(d, RB_A00 + 8, b) };
+                double acc[6] = { 0, 0, 0, 0, 0, 0 };
+                for (uint32_t q = d.byBodyOff[b] + t; q < d.byBodyOff[b + 1]; q += T)
+                {
+                    const uint32_t i = d.byBodyIdx[q];
+                    const float lx = d.rxrel[i], ly = d.ryrel[i], lz = d.rzrel[i];
+                    const float rx = __builtin_fmaf(a[0], lx, __builtin_fmaf(a[1], ly, a[2] * lz));
+                    const float ry = __builtin_fmaf(a[3], lx, __builtin_fmaf(a[4], ly, a[5] * lz));
+                    const float rz = __builtin_fmaf(a[6], lx, __builtin_fmaf(a[7], ly, a[8] * lz));
+                    const double fx = cr(d, CR_FX, i), fy = cr(d, CR_FY, i), fz = cr(d, CR_FZ, i);
+                    acc[0] += fx; acc[1] += fy; acc[2] += fz;
+                    acc[3] += static_cast<double>(ry) * fz - static_cast<double>(rz) * fy;
+                    acc[4] += static_cast<double>(rz) * fx - static_cast<double>(rx) * fz;
+                    acc[5] += static_cast<double>(rx) * fy - static_cast<double>(ry) * fx;
+                }
+                for (int f = 0; f < 6; f++)
+                {
+                    const double v = blockSum(acc[f], red);
+                    if (t == 0) { d.cLoad[f * d.nRB + b] = 0.0f + static_cast<float>(v); }
+                }
+            }
+            __syncthreads();
+            for (uint32_t i = r0 + t; i < r1; i += T) { velocityOne(d, i, k); }
+            __syncthreads();
+            double eR = 0.0, eB = 0.0;
+            uint32_t flags = 0;
+            for (uint32_t i = r0 + t; i < r1; i += T)
+            {
+                const double e = errorOne(d, contactSelf(d, false, i), k, false);
+                if (e != e) { flags |= kContactBadResidual; } else { eR += e; }
+            }
+            for (uint32_t c = c0 + t; c < c1; c += T)
+            {
+                const double e = errorOne(d, contactSelf(d, true, c), k, false);
+                if (e != e) { flags |= kContactBadResidual; } else { eB += e; }
+            }
+            if (flags) { atomicOr(&bad, flags); }
+            const double sum = blockSum(eR, red) + blockSum(eB, red);
+            avg = contacts > 0ull ? static_cast<float>(sum / static_cast<double>(contacts)) : 0.0f;
+            converged = avg <= k.contactDevTarget;
+            __syncthreads();
+            if (converged || iters == k.contactMaxIters || bad) { break; }
+            flags = 0;
+            for (uint32_t i = r0 + t; i < r1; i += T) { if (!jacobiOne(d, false, i)) { flags |= kContactBadPressure; } }
+            for (uint32_t c = c0 + t; c < c1; c += T) { if (!jacobiOne(d, true, c)) { flags |= kContactBadPressure; } }
+            if (flags) { atomicOr(&bad, flags); }
+            __syncthreads();
+            iters++;
+        }
+        __syncthreads();
+        const uint32_t fl = bad;
+        // the final contact loads added to every body of the scene
+        if (!fl)
+        {
+            for (uint32_t b = b0 + t; b < b1; b += T)
+            {
+                for (int f = 0; f < 6; f++) { d.rbForce[f * d.nRB + b] += d.cLoad[f * d.nRB + b]; }
+            }
+        }
+        if (t == 0)
+        {
+            so.contactBoundary = c1 - c0;
+            so.contactIters = iters;
+            so.contactAvg = avg;
+            so.contactConverged = converged ? 1u : 0u;
+            so.contacts = contacts;
+            so.contactBad = fl;
+            so.contactDone = 1u;
+        }
     }
 
     // ----------------------------------------------------------------------------------------
@@ -582,9 +561,9 @@ namespace tsp
         launchScan(s, d.rrOff, d.nR + 1, tmp);
     }
 
-    // lists, geometry, densities, diagonals, relaxation factors (the source terms follow after
-    // the rigid prediction); nCBMax bounds the contact boundary particles (the RB entries)
-    void launchContactSetup(hipStream_t s, const Dev& d, uint32_t nCBMax, uint32_t* tmp)
+    // the contact boundary particles and their BR lists for every scene, then one block per scene
+    // for the solve; nCBMax bounds the contact boundary particles (the RB entries)
+    void launchContactSolve(hipStream_t s, const Dev& d, uint32_t nCBMax, uint32_t* tmp)
     {
         (void)hipMemsetAsync(d.cbScan, 0, sizeof(uint32_t) * (d.nB + 1ull), s);
         (void)hipMemsetAsync(d.brOff, 0, sizeof(uint32_t) * (nCBMax + 1ull), s);
@@ -596,37 +575,6 @@ namespace tsp
         (void)hipMemcpyAsync(d.brPos, d.brOff, sizeof(uint32_t) * (nCBMax + 1ull), hipMemcpyDeviceToDevice, s);
         kContactBrFill<<<blocksN(d.nR), kBlock, 0, s>>>(d);
         if (nCBMax) { kContactBrSort<<<blocksN(nCBMax), kBlock, 0, s>>>(d); }
-        const uint64_t nGeo = std::max<uint64_t>(std::max<uint64_t>(d.nR, d.nRB), d.nScenes);
-        kContactGeometry<<<blocksN(nGeo), kBlock, 0, s>>>(d);
-        kContactDensity<<<blocksN(d.nR + static_cast<uint64_t>(nCBMax)), kBlock, 0, s>>>(d);
-        if (d.nRB) { kContactRelaxRigid<<<d.nRB, kBlock, 0, s>>>(d); }
-        if (nCBMax)
-        {
-            kContactRelaxBoundaryCount<<<blocksN(nCBMax), kBlock, 0, s>>>(d);
-            kContactRelaxBoundary<<<blocksN(nCBMax), kBlock, 0, s>>>(d);
-        }
-    }
-
-    void launchContactSource(hipStream_t s, const Dev& d, uint32_t nCBMax)
-    {
-        kContactError<<<blocksN(d.nR + static_cast<uint64_t>(nCBMax)), kBlock, 0, s>>>(d, true);
-    }
-
-    // one evaluation of the contact loads and residuals, the decision, and the Jacobi update of
-    // the scenes that continue
-    void launchContactIteration(hipStream_t s, const Dev& d, uint32_t nCBMax)
-    {
-        const uint32_t nAll = blocksN(d.nR + static_cast<uint64_t>(nCBMax));
-        kContactForces<<<blocksN(d.nR), kBlock, 0, s>>>(d);
-        if (d.nRB) { kContactLoads<<<d.nRB, kBlock, 0, s>>>(d); }
-        kContactVelocities<<<blocksN(d.nR), kBlock, 0, s>>>(d);
-        kContactError<<<nAll, kBlock, 0, s>>>(d, false);
-        kContactFinalize<<<d.nScenes, kBlock, 0, s>>>(d);
-        kContactJacobi<<<nAll, kBlock, 0, s>>>(d);
-    }
-
-    void launchContactApply(hipStream_t s, const Dev& d)
-    {
-        if (d.nRB) { kContactApply<<<blocksN(d.nRB), kBlock, 0, s>>>(d); }
+        kContactSolve<<<d.nScenes, kSolveThreads, 0, s>>>(d);
     }
 }

@@ -40,10 +40,7 @@ namespace tsp
     void launchQueryFill(hipStream_t, const Dev&);
     void launchQueryRigid(hipStream_t, const Dev&, int, uint32_t*);
     void launchContactOffsets(hipStream_t, const Dev&, uint32_t*);
-    void launchContactSetup(hipStream_t, const Dev&, uint32_t, uint32_t*);
-    void launchContactSource(hipStream_t, const Dev&, uint32_t);
-    void launchContactIteration(hipStream_t, const Dev&, uint32_t);
-    void launchContactApply(hipStream_t, const Dev&);
+    void launchContactSolve(hipStream_t, const Dev&, uint32_t, uint32_t*);
     void launchExportSurface(hipStream_t, const Dev&, uint32_t, uint32_t, uint32_t, uint32_t*, uint32_t*, float*, float*);
     void launchBuildFluid(hipStream_t, const Dev&, uint32_t, uint32_t, const BuildFluid&);
     void launchBuildBoundary(hipStream_t, const BuildBoundary&, uint32_t, float4*, float*, uint64_t*, float*, uint32_t*,
@@ -306,7 +303,6 @@ struct tsp_ctx
     bool contactLists = false;      // RB or RR entries exist: the lists are stored
     uint64_t contactRB = 0;         // RB entries (bounds the contact boundary particles)
     uint32_t* contactScanTmp = nullptr;
-    uint32_t lastContactEvals = 1;  // evaluations of the last contact solve (the first group)
 
     bool layoutDirty = true;        // key bases and cell offsets must be assigned again
     bool listsValid = false;        // a neighbor search ran since the last layout
@@ -546,48 +542,13 @@ struct tsp_ctx
     }
 
     // SolveRigidContacts (CPU/backend/rigid_solver.cpp:602-799) for every scene whose rigid
-    // particles have RB or RR entries: set-up, source terms from the predicted body motion, then
-    // groups of evaluations (each scene stops on its own), then the loads added to the bodies
+    // particles have RB or RR entries: the contact lists of all scenes, then one block per scene
+    // runs its solve to the end and adds the loads to its bodies. Nothing is read back here; the
+    // reports and any failure come with the status read at the end of the step call.
     void solveContacts()
     {
-        uint32_t maxEvals = 0;
-        for (uint32_t s = 0; s < nScenes(); s++)
-        {
-            if (!hostSolve[s].contactActive) { continue; }
-            const tsp_params& p = scenes[s].params;
-            validateContactParams(p);
-            maxEvals = std::max(maxEvals, p.rigid_contact_max_solver_iters + 1u);
-        }
-        if (maxEvals == 0) { return; }
         Scope sc(prof, stream, "contact");
-        const uint32_t nCBMax = static_cast<uint32_t>(contactRB);
-        launchContactSetup(stream, d, nCBMax, contactScanTmp);
-        launchRigidPredict(stream, d);
-        launchRigidUpdate(stream, d, true);
-        launchContactSource(stream, d, nCBMax);
-        uint32_t group = std::max(1u, std::min(lastContactEvals, maxEvals));
-        uint32_t queued = 0, evals = 0;
-        for (;;)
-        {
-            const uint32_t n = std::min(group, maxEvals - queued);
-            for (uint32_t q = 0; q < n; q++) { launchContactIteration(stream, d, nCBMax); }
-            queued += n;
-            readStatus();
-            bool all = true;
-            evals = 0;
-            for (uint32_t s = 0; s < nScenes(); s++)
-            {
-                const Solve& so = hostSolve[s];
-                if (!so.contactActive) { continue; }
-                all = all && so.contactDone;
-                evals = std::max(evals, so.contactIters + 1u);
-            }
-            if (all) { break; }
-            if (queued >= maxEvals) { throw Error{ TSP_ERR_STATE, "contact solve did not stop at its iteration limit" }; }
-            group = std::min(2u * group, 16u);
-        }
-        lastContactEvals = evals;
-        launchContactApply(stream, d);
+        launchContactSolve(stream, d, static_cast<uint32_t>(contactRB), contactScanTmp);
     }
 
     // --- the arena: the particles and bodies of every scene, laid out anew. Resident scenes
@@ -1415,6 +1376,12 @@ struct tsp_ctx
         readStatus();
         for (uint32_t s = 0; s < nScenes(); s++)
         {
+            // the last step's contact report (the contact solve runs after the step's other reads)
+            Solve& l = scenes[s].last;
+            const Solve& so = hostSolve[s];
+            l.contactActive = so.contactActive; l.contactDone = so.contactDone; l.contactIters = so.contactIters;
+            l.contactConverged = so.contactConverged; l.contactAvg = so.contactAvg; l.contactBoundary = so.contactBoundary;
+            l.contactBad = so.contactBad; l.contacts = so.contacts;
             if (hostSolve[s].rigidDegenerate) { throw Error{ TSP_ERR_NUMERIC, "Degenerate rigid body orientation." }; }
             const uint32_t bad = hostSolve[s].contactBad;
             if (bad & kContactManyBodies)

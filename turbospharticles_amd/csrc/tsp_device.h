@@ -581,4 +581,59 @@ namespace tsp
         else { visitEntries<false>(l, fg); }
     }
 
+    // --- rigid bodies, one body or particle (shared by the rigid kernels and the contact solve) ---
+
+    // UpdateRigidParticles(predicted), cpu_backend.cpp:2009-2063
+    __device__ __forceinline__ void rigidUpdateOne(const Dev& d, uint32_t i, bool predicted)
+    {
+        const uint32_t b = d.rbody[i];
+        const float lx = d.rxrel[i], ly = d.ryrel[i], lz = d.rzrel[i];
+        const float rx = __builtin_fmaf(rbv(d, RB_A00 + 0, b), lx, __builtin_fmaf(rbv(d, RB_A00 + 1, b), ly, rbv(d, RB_A00 + 2, b) * lz));
+        const float ry = __builtin_fmaf(rbv(d, RB_A00 + 3, b), lx, __builtin_fmaf(rbv(d, RB_A00 + 4, b), ly, rbv(d, RB_A00 + 5, b) * lz));
+        const float rz = __builtin_fmaf(rbv(d, RB_A00 + 6, b), lx, __builtin_fmaf(rbv(d, RB_A00 + 7, b), ly, rbv(d, RB_A00 + 8, b) * lz));
+        if (!predicted)
+        {
+            const float psi = d.rpos[i].w;
+            d.rpos[i] = make_float4(rbv(d, RB_XCM + 0, b) + rx, rbv(d, RB_XCM + 1, b) + ry, rbv(d, RB_XCM + 2, b) + rz, psi);
+        }
+        const uint32_t nb = d.nRB;
+        const float wx = predicted ? d.rbPred[6 * nb + b] : rbv(d, RB_WX + 0, b);
+        const float wy = predicted ? d.rbPred[7 * nb + b] : rbv(d, RB_WX + 1, b);
+        const float wz = predicted ? d.rbPred[8 * nb + b] : rbv(d, RB_WX + 2, b);
+        const float vbx = predicted ? d.rbPred[0 * nb + b] : rbv(d, RB_VX + 0, b);
+        const float vby = predicted ? d.rbPred[1 * nb + b] : rbv(d, RB_VX + 1, b);
+        const float vbz = predicted ? d.rbPred[2 * nb + b] : rbv(d, RB_VX + 2, b);
+        float4* out = predicted ? d.rvp : d.rvel;
+        out[i] = make_float4(vbx + wy * rz - wz * ry, vby + wz * rx - wx * rz, vbz + wx * ry - wy * rx, out[i].w);
+    }
+
+    // PredictRigidBodies, cpu_backend.cpp:2174-2235
+    __device__ __forceinline__ void rigidPredictOne(const Dev& d, uint32_t b)
+    {
+        const Consts& k = d.consts[d.bodyScene[b]];
+        const uint32_t nb = d.nRB;
+        const float mi = rbv(d, RB_MASS_INV, b);
+        float lp[3];
+        for (int a = 0; a < 3; a++)
+        {
+            d.rbPred[a * nb + b] = rbv(d, RB_VX + a, b) + k.dt * d.rbForce[a * nb + b] * mi;
+            lp[a] = rbv(d, RB_LX + a, b) + k.dt * d.rbForce[(3 + a) * nb + b];
+            d.rbPred[(3 + a) * nb + b] = lp[a];
+        }
+        float A[9], B[9], T[9], I[9];
+        for (int q = 0; q < 9; q++) { A[q] = rbv(d, RB_A00 + q, b); B[q] = rbv(d, RB_I00 + q, b); }
+        for (int r = 0; r < 3; r++)
+        {
+            for (int c = 0; c < 3; c++) { T[3 * r + c] = A[3 * r] * B[c] + A[3 * r + 1] * B[3 + c] + A[3 * r + 2] * B[6 + c]; }
+        }
+        for (int r = 0; r < 3; r++)
+        {
+            for (int c = 0; c < 3; c++) { I[3 * r + c] = T[3 * r] * A[3 * c] + T[3 * r + 1] * A[3 * c + 1] + T[3 * r + 2] * A[3 * c + 2]; }
+        }
+        for (int r = 0; r < 3; r++)
+        {
+            d.rbPred[(6 + r) * nb + b] = __builtin_fmaf(I[3 * r], lp[0], __builtin_fmaf(I[3 * r + 1], lp[1], I[3 * r + 2] * lp[2]));
+        }
+    }
+
 }
