@@ -303,6 +303,9 @@ class HipBatch:
     def profile_enable(self, on=True):
         self._check(self.lib.tsp_profile_enable(self.h, 1 if on else 0), "tsp_profile_enable")
 
+    # the per-launch-group times (HipBackend.profile_read)
+    profile_read = HipBackend.profile_read
+
     @staticmethod
     def footprint(n):
         """device bytes a scene with counts n = (fluid, boundary, rigid, bodies) holds while resident"""
