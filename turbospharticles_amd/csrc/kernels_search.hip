@@ -474,6 +474,103 @@ namespace tsp
         return v;
     }
 
+    // The candidates of the stencil, per relation in list order: cells in z-y-x raster order,
+    // ascending index inside a cell. onF(j, pos), onB(j, bpos), onR(j, rpos) see each candidate of
+    // their relation once; the relations are separate lists, so only the order inside each one
+    // matters. withF = false skips the fluid candidates.
+#if TSP_TUNE_SEARCH_ROW
+    // Row form: the ranges of up to three x-adjacent cells load together, and each relation's
+    // candidates along the row form one stream whose loads issue four at a time before the four
+    // tests.
+    template <typename Index, typename Load, typename Fn>
+    __device__ __forceinline__ void rowStream(const uint2 (&r)[3], Index index, Load load, Fn fn)
+    {
+        const uint32_t n0 = r[0].y - r[0].x, n1 = r[1].y - r[1].x, n2 = r[2].y - r[2].x;
+        const uint32_t n01 = n0 + n1, total = n01 + n2;
+        for (uint32_t t = 0; t < total; t += 4)
+        {
+            uint32_t j[4] = { 0u, 0u, 0u, 0u };
+            float4 p[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++)
+            {
+                const uint32_t q = t + u;
+                const uint32_t e = q < n0 ? r[0].x + q : (q < n01 ? r[1].x + (q - n0) : r[2].x + (q - n01));
+                p[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (q < total) { j[u] = index(e); p[u] = load(j[u]); }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++)
+            {
+                if (t + u < total) { fn(j[u], p[u]); }
+            }
+        }
+    }
+
+    template <typename OnF, typename OnB, typename OnR>
+    __device__ __forceinline__ void forStencil(const Dev& d, const Frame& f, const Stencil& s, bool withF, OnF onF,
+                                               OnB onB, OnR onR)
+    {
+        const auto same = [](uint32_t e) { return e; };
+        for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
+        {
+            for (int cy = s.lo[1]; cy <= s.hi[1]; cy++)
+            {
+                for (int cx = s.lo[0]; cx <= s.hi[0]; cx += 3)
+                {
+                    const int nx = min(3, s.hi[0] - cx + 1);
+                    const uint32_t lin = linearCell(f, cx, cy, cz);
+                    uint2 rf[3], rb[3], rr[3];
+#pragma unroll
+                    for (int c = 0; c < 3; c++)
+                    {
+                        const bool in = c < nx;
+                        rf[c] = (in && withF) ? d.cellF[lin + c] : make_uint2(0u, 0u);
+                        rb[c] = in ? d.cellB[lin + c] : make_uint2(0u, 0u);
+                        rr[c] = (in && d.nR) ? d.cellR[lin + c] : make_uint2(0u, 0u);
+                    }
+                    rowStream(rf, same, [&](uint32_t j) { return d.pos[j]; }, onF);
+                    rowStream(rb, same, [&](uint32_t j) { return d.bpos[j]; }, onB);
+                    rowStream(rr, [&](uint32_t e) { return d.idxR[e]; }, [&](uint32_t j) { return d.rpos[j]; }, onR);
+                }
+            }
+        }
+    }
+#else
+    // Cell form: one cell at a time, its candidates one at a time.
+    template <typename OnF, typename OnB, typename OnR>
+    __device__ __forceinline__ void forStencil(const Dev& d, const Frame& f, const Stencil& s, bool withF, OnF onF,
+                                               OnB onB, OnR onR)
+    {
+        for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
+        {
+            for (int cy = s.lo[1]; cy <= s.hi[1]; cy++)
+            {
+                for (int cx = s.lo[0]; cx <= s.hi[0]; cx++)
+                {
+                    const uint32_t lin = linearCell(f, cx, cy, cz);
+                    if (withF)
+                    {
+                        const uint2 rf = d.cellF[lin];
+                        for (uint32_t j = rf.x; j < rf.y; j++) { onF(j, d.pos[j]); }
+                    }
+                    const uint2 rb = d.cellB[lin];
+                    for (uint32_t j = rb.x; j < rb.y; j++) { onB(j, d.bpos[j]); }
+                    if (d.nR)
+                    {
+                        const uint2 rr = d.cellR[lin];
+                        for (uint32_t q = rr.x; q < rr.y; q++)
+                        {
+                            const uint32_t j = d.idxR[q];
+                            onR(j, d.rpos[j]);
+                        }
+                    }
+                }
+            }
+        }
+    }
+#endif
+
     __global__ void __launch_bounds__(kBlock) kQueryCount(Dev d, uint32_t nSlices)
     {
         const uint32_t blk = xcdBlock();
@@ -489,46 +586,23 @@ namespace tsp
         if (i < d.nF && stencilOf(f, d.keyF[i], s))
         {
             const float4 pi = d.pos[i];
-            for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
-            {
-                for (int cy = s.lo[1]; cy <= s.hi[1]; cy++)
+            forStencil(d, f, s, true,
+                [&](uint32_t j, const float4& pj)
                 {
-                    for (int cx = s.lo[0]; cx <= s.hi[0]; cx++)
+                    if (within(pi.x, pi.y, pi.z, pj, r2))
                     {
-                        const uint32_t lin = linearCell(f, cx, cy, cz);
-                        const uint2 rf = d.cellF[lin];
-                        for (uint32_t j = rf.x; j < rf.y; j++)
-                        {
-                            const float4 pj = d.pos[j];
-                            if (within(pi.x, pi.y, pi.z, pj, r2))
-                            {
-                                cnt[0]++; mn[0] = min(mn[0], j); mx[0] = max(mx[0], j);
-                                tiny = tiny || tinyDist2(dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z));
-                            }
-                        }
-                        const uint2 rb = d.cellB[lin];
-                        for (uint32_t j = rb.x; j < rb.y; j++)
-                        {
-                            if (within(pi.x, pi.y, pi.z, d.bpos[j], r2))
-                            {
-                                cnt[1]++; mn[1] = min(mn[1], j); mx[1] = max(mx[1], j);
-                            }
-                        }
-                        if (d.nR)
-                        {
-                            const uint2 rr = d.cellR[lin];
-                            for (uint32_t t = rr.x; t < rr.y; t++)
-                            {
-                                const uint32_t j = d.idxR[t];
-                                if (within(pi.x, pi.y, pi.z, d.rpos[j], r2))
-                                {
-                                    cnt[2]++; mn[2] = min(mn[2], j); mx[2] = max(mx[2], j);
-                                }
-                            }
-                        }
+                        cnt[0]++; mn[0] = min(mn[0], j); mx[0] = max(mx[0], j);
+                        tiny = tiny || tinyDist2(dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z));
                     }
-                }
-            }
+                },
+                [&](uint32_t j, const float4& pj)
+                {
+                    if (within(pi.x, pi.y, pi.z, pj, r2)) { cnt[1]++; mn[1] = min(mn[1], j); mx[1] = max(mx[1], j); }
+                },
+                [&](uint32_t j, const float4& pj)
+                {
+                    if (within(pi.x, pi.y, pi.z, pj, r2)) { cnt[2]++; mn[2] = min(mn[2], j); mx[2] = max(mx[2], j); }
+                });
         }
         if (i < d.nF)
         {
@@ -626,43 +700,24 @@ namespace tsp
         {
             const float4 pi = d.pos[i];
             uint32_t kF = 0, kB = 0, kR = 0;
-            for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
-            {
-                for (int cy = s.lo[1]; cy <= s.hi[1]; cy++)
+            forStencil(d, f, s, !wide,
+                [&](uint32_t j, const float4& pj)
                 {
-                    for (int cx = s.lo[0]; cx <= s.hi[0]; cx++)
+                    if (within(pi.x, pi.y, pi.z, pj, r2))
                     {
-                        const uint32_t lin = linearCell(f, cx, cy, cz);
-                        if (!wide)
-                        {
-                            const uint2 rf = d.cellF[lin];
-                            for (uint32_t j = rf.x; j < rf.y; j++)
-                            {
-                                if (within(pi.x, pi.y, pi.z, d.pos[j], r2))
-                                {
-                                    if (staged && kF < kStage) { stage[kF][t] = static_cast<uint16_t>(j - mF.base); }
-                                    else { sellWrite(d.pool[REL_FF], mF, lane, kF, j); }
-                                    kF++;
-                                }
-                            }
-                        }
-                        const uint2 rb = d.cellB[lin];
-                        for (uint32_t j = rb.x; j < rb.y; j++)
-                        {
-                            if (within(pi.x, pi.y, pi.z, d.bpos[j], r2)) { sellWrite(d.pool[REL_FB], mB, lane, kB++, j); }
-                        }
-                        if (d.nR)
-                        {
-                            const uint2 rr = d.cellR[lin];
-                            for (uint32_t q = rr.x; q < rr.y; q++)
-                            {
-                                const uint32_t j = d.idxR[q];
-                                if (within(pi.x, pi.y, pi.z, d.rpos[j], r2)) { sellWrite(d.pool[REL_FR], mR, lane, kR++, j); }
-                            }
-                        }
+                        if (staged && kF < kStage) { stage[kF][t] = static_cast<uint16_t>(j - mF.base); }
+                        else { sellWrite(d.pool[REL_FF], mF, lane, kF, j); }
+                        kF++;
                     }
-                }
-            }
+                },
+                [&](uint32_t j, const float4& pj)
+                {
+                    if (within(pi.x, pi.y, pi.z, pj, r2)) { sellWrite(d.pool[REL_FB], mB, lane, kB++, j); }
+                },
+                [&](uint32_t j, const float4& pj)
+                {
+                    if (within(pi.x, pi.y, pi.z, pj, r2)) { sellWrite(d.pool[REL_FR], mR, lane, kR++, j); }
+                });
             if (wide)
             {
                 // DictDelta16x2 split of a per-particle dict (neighbor_lists.cpp:838-858): the neighbors

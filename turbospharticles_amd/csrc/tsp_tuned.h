@@ -1,5 +1,7 @@
 // tsp_tuned.h -- build-time variant chosen by turbospharticles_amd/tune.py on MI355X
-// (1M rigid cube drop: 4.242 ms per step; every candidate checked bit-identical)
+// (1M rigid cube drop: 4.242 ms per step; every candidate checked bit-identical; the row walk of the
+// neighbor query, measured after that run against the cell walk: 4.095 against 4.148 ms per step, and
+// 10.31 against 10.94 ms per step of the 256 batched dam breaks, results identical)
 #ifndef TSP_TUNE_GRAPHS
 #define TSP_TUNE_GRAPHS 1
 #endif
@@ -20,4 +22,7 @@
 #endif
 #ifndef TSP_TUNE_PREFETCH
 #define TSP_TUNE_PREFETCH 1
+#endif
+#ifndef TSP_TUNE_SEARCH_ROW
+#define TSP_TUNE_SEARCH_ROW 1
 #endif
