@@ -16,6 +16,7 @@ on a bounded sample. The batched and served sections compare mixed-size scenes i
 against one context per scene (see batched_bench, served_bench).
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -80,6 +81,14 @@ def cpu_baseline(state, params, warmup, steps, threads):
     return state.n[0] * steps / dt, dt, sweeps / steps
 
 
+def state_digest(st):
+    """SHA-256 over a state's fluid ids and fields (both forms compare their final states)."""
+    h = hashlib.sha256(st.fluid_id.tobytes())
+    for f in sorted(st.fluid):
+        h.update(st.fluid[f].tobytes())
+    return h.hexdigest()
+
+
 def batched_bench(args):
     """Mixed-size scenes in one context against the same scenes one context each: 256 dam breaks,
     fluid counts log-uniform in [2,000, 200,000] (seeded), each tank scaled to its block, h = 0.01.
@@ -102,9 +111,11 @@ def batched_bench(args):
     b.synchronize()
     t_batch = time.perf_counter() - t0
     sweeps = sum(b.info(i).sweeps_total for i in ids)
+    digests = [state_digest(b.push(i)) for i in ids]
     b.close()
     progress(f"batched: {t_batch:.2f} s for {args.batch_steps} steps; plain form next")
     t_plain = 0.0
+    same = 0
     for q, st in enumerate(states):
         if q % 64 == 0:
             progress(f"batched: plain form, scene {q}/{len(states)}")
@@ -117,6 +128,7 @@ def batched_bench(args):
         g.step(args.batch_steps)
         g.synchronize()
         t_plain += time.perf_counter() - t0
+        same += int(state_digest(g.push()) == digests[q])
         g.close()
     v_b = n_total * args.batch_steps / t_batch
     v_p = n_total * args.batch_steps / t_plain
@@ -126,7 +138,8 @@ def batched_bench(args):
             "value": round(v_b, 1), "unit": "particle-steps/s", "plain_value": round(v_p, 1),
             "speedup_vs_plain": round(v_b / v_p, 2), "ms_per_step": round(t_batch * 1e3 / args.batch_steps, 3),
             "plain_ms_per_step_all_scenes": round(t_plain * 1e3 / args.batch_steps, 3),
-            "mean_jacobi_sweeps_per_scene_step": round(sweeps / len(ids) / (args.batch_warmup + args.batch_steps), 2)}
+            "mean_jacobi_sweeps_per_scene_step": round(sweeps / len(ids) / (args.batch_warmup + args.batch_steps), 2),
+            "results_identical_to_plain": f"{same}/{len(states)}"}
 
 
 def served_bench(args):
