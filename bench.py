@@ -215,7 +215,8 @@ def main():
     ap.add_argument("--n-side", type=int, default=N_SIDE)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain mixed scenes")
     ap.add_argument("--batch-scenes", type=int, default=256)
-    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-steps", type=int, default=295,
+                    help="timed steps of the batched section (with the warm-up, 300 steps per scene)")
     ap.add_argument("--batch-warmup", type=int, default=5)
     ap.add_argument("--no-serve", action="store_true", help="skip the served request stream")
     ap.add_argument("--serve-requests", type=int, default=8)
