@@ -480,27 +480,29 @@ namespace tsp
     // matters. withF = false skips the fluid candidates.
 #if TSP_TUNE_SEARCH_ROW
     // Row form: the ranges of up to three x-adjacent cells load together, and each relation's
-    // candidates along the row form one stream whose loads issue four at a time before the four
-    // tests.
+    // candidates along the row form one stream whose loads issue TSP_TUNE_SEARCH_BATCH at a time
+    // before their tests.
     template <typename Index, typename Load, typename Fn>
     __device__ __forceinline__ void rowStream(const uint2 (&r)[3], Index index, Load load, Fn fn)
     {
         const uint32_t n0 = r[0].y - r[0].x, n1 = r[1].y - r[1].x, n2 = r[2].y - r[2].x;
         const uint32_t n01 = n0 + n1, total = n01 + n2;
-        for (uint32_t t = 0; t < total; t += 4)
+        constexpr uint32_t kW = TSP_TUNE_SEARCH_BATCH;
+        for (uint32_t t = 0; t < total; t += kW)
         {
-            uint32_t j[4] = { 0u, 0u, 0u, 0u };
-            float4 p[4];
+            uint32_t j[kW];
+            float4 p[kW];
 #pragma unroll
-            for (uint32_t u = 0; u < 4; u++)
+            for (uint32_t u = 0; u < kW; u++)
             {
                 const uint32_t q = t + u;
                 const uint32_t e = q < n0 ? r[0].x + q : (q < n01 ? r[1].x + (q - n0) : r[2].x + (q - n01));
+                j[u] = 0u;
                 p[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 if (q < total) { j[u] = index(e); p[u] = load(j[u]); }
             }
 #pragma unroll
-            for (uint32_t u = 0; u < 4; u++)
+            for (uint32_t u = 0; u < kW; u++)
             {
                 if (t + u < total) { fn(j[u], p[u]); }
             }

@@ -30,7 +30,7 @@
 //   TSP_TUNE_PA_WAVES, TSP_TUNE_JACOBI_WAVES   waves per SIMD the compiler must leave room for
 //                          (register budget of the sweep kernels); 0 leaves the choice to it
 //   TSP_TUNE_SEARCH_ROW    the neighbor query walks the stencil by rows of three cells, loading
-//                          the rows' ranges together and the candidates four at a time
+//                          the rows' ranges together and TSP_TUNE_SEARCH_BATCH candidates at a time
 // the settings chosen by turbospharticles_amd/tune.py, when present, come first
 #if __has_include("tsp_tuned.h")
 #include "tsp_tuned.h"
@@ -58,6 +58,9 @@
 #endif
 #ifndef TSP_TUNE_SEARCH_ROW
 #define TSP_TUNE_SEARCH_ROW 1
+#endif
+#ifndef TSP_TUNE_SEARCH_BATCH
+#define TSP_TUNE_SEARCH_BATCH 4
 #endif
 #define TSP_WAVES_ATTR_0
 #define TSP_WAVES_ATTR_1 __attribute__((amdgpu_waves_per_eu(1)))

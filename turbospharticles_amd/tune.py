@@ -3,7 +3,8 @@
 Every variant gives the same bits; they differ in how the work maps onto the GPU: prefetched list
 entries, 24- or 32-byte Jacobi halo records, stored or recomputed gradient factors, the halo cap
 (LDS per sweep block), register budgets of the sweep kernels, sweep groups as HIP graphs, the
-neighbor query's walk of the stencil (cell by cell, or rows of three cells). The
+neighbor query's walk of the stencil (cell by cell, or rows of three cells) and its candidate loads
+in flight. The
 tuner walks the switches one at a time from the current settings (coordinate descent), keeping a
 change when it shortens the measured step.
 
@@ -41,6 +42,7 @@ SPACE = {
     "TSP_TUNE_PA_WAVES": [0, 6, 8],
     "TSP_TUNE_JACOBI_WAVES": [0, 6],
     "TSP_TUNE_SEARCH_ROW": [1, 0],
+    "TSP_TUNE_SEARCH_BATCH": [4, 8],
 }
 
 
