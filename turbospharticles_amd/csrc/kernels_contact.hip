@@ -518,7 +518,11 @@ namespace tsp
             avg = contacts > 0ull ? static_cast<float>(sum / static_cast<double>(contacts)) : 0.0f;
             converged = avg <= k.contactDevTarget;
             __syncthreads();
-            if (converged || iters == k.contactMaxIters || bad) { break; }
+            // every wave reads the flags before any wave of the sweep below can raise them, so all
+            // waves leave the loop at the same iteration
+            const bool stop = converged || iters == k.contactMaxIters || bad != 0u;
+            __syncthreads();
+            if (stop) { break; }
             flags = 0;
             for (uint32_t i = r0 + t; i < r1; i += T) { if (!jacobiOne(d, false, i)) { flags |= kContactBadPressure; } }
             for (uint32_t c = c0 + t; c < c1; c += T) { if (!jacobiOne(d, true, c)) { flags |= kContactBadPressure; } }
