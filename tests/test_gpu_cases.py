@@ -5,6 +5,8 @@
 - A 48^3 dam break (110,592 fluid particles): FF lists whose indices span more than 0xffff take
   the DictDelta16x2 order of the reference (low group, then the high group reversed); lists and
   one step must match the oracle bit for bit.
+- A near-coincident fluid-fluid or fluid-wall pair (0 < d^2 < 2^-96), which takes the general
+  square root instead of the short exact one; one step must match the oracle bit for bit.
 - Warm start, and the four termination conditions, over several steps.
 - A rigid body in free fall follows the semi-implicit Euler closed form; a spinning body keeps an
   orthonormal rotation.
@@ -72,6 +74,38 @@ def test_wide_index_span_lists_and_step_bit_exact():
     assert g.info().solver_iters == o.info().solver_iters
     for f in FIELDS_STEP:
         assert np.array_equal(sg.fluid[f], so.fluid[f]), f
+
+
+@pytest.mark.parametrize("near", ["fluid", "boundary"])
+def test_near_coincident_pair_takes_the_general_root_bit_exact(near):
+    """A pair with 0 < d^2 < 2^-96 needs the general square root: the scene is shifted so that a
+    fluid particle (or the wall particle nearest to one) sits at the origin, and a fluid particle
+    moves 1e-16 from it. The search flags the pair, the step takes the general lowering, and the
+    step must still match the oracle bit for bit."""
+    print(f"case: near-coincident fluid-{near} pair at the origin, one step", flush=True)
+    st = scenes.build_scene(scenes.dam_break_spec(0, H, counts=(8, 8, 8), tank=(0.3, 0.3, 0.3)))
+    f, b = st.fluid, st.boundary
+    p0 = np.array([f["x"][0], f["y"][0], f["z"][0]], np.float64)
+    if near == "fluid":
+        anchor, moved = p0, 1
+    else:
+        bp = np.stack([b["x"], b["y"], b["z"]], 1).astype(np.float64)
+        anchor, moved = bp[int(np.argmin(((bp - p0) ** 2).sum(1)))], 0
+    anchor = anchor.astype(np.float32)
+    for arr in (f, b):
+        for a, key in enumerate("xyz"):
+            arr[key][:] = arr[key] - anchor[a]
+    st.domain = (st.domain[0] - anchor, st.domain[1] - anchor)
+    f["x"][moved], f["y"][moved], f["z"][moved] = np.float32(1e-16), np.float32(0.0), np.float32(0.0)
+    p = scenes.make_params(H, termination_cond=scenes.TERM_POS)
+    g, o = _pair(st, p)
+    g.step(1)
+    o.step()
+    sg, so = g.push(), o.push()
+    assert g.info().solver_iters == o.info().solver_iters
+    assert np.array_equal(sg.fluid_id, so.fluid_id)
+    for fld in FIELDS_STEP:
+        assert np.array_equal(sg.fluid[fld], so.fluid[fld]), fld
 
 
 @pytest.mark.parametrize("cond,warm", [(scenes.TERM_RAW, 0.0), (scenes.TERM_ABS, 0.0), (scenes.TERM_FIXED, 0.0),

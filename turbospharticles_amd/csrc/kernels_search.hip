@@ -599,7 +599,11 @@ namespace tsp
                 },
                 [&](uint32_t j, const float4& pj)
                 {
-                    if (within(pi.x, pi.y, pi.z, pj, r2)) { cnt[1]++; mn[1] = min(mn[1], j); mx[1] = max(mx[1], j); }
+                    if (within(pi.x, pi.y, pi.z, pj, r2))
+                    {
+                        cnt[1]++; mn[1] = min(mn[1], j); mx[1] = max(mx[1], j);
+                        tiny = tiny || tinyDist2(dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z));
+                    }
                 },
                 [&](uint32_t j, const float4& pj)
                 {
@@ -614,7 +618,7 @@ namespace tsp
             d.cnt[REL_FR][i] = static_cast<uint16_t>(cnt[2]);
             if (wide) { d.ffWideMin[i] = mn[0]; }
             if (cnt[0] >= kWideSpanBit || cnt[1] > 0xffffu || cnt[2] > 0xffffu) { atomicOr(&d.status->overflow, 2u); }
-            if (tiny) { atomicOr(&d.status->ffTiny, 1u); }
+            if (tiny) { atomicOr(&d.status->pairTiny, 1u); }
         }
         // per slice: pitch, base (min index), format, entries
         const uint32_t slice = i / kSlice;

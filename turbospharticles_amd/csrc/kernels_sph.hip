@@ -56,7 +56,7 @@ namespace tsp
         visit(fb, [&](uint32_t j, uint32_t)
         {
             const float4 bj = d.bpos[j];
-            dB = __builtin_fmaf(bj.w, kwNormal(xi - bj.x, yi - bj.y, zi - bj.z, k), dB);
+            dB = __builtin_fmaf(bj.w, kwNormal<kFast>(xi - bj.x, yi - bj.y, zi - bj.z, k), dB);
         });
         const List fr = listOf(d, REL_FR, i);
         visit(fr, [&](uint32_t j, uint32_t)
@@ -67,6 +67,7 @@ namespace tsp
         d.vel[i].w = dF + dB + dR;
     }
 
+    template <bool kFast>
     __global__ void __launch_bounds__(kBlock) kNonPressure(Dev d)
     {
         __shared__ float4 hp[kHaloMax], hv[kHaloMax];
@@ -86,7 +87,7 @@ namespace tsp
             const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
             const float dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
             const float d2 = dist2(dx, dy, dz);
-            const float kg = kgFactor(d2, k);
+            const float kg = kgFactor<kFast>(d2, k);
             const float dot = __builtin_fmaf(dvx, dx, __builtin_fmaf(dvy, dy, dvz * dz));
             const float rj = vmax(vj.w, 1e-7f);
             const float coeff = dot / (rj * (d2 + k.hsqEps)) * kg;
@@ -104,7 +105,7 @@ namespace tsp
             const float4 bj = d.bpos[j];
             const float dx = xi - bj.x, dy = yi - bj.y, dz = zi - bj.z;
             const float d2 = dist2(dx, dy, dz);
-            const float f = kgFactor(d2, k);
+            const float f = kgFactor<kFast>(d2, k);
             const float gx = f * dx, gy = f * dy, gz = f * dz;
             const float dvx = vxi - 0.0f, dvy = vyi - 0.0f, dvz = vzi - 0.0f;
             const float dot = __builtin_fmaf(dvx, dx, __builtin_fmaf(dvy, dy, dvz * dz));
@@ -177,7 +178,7 @@ namespace tsp
         {
             const float4 bj = d.bpos[j];
             const float dx = xi - bj.x, dy = yi - bj.y, dz = zi - bj.z;
-            const float f = kgFactor(dist2(dx, dy, dz), k);
+            const float f = kgFactor<kFast>(dist2(dx, dy, dz), k);
             const float gx = f * dx, gy = f * dy, gz = f * dz;
             const float dvx = vxi - 0.0f, dvy = vyi - 0.0f, dvz = vzi - 0.0f;
             const float st = __builtin_fmaf(dvx, gx, __builtin_fmaf(dvy, gy, dvz * gz));
@@ -386,7 +387,7 @@ namespace tsp
         {
             const float4 bj = d.bpos[j];
             const float dx = xi - bj.x, dy = yi - bj.y, dz = zi - bj.z;
-            const float f = kgFactor(dist2(dx, dy, dz), k);
+            const float f = kgFactor<kFast>(dist2(dx, dy, dz), k);
             const float coeff = k.presOuterB * bj.w * pi;
             ax = __builtin_fmaf(coeff, f * dx, ax);
             ay = __builtin_fmaf(coeff, f * dy, ay);
@@ -485,7 +486,7 @@ namespace tsp
             {
                 const float4 bj = d.bpos[j];
                 const float dx = xi - bj.x, dy = yi - bj.y, dz = zi - bj.z;
-                const float f = kgFactor(dist2(dx, dy, dz), k);
+                const float f = kgFactor<kFast>(dist2(dx, dy, dz), k);
                 const float sv = __builtin_fmaf(axi, f * dx, __builtin_fmaf(ayi, f * dy, azi * (f * dz)));
                 sb += bj.w * sv;
             });
@@ -742,15 +743,20 @@ namespace tsp
     void launchDensity(hipStream_t s, const Dev& d)
     {
         if (!d.nF) { return; }
-        if (d.fastFF) { kDensity<true><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
+        if (d.fastPairs) { kDensity<true><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
         else { kDensity<false><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
     }
-    void launchNonPressure(hipStream_t s, const Dev& d) { if (d.nF) { kNonPressure<<<blocksOf(d.nF), kBlock, 0, s>>>(d); } }
+    void launchNonPressure(hipStream_t s, const Dev& d)
+    {
+        if (!d.nF) { return; }
+        if (d.fastPairs) { kNonPressure<true><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
+        else { kNonPressure<false><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
+    }
     void launchSourceDiag(hipStream_t s, const Dev& d)
     {
         if (d.nF)
         {
-            if (d.fastFF) { kSourceDiag<true><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
+            if (d.fastPairs) { kSourceDiag<true><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
             else { kSourceDiag<false><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
         }
         if (d.nScenes) { kSolverInit<<<1, kBlock, 0, s>>>(d); }
@@ -758,13 +764,13 @@ namespace tsp
     void launchPressureAcc(hipStream_t s, const Dev& d, bool sweep)
     {
         if (!d.nF) { return; }
-        if (d.fastFF) { kPressureAcc<true><<<blocksOf(d.nF), kBlock, 0, s>>>(d, sweep); }
+        if (d.fastPairs) { kPressureAcc<true><<<blocksOf(d.nF), kBlock, 0, s>>>(d, sweep); }
         else { kPressureAcc<false><<<blocksOf(d.nF), kBlock, 0, s>>>(d, sweep); }
     }
     void launchJacobi(hipStream_t s, const Dev& d)
     {
         if (!d.nF) { return; }
-        if (d.fastFF) { kJacobi<true><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
+        if (d.fastPairs) { kJacobi<true><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
         else { kJacobi<false><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
     }
     void launchFinalize(hipStream_t s, const Dev& d)

@@ -1173,8 +1173,8 @@ struct tsp_ctx
                 layoutFrames();
                 continue;
             }
-            // the short exact root and quotient cover this search's FF pairs unless one is tiny
-            d.fastFF = (d.fastDiv && !last.ffTiny) ? 1u : 0u;
+            // the short exact root and quotient cover this search's FF and FB pairs unless one is tiny
+            d.fastPairs = (d.fastDiv && !last.pairTiny) ? 1u : 0u;
             if (last.overflow & 2u) { throw Error{ TSP_ERR_CAPACITY, "neighbor count exceeds the u16 count storage" }; }
             if (last.overflow & 5u)
             {

@@ -104,7 +104,7 @@ namespace tsp
         uint32_t abort;          // some scene's particle left its inner cell frame: restart the search
         uint32_t overflow;       // a SELL pool is too small
         uint32_t bUnsorted;      // boundary ids are not non-decreasing under the frames
-        uint32_t ffTiny;         // some FF pair has 0 < d2 < 2^-96: the sweeps take the general sqrt
+        uint32_t pairTiny;       // some FF or FB pair has 0 < d2 < 2^-96: the kernels take the general sqrt
         uint32_t pad0[2];
         uint64_t poolUnits[3];   // u16 units used by FF, FB, FR
         uint64_t entries[3];     // entries (without padding) of FF, FB, FR
@@ -251,7 +251,7 @@ namespace tsp
         uint32_t nF, nB, nR, nRB;    // array lengths over all scenes (nF counts padding slots)
         uint32_t nScenes;
         uint32_t fastDiv;            // every scene's kernel constants lie in divCR's range
-        uint32_t fastFF;             // this search's FF pairs may take sqrtCR / divCR (kgFactor<true>)
+        uint32_t fastPairs;          // this search's FF and FB pairs may take sqrtCR / divCR (kgFactor<true>)
         Scene *scenes;
         Frame *frames;
         Consts *consts;
@@ -421,11 +421,12 @@ namespace tsp
         return __builtin_fmaf(r, y, q);
     }
 
-    // KernelWeight_CubicSpline_SIMD, CPU/simd.h:279-309
+    // KernelWeight_CubicSpline_SIMD, CPU/simd.h:279-309 (with kFast the root takes sqrtCR)
+    template <bool kFast = false>
     __device__ __forceinline__ float kwNormal(float dx, float dy, float dz, const Consts& k)
     {
         const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-        const float dist = __builtin_sqrtf(d2);
+        const float dist = kFast ? sqrtCR(d2) : __builtin_sqrtf(d2);
         const float q = dist * k.hinv;
         const float t1 = vmax(1.0f - q, 0.0f);
         const float t2 = vmax(2.0f - q, 0.0f);
@@ -446,9 +447,11 @@ namespace tsp
     }
 
     // KernelGradientFactorNoMinDist_CubicSpline_SIMD, CPU/simd.h:374-388. With kFast the root
-    // and the quotient take sqrtCR and divCR. The host allows it for a launch when no FF pair of
-    // the search has 0 < d2 < 2^-96 and every scene has |koutScale| and h in divCR's range (the
-    // quotient's operands are koutScale and dist, and it only matters where dist >= h).
+    // and the quotient take sqrtCR and divCR. The host allows it for a launch when no FF or FB
+    // pair of the search has 0 < d2 < 2^-96 and every scene has |koutScale| and h in divCR's
+    // range (the quotient's operands are koutScale and dist, and it only matters where
+    // dist >= h). FR pairs keep the general lowering: rigid particles move within a step, after
+    // the search has seen their distances.
     template <bool kFast = false>
     __device__ __forceinline__ float kgFactor(float d2, const Consts& k)
     {
