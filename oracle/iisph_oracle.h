@@ -28,14 +28,22 @@
  * writes it, explicit fma where the reference calls fma, no contraction elsewhere) and
  * *_f64 (the same algorithm with every value in double: the FP64 oracle).
  *
- * Differences from the reference binary that no test can pin here (the reference cannot be
- * built in this container): TBB's parallel_reduce join order for the FP64 Jacobi error sums
- * and for the rigid force sums is run-dependent in the reference; the oracle sums serially
- * in particle order (the rigid force sums in FP64 unless set_rigid_sums_f64(0)). The same
- * holds for the rigid contact solver's FP64 error sums and its contact force sums.
+ * It also restates:
  *   - source/simulation/CPU/backend/rigid_solver.cpp:23-799 (rigid-boundary and rigid-rigid
  *     contact IISPH: contact lists, geometry, density and diagonal, relaxation, pressure
  *     forces, velocities, density error, relaxed Jacobi, SolveRigidContacts)
+ *   - source/simulation/core/backend_sim.h:109-117 and CPU/backend/cpu_backend.cpp:910-981
+ *     (IsNearSurfaceFluidParticle and the export snapshot's near-surface particles with their
+ *     FF neighbors)
+ *
+ * Parity unpinned (the reference cannot be built in this container, so no test can pin these
+ * against it):
+ *   - TBB's parallel_reduce join order for the FP64 Jacobi error sums and for the rigid force
+ *     sums is run-dependent in the reference; the oracle sums serially in particle order (the
+ *     rigid force sums in FP64 unless set_rigid_sums_f64(0)). The same holds for the rigid
+ *     contact solver's FP64 error sums and its contact force sums.
+ *   - The DictDelta16x2 split order of wide-span FF lists is pinned against this restatement
+ *     of neighbor_lists.cpp:838-858 and neighbor_lists.h:171-197 only.
  */
 #ifndef TSP_IISPH_ORACLE_H
 #define TSP_IISPH_ORACLE_H
