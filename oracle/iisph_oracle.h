@@ -32,7 +32,7 @@
  *   - source/simulation/CPU/backend/rigid_solver.cpp:23-799 (rigid-boundary and rigid-rigid
  *     contact IISPH: contact lists, geometry, density and diagonal, relaxation, pressure
  *     forces, velocities, density error, relaxed Jacobi, SolveRigidContacts)
- *   - source/simulation/core/backend_sim.h:109-117 and CPU/backend/cpu_backend.cpp:910-981
+ *   - source/simulation/core/backend_sim.h:109-117 and CPU/backend/cpu_backend.cpp:829-981
  *     (IsNearSurfaceFluidParticle and the export snapshot's near-surface particles with their
  *     FF neighbors)
  *
