@@ -340,19 +340,56 @@ namespace tsp
     // a sweep's pressure accelerations; block 0 of a sweep launch first finishes the bookkeeping of
     // the previous Jacobi sweep, which the next Jacobi launch reads. Blocks that see the solve
     // converged skip their particles: the final launch (sweep = false) recomputes every a^p.
+    // The sweeps walk the 256-particle tiles of all scenes. One block per tile, in the XCD-aware
+    // order of xcdBlock(); or (TSP_TUNE_PERSISTENT) a grid of at most kSweepGrid resident blocks,
+    // each walking its XCD's contiguous range of tiles with a stride of the blocks per XCD.
+    constexpr uint32_t kSweepGrid = 256u * 6u;
+
+    template <typename Fn>
+    __device__ __forceinline__ void forTiles(uint32_t nTiles, Fn tile)
+    {
+#if TSP_TUNE_PERSISTENT
+        if (gridDim.x >= nTiles)
+        {
+            tile(xcdBlock());
+            return;
+        }
+        const uint32_t b = blockIdx.x, xcd = b & 7u, stride = gridDim.x >> 3;
+        const uint32_t per = nTiles >> 3, rem = nTiles & 7u;
+        const uint32_t start = xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per;
+        const uint32_t end = start + per + (xcd < rem ? 1u : 0u);
+        for (uint32_t t = start + (b >> 3); t < end; t += stride)
+        {
+            tile(t);
+            // the next tile restages the LDS records and flags this one has read
+            __syncthreads();
+        }
+#else
+        (void)nTiles;
+        tile(xcdBlock());
+#endif
+    }
+
+    static inline uint32_t sweepGrid(uint32_t nTiles)
+    {
+#if TSP_TUNE_PERSISTENT
+        return nTiles < kSweepGrid ? nTiles : kSweepGrid;
+#else
+        return nTiles;
+#endif
+    }
+
+    // one tile of a sweep's pressure accelerations
     template <bool kFast>
-    __global__ void __launch_bounds__(kBlock) TSP_WAVES_ATTR(TSP_TUNE_PA_WAVES) kPressureAcc(Dev d, bool sweep)
+    __device__ __forceinline__ void pressureAccTile(const Dev& d, uint32_t blk, bool sweep)
     {
         __shared__ float4 hp[kHaloMax];
         __shared__ uint32_t skip;
-        const uint32_t blk = xcdBlock();
         const uint32_t i = blk * kBlock + threadIdx.x;
         const uint32_t sc = d.blockScene[blk];
         const Consts& k = d.consts[sc];
         if (sweep)
         {
-            // block b of the launch finishes the previous sweep of scenes b, b + grid, ...
-            for (uint32_t q = blockIdx.x; q < d.nScenes; q += gridDim.x) { finalizeSweep(d, q, gridDim.x); }
             // one read per block, so the whole block leaves together
             if (threadIdx.x == 0) { skip = __hip_atomic_load(&d.solve[sc].converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
             __syncthreads();
@@ -407,9 +444,21 @@ namespace tsp
         d.ap[i] = make_float4(0.0f - ax, 0.0f - ay, 0.0f - az, 0.0f);
     }
 
-
     template <bool kFast>
-    __global__ void __launch_bounds__(kBlock) TSP_WAVES_ATTR(TSP_TUNE_JACOBI_WAVES) kJacobi(Dev d)
+    __global__ void __launch_bounds__(kBlock) TSP_WAVES_ATTR(TSP_TUNE_PA_WAVES) kPressureAcc(Dev d, bool sweep, uint32_t nTiles)
+    {
+        // block b of the launch finishes the previous sweep of scenes b, b + grid, ...
+        if (sweep)
+        {
+            for (uint32_t q = blockIdx.x; q < d.nScenes; q += gridDim.x) { finalizeSweep(d, q, nTiles); }
+        }
+        forTiles(nTiles, [&](uint32_t blk) { pressureAccTile<kFast>(d, blk, sweep); });
+    }
+
+
+    // one tile of a Jacobi sweep; nTot is the number of tiles (the stride of the error partials)
+    template <bool kFast>
+    __device__ __forceinline__ void jacobiTile(const Dev& d, uint32_t blk, uint32_t nTot)
     {
         __shared__ double red[3][kBlock / 64];
 #if TSP_TUNE_JACOBI_REC24
@@ -419,7 +468,6 @@ namespace tsp
 #else
         __shared__ float4 hp[kHaloMax], ha[kHaloMax];
 #endif
-        const uint32_t blk = xcdBlock();
         const uint32_t i = blk * kBlock + threadIdx.x;
         const uint32_t sc = d.blockScene[blk];
         const Consts& k = d.consts[sc];
@@ -526,9 +574,15 @@ namespace tsp
             {
                 double s = 0.0;
                 for (uint32_t q = 0; q < kBlock / 64; q++) { s += red[r][q]; }
-                d.errPartial[static_cast<size_t>(r) * gridDim.x + blk] = s;
+                d.errPartial[static_cast<size_t>(r) * nTot + blk] = s;
             }
         }
+    }
+
+    template <bool kFast>
+    __global__ void __launch_bounds__(kBlock) TSP_WAVES_ATTR(TSP_TUNE_JACOBI_WAVES) kJacobi(Dev d, uint32_t nTiles)
+    {
+        forTiles(nTiles, [&](uint32_t blk) { jacobiTile<kFast>(d, blk, nTiles); });
     }
 
     // the bookkeeping of the last sweep of a group, one block per scene, before the host reads
@@ -764,14 +818,16 @@ namespace tsp
     void launchPressureAcc(hipStream_t s, const Dev& d, bool sweep)
     {
         if (!d.nF) { return; }
-        if (d.fastPairs) { kPressureAcc<true><<<blocksOf(d.nF), kBlock, 0, s>>>(d, sweep); }
-        else { kPressureAcc<false><<<blocksOf(d.nF), kBlock, 0, s>>>(d, sweep); }
+        const uint32_t n = blocksOf(d.nF);
+        if (d.fastPairs) { kPressureAcc<true><<<sweepGrid(n), kBlock, 0, s>>>(d, sweep, n); }
+        else { kPressureAcc<false><<<sweepGrid(n), kBlock, 0, s>>>(d, sweep, n); }
     }
     void launchJacobi(hipStream_t s, const Dev& d)
     {
         if (!d.nF) { return; }
-        if (d.fastPairs) { kJacobi<true><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
-        else { kJacobi<false><<<blocksOf(d.nF), kBlock, 0, s>>>(d); }
+        const uint32_t n = blocksOf(d.nF);
+        if (d.fastPairs) { kJacobi<true><<<sweepGrid(n), kBlock, 0, s>>>(d, n); }
+        else { kJacobi<false><<<sweepGrid(n), kBlock, 0, s>>>(d, n); }
     }
     void launchFinalize(hipStream_t s, const Dev& d)
     {

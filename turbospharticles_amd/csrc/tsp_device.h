@@ -29,6 +29,8 @@
 //   TSP_TUNE_GRAPHS        each group of solver sweeps runs as one instantiated HIP graph
 //   TSP_TUNE_PA_WAVES, TSP_TUNE_JACOBI_WAVES   waves per SIMD the compiler must leave room for
 //                          (register budget of the sweep kernels); 0 leaves the choice to it
+//   TSP_TUNE_PERSISTENT    the solver sweeps run as persistent kernels: a resident grid whose
+//                          blocks walk the 256-particle tiles, instead of one block per tile
 //   TSP_TUNE_SEARCH_ROW    the neighbor query walks the stencil by rows of three cells, loading
 //                          the rows' ranges together and TSP_TUNE_SEARCH_BATCH candidates at a time
 // the settings chosen by turbospharticles_amd/tune.py, when present, come first
@@ -58,6 +60,9 @@
 #endif
 #ifndef TSP_TUNE_SEARCH_ROW
 #define TSP_TUNE_SEARCH_ROW 1
+#endif
+#ifndef TSP_TUNE_PERSISTENT
+#define TSP_TUNE_PERSISTENT 0
 #endif
 #ifndef TSP_TUNE_SEARCH_BATCH
 #define TSP_TUNE_SEARCH_BATCH 4
