@@ -40,15 +40,19 @@ def _run(cmd):
     return r
 
 
-def build_hip(verbose=False):
-    os.makedirs(LIB, exist_ok=True)
+def hip_command(out, defines=()):
+    """The hipcc command that builds libtsp_hip.so at `out` (with extra -D defines)."""
     hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
-    out = os.path.join(LIB, "libtsp_hip.so")
     # the scene plan is shared with the host builder (plain C++, compiled here too)
     srcs = [os.path.join(HERE, "csrc", s) for s in HIP_SOURCES] + [os.path.join(HERE, "host", "scene_plan.cpp")]
-    cmd = [hipcc] + HIPCC_FLAGS + ["-I" + INC, "-I" + os.path.join(HERE, "csrc"), "-I" + os.path.join(HERE, "host"),
-                                   "-shared", "-o", out + ".tmp"] + srcs
-    r = _run(cmd)
+    return [hipcc] + HIPCC_FLAGS + list(defines) + ["-I" + INC, "-I" + os.path.join(HERE, "csrc"),
+                                                     "-I" + os.path.join(HERE, "host"), "-shared", "-o", out] + srcs
+
+
+def build_hip(verbose=False):
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libtsp_hip.so")
+    r = _run(hip_command(out + ".tmp"))
     os.replace(out + ".tmp", out)
     if verbose and r.stderr:
         print(r.stderr)

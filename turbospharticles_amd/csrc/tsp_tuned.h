@@ -1,7 +1,5 @@
 // tsp_tuned.h -- build-time variant chosen by turbospharticles_amd/tune.py on MI355X
-// (1M rigid cube drop: 4.242 ms per step; every candidate checked bit-identical; the row walk of the
-// neighbor query, measured after that run against the cell walk: 4.095 against 4.148 ms per step, and
-// 10.31 against 10.94 ms per step of the 256 batched dam breaks, results identical)
+// (1M rigid cube drop: 4.082 ms per step; every candidate checked bit-identical)
 #ifndef TSP_TUNE_GRAPHS
 #define TSP_TUNE_GRAPHS 1
 #endif
@@ -18,10 +16,16 @@
 #define TSP_TUNE_KG_STORE 0
 #endif
 #ifndef TSP_TUNE_PA_WAVES
-#define TSP_TUNE_PA_WAVES 0
+#define TSP_TUNE_PA_WAVES 6
+#endif
+#ifndef TSP_TUNE_PERSISTENT
+#define TSP_TUNE_PERSISTENT 0
 #endif
 #ifndef TSP_TUNE_PREFETCH
 #define TSP_TUNE_PREFETCH 1
+#endif
+#ifndef TSP_TUNE_SEARCH_BATCH
+#define TSP_TUNE_SEARCH_BATCH 4
 #endif
 #ifndef TSP_TUNE_SEARCH_ROW
 #define TSP_TUNE_SEARCH_ROW 1

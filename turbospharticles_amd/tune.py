@@ -79,21 +79,22 @@ def build_variant(cfg):
     out_dir = os.path.join(TUNE_DIR, name_of(cfg))
     os.makedirs(out_dir, exist_ok=True)
     out = os.path.join(out_dir, "libtsp_hip.so")
-    hipcc = B.shutil.which("hipcc") or os.path.join(B.ROCM, "bin", "hipcc")
-    srcs = [os.path.join(HERE, "csrc", s) for s in B.HIP_SOURCES]
-    defs = [f"-D{k}={v}" for k, v in cfg.items()]
-    B._run([hipcc] + B.HIPCC_FLAGS + defs + ["-I" + B.INC, "-I" + os.path.join(HERE, "csrc"), "-shared",
-                                             "-o", out + ".tmp"] + srcs)
+    B._run(B.hip_command(out + ".tmp", [f"-D{k}={v}" for k, v in cfg.items()]))
     os.replace(out + ".tmp", out)
     with open(os.path.join(out_dir, "config.json"), "w") as f:
         json.dump(cfg, f)
     return out
 
 
-def cmd_build(_args):
-    for cfg in candidates():
-        print("build", name_of(cfg), flush=True)
+def cmd_build(args):
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(cfg):
         build_variant(cfg)
+        print("built", name_of(cfg), flush=True)
+
+    with ThreadPoolExecutor(max_workers=max(1, args.jobs)) as pool:
+        list(pool.map(one, candidates()))
 
 
 CHECK = r"""
@@ -166,7 +167,8 @@ def cmd_apply(args):
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
-    sub.add_parser("build")
+    b = sub.add_parser("build")
+    b.add_argument("--jobs", type=int, default=4, help="candidates compiled at once")
     m = sub.add_parser("measure")
     m.add_argument("--steps", type=int, default=30)
     m.add_argument("--out", default=None, help="results file (default lib/tune/results.json)")
