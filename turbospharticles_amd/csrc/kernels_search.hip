@@ -485,9 +485,36 @@ namespace tsp
     template <typename Index, typename Load, typename Fn>
     __device__ __forceinline__ void rowStream(const uint2 (&r)[3], Index index, Load load, Fn fn)
     {
+        constexpr uint32_t kW = TSP_TUNE_SEARCH_BATCH;
+#if TSP_TUNE_SEARCH_ROW == 2
+        // cell by cell along the row, kW candidates at a time (a cell's candidates are contiguous)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+        {
+            const uint32_t b = r[c].x, n = r[c].y - r[c].x;
+            for (uint32_t t = 0; t < n; t += kW)
+            {
+                uint32_t j[kW];
+                float4 p[kW];
+#pragma unroll
+                for (uint32_t u = 0; u < kW; u++)
+                {
+                    j[u] = 0u;
+                    p[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if (t + u < n) { j[u] = index(b + t + u); p[u] = load(j[u]); }
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < kW; u++)
+                {
+                    if (t + u < n) { fn(j[u], p[u]); }
+                }
+            }
+        }
+#else
+        // the row's three ranges as one stream: candidate q is q + the offset of its range
         const uint32_t n0 = r[0].y - r[0].x, n1 = r[1].y - r[1].x, n2 = r[2].y - r[2].x;
         const uint32_t n01 = n0 + n1, total = n01 + n2;
-        constexpr uint32_t kW = TSP_TUNE_SEARCH_BATCH;
+        const uint32_t o0 = r[0].x, o1 = r[1].x - n0, o2 = r[2].x - n01;
         for (uint32_t t = 0; t < total; t += kW)
         {
             uint32_t j[kW];
@@ -496,7 +523,7 @@ namespace tsp
             for (uint32_t u = 0; u < kW; u++)
             {
                 const uint32_t q = t + u;
-                const uint32_t e = q < n0 ? r[0].x + q : (q < n01 ? r[1].x + (q - n0) : r[2].x + (q - n01));
+                const uint32_t e = q + (q < n0 ? o0 : (q < n01 ? o1 : o2));
                 j[u] = 0u;
                 p[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 if (q < total) { j[u] = index(e); p[u] = load(j[u]); }
@@ -507,6 +534,7 @@ namespace tsp
                 if (t + u < total) { fn(j[u], p[u]); }
             }
         }
+#endif
     }
 
     template <typename OnF, typename OnB, typename OnR>

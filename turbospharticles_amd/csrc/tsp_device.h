@@ -33,6 +33,7 @@
 //                          blocks walk the 256-particle tiles, instead of one block per tile
 //   TSP_TUNE_SEARCH_ROW    the neighbor query walks the stencil by rows of three cells, loading
 //                          the rows' ranges together and TSP_TUNE_SEARCH_BATCH candidates at a time
+//                          (1: the row's candidates as one stream, 2: cell by cell; 0: the cell walk)
 // the settings chosen by turbospharticles_amd/tune.py, when present, come first
 #if __has_include("tsp_tuned.h")
 #include "tsp_tuned.h"

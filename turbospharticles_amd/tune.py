@@ -41,7 +41,7 @@ SPACE = {
     "TSP_TUNE_GRAPHS": [1, 0],
     "TSP_TUNE_PA_WAVES": [0, 6, 8],
     "TSP_TUNE_JACOBI_WAVES": [0, 6],
-    "TSP_TUNE_SEARCH_ROW": [1, 0],
+    "TSP_TUNE_SEARCH_ROW": [1, 0, 2],
     "TSP_TUNE_SEARCH_BATCH": [4, 8],
     "TSP_TUNE_PERSISTENT": [0, 1],
 }
