@@ -7,6 +7,8 @@
   one step must match the oracle bit for bit.
 - A near-coincident fluid-fluid or fluid-wall pair (0 < d^2 < 2^-96), which takes the general
   square root instead of the short exact one; one step must match the oracle bit for bit.
+- 35,937 particles inside one support radius: FF counts beyond the count storage stop the search
+  with TSP_ERR_CAPACITY before any list is written.
 - Warm start, and the four termination conditions, over several steps.
 - A rigid body in free fall follows the semi-implicit Euler closed form; a spinning body keeps an
   orthonormal rotation.
@@ -106,6 +108,23 @@ def test_near_coincident_pair_takes_the_general_root_bit_exact(near):
     assert np.array_equal(sg.fluid_id, so.fluid_id)
     for fld in FIELDS_STEP:
         assert np.array_equal(sg.fluid[fld], so.fluid[fld]), fld
+
+
+def test_neighbor_count_beyond_the_count_storage_is_an_error():
+    """35,937 fluid particles packed within 0.002 of each other (h = 0.01): every FF count is about
+    36,000, beyond the 15-bit count storage (bit 15 of the u16 count marks wide-span lists; the
+    reference's u16 counts, neighbor_lists.h:43, stop at 65,535). The search must stop after the
+    count pass with TSP_ERR_CAPACITY, before any list is written."""
+    print("case: 35,937 particles inside one support radius (count overflow)", flush=True)
+    st = scenes.build_scene(scenes.dam_break_spec(0, H, counts=(33, 33, 33), tank=(0.3, 0.3, 0.3), jitter_frac=0.0))
+    rng = np.random.default_rng(3)
+    for a, key in enumerate("xyz"):
+        st.fluid[key][:] = (0.15 + rng.uniform(-0.001, 0.001, st.n[0])).astype(np.float32)
+    g = HipBackend(0)
+    g.set_params(scenes.make_params(H, termination_cond=scenes.TERM_POS))
+    with pytest.raises(RuntimeError, match=r"\(-4\)"):
+        g.pull(st)
+        g.step(1)
 
 
 @pytest.mark.parametrize("cond,warm", [(scenes.TERM_RAW, 0.0), (scenes.TERM_ABS, 0.0), (scenes.TERM_FIXED, 0.0),
