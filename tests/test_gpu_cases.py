@@ -7,6 +7,8 @@
   one step must match the oracle bit for bit.
 - A near-coincident fluid-fluid or fluid-wall pair (0 < d^2 < 2^-96), which takes the general
   square root instead of the short exact one; one step must match the oracle bit for bit.
+- A block squeezed to 0.8 of its spacing (about 67 FF neighbors per particle) overflows the
+  first pool size: the pools grow, and lists and one step must match the oracle bit for bit.
 - 35,937 particles inside one support radius: FF counts beyond the count storage stop the search
   with TSP_ERR_CAPACITY before any list is written.
 - Warm start, and the four termination conditions, over several steps.
@@ -101,6 +103,32 @@ def test_near_coincident_pair_takes_the_general_root_bit_exact(near):
     f["x"][moved], f["y"][moved], f["z"][moved] = np.float32(1e-16), np.float32(0.0), np.float32(0.0)
     p = scenes.make_params(H, termination_cond=scenes.TERM_POS)
     g, o = _pair(st, p)
+    g.step(1)
+    o.step()
+    sg, so = g.push(), o.push()
+    assert g.info().solver_iters == o.info().solver_iters
+    assert np.array_equal(sg.fluid_id, so.fluid_id)
+    for fld in FIELDS_STEP:
+        assert np.array_equal(sg.fluid[fld], so.fluid[fld]), fld
+
+
+def test_lists_beyond_the_first_pool_size_grow_the_pools_bit_exact():
+    """A 16^3 block squeezed to 0.8 of its spacing holds about 60 FF neighbors per particle, more
+    than the 48 per particle the pools start with: the first search overflows the FF pool, the
+    pools grow and the search runs again. Lists and one step must match the oracle bit for bit."""
+    print("case: 16^3 block at 0.8 spacing (pool growth)", flush=True)
+    st = scenes.build_scene(scenes.dam_break_spec(0, H, counts=(16, 16, 16), tank=(0.4, 0.3, 0.3)))
+    for key in "xyz":
+        v = st.fluid[key]
+        v[:] = (v.min() + (v - v.min()) * np.float32(0.8)).astype(np.float32)
+    p = scenes.make_params(H, termination_cond=scenes.TERM_POS)
+    g, o = _pair(st, p)
+    g.neighbor_search()
+    o.neighbor_search()
+    cg, ig = g.neighbors(O.FF)
+    co, io, _ = o.lists(O.FF)
+    assert cg.mean() > 48, "the scene must exceed the first pool size"
+    assert np.array_equal(cg, co) and np.array_equal(ig, io)
     g.step(1)
     o.step()
     sg, so = g.push(), o.push()
