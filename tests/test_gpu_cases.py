@@ -113,7 +113,7 @@ def test_near_coincident_pair_takes_the_general_root_bit_exact(near):
 
 
 def test_lists_beyond_the_first_pool_size_grow_the_pools_bit_exact():
-    """A 16^3 block squeezed to 0.8 of its spacing holds about 60 FF neighbors per particle, more
+    """A 16^3 block squeezed to 0.8 of its spacing holds about 67 FF neighbors per particle, more
     than the 48 per particle the pools start with: the first search overflows the FF pool, the
     pools grow and the search runs again. Lists and one step must match the oracle bit for bit."""
     print("case: 16^3 block at 0.8 spacing (pool growth)", flush=True)
