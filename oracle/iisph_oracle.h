@@ -42,8 +42,10 @@
  *     sums is run-dependent in the reference; the oracle sums serially in particle order (the
  *     rigid force sums in FP64 unless set_rigid_sums_f64(0)). The same holds for the rigid
  *     contact solver's FP64 error sums and its contact force sums.
- *   - The DictDelta16x2 split order of wide-span FF lists is pinned against this restatement
- *     of neighbor_lists.cpp:838-858 and neighbor_lists.h:171-197 only.
+ *
+ * The ordering parts are pinned by tests/test_oracle_order.py against the reference's rules
+ * restated there independently (cell_frame.cpp known answers, frame origin and hysteresis,
+ * stable sort of interleaved Morton ids, z-y-x raster list order, the DictDelta16x2 split).
  */
 #ifndef TSP_IISPH_ORACLE_H
 #define TSP_IISPH_ORACLE_H
@@ -138,6 +140,9 @@ enum { ORC_FF, ORC_FB, ORC_FR, ORC_RF, ORC_RB, ORC_RR };
        of the single kernels pass the reference tests' alpha */                          \
     void P##_set_kernel_alpha(P##_ctx *c, float alpha);                                  \
     int P##_pull(P##_ctx *c, const orc_state *s);                                        \
+    /* moves the fluid particles (current order) and keeps the cell frame (tests) */      \
+    void P##_set_fluid_positions(P##_ctx *c, const float *x, const float *y,             \
+                                 const float *z);                                        \
     int P##_push(P##_ctx *c, orc_state *s);                                              \
     int P##_neighbor_search(P##_ctx *c);                                                 \
     int P##_step(P##_ctx *c);                                                            \
@@ -180,6 +185,9 @@ enum {
 
 /* SplitMix64-style jitter of scene/builder.cpp:1486-1501 */
 float orc_jitter3d(uint64_t base_seed, uint64_t key, float amp);
+/* TryBuildCellCoordFrame of core/cell_frame.h:32-66 (1: built; 0: rejected, frame untouched) */
+int orc_try_build_cell_frame(const int64_t cell_min[3], const int64_t cell_max[3], int32_t orig[3],
+                             uint32_t dim[3]);
 /* Morton spread of neighbor_search.cpp:2057-2070 */
 uint64_t orc_spread_bits_21to63(uint32_t v);
 

@@ -27,6 +27,24 @@ uint64_t orc_spread_bits_21to63(uint32_t v)
     return x;
 }
 
+/* TryBuildCellCoordFrame, core/cell_frame.h:32-66: a frame from inclusive cell bounds; each axis
+   must be ordered, inside the int32 range and at most 2^21 cells long. On failure the frame
+   arrays keep what they held. */
+int orc_try_build_cell_frame(const int64_t cell_min[3], const int64_t cell_max[3], int32_t orig[3],
+                             uint32_t dim[3])
+{
+    for (int a = 0; a < 3; a++) {
+        if (cell_min[a] > cell_max[a]) return 0;
+        if (cell_min[a] < (int64_t)INT32_MIN || cell_max[a] > (int64_t)INT32_MAX) return 0;
+        if (cell_max[a] - cell_min[a] + 1 > (int64_t)(1u << 21)) return 0;
+    }
+    for (int a = 0; a < 3; a++) {
+        orig[a] = (int32_t)cell_min[a];
+        dim[a] = (uint32_t)(cell_max[a] - cell_min[a] + 1);
+    }
+    return 1;
+}
+
 /* Jitter3D, scene/builder.cpp:1486-1501 */
 float orc_jitter3d(uint64_t base_seed, uint64_t key, float amp)
 {

@@ -74,6 +74,7 @@ def lib():
                 "set_threads": (None, [vp, ctypes.c_int]),
                 "set_kernel_alpha": (None, [vp, ctypes.c_float]),
                 "pull": (ctypes.c_int, [vp, ctypes.POINTER(OrcState)]),
+                "set_fluid_positions": (None, [vp, _fp, _fp, _fp]),
                 "push": (ctypes.c_int, [vp, ctypes.POINTER(OrcState)]),
                 "neighbor_search": (ctypes.c_int, [vp]),
                 "step": (ctypes.c_int, [vp]),
@@ -95,6 +96,9 @@ def lib():
                 fn.argtypes = args
         L.orc_jitter3d.restype = ctypes.c_float
         L.orc_jitter3d.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_float]
+        L.orc_try_build_cell_frame.restype = ctypes.c_int
+        L.orc_try_build_cell_frame.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                               ctypes.POINTER(ctypes.c_int32), _u32p]
         L.orc_spread_bits_21to63.restype = ctypes.c_uint64
         L.orc_spread_bits_21to63.argtypes = [ctypes.c_uint32]
         _lib = L
@@ -157,6 +161,11 @@ class Oracle:
         rc = self.fn("push")(self.h, ctypes.byref(o))
         assert rc == 0, rc
         return st
+
+    def set_fluid_positions(self, x, y, z):
+        """Moves the fluid particles (current order) without invalidating the cell frame."""
+        a = [np.ascontiguousarray(v, np.float32) for v in (x, y, z)]
+        self.fn("set_fluid_positions")(self.h, *(_capi.fptr(v) for v in a))
 
     def neighbor_search(self):
         assert self.fn("neighbor_search")(self.h) == 0
@@ -259,3 +268,15 @@ def kernel_weight(params, offsets, precision=32):
     getattr(L, f"orc{precision}_kernel_weight")(ctypes.byref(op), _capi.fptr(off), n, _capi.fptr(w),
                                                 _capi.fptr(g), _capi.fptr(wu))
     return w, g.reshape(-1, 3), wu
+
+
+def try_build_cell_frame(cell_min, cell_max, frame=(0, 0, 0, 0, 0, 0)):
+    """orc_try_build_cell_frame: (built, (origX, origY, origZ, dimX, dimY, dimZ)); a rejected
+    build returns the frame it was given."""
+    L = lib()
+    mn = (ctypes.c_int64 * 3)(*cell_min)
+    mx = (ctypes.c_int64 * 3)(*cell_max)
+    orig = (ctypes.c_int32 * 3)(*frame[:3])
+    dim = (ctypes.c_uint32 * 3)(*frame[3:])
+    ok = L.orc_try_build_cell_frame(mn, mx, orig, dim)
+    return bool(ok), tuple(orig) + tuple(dim)
