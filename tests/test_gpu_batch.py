@@ -160,3 +160,67 @@ def test_served_stream_of_scene_specs_matches_host_built_states():
     for a, c, e in zip(rb, rp, rs):
         _same_state(a.result, c.result, f"request {a.rid}")
         _same_state(e.result, c.result, f"request {e.rid} (plain, from its spec)")
+
+
+def _far_pair(q):
+    """Two fluid particles 19.6 m apart along x: a cell frame of 998 x 19 x 19 cells at h 0.01,
+    whose Morton ids take 30 bits; a context's 32-bit key space holds three such scenes."""
+    st = _capi.HostState(2, 0, 0, 0)
+    st.fluid_id[:] = [2 * q, 2 * q + 1]
+    st.fluid["x"][:] = [0.0, 19.6]
+    st.fluid["y"][:] = [0.5 + 0.01 * q, 0.5]
+    st.fluid["z"][:] = [0.5, 0.5 - 0.01 * q]
+    return st
+
+
+def test_served_stream_waits_when_the_key_space_is_full():
+    """Six requests of scenes with large cell frames arrive together; the context can hold three
+    of them at a time, so admissions are refused (TSP_ERR_CAPACITY) and requeued until scenes
+    leave. Dam breaks in between are admitted as usual; every result equals its plain run."""
+    from turbospharticles_amd import serving
+    print("case: served stream, key space for three large-frame scenes", flush=True)
+    p = scenes.make_params(0.01, termination_cond=scenes.TERM_POS)
+
+    def make():
+        reqs = []
+        for q in range(6):
+            reqs.append(serving.Request(2 * q, 0.0, p, _far_pair(q), 3 + q))
+            reqs.append(serving.Request(2 * q + 1, 0.0, p, None, 2 + q % 3,
+                                        spec=scenes.scaled_dam_break_spec(6 + q, 0.01, jitter_seed=q)))
+        return reqs
+
+    rb = make()
+    serving.serve_batched(rb, 200e9, chunk_steps=2)
+    rp = make()
+    serving.serve_plain(rp)
+    far = [r for r in rb if r.rid % 2 == 0]
+    assert max(r.admitted_s for r in far) >= min(r.done_s for r in far), "a large-frame scene must wait"
+    for a, c in zip(rb, rp):
+        assert a.result.step == a.steps
+        _same_state(a.result, c.result, f"request {a.rid}")
+
+
+def test_refused_admission_keeps_the_resident_scenes_and_drops_removed_ids():
+    """A batch update that removes one scene and adds one that does not fit the key space: the
+    context refuses the new scene, the removal takes effect, the remaining scenes step on as if
+    nothing happened, and the wrapper's scene table follows the context."""
+    from turbospharticles_amd.backend import TspError
+    print("case: refused admission with a removal", flush=True)
+    p = scenes.make_params(0.01, termination_cond=scenes.TERM_POS)
+    b = HipBatch(0)
+    ids = b.update(add=[(p, _far_pair(q)) for q in range(3)])
+    dam = scenes.build_scene(scenes.dam_break_spec(8, 0.01, tank=(0.3, 0.2, 0.2)))
+    (kept,) = b.update(add=[(p, dam)])
+    b.step(2)
+    with pytest.raises(TspError) as e:
+        b.update(remove=[ids[0]], add=[(p, _far_pair(7)), (p, _far_pair(8))])
+    assert e.value.code == _capi.TSP_ERR_CAPACITY
+    assert sorted(b.counts) == sorted(b.ids()) == sorted([ids[1], ids[2], kept])
+    b.step(3)
+    g, _ = _alone(p, dam, 5)
+    _same_state(g.push(), b.push(kept), "dam break kept through the refusal")
+    g.close()
+    g, _ = _alone(p, _far_pair(2), 5)
+    _same_state(g.push(), b.push(ids[2]), "large-frame scene kept through the refusal")
+    g.close()
+    b.close()

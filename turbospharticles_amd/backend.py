@@ -12,6 +12,15 @@ import numpy as np
 from . import _capi
 
 
+class TspError(RuntimeError):
+    """A failed tsp_* call: `code` is its tsp_status (TSP_ERR_CAPACITY = -4 for a count beyond
+    a fixed limit, e.g. a context whose scenes' cell frames do not fit its key space)."""
+
+    def __init__(self, what, code, msg):
+        super().__init__(f"{what} failed ({code}): {msg}")
+        self.code = code
+
+
 class HipBackend:
     def __init__(self, device=0):
         self.lib = _capi.hip_lib()
@@ -35,8 +44,7 @@ class HipBackend:
 
     def _check(self, rc, what):
         if rc != 0:
-            msg = self.lib.tsp_last_error(self.h).decode()
-            raise RuntimeError(f"{what} failed ({rc}): {msg}")
+            raise TspError(what, rc, self.lib.tsp_last_error(self.h).decode())
 
     # I_SimBackend::Name
     def name(self):
@@ -190,8 +198,7 @@ class HipBatch:
 
     def _check(self, rc, what):
         if rc != 0:
-            msg = self.lib.tsp_last_error(self.h).decode()
-            raise RuntimeError(f"{what} failed ({rc}): {msg}")
+            raise TspError(what, rc, self.lib.tsp_last_error(self.h).decode())
 
     def update(self, add=(), remove=()):
         """Removes the scenes `remove` (ids), then appends `add`, a sequence of (params, state)
@@ -211,15 +218,28 @@ class HipBatch:
                 dom[6 * q:6 * q + 3] = st.domain[0]
                 dom[6 * q + 3:6 * q + 6] = st.domain[1]
         ids = np.zeros(max(n, 1), np.uint32)
-        self._check(self.lib.tsp_batch_update(self.h, len(rm), _capi.u32ptr(rm) if len(rm) else None, n, params,
-                                              vptrs, _capi.fptr(dom), _capi.u32ptr(ids)),
-                    "tsp_batch_update")
+        rc = self.lib.tsp_batch_update(self.h, len(rm), _capi.u32ptr(rm) if len(rm) else None, n, params, vptrs,
+                                       _capi.fptr(dom), _capi.u32ptr(ids))
+        self._check_update(rc, "tsp_batch_update")
         for r in rm:
             self.counts.pop(int(r), None)
         out = [int(v) for v in ids[:n]]
         for sid, (_, st) in zip(out, add):
             self.counts[sid] = st.n
         return out
+
+    def _check_update(self, rc, what):
+        """A refused update adds no scene, but the removals it was given may have taken effect
+        (a refused admission keeps only the scenes that stay): the scene table follows the
+        context's ids before the error is raised."""
+        if rc == 0:
+            return
+        err = TspError(what, rc, self.lib.tsp_last_error(self.h).decode())
+        live = set(self.ids())
+        for sid in list(self.counts):
+            if sid not in live:
+                self.counts.pop(sid)
+        raise err
 
     def add(self, params, state):
         return self.update(add=[(params, state)])[0]
@@ -236,8 +256,9 @@ class HipBatch:
             params[q] = p
             sptrs[q] = ctypes.pointer(sp)
         ids = np.zeros(max(n, 1), np.uint32)
-        self._check(self.lib.tsp_batch_update_specs(self.h, len(rm), _capi.u32ptr(rm) if len(rm) else None, n, params,
-                                                    sptrs, _capi.u32ptr(ids)), "tsp_batch_update_specs")
+        rc = self.lib.tsp_batch_update_specs(self.h, len(rm), _capi.u32ptr(rm) if len(rm) else None, n, params,
+                                             sptrs, _capi.u32ptr(ids))
+        self._check_update(rc, "tsp_batch_update_specs")
         for r in rm:
             self.counts.pop(int(r), None)
         out = [int(v) for v in ids[:n]]

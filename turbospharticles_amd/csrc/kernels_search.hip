@@ -670,13 +670,24 @@ namespace tsp
         }
     }
 
-    // storage units of every slice into a scan buffer of nSlices + 1 entries per relation
+    // storage units of every slice into a scan buffer of nSlices + 1 entries per relation, and
+    // their total in 64 bits (the u32 scan of the offsets wraps beyond 2^32 units; the host
+    // refuses such a pool before any list is written)
     __global__ void __launch_bounds__(kBlock) kSliceUnits(Dev d, uint32_t nSlices, uint32_t* __restrict__ units)
     {
+        __shared__ unsigned long long red[kBlock];
         const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
         const uint32_t rel = blockIdx.y;
-        if (q > nSlices) { return; }
-        units[rel * (nSlices + 1) + q] = q < nSlices ? d.meta[rel][q].offset : 0u;
+        const uint32_t u = q < nSlices ? d.meta[rel][q].offset : 0u;
+        if (q <= nSlices) { units[rel * (nSlices + 1) + q] = u; }
+        red[threadIdx.x] = u;
+        __syncthreads();
+        for (uint32_t o = kBlock / 2; o > 0; o >>= 1)
+        {
+            if (threadIdx.x < o) { red[threadIdx.x] += red[threadIdx.x + o]; }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0 && red[0]) { atomicAdd(reinterpret_cast<unsigned long long*>(&d.status->poolUnits[rel]), red[0]); }
     }
 
     // writes the scanned offsets back, sums the entries and checks the pool capacity
@@ -705,8 +716,7 @@ namespace tsp
         }
         if (q == 0)
         {
-            const uint64_t total = offs[rel * (nSlices + 1) + nSlices];
-            d.status->poolUnits[rel] = total;
+            const uint64_t total = d.status->poolUnits[rel];   // summed in 64 bits by kSliceUnits
             if (total > d.poolCap[rel]) { atomicOr(&d.status->overflow, 1u); }
         }
     }

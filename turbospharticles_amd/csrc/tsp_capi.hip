@@ -1175,16 +1175,29 @@ struct tsp_ctx
             }
             // the short exact root and quotient cover this search's FF and FB pairs unless one is tiny
             d.fastPairs = (d.fastDiv && !last.pairTiny) ? 1u : 0u;
-            if (last.overflow & 2u) { throw Error{ TSP_ERR_CAPACITY, "neighbor count exceeds the u16 count storage" }; }
+            // capacity errors leave the particles in their order before the search (the lists
+            // of this search are not written)
+            const auto refuse = [&](const char* why)
+            {
+                swapFields();
+                listsValid = false;
+                throw Error{ TSP_ERR_CAPACITY, why };
+            };
+            if (last.overflow & 2u) { refuse("neighbor count exceeds the u16 count storage"); }
+            for (int r = 0; r < 3; r++)
+            {
+                // slice offsets are u32 in u16 units
+                if (last.poolUnits[r] > 0xffffffffull) { refuse("SELL pool exceeds 2^32 u16 units"); }
+            }
             if (last.overflow & 5u)
             {
                 uint64_t cap[3];
-                for (int r = 0; r < 3; r++) { cap[r] = std::max<uint64_t>(d.poolCap[r], last.poolUnits[r] + last.poolUnits[r] / 8 + 64); }
-                const uint64_t rfCap = std::max<uint64_t>(d.rfCap, last.rfTotal + last.rfTotal / 8 + 64);
                 for (int r = 0; r < 3; r++)
                 {
-                    if (cap[r] > 0xffffffffull) { throw Error{ TSP_ERR_CAPACITY, "SELL pool exceeds 2^32 u16 units" }; }
+                    cap[r] = std::max<uint64_t>(d.poolCap[r], last.poolUnits[r] + last.poolUnits[r] / 8 + 64);
+                    cap[r] = std::min<uint64_t>(cap[r], 0xffffffffull);
                 }
+                const uint64_t rfCap = std::max<uint64_t>(d.rfCap, last.rfTotal + last.rfTotal / 8 + 64);
                 allocPools(cap, rfCap);
                 for (SceneHost& h : scenes) { h.flags |= TSP_FLAG_LIST_REGROWN; }
             }

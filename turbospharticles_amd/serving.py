@@ -14,11 +14,17 @@ in arrival order: the reference's one-simulation-per-backend model.
 A request's scene is either a host state or a scene spec; a spec is generated on the device when
 the request is admitted (tsp_batch_update_specs / tsp_pull_spec), so no host state is built or
 copied for it.
+
+Besides HBM, a context's 32-bit key space is an admission resource: every resident scene holds
+a key range sized by its cell frame (2^(3 bits) for frames of up to 2^bits cells per axis). A
+context refuses scenes whose frames do not fit (TSP_ERR_CAPACITY) and keeps its resident scenes;
+the loop then admits fewer of the waiting requests, or holds them until a resident scene leaves.
 """
 import collections
 import time
 
-from .backend import HipBackend, HipBatch, spec_counts
+from . import _capi
+from .backend import HipBackend, HipBatch, TspError, spec_counts
 
 
 class Request:
@@ -55,6 +61,22 @@ def _wait_until(t0, t):
         time.sleep(dt)
 
 
+def _admit(batch, admit):
+    """One admission of host states and scene specs; either all of them join or none does.
+    Returns the requests in the order of their new scene ids, and the ids."""
+    hosted = [r for r in admit if r.state is not None]
+    specced = [r for r in admit if r.state is None]
+    ids = batch.update(add=[(r.params, r.state) for r in hosted]) if hosted else []
+    try:
+        if specced:
+            ids += batch.update_specs(add=[(r.params, r.spec) for r in specced])
+    except Exception:
+        if ids:
+            batch.update(remove=ids)
+        raise
+    return hosted + specced, ids
+
+
 def serve_batched(requests, hbm_budget_bytes, chunk_steps=10, device=0, keep_results=True):
     """Serves `requests` (arrival times relative to the call) with continuous batching; returns
     the wall time from the start to the last completion. Fills admitted_s, done_s and result."""
@@ -63,6 +85,7 @@ def serve_batched(requests, hbm_budget_bytes, chunk_steps=10, device=0, keep_res
     waiting = collections.deque()
     active = {}            # scene id -> [request, steps left]
     used = 0
+    hold = False           # the key space refused the head of the queue: wait for a completion
     t0 = time.perf_counter()
     try:
         while pending or waiting or active:
@@ -70,23 +93,37 @@ def serve_batched(requests, hbm_budget_bytes, chunk_steps=10, device=0, keep_res
             while pending and pending[0].arrival_s <= now:
                 waiting.append(pending.popleft())
             admit = []
-            while waiting and used + waiting[0].footprint <= hbm_budget_bytes:
+            while not hold and waiting and used + waiting[0].footprint <= hbm_budget_bytes:
                 r = waiting.popleft()
                 used += r.footprint
                 admit.append(r)
             if waiting and not active and not admit:
                 r = waiting[0]
                 raise RuntimeError(f"request {r.rid} needs {r.footprint} B, more than the {hbm_budget_bytes} B budget")
-            if admit:
-                hosted = [r for r in admit if r.state is not None]
-                specced = [r for r in admit if r.state is None]
-                ids = batch.update(add=[(r.params, r.state) for r in hosted]) if hosted else []
-                ids += batch.update_specs(add=[(r.params, r.spec) for r in specced]) if specced else []
-                admit = hosted + specced
+            while admit:
+                try:
+                    admitted, ids = _admit(batch, admit)
+                except TspError as e:
+                    if e.code != _capi.TSP_ERR_CAPACITY or (len(admit) == 1 and not active):
+                        raise
+                    # put the requests back at the head of the queue, in order; try the first
+                    # alone, and hold the queue when even that one does not fit
+                    for r in reversed(admit):
+                        waiting.appendleft(r)
+                        used -= r.footprint
+                    if len(admit) == 1:
+                        hold = True
+                        admit = []
+                    else:
+                        r = waiting.popleft()
+                        used += r.footprint
+                        admit = [r]
+                    continue
                 now = time.perf_counter() - t0
-                for sid, r in zip(ids, admit):
+                for sid, r in zip(ids, admitted):
                     r.admitted_s = now
                     active[sid] = [r, r.steps]
+                admit = []
             if not active:
                 if pending:
                     _wait_until(t0, pending[0].arrival_s)
@@ -109,6 +146,7 @@ def serve_batched(requests, hbm_budget_bytes, chunk_steps=10, device=0, keep_res
                     r = active.pop(sid)[0]
                     r.done_s = now
                     used -= r.footprint
+                hold = False
         return time.perf_counter() - t0
     finally:
         batch.close()
