@@ -49,26 +49,48 @@ def algorithmic_bytes(info, n_f):
     }
 
 
-def measured_traffic(kernel, n_side):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary of this workload (profiles/<kernel>_1M_cube_traffic.json), or None."""
+VALU_CLOCK_HZ = 2.4e9      # MI355X peak engine clock
+N_SIMDS = 1024             # 256 CUs x 4 SIMDs; a wave64 VALU instruction takes 4 cycles of one SIMD
+
+
+def measured_counters(kernel, n_side):
+    """The committed rocprofv3 counter summary of `kernel` on this workload
+    (profiles/<kernel>_1M_cube_traffic.json): HBM bytes per launch (FETCH_SIZE / WRITE_SIZE scaled
+    by the copy calibration) and VALU instructions per wave (SQ_INSTS_VALU / SQ_WAVES), or {}."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_1M_cube_traffic.json")
     if n_side != N_SIDE or not os.path.exists(path):
-        return None
+        return {}
     with open(path) as f:
-        return int(json.load(f)["hbm_bytes_per_launch"])
+        return json.load(f)
 
 
 def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(state, params, warmup, steps, threads):
-    """The oracle on the host cores: the same warm-up steps untimed, then the first `steps` of the
-    GPU's timed window."""
+def host_cpu():
+    """(model name, fp32 SIMD lanes, oracle build): 16 lanes and the AVX-512 build where the host
+    has AVX-512, else 8 lanes and the AVX2 build."""
+    model, flags = "unknown", ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                if line.startswith("flags") and not flags:
+                    flags = line
+    except OSError:
+        pass
+    return (model, 16, "v4") if " avx512f" in flags else (model, 8, "v3")
+
+
+def cpu_baseline(state, params, warmup, steps, threads, lanes, variant):
+    """The oracle (the TBB path restated, -O3 build for the host's ISA, chunk width = the host's
+    fp32 SIMD lanes, as the reference's SIMD chunks) on the host cores: the same warm-up steps
+    untimed, then the first `steps` of the GPU's timed window."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    o = O.Oracle(params, chunk_width=1, threads=threads)
+    o = O.Oracle(params, chunk_width=lanes, threads=threads, variant=variant)
     o.pull(state)
     for _ in range(warmup):
         o.step()
@@ -180,12 +202,13 @@ def served_bench(args):
         same += int(all(np.array_equal(a.result.fluid[f], c.result.fluid[f]) for f in a.result.fluid)
                     and np.array_equal(a.result.fluid_id, c.result.fluid_id))
     lb, lp = serving.latency_summary(rb), serving.latency_summary(rp)
-    return {"workload": f"{n} requests, Poisson arrivals at {args.serve_rate}/s (seed {args.serve_seed}), "
+    return {"requests": n,
+            "workload": f"{n} requests, Poisson arrivals at {args.serve_rate}/s (seed {args.serve_seed}), "
                         f"dam break / cube drop 50/50, fluid {min(s ** 3 for s in sides)}..{max(s ** 3 for s in sides)}, "
                         f"steps {int(steps.min())}..{int(steps.max())}, 200 GB admission budget, joins and leaves "
                         f"every <= {args.serve_chunk} steps, scenes generated on the device at admission",
             "value": round(work / wall_b, 1), "unit": "particle-steps/s",
-            "p50_latency_s": lb["p50_s"], "p99_latency_s": lb["p99_s"],
+            "p50_latency_s": lb["p50_s"], "p99_latency_s": lb["p99_s"], "latency_samples": n,
             "plain_value": round(work / wall_p, 1), "plain_p50_latency_s": lp["p50_s"],
             "plain_p99_latency_s": lp["p99_s"], "speedup_vs_plain": round(wall_p / wall_b, 2),
             "results_identical_to_plain": f"{same}/{n}"}
@@ -210,7 +233,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=5,
                     help="oracle steps timed for the CPU baseline (after the same warm-up as the GPU)")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="host threads of the CPU baseline (a GPU box's share of its host is 16 cores)")
+    ap.add_argument("--full-steps", type=int, default=500,
+                    help="steps of the whole-run figure (the workload's full 500-step drop from its initial state)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n-side", type=int, default=N_SIDE)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain mixed scenes")
@@ -219,7 +245,7 @@ def main():
                     help="timed steps of the batched section (with the warm-up, 300 steps per scene)")
     ap.add_argument("--batch-warmup", type=int, default=5)
     ap.add_argument("--no-serve", action="store_true", help="skip the served request stream")
-    ap.add_argument("--serve-requests", type=int, default=8)
+    ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-seed", type=int, default=5)
     ap.add_argument("--serve-chunk", type=int, default=10)
@@ -265,11 +291,32 @@ def main():
         gpu.close()
         return t1 - t0, info, prof, (info.sweeps_total - sweeps0) / args.steps
 
+    def full_run(steps):
+        # the whole drop from the initial state, step by step (the cube lands and rests on the
+        # floor late in it, and the contact solve runs): wall time, sweeps and contact steps
+        gpu = HipBackend(0)
+        gpu.set_params(params)
+        gpu.pull(state)
+        gpu.synchronize()
+        contact = 0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            gpu.step(1)
+            contact += int(gpu.info().flags & 4 != 0)
+        gpu.synchronize()
+        wall_full = time.perf_counter() - t0
+        info = gpu.info()
+        gpu.close()
+        return {"steps": steps, "ms_per_step": round(wall_full * 1e3 / steps, 3),
+                "value": round(n_f * steps / wall_full, 1), "mean_jacobi_sweeps": round(info.sweeps_total / steps, 2),
+                "contact_steps": contact, "note": "steps 1..%d from the initial state, one tsp_step call per step" % steps}
+
     # the value comes from the plain timed steps; the per-kernel times from the same steps run
     # again with the events (which add their own gaps between launches)
     wall, info, _, k_bar = window(False)
     wall_prof, _, prof, _ = window(True)
     value = n_f * args.steps / wall
+    full = full_run(args.full_steps) if args.full_steps > 0 else None
 
     ab = algorithmic_bytes(info, n_f)
     candidates = {k: v for k, v in prof.items() if k in ab}
@@ -280,19 +327,31 @@ def main():
         avg_s = ms / 1e3 / launches
         bytes_launch = ab[dominant] * n_f
         achieved = bytes_launch / avg_s / 1e9
+        counters = measured_counters(dominant, args.n_side)
         roof = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(dominant, args.n_side),
+                "traffic": int(counters["hbm_bytes_per_launch"]) if counters else None,
                 "bytes_per_launch": int(bytes_launch), "avg_launch_us": round(avg_s * 1e6, 2)}
+        if counters.get("valu_insts_per_wave"):
+            # instruction issue: the kernel's VALU instructions (per wave, from the counters) over
+            # what the 1024 SIMDs issue in the launch time measured here (4 cycles per wave64
+            # instruction at the peak clock); the sweeps' pair arithmetic, not HBM, bounds them
+            waves = (n_f + 63) // 64
+            valu = counters["valu_insts_per_wave"] * waves * 4 / (N_SIMDS * VALU_CLOCK_HZ * avg_s)
+            roof["valu_issue_frac"] = round(valu, 4)
+            roof["valu_insts_per_wave"] = counters["valu_insts_per_wave"]
     b_step = 846.0 + 214.0 * k_bar
     step_gbs = n_f * b_step * args.steps / wall / 1e9
 
     cpu = None
     if not args.no_cpu:
         progress("CPU baseline (oracle)")
-        v, dt, kc = cpu_baseline(state, params, args.warmup, args.cpu_steps, args.cpu_threads)
+        model, lanes, variant = host_cpu()
+        v, dt, kc = cpu_baseline(state, params, args.warmup, args.cpu_steps, args.cpu_threads, lanes, variant)
         cpu = {"value": round(v, 1), "unit": "particle-steps/s", "cores": args.cpu_threads, "kind": "port",
-               "sample": f"oracle (C restatement of the TBB path, OpenMP) on steps {args.warmup + 1}.."
+               "cpu_model": model, "simd_fp32_lanes": lanes,
+               "sample": f"oracle (C restatement of the TBB path, OpenMP, gcc -O3 -march=x86-64-{variant}, "
+                         f"chunk width {lanes}) on steps {args.warmup + 1}.."
                          f"{args.warmup + args.cpu_steps} of the same workload (the first {args.cpu_steps} of the GPU's "
                          f"timed window, after the same {args.warmup} warm-up steps): {dt:.1f} s, "
                          f"mean {kc:.1f} Jacobi sweeps/step"}
@@ -309,6 +368,7 @@ def main():
         "config": {"workload": f"rigid cube drop, {n_f} fluid ({args.n_side}^3, h={h:g}) + {state.n[1]} boundary "
                                f"+ {state.n[2]} rigid, dt 1e-3, 0.1% avg compression error, 2..100 iterations",
                    "mean_jacobi_sweeps": round(k_bar, 2),
+                   "full_run": full,
                    "profiled_ms_per_step": round(wall_prof * 1e3 / args.steps, 3),
                    "step_algorithmic_GBps": round(step_gbs, 1),
                    "kernel_ms": {k: round(v[1], 3) for k, v in prof.items()}},

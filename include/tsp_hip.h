@@ -197,6 +197,19 @@ typedef struct tsp_statistics {
 /* I_SimBackend::FillStatisticsRecord (core/backend_sim.h:243) for the first scene */
 int tsp_get_statistics(tsp_ctx *ctx, tsp_statistics *out);
 
+/* Device memory a context holds, by what it stores, in bytes (I_SimBackend::FillMemoryRecord,
+   core/backend_sim.h:239, with the buckets of eval/memory/categories.h:59-78). */
+typedef struct tsp_memory {
+    uint64_t particles[3];      /* fluid, boundary, rigid particle (and body) arrays */
+    uint64_t search[3];         /* sort keys and indices, cell tables: fluid, boundary, rigid */
+    uint64_t lists[3];          /* fluid-source lists (SELL-32 pools, counts, slice metadata,
+                                   halos); none for boundary sources; rigid-source lists */
+    uint64_t misc;              /* scans, reductions, solver state, statistics and export scratch */
+    uint64_t alloc_bytes;       /* all of the above */
+    uint64_t peak_alloc_bytes;  /* the largest alloc_bytes since tsp_create */
+} tsp_memory;
+int tsp_get_memory(tsp_ctx *ctx, tsp_memory *out);
+
 /* Sync::ExportSnapshot (sync/snapshots.h:37-66) as the caller's arrays. The fluid arrays hold
    the exported particles: those near the surface (IsNearSurfaceFluidParticle, core/backend_sim.h:
    109-117: a boundary or rigid neighbor, or fewer FF neighbors than the threshold) and their FF

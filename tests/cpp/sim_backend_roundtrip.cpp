@@ -1,10 +1,13 @@
-// sim_backend_roundtrip.cpp -- drives SimBackend_HIP the way the reference's Simulation drives a
-// backend: build a scene into a universal state, pull, step, push. Writes the pushed fluid state
-// (ids, then the 16 SoA fields, each N_F values) to the file named by argv[1], so that a test can
-// compare it with the Python binding of the same C ABI.
+// sim_backend_roundtrip.cpp -- drives the gfx950 backend the way the reference's SimulationMgr
+// drives a backend (core/simulation.cpp:338-503): through an I_SimBackend* from MakeSimBackend,
+// a scene built into a universal state, pull, neighbor search, steps, push, and every reporting
+// method of the interface. Writes the pushed fluid state (ids, then the 16 SoA fields, each N_F
+// values) to the file named by argv[1], so that a test can compare it with the Python binding of
+// the same C ABI, and prints one JSON line with what the reporting methods returned.
 //
 // usage: sim_backend_roundtrip OUT n_side steps
 // exit codes: 0 ok, 2 usage, 3 backend error (message on stderr)
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <exception>
@@ -13,6 +16,8 @@
 #include "../../include/tsp_scene.h"
 #include "../../turbospharticles_amd/host/sim_backend_hip.h"
 
+using namespace tsp;
+
 int main(int argc, char** argv)
 {
     if (argc != 4) { std::fprintf(stderr, "usage: %s OUT n_side steps\n", argv[0]); return 2; }
@@ -20,7 +25,8 @@ int main(int argc, char** argv)
     const int steps = std::atoi(argv[3]);
     try
     {
-        tsp::SimBackend_HIP backend(0);
+        Bench::BenchmarksTracker tracker;
+        std::unique_ptr<Sim::I_SimBackend> backend = Sim::MakeSimBackend(Base::BackendType::GPU, tracker);
         tsp_scene_spec spec{};
         spec.particle_size = 0.01f;
         spec.rest_density = 1000.0f;
@@ -33,7 +39,7 @@ int main(int argc, char** argv)
         if (tsp_scene_build(&spec, &scene) != 0) { std::fprintf(stderr, "scene build failed\n"); return 3; }
         uint32_t n[4];
         tsp_scene_counts(scene, n);
-        tsp::Sync::UniversalSimState state;
+        Sync::UniversalSimState state;
         auto& F = state.particlesF;
         auto& B = state.particlesB;
         F.N_ = n[0];
@@ -58,29 +64,61 @@ int main(int argc, char** argv)
         tsp_scene_domain(scene, mn, mx);
         tsp_scene_free(scene);
 
-        tsp::SimParams params;
+        SimParams params;
         params.SetParticleSize(0.01f);
-        params.solverTerminationCond = tsp::SolverTerminationCond::RelDensityError_Pos;
-        backend.SetSimDomainBounds(mn, mx);
-        backend.PullFromUniversalSimState(state, params);
-        backend.ExecuteFullSimStep(params, static_cast<uint32_t>(steps));
-        tsp::Sync::UniversalSimState out;
-        backend.PushToUniversalSimState(out, true);
-        tsp::StatisticsPartial stats;
-        backend.FillStatisticsPartial(stats, params);
+        params.solverTerminationCond = SolverTerminationCond::RelDensityError_Pos;
+        backend->SetSimDomainBounds(vec3{ mn[0], mn[1], mn[2] }, vec3{ mx[0], mx[1], mx[2] });
+        backend->PullFromUniversalSimState(state, params);
+        backend->ExecuteNeighborSearch(params, true);
+        for (int s = 0; s < steps; s++) { backend->ExecuteFullSimStep(params, 1); }
+        Sync::UniversalSimState out;
+        backend->PushToUniversalSimState(out, false);
+
+        Stats::StatRecordStep stats;
+        backend->FillStatisticsRecord(stats, params);
+        Mem::MemRecordStep mem;
+        backend->FillMemoryRecord(mem);
+        Sync::ExportSnapshot exp;
+        backend->PackExportSnapshot(exp, params, true, std::nullopt);
+        Sync::VisualSnapshot vis;
+        vis.scalarModeF = Base::ColorCodingMode::VelocityMag;
+        backend->PackVisualSnapshot(vis, true, true);
+        Bench::BenchOpList ops;
+        backend->GetBenchOps(ops);
+
+        // the visual pack's positions and speeds are the pushed state's
+        auto& G = out.particlesF;
+        uint32_t visMismatch = vis.N_F == G.N_ && vis.scalarDataF.size() == G.N_ ? 0u : 1u;
+        for (uint32_t i = 0; !visMismatch && i < G.N_; i++)
+        {
+            const float sp = std::sqrt(G.xVel[i] * G.xVel[i] + G.yVel[i] * G.yVel[i] + G.zVel[i] * G.zVel[i]);
+            visMismatch += vis.posF[i].x != G.xPos[i] || vis.posF[i].y != G.yPos[i] || vis.posF[i].z != G.zPos[i]
+                           || vis.scalarDataF[i] != sp;
+        }
+        uint64_t vramBuckets = 0;
+        for (const auto& u : mem.mem[Mem::GetIdx(Mem::MemSpace::VRAM)]) { vramBuckets += u.N_allocBytes; }
 
         FILE* f = std::fopen(argv[1], "wb");
         if (!f) { std::fprintf(stderr, "cannot open %s\n", argv[1]); return 3; }
-        std::fwrite(out.particlesF.id.data(), sizeof(uint64_t), out.particlesF.N_, f);
-        auto& G = out.particlesF;
+        std::fwrite(G.id.data(), sizeof(uint64_t), G.N_, f);
         std::vector<float>* outF[TSP_F_COUNT] = { &G.xPos, &G.yPos, &G.zPos, &G.xVel, &G.yVel, &G.zVel, &G.xVelPred,
                                                   &G.yVelPred, &G.zVelPred, &G.xAccPress, &G.yAccPress, &G.zAccPress,
                                                   &G.sourceTerm, &G.diagElem, &G.pressure, &G.density };
         for (auto* v : outF) { std::fwrite(v->data(), sizeof(float), v->size(), f); }
         std::fclose(f);
-        std::printf("{\"backend\": \"%s\", \"n_fluid\": %u, \"solver_iters\": %u, \"nbr_ff_sum\": %llu, \"nbr_ff_max\": %u}\n",
-                    backend.Name(), G.N_, stats.N_solverIters, static_cast<unsigned long long>(stats.nbrCountFF.sum),
-                    stats.nbrCountFF.max);
+        std::printf("{\"backend\": \"%s\", \"n_fluid\": %u, \"n_boundary\": %u, \"stats\": [", backend->Name(),
+                    backend->GetNumFluidParticles(), backend->GetNumBoundaryParticles());
+        for (size_t q = 0; q < stats.stats.size(); q++) { std::printf("%s%.17g", q ? ", " : "", stats.stats[q]); }
+        std::printf("], \"stats_step\": %llu, \"export_fluid\": %u, \"export_boundary\": %u, \"export_rigid\": %u, "
+                    "\"visual_fluid\": %u, \"visual_mismatch\": %u, \"vram_buckets\": %llu, \"vram_alloc\": %llu, "
+                    "\"vram_peak\": %llu, \"bench_ops\": %zu, \"tracked_total\": %llu, \"tracked_query\": %llu, "
+                    "\"distributed\": %d}\n",
+                    static_cast<unsigned long long>(stats.simStep), exp.N_F, exp.N_B, exp.N_R, vis.N_F, visMismatch,
+                    static_cast<unsigned long long>(vramBuckets), static_cast<unsigned long long>(mem.device.N_allocBytes),
+                    static_cast<unsigned long long>(mem.device.N_peakAllocBytes), ops.size(),
+                    static_cast<unsigned long long>(tracker.Count(Bench::BenchOp::Total)),
+                    static_cast<unsigned long long>(tracker.Count(Bench::BenchOp::Neighbors_Query)),
+                    backend->GetDistributedParticleBackend() != nullptr);
         return 0;
     }
     catch (const std::exception& e)

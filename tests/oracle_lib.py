@@ -49,19 +49,21 @@ class OrcInfo(ctypes.Structure):
     ]
 
 
-_lib = None
+_libs = {}
 
 
 def build():
     subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(ORACLE_LIB):
+def lib(variant=None):
+    """The checker build (variant None), or an optimized build of the same source for the CPU
+    baseline: "v3" (-O3, AVX2) or "v4" (-O3, AVX-512)."""
+    if variant not in _libs:
+        path = ORACLE_LIB if variant is None else os.path.join(ORACLE_DIR, "_build", f"liboracle_{variant}.so")
+        if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(ORACLE_LIB)
+        L = ctypes.CDLL(path)
         vp = ctypes.c_void_p
         for p in ("orc32", "orc64"):
             sig = {
@@ -101,8 +103,8 @@ def lib():
                                                ctypes.POINTER(ctypes.c_int32), _u32p]
         L.orc_spread_bits_21to63.restype = ctypes.c_uint64
         L.orc_spread_bits_21to63.argtypes = [ctypes.c_uint32]
-        _lib = L
-    return _lib
+        _libs[variant] = L
+    return _libs[variant]
 
 
 def to_orc_params(p):
@@ -114,8 +116,9 @@ def to_orc_params(p):
 class Oracle:
     """One oracle context (prefix 'orc32' for the FP32 restatement, 'orc64' for FP64)."""
 
-    def __init__(self, params, *, precision=32, chunk_width=1, threads=None, rigid_f64=True, alpha=0.0):
-        self.L = lib()
+    def __init__(self, params, *, precision=32, chunk_width=1, threads=None, rigid_f64=True, alpha=0.0,
+                 variant=None):
+        self.L = lib(variant)
         self.p = f"orc{precision}"
         self.h = self.fn("create")()
         self.params = to_orc_params(params)

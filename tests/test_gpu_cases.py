@@ -9,8 +9,10 @@
   square root instead of the short exact one; one step must match the oracle bit for bit.
 - A block squeezed to 0.8 of its spacing (about 67 FF neighbors per particle) overflows the
   first pool size: the pools grow, and lists and one step must match the oracle bit for bit.
-- 35,937 particles inside one support radius: FF counts beyond the count storage stop the search
-  with TSP_ERR_CAPACITY before any list is written.
+- 68,921 particles inside one support radius: FF counts beyond the u16 count storage stop the
+  search with TSP_ERR_CAPACITY before any list is written, the particle order unchanged.
+- One particle with 33,000 FF neighbors (12 clusters around it): counts above 15 bits, and FF
+  lists beyond the reference's 256-entry dictionary, which take its normal formulation.
 - Warm start, and the four termination conditions, over several steps.
 - A rigid body in free fall follows the semi-implicit Euler closed form; a spinning body keeps an
   orthonormal rotation.
@@ -139,20 +141,72 @@ def test_lists_beyond_the_first_pool_size_grow_the_pools_bit_exact():
 
 
 def test_neighbor_count_beyond_the_count_storage_is_an_error():
-    """35,937 fluid particles packed within 0.002 of each other (h = 0.01): every FF count is about
-    36,000, beyond the 15-bit count storage (bit 15 of the u16 count marks wide-span lists; the
-    reference's u16 counts, neighbor_lists.h:43, stop at 65,535). The search must stop after the
-    count pass with TSP_ERR_CAPACITY, before any list is written."""
-    print("case: 35,937 particles inside one support radius (count overflow)", flush=True)
-    st = scenes.build_scene(scenes.dam_break_spec(0, H, counts=(33, 33, 33), tank=(0.3, 0.3, 0.3), jitter_frac=0.0))
+    """68,921 fluid particles packed within 0.002 of each other (h = 0.01): every FF count is
+    68,921, beyond the u16 count storage (the reference's u16 counts, neighbor_lists.h:43, stop at
+    65,535). The search must stop after the count pass with TSP_ERR_CAPACITY, before any list is
+    written, and leave the particles in their order before the step."""
+    print("case: 68,921 particles inside one support radius (count overflow)", flush=True)
+    st = scenes.build_scene(scenes.dam_break_spec(0, H, counts=(41, 41, 41), tank=(0.3, 0.3, 0.3), jitter_frac=0.0))
     rng = np.random.default_rng(3)
     for a, key in enumerate("xyz"):
         st.fluid[key][:] = (0.15 + rng.uniform(-0.001, 0.001, st.n[0])).astype(np.float32)
     g = HipBackend(0)
     g.set_params(scenes.make_params(H, termination_cond=scenes.TERM_POS))
+    g.pull(st)
+    before = g.push()
     with pytest.raises(RuntimeError, match=r"\(-4\)"):
-        g.pull(st)
         g.step(1)
+    after = g.push()
+    assert np.array_equal(after.fluid_id, before.fluid_id)
+    for f in FIELDS_STEP:
+        assert np.array_equal(after.fluid[f], before.fluid[f]), f
+    g.close()
+
+
+def _spots(n_per_spot=2750, seed=5):
+    """A fluid particle with 12 tiny clusters of n_per_spot particles around it at the vertices of
+    an icosahedron of radius 0.99 r (r = 2h): adjacent clusters are 1.04 r apart, so a cluster
+    particle has its own cluster and the centre as neighbors, and the centre has all of them."""
+    from turbospharticles_amd import _capi
+    phi = (1.0 + 5.0 ** 0.5) / 2.0
+    verts = np.array([(0, 1, phi), (0, -1, phi), (0, 1, -phi), (0, -1, -phi), (1, phi, 0), (-1, phi, 0),
+                      (1, -phi, 0), (-1, -phi, 0), (phi, 0, 1), (-phi, 0, 1), (phi, 0, -1), (-phi, 0, -1)], np.float64)
+    verts *= 0.99 * 2 * H / np.linalg.norm(verts[0])
+    rng = np.random.default_rng(seed)
+    centre = np.array([0.15, 0.15, 0.15])
+    pts = [centre] + [centre + v + rng.uniform(-2e-5, 2e-5, (n_per_spot, 3)) for v in verts]
+    pts = np.vstack(pts).astype(np.float32)
+    st = _capi.HostState(len(pts), 0, 0, 0)
+    st.fluid_id[:] = np.arange(len(pts), dtype=np.uint64)
+    for a, key in enumerate("xyz"):
+        st.fluid[key][:] = pts[:, a]
+    st.fluid["density"][:] = 1000.0
+    return st
+
+
+def test_neighbor_counts_between_32768_and_65535_and_lists_beyond_the_dictionary_bit_exact():
+    """The centre particle of 12 clusters has 33,000 FF neighbors (more than 15 bits), every
+    cluster particle 2,751: all lists exceed the reference's 256-entry dictionary, so every FF sum
+    takes its normal (per-neighbor scaled) formulation, and the centre's list spans more than
+    0xffff indices in candidate order. Lists and one step must match the oracle bit for bit."""
+    print("case: 12 clusters of 2,750 around one particle (33,000 neighbors), one step", flush=True)
+    st = _spots()
+    p = scenes.make_params(H, termination_cond=scenes.TERM_POS)
+    g, o = _pair(st, p)
+    g.neighbor_search()
+    o.neighbor_search()
+    cg, ig = g.neighbors(O.FF)
+    co, io, dict_flags = o.lists(O.FF)
+    assert cg.max() == 33_001 and not dict_flags.any()   # itself and the 33,000 cluster particles
+    assert np.array_equal(cg, co) and np.array_equal(ig, io)
+    g.step(1)
+    o.step()
+    sg, so = g.push(), o.push()
+    assert g.info().solver_iters == o.info().solver_iters
+    assert np.array_equal(sg.fluid_id, so.fluid_id)
+    for f in FIELDS_STEP:
+        assert np.array_equal(sg.fluid[f], so.fluid[f]), f
+    g.close()
 
 
 @pytest.mark.parametrize("cond,warm", [(scenes.TERM_RAW, 0.0), (scenes.TERM_ABS, 0.0), (scenes.TERM_FIXED, 0.0),
@@ -202,19 +256,24 @@ def test_spinning_body_orthonormal_on_gpu():
 
 
 def test_cpp_backend_matches_the_python_binding(tmp_path):
-    # the C++ SimBackend_HIP driver and the Python binding run the same scene through the same
-    # C ABI: three steps of a 12^3 dam break must agree bit for bit
-    print("case: C++ SimBackend_HIP round trip, 12^3, 3 steps", flush=True)
+    # the C++ backend, driven through an I_SimBackend* from MakeSimBackend, and the Python binding
+    # run the same scene through the same C ABI: a neighbor search and three steps of a 12^3 dam
+    # break must agree bit for bit, and every reporting method of the interface must return what
+    # the C ABI reports (statistics, export snapshot, memory record, visual pack, bench ops)
+    import json
+    print("case: C++ I_SimBackend round trip, 12^3, 3 steps", flush=True)
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     exe = os.path.join(root, "turbospharticles_amd", "lib", "sim_backend_roundtrip")
     out = tmp_path / "state.bin"
     r = subprocess.run([exe, str(out), "12", "3"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+    rep = json.loads(r.stdout)
     st = scenes.build_scene(scenes.dam_break_spec(12, H, tank=(0.4, 0.3, 0.3)))
     p = scenes.make_params(H, termination_cond=scenes.TERM_POS)
     g = HipBackend(0)
     g.set_params(p)
     g.pull(st)
+    g.neighbor_search()
     g.step(3)
     sg = g.push()
     n = st.n[0]
@@ -225,6 +284,25 @@ def test_cpp_backend_matches_the_python_binding(tmp_path):
     from turbospharticles_amd import _capi
     for k, f in enumerate(_capi.F_FIELDS):
         assert np.array_equal(fields[k], sg.fluid[f]), f
+    # FillStatisticsRecord: StatMetric order of eval/stats/metrics.h:23-86
+    s = g.statistics()
+    exp = [s.n_fluid, s.n_boundary, s.n_rigid]
+    for rel in range(6):
+        exp += [s.nbr_avg[rel], s.nbr_min[rel], s.nbr_max[rel], s.nbr_p99[rel]]
+    exp += list(s.dens_rel_err_avg) + [s.solver_iters, s.cfl, s.vel_avg, s.vel_min, s.vel_max]
+    assert rep["stats"] == pytest.approx(exp, rel=0, abs=0)
+    assert rep["stats_step"] == 3
+    snap = g.export_snapshot(include_boundary=True)
+    assert (rep["export_fluid"], rep["export_boundary"], rep["export_rigid"]) == \
+        (snap.fluid_vel_mag.size, st.n[1], 0)
+    assert rep["export_fluid"] > 0
+    assert rep["visual_fluid"] == n and rep["visual_mismatch"] == 0
+    assert rep["vram_buckets"] == rep["vram_alloc"] > 0 and rep["vram_peak"] >= rep["vram_alloc"]
+    m = g.memory()
+    assert m.alloc_bytes == sum(m.particles) + sum(m.search) + sum(m.lists) + m.misc
+    assert rep["bench_ops"] == 15 and rep["tracked_total"] == 3 and rep["tracked_query"] == 1
+    assert rep["distributed"] == 0
+    assert rep["n_fluid"] == n and rep["n_boundary"] == st.n[1]
 
 
 def test_statistics_on_the_device_match_the_state():

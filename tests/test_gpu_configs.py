@@ -58,12 +58,16 @@ def _steps_against_the_oracle(spec, steps, body_tol):
         o.step()
         gi, oi = g.info(), o.info()
         print(f"  step {s + 1}: iterations {gi.solver_iters} / {oi.solver_iters}, contact iterations "
-              f"{gi.contact_iters} / {oi.contact_iters}", flush=True)
+              f"{gi.contact_iters} / {oi.contact_iters}, contact boundary particles {gi.contact_boundary} / "
+              f"{oi.contact_boundary}", flush=True)
         assert gi.solver_iters == oi.solver_iters, s
         assert gi.contact_iters == oi.contact_iters, s
         assert gi.contact_converged == oi.contact_converged, s
         assert gi.contacts == oi.contacts, s
-        contact += gi.contact_iters > 0
+        assert gi.contact_boundary == oi.contact_boundary, s
+        # the contact solve ran (a resting body may converge in its first evaluation)
+        assert bool(gi.flags & 4) == (oi.n_rb + oi.n_rr > 0), s
+        contact += int(gi.flags & 4 != 0)
     sg, so = g.push(), o.push()
     g.close()
     assert np.array_equal(sg.fluid_id, so.fluid_id)

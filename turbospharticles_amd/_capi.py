@@ -103,6 +103,11 @@ class TspExportView(ctypes.Structure):
     ]
 
 
+class TspMemory(ctypes.Structure):
+    _fields_ = [("particles", ctypes.c_uint64 * 3), ("search", ctypes.c_uint64 * 3), ("lists", ctypes.c_uint64 * 3),
+                ("misc", ctypes.c_uint64), ("alloc_bytes", ctypes.c_uint64), ("peak_alloc_bytes", ctypes.c_uint64)]
+
+
 class TspStatistics(ctypes.Structure):
     _fields_ = [
         ("n_fluid", ctypes.c_uint32), ("n_boundary", ctypes.c_uint32), ("n_rigid", ctypes.c_uint32),
@@ -164,6 +169,7 @@ def _declare_hip(lib):
         "tsp_profile_read": (i, [vp, ctypes.c_char_p, _u64p, ctypes.POINTER(ctypes.c_double), i]),
         "tsp_synchronize": (i, [vp]),
         "tsp_get_statistics": (i, [vp, ctypes.POINTER(TspStatistics)]),
+        "tsp_get_memory": (i, [vp, ctypes.POINTER(TspMemory)]),
         "tsp_batch_scene_statistics": (i, [vp, ctypes.c_uint32, ctypes.POINTER(TspStatistics)]),
         "tsp_export_snapshot": (i, [vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(TspExportView)]),
         "tsp_batch_export_snapshot": (i, [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,

@@ -161,6 +161,12 @@ class HipBackend:
     def synchronize(self):
         self._check(self.lib.tsp_synchronize(self.h), "tsp_synchronize")
 
+    # I_SimBackend::FillMemoryRecord (device bytes by what they store)
+    def memory(self):
+        m = _capi.TspMemory()
+        self._check(self.lib.tsp_get_memory(self.h, ctypes.byref(m)), "tsp_get_memory")
+        return m
+
     # I_SimBackend::FillStatisticsRecord
     def statistics(self):
         st = _capi.TspStatistics()

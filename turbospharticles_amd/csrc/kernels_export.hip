@@ -21,7 +21,7 @@ namespace tsp
         const uint32_t local = blockIdx.x * kBlock + threadIdx.x;
         if (local >= n) { return; }
         const uint32_t i = off + local;
-        const uint32_t cFF = d.cnt[REL_FF][i] & 0x7fffu;
+        const uint32_t cFF = d.cnt[REL_FF][i];
         const uint32_t cFB = d.cnt[REL_FB][i], cFR = d.cnt[REL_FR][i];
         if (!(cFB > 0u || cFR > 0u || cFF < threshold)) { return; }
         flags[local] = 1u;

@@ -641,11 +641,12 @@ namespace tsp
         if (i < d.nF)
         {
             const bool wide = cnt[0] > 0 && mx[0] - mn[0] > 0xffffu;
-            d.cnt[REL_FF][i] = static_cast<uint16_t>(cnt[0] | (wide ? kWideSpanBit : 0u));
+            d.cnt[REL_FF][i] = static_cast<uint16_t>(cnt[0]);
             d.cnt[REL_FB][i] = static_cast<uint16_t>(cnt[1]);
             d.cnt[REL_FR][i] = static_cast<uint16_t>(cnt[2]);
-            if (wide) { d.ffWideMin[i] = mn[0]; }
-            if (cnt[0] >= kWideSpanBit || cnt[1] > 0xffffu || cnt[2] > 0xffffu) { atomicOr(&d.status->overflow, 2u); }
+            d.ffWideMin[i] = wide ? mn[0] : kEmpty32;
+            // u16 counts, as the reference's (neighbor_lists.h:43)
+            if (cnt[0] > 0xffffu || cnt[1] > 0xffffu || cnt[2] > 0xffffu) { atomicOr(&d.status->overflow, 2u); }
             if (tiny) { atomicOr(&d.status->pairTiny, 1u); }
         }
         // per slice: pitch, base (min index), format, entries
@@ -736,7 +737,7 @@ namespace tsp
         Stencil s;
         const bool active = i < d.nF && stencilOf(f, d.keyF[i], s);
         const uint32_t cF = i < d.nF ? d.cnt[REL_FF][i] : 0u;
-        const bool wide = (cF & kWideSpanBit) != 0u;
+        const bool wide = i < d.nF && d.ffWideMin[i] != kEmpty32;
         SliceMeta mF{}, mB{}, mR{};
         if (i < d.nF) { mF = d.meta[REL_FF][slice]; mB = d.meta[REL_FB][slice]; mR = d.meta[REL_FR][slice]; }
         const bool staged = !wide && mF.format == kFmtDelta16;
@@ -766,11 +767,12 @@ namespace tsp
             {
                 // DictDelta16x2 split of a per-particle dict (neighbor_lists.cpp:838-858): the neighbors
                 // within 0xffff of the smallest form the low group, the rest the high group, which is
-                // stored from the back when it fits 0xffff itself, else the candidate order stays
-                const uint32_t n = cF & 0x7fffu;
+                // stored from the back when it fits 0xffff itself, else the candidate order stays.
+                // A list of more than kDictMax entries is no dict: candidate order (pass 1 only).
+                const uint32_t n = cF;
                 const uint32_t minRef = d.ffWideMin[i];
                 uint32_t base2 = kEmpty32, maxRef = 0;
-                for (int pass = 0; pass < 3; pass++)
+                for (int pass = n > kDictMax ? 1 : 0; pass < 3; pass++)
                 {
                     uint32_t kLow = 0, kHigh = n;
                     for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
@@ -809,7 +811,7 @@ namespace tsp
         }
         // the staged FF entries leave as whole slice rows: column k of the 32 lanes per store
         __syncthreads();
-        const uint32_t own = (active && staged) ? min(cF & 0x7fffu, kStage) : 0u;
+        const uint32_t own = (active && staged) ? min(cF, kStage) : 0u;
         const uint32_t rows = sliceMax(own);
         uint16_t* row = d.pool[REL_FF] + mF.offset + lane;
         for (uint32_t k = 0; k < rows; k++)
@@ -852,7 +854,7 @@ namespace tsp
         SliceMeta m{};
         if (i < d.nF)
         {
-            n = d.cnt[REL_FF][i] & 0x7fffu;
+            n = d.cnt[REL_FF][i];
             m = d.meta[REL_FF][i / kSlice];
         }
         const uint32_t lane = i & (kSlice - 1);

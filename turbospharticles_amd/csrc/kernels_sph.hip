@@ -47,10 +47,23 @@ namespace tsp
         const float4 pi = d.pos[i];
         const float xi = pi.x, yi = pi.y, zi = pi.z;
         const List ff = listOf(d, REL_FF, i);
-        float wsum = 0.0f;
-        const auto ffTerm = [&](const float4& pj) { wsum += kwUnscaled<kFast>(dist2(xi - pj.x, yi - pj.y, zi - pj.z), k); };
-        visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(hp[q]); }, [&](uint32_t j, uint32_t) { ffTerm(d.pos[j]); });
-        const float dF = wsum * k.weightScale;
+        float dF;
+        if (ff.n <= kDictMax)
+        {
+            // dict chunk: unscaled weights, scaled once (sph_solver.cpp:797-810)
+            float wsum = 0.0f;
+            const auto ffTerm = [&](const float4& pj) { wsum += kwUnscaled<kFast>(dist2(xi - pj.x, yi - pj.y, zi - pj.z), k); };
+            visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(hp[q]); }, [&](uint32_t j, uint32_t) { ffTerm(d.pos[j]); });
+            dF = wsum * k.weightScale;
+        }
+        else
+        {
+            // normal chunk: the alpha-scaled weight per neighbor (sph_solver.cpp:826-831)
+            float s = 0.0f;
+            const auto ffTerm = [&](const float4& pj) { s = __builtin_fmaf(k.mass, kwNormal<kFast>(xi - pj.x, yi - pj.y, zi - pj.z, k), s); };
+            visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(hp[q]); }, [&](uint32_t j, uint32_t) { ffTerm(d.pos[j]); });
+            dF = s;
+        }
         float dB = 0.0f, dR = 0.0f;
         const List fb = listOf(d, REL_FB, i);
         visit(fb, [&](uint32_t j, uint32_t)
@@ -82,6 +95,8 @@ namespace tsp
         const float xi = pi.x, yi = pi.y, zi = pi.z, vxi = vi.x, vyi = vi.y, vzi = vi.z;
         float aFx = 0.0f, aFy = 0.0f, aFz = 0.0f;
         const List ff = listOf(d, REL_FF, i);
+        // dict chunks scale by the mass after the sum, normal chunks per neighbor (sph_solver.cpp:974-1049)
+        const bool dict = ff.n <= kDictMax;
         const auto ffTerm = [&](const float4& pj, const float4& vj)
         {
             const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
@@ -90,14 +105,14 @@ namespace tsp
             const float kg = kgFactor<kFast>(d2, k);
             const float dot = __builtin_fmaf(dvx, dx, __builtin_fmaf(dvy, dy, dvz * dz));
             const float rj = vmax(vj.w, 1e-7f);
-            const float coeff = dot / (rj * (d2 + k.hsqEps)) * kg;
+            const float coeff = (dict ? dot : k.mass * dot) / (rj * (d2 + k.hsqEps)) * kg;
             aFx = __builtin_fmaf(coeff, dx, aFx);
             aFy = __builtin_fmaf(coeff, dy, aFy);
             aFz = __builtin_fmaf(coeff, dz, aFz);
         };
         visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(hp[q], hv[q]); },
                 [&](uint32_t j, uint32_t) { ffTerm(d.pos[j], d.vel[j]); });
-        aFx *= k.mass; aFy *= k.mass; aFz *= k.mass;
+        if (dict) { aFx *= k.mass; aFy *= k.mass; aFz *= k.mass; }
         float aBx = 0.0f, aBy = 0.0f, aBz = 0.0f;
         const List fb = listOf(d, REL_FB, i);
         visit(fb, [&](uint32_t j, uint32_t e)
@@ -402,6 +417,10 @@ namespace tsp
         const float xi = p4.x, yi = p4.y, zi = p4.z, pi = p4.w;
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         const List ff = listOf(d, REL_FF, i);
+        // dict chunks sum (p_i + p_j) kg and scale once; normal chunks scale each neighbor's
+        // pressure term (sph_solver.cpp:1573-1640)
+        const bool dict = ff.n <= kDictMax;
+        const float outerF = k.mass * pi * k.presDis;
         const auto ffTerm = [&](const float4& pj, uint32_t e)
         {
             const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
@@ -411,14 +430,14 @@ namespace tsp
             (void)e;
             const float kg = kgFactor<kFast>(dist2(dx, dy, dz), k);
 #endif
-            const float coeff = (pi + pj.w) * kg;
+            const float coeff = (dict ? pi + pj.w : outerF + k.mass * pj.w * k.presDis) * kg;
             sx = __builtin_fmaf(coeff, dx, sx);
             sy = __builtin_fmaf(coeff, dy, sy);
             sz = __builtin_fmaf(coeff, dz, sz);
         };
         visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t e) { ffTerm(hp[q], e); },
                 [&](uint32_t j, uint32_t e) { ffTerm(d.pos[j], e); });
-        float ax = sx * k.presOuterFF, ay = sy * k.presOuterFF, az = sz * k.presOuterFF;
+        float ax = dict ? sx * k.presOuterFF : sx, ay = dict ? sy * k.presOuterFF : sy, az = dict ? sz * k.presOuterFF : sz;
         const List fb = listOf(d, REL_FB, i);
         visit(fb, [&](uint32_t j, uint32_t)
         {

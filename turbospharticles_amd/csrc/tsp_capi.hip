@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -343,18 +344,39 @@ struct tsp_ctx
         if (stream) { (void)hipStreamDestroy(stream); }
     }
 
+    // device memory the context holds, by what it stores (tsp_memory)
+    enum MemCat : uint8_t { MEM_PF, MEM_PB, MEM_PR, MEM_SF, MEM_SB, MEM_SR, MEM_LF, MEM_LR, MEM_MISC, kMemCats };
+    std::unordered_map<void*, std::pair<uint64_t, uint8_t>> allocated;
+    uint64_t memBytes[kMemCats] = {};
+    uint64_t memTotal = 0, memPeak = 0;
+
     template <typename T>
-    T* alloc(size_t n, std::vector<void*>& list)
+    T* alloc(size_t n, std::vector<void*>& list, MemCat cat = MEM_MISC)
     {
         void* p = nullptr;
-        hipCheck(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)), "hipMalloc");
+        const uint64_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        hipCheck(hipMalloc(&p, bytes), "hipMalloc");
         list.push_back(p);
+        allocated[p] = { bytes, static_cast<uint8_t>(cat) };
+        memBytes[cat] += bytes;
+        memTotal += bytes;
+        memPeak = std::max(memPeak, memTotal);
         return static_cast<T*>(p);
     }
 
-    static void releaseList(std::vector<void*>& list)
+    void releaseList(std::vector<void*>& list)
     {
-        for (void* p : list) { (void)hipFree(p); }
+        for (void* p : list)
+        {
+            const auto it = allocated.find(p);
+            if (it != allocated.end())
+            {
+                memBytes[it->second.second] -= it->second.first;
+                memTotal -= it->second.first;
+                allocated.erase(it);
+            }
+            (void)hipFree(p);
+        }
         list.clear();
     }
 
@@ -422,12 +444,12 @@ struct tsp_ctx
         for (int r = 0; r < 3; r++)
         {
             d.poolCap[r] = cap[r];
-            d.pool[r] = alloc<uint16_t>(cap[r], poolAllocs);
+            d.pool[r] = alloc<uint16_t>(cap[r], poolAllocs, MEM_LF);
         }
         d.rfCap = rfCap;
-        d.rfIdx = alloc<uint32_t>(rfCap, poolAllocs);
+        d.rfIdx = alloc<uint32_t>(rfCap, poolAllocs, MEM_LR);
 #if TSP_TUNE_KG_STORE
-        d.kgFF = alloc<float>(cap[REL_FF], poolAllocs);
+        d.kgFF = alloc<float>(cap[REL_FF], poolAllocs, MEM_LF);
 #else
         d.kgFF = nullptr;
 #endif
@@ -528,12 +550,12 @@ struct tsp_ctx
         releaseList(contactAllocs);
         d.rbCap = rbCap;
         d.rrCap = rrCap;
-        d.rbIdx = alloc<uint32_t>(rbCap, contactAllocs);
-        d.rrIdx = alloc<uint32_t>(rrCap, contactAllocs);
-        d.cbLocal = alloc<uint32_t>(rbCap, contactAllocs);
-        d.brOff = alloc<uint32_t>(rbCap + 1, contactAllocs);
-        d.brPos = alloc<uint32_t>(rbCap + 1, contactAllocs);
-        d.brIdx = alloc<uint32_t>(rbCap, contactAllocs);
+        d.rbIdx = alloc<uint32_t>(rbCap, contactAllocs, MEM_LR);
+        d.rrIdx = alloc<uint32_t>(rrCap, contactAllocs, MEM_LR);
+        d.cbLocal = alloc<uint32_t>(rbCap, contactAllocs, MEM_LR);
+        d.brOff = alloc<uint32_t>(rbCap + 1, contactAllocs, MEM_LR);
+        d.brPos = alloc<uint32_t>(rbCap + 1, contactAllocs, MEM_LR);
+        d.brIdx = alloc<uint32_t>(rbCap, contactAllocs, MEM_LR);
         d.cB = alloc<float>(static_cast<uint64_t>(kCB) * rbCap, contactAllocs);
         d.cErrB = alloc<double>(rbCap, contactAllocs);
         // block sums of the contact scans (boundary flags, BR offsets)
@@ -589,36 +611,36 @@ struct tsp_ctx
         try
         {
             n.nF = nF; n.nB = nB; n.nR = nR; n.nRB = nRB; n.nScenes = S;
-            n.pos = alloc<float4>(nF, A); npos2 = alloc<float4>(nF, A);
-            n.vel = alloc<float4>(nF, A); nvel2 = alloc<float4>(nF, A);
-            n.vp = alloc<float4>(nF, A); nvp2 = alloc<float4>(nF, A);
-            n.ap = alloc<float4>(nF, A); nap2 = alloc<float4>(nF, A);
-            n.source = alloc<float>(nF, A); nsource2 = alloc<float>(nF, A);
-            n.diag = alloc<float>(nF, A); ndiag2 = alloc<float>(nF, A);
-            n.id = alloc<uint64_t>(nF, A); nidF2 = alloc<uint64_t>(nF, A);
+            n.pos = alloc<float4>(nF, A, MEM_PF); npos2 = alloc<float4>(nF, A, MEM_PF);
+            n.vel = alloc<float4>(nF, A, MEM_PF); nvel2 = alloc<float4>(nF, A, MEM_PF);
+            n.vp = alloc<float4>(nF, A, MEM_PF); nvp2 = alloc<float4>(nF, A, MEM_PF);
+            n.ap = alloc<float4>(nF, A, MEM_PF); nap2 = alloc<float4>(nF, A, MEM_PF);
+            n.source = alloc<float>(nF, A, MEM_PF); nsource2 = alloc<float>(nF, A, MEM_PF);
+            n.diag = alloc<float>(nF, A, MEM_PF); ndiag2 = alloc<float>(nF, A, MEM_PF);
+            n.id = alloc<uint64_t>(nF, A, MEM_PF); nidF2 = alloc<uint64_t>(nF, A, MEM_PF);
             // the padding slots of the fluid segments hold zeros
             zero(n.pos, sizeof(float4) * nF); zero(n.vel, sizeof(float4) * nF);
             zero(n.vp, sizeof(float4) * nF); zero(n.ap, sizeof(float4) * nF);
             zero(n.source, sizeof(float) * nF); zero(n.diag, sizeof(float) * nF);
             zero(n.id, sizeof(uint64_t) * nF);
-            n.bpos = alloc<float4>(nB, A); nbpos2 = alloc<float4>(nB, A);
-            n.bvol = alloc<float>(nB, A); nbvol2 = alloc<float>(nB, A);
-            nidB = alloc<uint64_t>(nB, A); nidB2 = alloc<uint64_t>(nB, A);
-            n.rxrel = alloc<float>(nR, A); n.ryrel = alloc<float>(nR, A); n.rzrel = alloc<float>(nR, A);
-            n.rpos = alloc<float4>(nR, A); n.rvel = alloc<float4>(nR, A); n.rvp = alloc<float4>(nR, A);
-            n.rfx = alloc<float>(nR, A); n.rfy = alloc<float>(nR, A); n.rfz = alloc<float>(nR, A);
-            n.rbody = alloc<uint32_t>(nR, A);
-            nidR = alloc<uint64_t>(nR, A);
-            n.rb = alloc<float>(static_cast<size_t>(TSP_RB_COUNT) * nRB1, A);
-            n.rbPred = alloc<float>(9 * nRB1, A);
-            n.rbForce = alloc<float>(6 * nRB1, A);
+            n.bpos = alloc<float4>(nB, A, MEM_PB); nbpos2 = alloc<float4>(nB, A, MEM_PB);
+            n.bvol = alloc<float>(nB, A, MEM_PB); nbvol2 = alloc<float>(nB, A, MEM_PB);
+            nidB = alloc<uint64_t>(nB, A, MEM_PB); nidB2 = alloc<uint64_t>(nB, A, MEM_PB);
+            n.rxrel = alloc<float>(nR, A, MEM_PR); n.ryrel = alloc<float>(nR, A, MEM_PR); n.rzrel = alloc<float>(nR, A, MEM_PR);
+            n.rpos = alloc<float4>(nR, A, MEM_PR); n.rvel = alloc<float4>(nR, A, MEM_PR); n.rvp = alloc<float4>(nR, A, MEM_PR);
+            n.rfx = alloc<float>(nR, A, MEM_PR); n.rfy = alloc<float>(nR, A, MEM_PR); n.rfz = alloc<float>(nR, A, MEM_PR);
+            n.rbody = alloc<uint32_t>(nR, A, MEM_PR);
+            nidR = alloc<uint64_t>(nR, A, MEM_PR);
+            n.rb = alloc<float>(static_cast<size_t>(TSP_RB_COUNT) * nRB1, A, MEM_PR);
+            n.rbPred = alloc<float>(9 * nRB1, A, MEM_PR);
+            n.rbForce = alloc<float>(6 * nRB1, A, MEM_PR);
             zero(n.rbPred, sizeof(float) * 9 * nRB1);
             zero(n.rbForce, sizeof(float) * 6 * nRB1);
-            n.byBodyOff = alloc<uint32_t>(nRB + 1ull, A);
-            n.byBodyIdx = alloc<uint32_t>(nR, A);
-            n.blockScene = alloc<uint32_t>(nF / kBlock, A);
-            n.bodyScene = alloc<uint32_t>(nRB, A);
-            n.sceneB = alloc<uint16_t>(nB, A);
+            n.byBodyOff = alloc<uint32_t>(nRB + 1ull, A, MEM_PR);
+            n.byBodyIdx = alloc<uint32_t>(nR, A, MEM_PR);
+            n.blockScene = alloc<uint32_t>(nF / kBlock, A, MEM_PF);
+            n.bodyScene = alloc<uint32_t>(nRB, A, MEM_PR);
+            n.sceneB = alloc<uint16_t>(nB, A, MEM_PB);
 
             const auto field = [](const float* f, uint32_t i, float dflt) { return f ? f[i] : dflt; };
             std::vector<uint64_t> seq;
@@ -788,22 +810,22 @@ struct tsp_ctx
         // buffers sized by the layout
         releaseList(work);
         const uint32_t nSort = std::max(nF, std::max(nB, nR));
-        d.keyF = alloc<uint32_t>(nSort, work);
-        d.keyF2 = alloc<uint32_t>(nSort, work);
-        d.idxF = alloc<uint32_t>(nSort, work);
-        d.idxF2 = alloc<uint32_t>(nSort, work);
-        d.keyR = alloc<uint32_t>(nR, work);
-        d.keyR2 = alloc<uint32_t>(nR, work);
-        d.idxR = alloc<uint32_t>(nR, work);
-        d.idxR2 = alloc<uint32_t>(nR, work);
+        d.keyF = alloc<uint32_t>(nSort, work, MEM_SF);
+        d.keyF2 = alloc<uint32_t>(nSort, work, MEM_SF);
+        d.idxF = alloc<uint32_t>(nSort, work, MEM_SF);
+        d.idxF2 = alloc<uint32_t>(nSort, work, MEM_SF);
+        d.keyR = alloc<uint32_t>(nR, work, MEM_SR);
+        d.keyR2 = alloc<uint32_t>(nR, work, MEM_SR);
+        d.idxR = alloc<uint32_t>(nR, work, MEM_SR);
+        d.idxR2 = alloc<uint32_t>(nR, work, MEM_SR);
         const uint64_t nTiles = (nSort + 2047ull) / 2048ull;
         radixHist = alloc<uint32_t>(256ull * nTiles, work);
         aabb = alloc<uint32_t>(6ull * S, work);
         const uint32_t nSlices = (nF + kSlice - 1) / kSlice;
         for (int r = 0; r < 3; r++)
         {
-            d.cnt[r] = alloc<uint16_t>(nF, work);
-            d.meta[r] = alloc<SliceMeta>(nSlices, work);
+            d.cnt[r] = alloc<uint16_t>(nF, work, MEM_LF);
+            d.meta[r] = alloc<SliceMeta>(nSlices, work, MEM_LF);
             zero(d.cnt[r], sizeof(uint16_t) * nF);
         }
         sliceUnits = alloc<uint32_t>(3ull * (nSlices + 1), work);
@@ -812,23 +834,23 @@ struct tsp_ctx
             const uint64_t m = std::max<uint64_t>(256ull * nTiles, std::max<uint64_t>(nSlices + 1ull, nR + 1ull));
             d.scanTmp = alloc<uint32_t>((m + 2047ull) / 2048ull + 1ull, work);
         }
-        d.ffWideMin = alloc<uint32_t>(nF, work);
+        d.ffWideMin = alloc<uint32_t>(nF, work, MEM_LF);
         {
             const uint64_t nBlocks = nF / kBlock;
-            d.haloIdx = alloc<uint32_t>(nBlocks * kHaloMax, work);
-            d.haloCount = alloc<uint32_t>(nBlocks, work);
+            d.haloIdx = alloc<uint32_t>(nBlocks * kHaloMax, work, MEM_LF);
+            d.haloCount = alloc<uint32_t>(nBlocks, work, MEM_LF);
             // no block has a halo until the first search builds one
             hipCheck(hipMemsetAsync(d.haloCount, 0xff, sizeof(uint32_t) * std::max<uint64_t>(nBlocks, 1), stream), "memset");
         }
-        d.rfCnt = alloc<uint32_t>(nR, work);
-        d.rbCnt = alloc<uint32_t>(nR, work);
-        d.rrCnt = alloc<uint32_t>(nR, work);
+        d.rfCnt = alloc<uint32_t>(nR, work, MEM_LR);
+        d.rbCnt = alloc<uint32_t>(nR, work, MEM_LR);
+        d.rrCnt = alloc<uint32_t>(nR, work, MEM_LR);
         zero(d.rfCnt, sizeof(uint32_t) * nR);
         zero(d.rbCnt, sizeof(uint32_t) * nR);
         zero(d.rrCnt, sizeof(uint32_t) * nR);
-        d.rfOff = alloc<uint32_t>(nR + 1ull, work);
-        d.rbOff = alloc<uint32_t>(nR + 1ull, work);
-        d.rrOff = alloc<uint32_t>(nR + 1ull, work);
+        d.rfOff = alloc<uint32_t>(nR + 1ull, work, MEM_LR);
+        d.rbOff = alloc<uint32_t>(nR + 1ull, work, MEM_LR);
+        d.rrOff = alloc<uint32_t>(nR + 1ull, work, MEM_LR);
         d.cbScan = alloc<uint32_t>(nB + 1ull, work);
         zero(d.cbScan, sizeof(uint32_t) * (nB + 1ull));
         d.cRw = alloc<float>(3ull * nR, work);
@@ -1045,9 +1067,9 @@ struct tsp_ctx
         {
             releaseList(frameAllocs);
             cellCap = std::max<uint64_t>(cells, 1);
-            d.cellF = alloc<uint2>(cellCap, frameAllocs);
-            d.cellB = alloc<uint2>(cellCap, frameAllocs);
-            d.cellR = alloc<uint2>(cellCap, frameAllocs);
+            d.cellF = alloc<uint2>(cellCap, frameAllocs, MEM_SF);
+            d.cellB = alloc<uint2>(cellCap, frameAllocs, MEM_SB);
+            d.cellR = alloc<uint2>(cellCap, frameAllocs, MEM_SR);
         }
         launchClearAll(stream, d.cellF, cells);
         launchClearAll(stream, d.cellB, cells);
@@ -1551,7 +1573,7 @@ struct tsp_ctx
         const uint16_t* c16[3] = { d.cnt[REL_FF] + h.offF, d.cnt[REL_FB] + h.offF, d.cnt[REL_FR] + h.offF };
         for (int r = 0; r < 3; r++)
         {
-            launchStatsCounts(stream, c16[r], nullptr, r == REL_FF ? 0x7fffu : 0xffffu, h.nF, statHist, statCounts + 4 * r);
+            launchStatsCounts(stream, c16[r], nullptr, 0xffffu, h.nF, statHist, statCounts + 4 * r);
         }
         const uint32_t* c32[3] = { d.rfCnt + h.offR, d.rbCnt + h.offR, d.rrCnt + h.offR };
         for (int r = 0; r < 3; r++) { launchStatsCounts(stream, nullptr, c32[r], 0u, h.nR, statHist, statCounts + 4 * (3 + r)); }
@@ -1967,7 +1989,7 @@ int tsp_get_neighbors(tsp_ctx* ctx, int rel, uint32_t* counts, uint32_t* idx, ui
             hipCheck(hipMemcpy(cnt.data(), d.cnt[rel] + h.offF, sizeof(uint16_t) * n, hipMemcpyDeviceToHost), "D2H");
             hipCheck(hipMemcpy(meta.data(), d.meta[rel] + s0, sizeof(SliceMeta) * nSlices, hipMemcpyDeviceToHost), "D2H");
         }
-        const uint32_t mask = rel == 0 ? 0x7fffu : 0xffffu;
+        const uint32_t mask = 0xffffu;
         uint64_t tot = 0;
         for (uint32_t i = 0; i < n; i++) { tot += cnt[i] & mask; if (counts) { counts[i] = cnt[i] & mask; } }
         if (total) { *total = tot; }
@@ -2106,6 +2128,25 @@ int tsp_get_statistics(tsp_ctx* ctx, tsp_statistics* out)
         if (!out) { throw Error{ TSP_ERR_INVALID, "null statistics" }; }
         ctx->requireScene();
         ctx->statistics(0, out);
+        return TSP_OK;
+    });
+}
+
+int tsp_get_memory(tsp_ctx* ctx, tsp_memory* out)
+{
+    return guarded(ctx, [&]() {
+        if (!out) { throw Error{ TSP_ERR_INVALID, "null memory record" }; }
+        std::memset(out, 0, sizeof(*out));
+        for (int c = 0; c < 3; c++)
+        {
+            out->particles[c] = ctx->memBytes[tsp_ctx::MEM_PF + c];
+            out->search[c] = ctx->memBytes[tsp_ctx::MEM_SF + c];
+        }
+        out->lists[0] = ctx->memBytes[tsp_ctx::MEM_LF];
+        out->lists[2] = ctx->memBytes[tsp_ctx::MEM_LR];
+        out->misc = ctx->memBytes[tsp_ctx::MEM_MISC];
+        out->alloc_bytes = ctx->memTotal;
+        out->peak_alloc_bytes = ctx->memPeak;
         return TSP_OK;
     });
 }

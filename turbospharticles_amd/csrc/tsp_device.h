@@ -87,7 +87,10 @@ namespace tsp
     constexpr uint32_t kEmpty32 = 0xffffffffu;
     constexpr uint32_t kFmtDelta16 = 0;
     constexpr uint32_t kFmtAbs32 = 1;
-    constexpr uint32_t kWideSpanBit = 0x8000u; // FF count flag: neighbors span more than 0xffff
+    // FF lists of more than kDictMax entries take the reference's normal (per-neighbor scaled)
+    // formulation: its per-particle chunks become dict chunks only up to 256 distinct neighbors
+    // (N_SELL_DICT_SIZE_MAX, neighbor_lists.cpp:326-332)
+    constexpr uint32_t kDictMax = 256;
     // FF neighbor halo of a 256-particle block: the distinct FF neighbors of its particles,
     // ascending; the FF entries of the block then hold halo slots instead of particle indices
     constexpr uint32_t kHaloMax = TSP_TUNE_HALO_MAX; // 24 B x 1000 records in kJacobi: six blocks per CU
@@ -293,7 +296,7 @@ namespace tsp
         uint16_t *pool[3];
         uint64_t poolCap[3];     // u16 units
         float *kgFF;             // per FF entry (by its u16 position): gradient factor of the step (build variant)
-        uint32_t *ffWideMin;     // per particle: min FF index when the wide-span bit is set
+        uint32_t *ffWideMin;     // per particle: min FF index of a list spanning more than 0xffff, else kEmpty32
         uint32_t *haloIdx;       // per block: kHaloMax particle indices, ascending
         uint32_t *haloCount;     // per block: halo size, or kNoHalo
         // rigid-source CSR (RF), counts of RB and RR
@@ -527,7 +530,7 @@ namespace tsp
         l.pool = d.pool[rel];
         l.m = d.meta[rel][i / kSlice];
         l.lane = i & (kSlice - 1);
-        l.n = d.cnt[rel][i] & (rel == REL_FF ? 0x7fffu : 0xffffu);
+        l.n = d.cnt[rel][i];
         return l;
     }
 

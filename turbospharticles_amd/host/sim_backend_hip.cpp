@@ -1,9 +1,11 @@
-// sim_backend_hip.cpp -- SimBackend_HIP over the C ABI of libtsp_hip.so.
+// sim_backend_hip.cpp -- SimBackend_HIP over the C ABI of libtsp_hip.so, and MakeSimBackend.
 //
 // The state crosses the boundary without staging: a tsp_state_view points into the vectors of
 // the universal state, and the library copies them to and from the device.
 #include "sim_backend_hip.h"
 
+#include <chrono>
+#include <cmath>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -95,7 +97,7 @@ namespace tsp
         void ResizeAll(uint32_t n, V&... vs) { (vs.resize(n), ...); }
     }
 
-    SimBackend_HIP::SimBackend_HIP(int device)
+    SimBackend_HIP::SimBackend_HIP(int device, Bench::BenchmarksTracker* tracker) : tracker_(tracker)
     {
         const int rc = tsp_create(device, &ctx_);
         if (rc != TSP_OK) { throw std::runtime_error("SimBackend_HIP: no usable gfx950 device (tsp_create " + std::to_string(rc) + ")"); }
@@ -119,9 +121,11 @@ namespace tsp
         paramsSet_ = true;
     }
 
-    void SimBackend_HIP::SetSimDomainBounds(const float min[3], const float max[3])
+    void SimBackend_HIP::SetSimDomainBounds(const vec3& min, const vec3& max)
     {
-        Check(tsp_set_domain(ctx_, min, max), "SimBackend_HIP::SetSimDomainBounds");
+        const float mn[3] = { min.x, min.y, min.z };
+        const float mx[3] = { max.x, max.y, max.z };
+        Check(tsp_set_domain(ctx_, mn, mx), "SimBackend_HIP::SetSimDomainBounds");
     }
 
     void SimBackend_HIP::PullFromUniversalSimState(Sync::UniversalSimState& src, const SimParams& params)
@@ -140,6 +144,7 @@ namespace tsp
         tsp_state_view v = View(src);
         Check(tsp_pull(ctx_, &v), "SimBackend_HIP::PullFromUniversalSimState");
         n_[0] = F.N_; n_[1] = B.N_; n_[2] = R.N_; n_[3] = RB.N_;
+        cacheValid_ = false;
     }
 
     void SimBackend_HIP::PushToUniversalSimState(Sync::UniversalSimState& dst, bool backendWillBeRetired)
@@ -164,48 +169,249 @@ namespace tsp
         dst.simTime = v.time;
     }
 
+    const Sync::UniversalSimState& SimBackend_HIP::Current() const
+    {
+        tsp_step_info info{};
+        Check(tsp_get_step_info(ctx_, &info), "SimBackend_HIP::Current");
+        if (!cacheValid_ || cache_.simStep != info.step)
+        {
+            const_cast<SimBackend_HIP*>(this)->PushToUniversalSimState(cache_, false);
+            cacheValid_ = true;
+        }
+        return cache_;
+    }
+
+    void SimBackend_HIP::PackExportSnapshot(Sync::ExportSnapshot& snap, const SimParams& params, bool includeB,
+                                            std::optional<uint64_t> meshIdBeginB) const
+    {
+        const uint64_t mesh = meshIdBeginB ? *meshIdBeginB : UINT64_MAX;
+        std::vector<float> fxyz, fvel, bxyz, bvol, rxyz, rvol;
+        std::vector<uint8_t> bmesh;
+        std::vector<uint32_t> rbody;
+        bxyz.resize(includeB ? 3ull * n_[1] : 0);
+        bvol.resize(includeB ? n_[1] : 0);
+        bmesh.resize(includeB ? n_[1] : 0);
+        rxyz.resize(3ull * n_[2]);
+        rvol.resize(n_[2]);
+        rbody.resize(n_[2]);
+        tsp_export_view v{};
+        uint32_t cap = n_[0] / 4 + 1024;
+        for (int attempt = 0; attempt < 2; attempt++)
+        {
+            fxyz.resize(3ull * cap);
+            fvel.resize(cap);
+            v = tsp_export_view{};
+            v.cap_fluid = cap;
+            v.fluid_xyz = fxyz.data();
+            v.fluid_vel_mag = fvel.data();
+            v.boundary_xyz = includeB ? bxyz.data() : nullptr;
+            v.boundary_vol = includeB ? bvol.data() : nullptr;
+            v.boundary_is_mesh = includeB ? bmesh.data() : nullptr;
+            v.rigid_xyz = rxyz.data();
+            v.rigid_vol = rvol.data();
+            v.rigid_body = rbody.data();
+            const int rc = tsp_export_snapshot(ctx_, params.surfaceNbrCountThreshold, mesh, &v);
+            if (rc == TSP_ERR_CAPACITY && v.n_fluid > cap && attempt == 0) { cap = v.n_fluid; continue; }
+            Check(rc, "SimBackend_HIP::PackExportSnapshot");
+            break;
+        }
+        snap = Sync::ExportSnapshot{};
+        snap.simStep = v.step;
+        snap.simTime = v.time;
+        snap.N_F = v.n_fluid;
+        snap.xPosF.resize(v.n_fluid); snap.yPosF.resize(v.n_fluid); snap.zPosF.resize(v.n_fluid);
+        snap.velMagF.assign(fvel.begin(), fvel.begin() + v.n_fluid);
+        for (uint32_t i = 0; i < v.n_fluid; i++)
+        {
+            snap.xPosF[i] = fxyz[3ull * i]; snap.yPosF[i] = fxyz[3ull * i + 1]; snap.zPosF[i] = fxyz[3ull * i + 2];
+        }
+        snap.N_B = v.n_boundary;
+        if (includeB)
+        {
+            snap.xPosB.resize(v.n_boundary); snap.yPosB.resize(v.n_boundary); snap.zPosB.resize(v.n_boundary);
+            for (uint32_t i = 0; i < v.n_boundary; i++)
+            {
+                snap.xPosB[i] = bxyz[3ull * i]; snap.yPosB[i] = bxyz[3ull * i + 1]; snap.zPosB[i] = bxyz[3ull * i + 2];
+            }
+            snap.volB = bvol;
+            snap.isMeshB = bmesh;
+        }
+        snap.N_R = v.n_rigid;
+        snap.bodyIdR = rbody;
+        snap.volR = rvol;
+        snap.xPosR.resize(v.n_rigid); snap.yPosR.resize(v.n_rigid); snap.zPosR.resize(v.n_rigid);
+        for (uint32_t i = 0; i < v.n_rigid; i++)
+        {
+            snap.xPosR[i] = rxyz[3ull * i]; snap.yPosR[i] = rxyz[3ull * i + 1]; snap.zPosR[i] = rxyz[3ull * i + 2];
+        }
+    }
+
+    void SimBackend_HIP::PackVisualSnapshot(Sync::VisualSnapshot& snap, bool includeF, bool includeR) const
+    {
+        const Sync::UniversalSimState& s = Current();
+        const auto& F = s.particlesF;
+        snap.simStep = s.simStep;
+        snap.N_F = includeF ? F.N_ : 0;
+        snap.posF.resize(snap.N_F);
+        for (uint32_t i = 0; i < snap.N_F; i++) { snap.posF[i] = vec3{ F.xPos[i], F.yPos[i], F.zPos[i] }; }
+        if (includeF) { FillScalarData(snap, snap.scalarModeF); }
+        else { snap.scalarDataF.clear(); }
+        snap.N_R = includeR ? n_[2] : 0;
+        snap.posR.resize(snap.N_R);
+        snap.volR.resize(snap.N_R);
+        if (snap.N_R)
+        {
+            std::vector<float> xyz(3ull * n_[2]), vel(3ull * n_[2]);
+            Check(tsp_get_rigid_world(ctx_, xyz.data(), vel.data()), "SimBackend_HIP::PackVisualSnapshot");
+            for (uint32_t i = 0; i < snap.N_R; i++)
+            {
+                snap.posR[i] = vec3{ xyz[3ull * i], xyz[3ull * i + 1], xyz[3ull * i + 2] };
+                snap.volR[i] = s.particlesR.volR[i];
+            }
+        }
+        // PackVisualRigidBodyState: centre of mass and the rotation's columns
+        const auto& RB = s.rigidBodies;
+        snap.N_RB = includeR ? RB.N_ : 0;
+        snap.centerOfMassRB.resize(snap.N_RB);
+        snap.orientationRB.resize(snap.N_RB);
+        for (uint32_t b = 0; b < snap.N_RB; b++)
+        {
+            snap.centerOfMassRB[b] = vec3{ RB.xCM[b], RB.yCM[b], RB.zCM[b] };
+            snap.orientationRB[b].col[0] = vec3{ RB.a00[b], RB.a10[b], RB.a20[b] };
+            snap.orientationRB[b].col[1] = vec3{ RB.a01[b], RB.a11[b], RB.a21[b] };
+            snap.orientationRB[b].col[2] = vec3{ RB.a02[b], RB.a12[b], RB.a22[b] };
+        }
+    }
+
+    void SimBackend_HIP::FillScalarData(Sync::VisualSnapshot& snap, Base::ColorCodingMode mode) const
+    {
+        snap.scalarModeF = mode;
+        if (n_[0] == 0 || mode == Base::ColorCodingMode::None)
+        {
+            snap.scalarDataF.clear();
+            return;
+        }
+        const auto& F = Current().particlesF;
+        snap.scalarDataF.resize(F.N_);
+        for (uint32_t i = 0; i < F.N_; i++)
+        {
+            float v = 0.0f;
+            switch (mode)
+            {
+            case Base::ColorCodingMode::VelocityMag:
+                v = std::sqrt(F.xVel[i] * F.xVel[i] + F.yVel[i] * F.yVel[i] + F.zVel[i] * F.zVel[i]);
+                break;
+            case Base::ColorCodingMode::Density: v = F.density[i]; break;
+            case Base::ColorCodingMode::Pressure: v = F.pressure[i]; break;
+            default: break;
+            }
+            snap.scalarDataF[i] = v;
+        }
+    }
+
+    void SimBackend_HIP::FillMemoryRecord(Mem::MemRecordStep& record) const
+    {
+        using namespace Mem;
+        tsp_memory m{};
+        Check(tsp_get_memory(ctx_, &m), "SimBackend_HIP::FillMemoryRecord");
+        tsp_step_info info{};
+        Check(tsp_get_step_info(ctx_, &info), "SimBackend_HIP::FillMemoryRecord");
+        record.simStep = info.step;
+        record.simTime = info.time;
+        const auto set = [&](MemBucket b, uint64_t bytes) { Set(record, MemSpace::VRAM, b, MemUsage{ bytes, bytes }); };
+        set(MemBucket::ParticlePropertiesF, m.particles[0]);
+        set(MemBucket::ParticlePropertiesB, m.particles[1]);
+        set(MemBucket::ParticlePropertiesR, m.particles[2]);
+        set(MemBucket::Neighbors_SearchDataStructF, m.search[0]);
+        set(MemBucket::Neighbors_SearchDataStructB, m.search[1]);
+        set(MemBucket::Neighbors_SearchDataStructR, m.search[2]);
+        set(MemBucket::Neighbors_ListStorageF, m.lists[0]);
+        set(MemBucket::Neighbors_ListStorageB, m.lists[1]);
+        set(MemBucket::Neighbors_ListStorageR, m.lists[2]);
+        set(MemBucket::Misc, m.misc);
+        record.device.N_allocBytes = m.alloc_bytes;
+        record.device.N_peakAllocBytes = m.peak_alloc_bytes;
+        // host: the copy of the state this adapter keeps for visual packs
+        uint64_t host = 0;
+        if (cacheValid_)
+        {
+            const auto& F = cache_.particlesF;
+            host = F.id.capacity() * sizeof(uint64_t) + 16ull * F.xPos.capacity() * sizeof(float);
+        }
+        Set(record, MemSpace::RAM, MemBucket::ParticlePropertiesF, MemUsage{ host, host });
+    }
+
+    void SimBackend_HIP::FillStatisticsRecord(Stats::StatRecordStep& record, const SimParams& params) const
+    {
+        using namespace Stats;
+        (void)params;
+        tsp_statistics st{};
+        Check(tsp_get_statistics(ctx_, &st), "SimBackend_HIP::FillStatisticsRecord");
+        tsp_step_info info{};
+        Check(tsp_get_step_info(ctx_, &info), "SimBackend_HIP::FillStatisticsRecord");
+        record.simStep = info.step;
+        record.simTime = info.time;
+        Set(record, StatMetric::NumF, st.n_fluid);
+        Set(record, StatMetric::NumB, st.n_boundary);
+        Set(record, StatMetric::NumR, st.n_rigid);
+        for (int r = 0; r < 6; r++)
+        {
+            const uint8_t base = static_cast<uint8_t>(GetIdx(StatMetric::NumNbrFF_Avg) + 4 * r);
+            record.stats[base + 0] = st.nbr_avg[r];
+            record.stats[base + 1] = st.nbr_min[r];
+            record.stats[base + 2] = st.nbr_max[r];
+            record.stats[base + 3] = st.nbr_p99[r];
+        }
+        Set(record, StatMetric::DensRelError_Raw_Avg, st.dens_rel_err_avg[0]);
+        Set(record, StatMetric::DensRelError_Pos_Avg, st.dens_rel_err_avg[1]);
+        Set(record, StatMetric::DensRelError_Abs_Avg, st.dens_rel_err_avg[2]);
+        Set(record, StatMetric::NumSolverIters, st.solver_iters);
+        Set(record, StatMetric::CFL_Number, st.cfl);
+        Set(record, StatMetric::VelMag_Avg, st.vel_avg);
+        Set(record, StatMetric::VelMag_Min, st.vel_min);
+        Set(record, StatMetric::VelMag_Max, st.vel_max);
+    }
+
     void SimBackend_HIP::ExecuteNeighborSearch(const SimParams& params, bool bench)
     {
-        (void)bench;
         ApplyParams(params);
+        const auto t0 = std::chrono::steady_clock::now();
         Check(tsp_neighbor_search(ctx_), "SimBackend_HIP::ExecuteNeighborSearch");
+        Check(tsp_synchronize(ctx_), "SimBackend_HIP::ExecuteNeighborSearch");
+        if (bench && tracker_)
+        {
+            const auto ns = std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count();
+            tracker_->Record(Bench::BenchOp::Neighbors_Query, static_cast<uint64_t>(ns));
+        }
+        cacheValid_ = false;
     }
 
     void SimBackend_HIP::ExecuteFullSimStep(const SimParams& params, uint32_t N_steps)
     {
         ApplyParams(params);
+        const auto t0 = std::chrono::steady_clock::now();
         Check(tsp_step(ctx_, N_steps), "SimBackend_HIP::ExecuteFullSimStep");
+        cacheValid_ = false;
+        if (tracker_ && N_steps)
+        {
+            const auto ns = std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count();
+            tracker_->Record(Bench::BenchOp::Total, static_cast<uint64_t>(ns) / N_steps);
+        }
     }
 
-    bool SimBackend_HIP::FillStatisticsPartial(StatisticsPartial& partial, const SimParams& params) const
+    namespace Sim
     {
-        (void)params;
-        partial = StatisticsPartial{};
-        tsp_step_info info{};
-        Check(tsp_get_step_info(ctx_, &info), "SimBackend_HIP::FillStatisticsPartial");
-        partial.N_F = n_[0]; partial.N_B = n_[1]; partial.N_R = n_[2];
-        partial.N_solverIters = info.solver_iters;
-        partial.densRelErrorRawSum = info.err_sum_raw;
-        partial.densRelErrorPosSum = info.err_sum_pos;
-        partial.densRelErrorAbsSum = info.err_sum_abs;
-        DistributionPartial* dist[6] = { &partial.nbrCountFF, &partial.nbrCountFB, &partial.nbrCountFR,
-                                         &partial.nbrCountRF, &partial.nbrCountRB, &partial.nbrCountRR };
-        for (int rel = 0; rel < 6; rel++)
+        std::unique_ptr<I_SimBackend> MakeSimBackend(Base::BackendType backendType,
+                                                     Bench::BenchmarksTracker& benchmarksTracker)
         {
-            const uint32_t n = rel < 3 ? n_[0] : n_[2];
-            std::vector<uint32_t> counts(n);
-            uint64_t total = 0;
-            Check(tsp_get_neighbors(ctx_, rel, counts.data(), nullptr, 0, &total), "SimBackend_HIP::FillStatisticsPartial");
-            DistributionPartial& d = *dist[rel];
-            d.N = n;
-            d.sum = total;
-            if (n)
+            switch (backendType)
             {
-                d.min = counts[0];
-                d.max = counts[0];
-                for (uint32_t c : counts) { d.min = c < d.min ? c : d.min; d.max = c > d.max ? c : d.max; }
+            case Base::BackendType::GPU:
+                return std::make_unique<SimBackend_HIP>(0, &benchmarksTracker);
+            case Base::BackendType::CPU:
+                throw std::runtime_error("CPU sim backend requested, but this library holds only the gfx950 backend.");
             }
+            throw std::runtime_error("Unsupported sim backend type requested.");
         }
-        return true;
     }
 }
