@@ -95,11 +95,12 @@ def test_1m_cube_resting_on_the_floor_runs_the_contact_solve():
     assert contact == 3
 
 
-def _same_fluid(a, b, what):
+def _same_fluid(a, b, what, steps=True):
     assert np.array_equal(a.fluid_id, b.fluid_id), what
     for f in FIELDS_STEP:
         assert np.array_equal(a.fluid[f], b.fluid[f]), (what, f)
-    assert (a.step, a.time) == (b.step, b.time), what
+    if steps:
+        assert (a.step, a.time) == (b.step, b.time), what
 
 
 @pytest.mark.timeout(600)
@@ -131,7 +132,7 @@ def test_256_mixed_scenes_in_one_context_match_plain_runs_and_the_oracle():
             for _ in range(steps):
                 o.step()
             so = o.push()
-            _same_fluid(got, so, f"scene {q} against the oracle")
+            _same_fluid(got, so, f"scene {q} against the oracle", steps=False)   # the oracle keeps no step count
             assert b.info(sid).solver_iters == o.info().solver_iters
     b.close()
 
