@@ -196,6 +196,13 @@ typedef struct tsp_statistics {
 } tsp_statistics;
 /* I_SimBackend::FillStatisticsRecord (core/backend_sim.h:243) for the first scene */
 int tsp_get_statistics(tsp_ctx *ctx, tsp_statistics *out);
+/* tsp_get_statistics without any allocation. No counterpart in the reference: for callers that
+   cannot allocate device memory (serving loops, code around HIP graph capture). The kernels take
+   a static per-GPU scratch array of 3.3 MB (the library sets up its address at tsp_create) instead
+   of the context's buffers; scenes of more than 2^24 fluid particles are refused with
+   TSP_ERR_CAPACITY. Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+   (contexts on different streams, or threads) race. The results equal tsp_get_statistics'. */
+int tsp_get_statistics_noalloc(tsp_ctx *ctx, tsp_statistics *out);
 
 /* Device memory a context holds, by what it stores, in bytes (I_SimBackend::FillMemoryRecord,
    core/backend_sim.h:239, with the buckets of eval/memory/categories.h:59-78). */

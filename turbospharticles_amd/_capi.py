@@ -169,6 +169,7 @@ def _declare_hip(lib):
         "tsp_profile_read": (i, [vp, ctypes.c_char_p, _u64p, ctypes.POINTER(ctypes.c_double), i]),
         "tsp_synchronize": (i, [vp]),
         "tsp_get_statistics": (i, [vp, ctypes.POINTER(TspStatistics)]),
+        "tsp_get_statistics_noalloc": (i, [vp, ctypes.POINTER(TspStatistics)]),
         "tsp_get_memory": (i, [vp, ctypes.POINTER(TspMemory)]),
         "tsp_batch_scene_statistics": (i, [vp, ctypes.c_uint32, ctypes.POINTER(TspStatistics)]),
         "tsp_export_snapshot": (i, [vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(TspExportView)]),

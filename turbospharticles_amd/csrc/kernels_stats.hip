@@ -3,13 +3,15 @@
 // and 99th percentile from an exact histogram of the counts, the averages of the relative density
 // error rho/rho0 - 1 (raw, positive part, absolute), and the velocity magnitude's sum, minimum and
 // maximum. A served scene is monitored without copying its particles to the host.
+#include <mutex>
+
 #include "tsp_device.h"
 
 namespace tsp
 {
     constexpr uint32_t kHistBins = 65536;   // neighbor counts are stored in 16 bits
 
-    // counts of one relation of one scene: u16 (FF with its wide-span bit masked, FB, FR) or u32
+    // counts of one relation of one scene: u16 (FF, FB, FR) or u32 (RF, RB, RR)
     __global__ void __launch_bounds__(kBlock) kStatsHist(const uint16_t* __restrict__ c16, const uint32_t* __restrict__ c32,
         uint32_t mask, uint32_t n, uint32_t* __restrict__ hist)
     {
@@ -133,13 +135,19 @@ namespace tsp
         out[4] = lo; out[5] = hi;
     }
 
+    __global__ void __launch_bounds__(kBlock) kStatsClear(uint32_t* __restrict__ hist)
+    {
+        hist[blockIdx.x * kBlock + threadIdx.x] = 0u;
+    }
+
     static inline uint32_t blocksN(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
 
-    // hist: kHistBins words; out4: min, max, p99, sum
+    // hist: kHistBins words (cleared by a kernel, not a memset node, so the sequence can be
+    // captured); out4: min, max, p99, sum
     void launchStatsCounts(hipStream_t s, const uint16_t* c16, const uint32_t* c32, uint32_t mask, uint32_t n,
                            uint32_t* hist, uint64_t* out4)
     {
-        (void)hipMemsetAsync(hist, 0, sizeof(uint32_t) * kHistBins, s);
+        kStatsClear<<<kHistBins / kBlock, kBlock, 0, s>>>(hist);
         if (n) { kStatsHist<<<blocksN(n), kBlock, 0, s>>>(c16, c32, mask, n, hist); }
         kStatsHistScan<<<1, kBlock, 0, s>>>(hist, n, out4);
     }
@@ -149,5 +157,57 @@ namespace tsp
     {
         if (n) { kStatsFluid<<<blocksN(n), kBlock, 0, s>>>(vel, off, n, rho0, partial); }
         kStatsFluidSum<<<1, kBlock, 0, s>>>(partial, blocksN(n), out6);
+    }
+
+    // --- the statistics scratch of the allocation-free entry point (tsp_get_statistics_noalloc):
+    //     one static array per GPU, for callers that cannot allocate device memory. Layout: the
+    //     count histogram, the relations' (min, max, p99, sum), the fluid sums, then 6 FP64 block
+    //     partials per 256 fluid particles of a scene of at most kStatsMaxFluid particles.
+    //     Every call on one GPU shares that GPU's array: calls that overlap on one GPU (contexts
+    //     on different streams) race, and each call leaves its values in it. ---
+    constexpr uint32_t kStatsMaxFluid = 1u << 24;
+    constexpr size_t kStatsHistOff = 0;
+    constexpr size_t kStatsCountsOff = kStatsHistOff + sizeof(uint32_t) * kHistBins;
+    constexpr size_t kStatsFluidOff = kStatsCountsOff + 256;
+    constexpr size_t kStatsPartialOff = kStatsFluidOff + 256;
+    constexpr size_t kStatsScratchBytes = kStatsPartialOff + sizeof(double) * 6 * (kStatsMaxFluid / kBlock);
+    static_assert(sizeof(uint64_t) * 24 <= 256 && sizeof(double) * 6 <= 256, "scratch layout");
+
+    static __device__ __align__(256) unsigned char scratch[kStatsScratchBytes];
+
+    constexpr int kStatsMaxDevices = 64;
+    static void* scratchPtr[kStatsMaxDevices];
+    static bool scratchReady[kStatsMaxDevices];
+    static std::mutex scratchMutex;
+
+    // once per device, with that device current: the device address of its copy of the array
+    // (the array's name in host code is a host-side stand-in, not a device address)
+    hipError_t statsScratchInit(int device)
+    {
+        if (device < 0 || device >= kStatsMaxDevices) { return hipErrorInvalidDevice; }
+        std::lock_guard<std::mutex> lock(scratchMutex);
+        if (scratchReady[device]) { return hipSuccess; }
+        void* ptr = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (rc != hipSuccess) { return rc; }
+        scratchPtr[device] = ptr;
+        scratchReady[device] = true;
+        return hipSuccess;
+    }
+
+    // the scratch buffers of one device (statsScratchInit ran for it), or false
+    bool statsScratch(int device, uint32_t** hist, uint64_t** counts, double** fluid, double** partial,
+                      uint32_t* maxFluid)
+    {
+        if (device < 0 || device >= kStatsMaxDevices) { return false; }
+        std::lock_guard<std::mutex> lock(scratchMutex);
+        if (!scratchReady[device]) { return false; }
+        unsigned char* base = static_cast<unsigned char*>(scratchPtr[device]);
+        *hist = reinterpret_cast<uint32_t*>(base + kStatsHistOff);
+        *counts = reinterpret_cast<uint64_t*>(base + kStatsCountsOff);
+        *fluid = reinterpret_cast<double*>(base + kStatsFluidOff);
+        *partial = reinterpret_cast<double*>(base + kStatsPartialOff);
+        *maxFluid = kStatsMaxFluid;
+        return true;
     }
 }

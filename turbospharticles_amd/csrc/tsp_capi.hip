@@ -63,6 +63,8 @@ namespace tsp
     void launchRigidIntegrate(hipStream_t, const Dev&);
     void launchStatsCounts(hipStream_t, const uint16_t*, const uint32_t*, uint32_t, uint32_t, uint32_t*, uint64_t*);
     void launchStatsFluid(hipStream_t, const float4*, uint32_t, uint32_t, float, double*, double*);
+    hipError_t statsScratchInit(int);
+    bool statsScratch(int, uint32_t**, uint64_t**, double**, double**, uint32_t*);
 }
 
 using namespace tsp;
@@ -1556,19 +1558,37 @@ struct tsp_ctx
     double* statPartial = nullptr;
     uint64_t statPartialCap = 0;
 
-    void statistics(uint32_t si, tsp_statistics* out)
+    // with noAlloc the kernels take the device's static statistics scratch instead of the
+    // context's buffers, and nothing is allocated
+    void statistics(uint32_t si, tsp_statistics* out, bool noAlloc = false)
     {
         const SceneHost& h = scenes[si];
         const uint64_t nb = (static_cast<uint64_t>(h.nF) + kBlock - 1) / kBlock;
-        if (!statHist || statPartialCap < nb)
+        uint32_t* statHist = this->statHist;
+        uint64_t* statCounts = this->statCounts;
+        double* statFluid = this->statFluid;
+        double* statPartial = this->statPartial;
+        if (noAlloc)
+        {
+            uint32_t maxFluid = 0;
+            if (!statsScratch(device, &statHist, &statCounts, &statFluid, &statPartial, &maxFluid))
+            {
+                throw Error{ TSP_ERR_STATE, "the statistics scratch of this device is not set up" };
+            }
+            if (h.nF > maxFluid)
+            {
+                throw Error{ TSP_ERR_CAPACITY, "the allocation-free statistics take scenes of at most 2^24 fluid particles" };
+            }
+        }
+        else if (!statHist || statPartialCap < nb)
         {
             sync();
             releaseList(statAllocs);
-            statHist = alloc<uint32_t>(65536, statAllocs);
-            statCounts = alloc<uint64_t>(24, statAllocs);
-            statFluid = alloc<double>(6, statAllocs);
+            statHist = this->statHist = alloc<uint32_t>(65536, statAllocs);
+            statCounts = this->statCounts = alloc<uint64_t>(24, statAllocs);
+            statFluid = this->statFluid = alloc<double>(6, statAllocs);
             statPartialCap = std::max<uint64_t>(nb, 1024);
-            statPartial = alloc<double>(6 * statPartialCap, statAllocs);
+            statPartial = this->statPartial = alloc<double>(6 * statPartialCap, statAllocs);
         }
         const uint16_t* c16[3] = { d.cnt[REL_FF] + h.offF, d.cnt[REL_FB] + h.offF, d.cnt[REL_FR] + h.offF };
         for (int r = 0; r < 3; r++)
@@ -1781,7 +1801,9 @@ int tsp_create(int device, tsp_ctx** out)
     if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) { return TSP_ERR_HIP; }
     tsp_ctx* c = new tsp_ctx();
     c->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess
+    // statsScratchInit: the device address of this device's statistics scratch, once per device
+    if (hipSetDevice(device) != hipSuccess || statsScratchInit(device) != hipSuccess
+        || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess
         || hipHostMalloc(reinterpret_cast<void**>(&c->hostBlock), tsp_ctx::kSolveOff) != hipSuccess)
     {
         delete c;
@@ -2147,6 +2169,16 @@ int tsp_get_memory(tsp_ctx* ctx, tsp_memory* out)
         out->misc = ctx->memBytes[tsp_ctx::MEM_MISC];
         out->alloc_bytes = ctx->memTotal;
         out->peak_alloc_bytes = ctx->memPeak;
+        return TSP_OK;
+    });
+}
+
+int tsp_get_statistics_noalloc(tsp_ctx* ctx, tsp_statistics* out)
+{
+    return guarded(ctx, [&]() {
+        if (!out) { throw Error{ TSP_ERR_INVALID, "null statistics" }; }
+        ctx->requireScene();
+        ctx->statistics(0, out, true);
         return TSP_OK;
     });
 }
