@@ -190,7 +190,7 @@ namespace tsp
     }
 
     // ----------------------------------------------------------------------------------------
-    // stable LSD radix sort, 8-bit digits, tiles of 2048 keys
+    // stable LSD radix sort, kRadixBits-bit digits (kRadixBins <= 256 buckets), tiles of 2048 keys
 
     constexpr uint32_t kTile = 2048;
     constexpr uint32_t kRounds = kTile / kBlock;
@@ -198,29 +198,32 @@ namespace tsp
     __global__ void __launch_bounds__(kBlock) kRadixHist(const uint32_t* __restrict__ key, uint32_t n, uint32_t shift,
         uint32_t nTiles, uint32_t* __restrict__ hist)
     {
-        __shared__ uint32_t cnt[256];
-        cnt[threadIdx.x] = 0;
+        __shared__ uint32_t cnt[kRadixBins];
+        if (threadIdx.x < kRadixBins) { cnt[threadIdx.x] = 0; }
         __syncthreads();
         const uint32_t base = blockIdx.x * kTile;
         for (uint32_t r = 0; r < kRounds; r++)
         {
             const uint32_t i = base + r * kBlock + threadIdx.x;
-            if (i < n) { atomicAdd(&cnt[(key[i] >> shift) & 255u], 1u); }
+            if (i < n) { atomicAdd(&cnt[(key[i] >> shift) & (kRadixBins - 1u)], 1u); }
         }
         __syncthreads();
-        hist[threadIdx.x * nTiles + blockIdx.x] = cnt[threadIdx.x];
+        if (threadIdx.x < kRadixBins) { hist[threadIdx.x * nTiles + blockIdx.x] = cnt[threadIdx.x]; }
     }
 
     __global__ void __launch_bounds__(kBlock) kRadixScatter(const uint32_t* __restrict__ keyIn,
         const uint32_t* __restrict__ idxIn, uint32_t* __restrict__ keyOut, uint32_t* __restrict__ idxOut, uint32_t n,
         uint32_t shift, uint32_t nTiles, const uint32_t* __restrict__ offsets)
     {
-        __shared__ uint32_t run[256];
-        __shared__ uint32_t waveCnt[kBlock / 64][256];
+        __shared__ uint32_t run[kRadixBins];
+        __shared__ uint32_t waveCnt[kBlock / 64][kRadixBins];
         const uint32_t t = threadIdx.x;
         const uint32_t wave = t >> 6, lane = t & 63u;
-        run[t] = offsets[t * nTiles + blockIdx.x];
-        for (uint32_t w = 0; w < kBlock / 64; w++) { waveCnt[w][t] = 0; }
+        if (t < kRadixBins)
+        {
+            run[t] = offsets[t * nTiles + blockIdx.x];
+            for (uint32_t w = 0; w < kBlock / 64; w++) { waveCnt[w][t] = 0; }
+        }
         __syncthreads();
         const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
         const uint32_t base = blockIdx.x * kTile;
@@ -229,10 +232,10 @@ namespace tsp
             const uint32_t i = base + r * kBlock + t;
             const bool valid = i < n;
             const uint32_t k = valid ? keyIn[i] : 0u;
-            const uint32_t d = (k >> shift) & 255u;
+            const uint32_t d = (k >> shift) & (kRadixBins - 1u);
             // lanes of this wave holding the same digit
             uint64_t match = __ballot(valid);
-            for (uint32_t bit = 0; bit < 8; bit++)
+            for (uint32_t bit = 0; bit < kRadixBits; bit++)
             {
                 const bool set = (d >> bit) & 1u;
                 const uint64_t bs = __ballot(set);
@@ -248,9 +251,12 @@ namespace tsp
                 for (uint32_t w = 0; w < wave; w++) { rank += waveCnt[w][d]; }
             }
             __syncthreads();
-            uint32_t tot = 0;
-            for (uint32_t w = 0; w < kBlock / 64; w++) { tot += waveCnt[w][t]; waveCnt[w][t] = 0; }
-            run[t] += tot;
+            if (t < kRadixBins)
+            {
+                uint32_t tot = 0;
+                for (uint32_t w = 0; w < kBlock / 64; w++) { tot += waveCnt[w][t]; waveCnt[w][t] = 0; }
+                run[t] += tot;
+            }
             __syncthreads();
             if (valid)
             {
@@ -275,16 +281,16 @@ namespace tsp
         uint32_t* __restrict__ key2, uint32_t* __restrict__ idx2, uint32_t n, uint32_t bits)
     {
         constexpr uint32_t kWaves = kSmallThreads / 64;
-        __shared__ uint32_t run[256];
+        __shared__ uint32_t run[kRadixBins];
         __shared__ uint32_t waveTot[4];
-        __shared__ uint32_t cnt[kWaves][256];
-        __shared__ uint32_t pre[kWaves][256];
+        __shared__ uint32_t cnt[kWaves][kRadixBins];
+        __shared__ uint32_t pre[kWaves][kRadixBins];
         const uint32_t t = threadIdx.x;
         const uint32_t wave = t >> 6, lane = t & 63u;
         const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
-        for (uint32_t w = 0; w < kWaves; w++) { if (t < 256) { cnt[w][t] = 0; } }
+        for (uint32_t w = 0; w < kWaves; w++) { if (t < kRadixBins) { cnt[w][t] = 0; } }
         uint32_t* kIn = key; uint32_t* iIn = idx; uint32_t* kOut = key2; uint32_t* iOut = idx2;
-        for (uint32_t shift = 0; shift < bits; shift += 8)
+        for (uint32_t shift = 0; shift < bits; shift += kRadixBits)
         {
             uint32_t kr[kSmallPer], ir[kSmallPer];
 #pragma unroll
@@ -294,17 +300,17 @@ namespace tsp
                 kr[c] = i < n ? kIn[i] : 0u;
                 ir[c] = i < n ? iIn[i] : 0u;
             }
-            if (t < 256) { run[t] = 0; }
+            if (t < kRadixBins) { run[t] = 0; }
             __syncthreads();
 #pragma unroll
             for (uint32_t c = 0; c < kSmallPer; c++)
             {
-                if (c * kSmallThreads + t < n) { atomicAdd(&run[(kr[c] >> shift) & 255u], 1u); }
+                if (c * kSmallThreads + t < n) { atomicAdd(&run[(kr[c] >> shift) & (kRadixBins - 1u)], 1u); }
             }
             __syncthreads();
             // exclusive scan of the 256 digit counts: inclusive scan per wave, then the wave totals
             uint32_t v = 0, incl = 0;
-            if (t < 256)
+            if (t < kRadixBins)
             {
                 v = run[t];
                 incl = v;
@@ -316,7 +322,7 @@ namespace tsp
                 if (lane == 63) { waveTot[wave] = incl; }
             }
             __syncthreads();
-            if (t < 256)
+            if (t < kRadixBins)
             {
                 uint32_t off = 0;
                 for (uint32_t w = 0; w < wave; w++) { off += waveTot[w]; }
@@ -330,9 +336,9 @@ namespace tsp
                 const uint32_t i = c * kSmallThreads + t;
                 const bool valid = i < n;
                 const uint32_t k = kr[c];
-                const uint32_t dg = (k >> shift) & 255u;
+                const uint32_t dg = (k >> shift) & (kRadixBins - 1u);
                 uint64_t match = __ballot(valid);
-                for (uint32_t bit = 0; bit < 8; bit++)
+                for (uint32_t bit = 0; bit < kRadixBits; bit++)
                 {
                     const bool set = (dg >> bit) & 1u;
                     const uint64_t bs = __ballot(set);
@@ -342,7 +348,7 @@ namespace tsp
                 if (valid && rankInWave == 0) { cnt[wave][dg] = __popcll(match); }
                 __syncthreads();
                 // per digit: offsets of the waves' runs (and the counts cleared for the next chunk)
-                if (t < 256)
+                if (t < kRadixBins)
                 {
                     uint32_t cw[kWaves];
 #pragma unroll
@@ -455,22 +461,23 @@ namespace tsp
         return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) <= r2;
     }
 
-    // reductions over the 32 lanes of one SELL slice (the two halves of a wave64 are separate)
+    // reductions over the lanes of one SELL slice (with 32-lane slices the two halves of a wave64
+    // are separate)
     __device__ __forceinline__ uint32_t sliceMax(uint32_t v)
     {
-        for (int o = 16; o > 0; o >>= 1) { v = max(v, static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o, 32))); }
+        for (int o = kSlice / 2; o > 0; o >>= 1) { v = max(v, static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o, kSlice))); }
         return v;
     }
 
     __device__ __forceinline__ uint32_t sliceMin(uint32_t v)
     {
-        for (int o = 16; o > 0; o >>= 1) { v = min(v, static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o, 32))); }
+        for (int o = kSlice / 2; o > 0; o >>= 1) { v = min(v, static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o, kSlice))); }
         return v;
     }
 
     __device__ __forceinline__ uint32_t sliceSum(uint32_t v)
     {
-        for (int o = 16; o > 0; o >>= 1) { v += static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o, 32)); }
+        for (int o = kSlice / 2; o > 0; o >>= 1) { v += static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o, kSlice)); }
         return v;
     }
 
@@ -488,6 +495,7 @@ namespace tsp
         constexpr uint32_t kW = TSP_TUNE_SEARCH_BATCH;
 #if TSP_TUNE_SEARCH_ROW == 2
         // cell by cell along the row, kW candidates at a time (a cell's candidates are contiguous)
+        uint32_t q0 = 0;
 #pragma unroll
         for (int c = 0; c < 3; c++)
         {
@@ -506,9 +514,10 @@ namespace tsp
 #pragma unroll
                 for (uint32_t u = 0; u < kW; u++)
                 {
-                    if (t + u < n) { fn(j[u], p[u]); }
+                    if (t + u < n) { fn(j[u], p[u], q0 + t + u); }
                 }
             }
+            q0 += n;
         }
 #else
         // the row's three ranges as one stream: candidate q is q + the offset of its range
@@ -531,22 +540,32 @@ namespace tsp
 #pragma unroll
             for (uint32_t u = 0; u < kW; u++)
             {
-                if (t + u < total) { fn(j[u], p[u]); }
+                if (t + u < total) { fn(j[u], p[u], t + u); }
             }
         }
 #endif
     }
 
-    template <typename OnF, typename OnB, typename OnR>
+    // the particle index of the q-th fluid candidate of a row stream (the three ranges in order)
+    __device__ __forceinline__ uint32_t rowCandidate(const uint2 (&r)[3], uint32_t q)
+    {
+        const uint32_t n0 = r[0].y - r[0].x, n01 = n0 + (r[1].y - r[1].x);
+        return q < n0 ? r[0].x + q : (q < n01 ? r[1].x + (q - n0) : r[2].x + (q - n01));
+    }
+
+    // preF(row, rf) may take a row's fluid candidates itself (returns true) instead of the stream;
+    // postF(row) follows the row's fluid part. row counts the rows of three cells in stencil order.
+    template <typename OnF, typename OnB, typename OnR, typename PreF, typename PostF>
     __device__ __forceinline__ void forStencil(const Dev& d, const Frame& f, const Stencil& s, bool withF, OnF onF,
-                                               OnB onB, OnR onR)
+                                               OnB onB, OnR onR, PreF preF, PostF postF)
     {
         const auto same = [](uint32_t e) { return e; };
+        uint32_t row = 0;
         for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
         {
             for (int cy = s.lo[1]; cy <= s.hi[1]; cy++)
             {
-                for (int cx = s.lo[0]; cx <= s.hi[0]; cx += 3)
+                for (int cx = s.lo[0]; cx <= s.hi[0]; cx += 3, row++)
                 {
                     const int nx = min(3, s.hi[0] - cx + 1);
                     const uint32_t lin = linearCell(f, cx, cy, cz);
@@ -559,18 +578,29 @@ namespace tsp
                         rb[c] = in ? d.cellB[lin + c] : make_uint2(0u, 0u);
                         rr[c] = (in && d.nR) ? d.cellR[lin + c] : make_uint2(0u, 0u);
                     }
-                    rowStream(rf, same, [&](uint32_t j) { return d.pos[j]; }, onF);
-                    rowStream(rb, same, [&](uint32_t j) { return d.bpos[j]; }, onB);
-                    rowStream(rr, [&](uint32_t e) { return d.idxR[e]; }, [&](uint32_t j) { return d.rpos[j]; }, onR);
+                    if (withF)
+                    {
+                        if (!preF(row, rf))
+                        {
+                            rowStream(rf, same, [&](uint32_t j) { return d.pos[j]; },
+                                      [&](uint32_t j, const float4& p, uint32_t q) { onF(j, p, row, q); });
+                        }
+                        postF(row);
+                    }
+                    rowStream(rb, same, [&](uint32_t j) { return d.bpos[j]; },
+                              [&](uint32_t j, const float4& p, uint32_t) { onB(j, p); });
+                    rowStream(rr, [&](uint32_t e) { return d.idxR[e]; }, [&](uint32_t j) { return d.rpos[j]; },
+                              [&](uint32_t j, const float4& p, uint32_t) { onR(j, p); });
                 }
             }
         }
     }
 #else
-    // Cell form: one cell at a time, its candidates one at a time.
-    template <typename OnF, typename OnB, typename OnR>
+    // Cell form: one cell at a time, its candidates one at a time (no hit masks: every row is
+    // tested, row numbers are not passed)
+    template <typename OnF, typename OnB, typename OnR, typename PreF, typename PostF>
     __device__ __forceinline__ void forStencil(const Dev& d, const Frame& f, const Stencil& s, bool withF, OnF onF,
-                                               OnB onB, OnR onR)
+                                               OnB onB, OnR onR, PreF, PostF)
     {
         for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
         {
@@ -582,7 +612,7 @@ namespace tsp
                     if (withF)
                     {
                         const uint2 rf = d.cellF[lin];
-                        for (uint32_t j = rf.x; j < rf.y; j++) { onF(j, d.pos[j]); }
+                        for (uint32_t j = rf.x; j < rf.y; j++) { onF(j, d.pos[j], kMaskRows, 0u); }
                     }
                     const uint2 rb = d.cellB[lin];
                     for (uint32_t j = rb.x; j < rb.y; j++) { onB(j, d.bpos[j]); }
@@ -613,16 +643,21 @@ namespace tsp
         uint32_t mn[3] = { kEmpty32, kEmpty32, kEmpty32 };
         uint32_t mx[3] = { 0, 0, 0 };
         Stencil s;
+        // rows the masks do not cover (all of them unless the row form walks a reach-1 stencil)
+        uint32_t uncovered = (1u << kMaskRows) - 1u, rowMask = 0;
         if (i < d.nF && stencilOf(f, d.keyF[i], s))
         {
             const float4 pi = d.pos[i];
+            const bool masks = TSP_TUNE_SEARCH_ROW == 1 && f.reach == 1;
+            if (masks) { uncovered = 0; }
             forStencil(d, f, s, true,
-                [&](uint32_t j, const float4& pj)
+                [&](uint32_t j, const float4& pj, uint32_t row, uint32_t q)
                 {
                     if (within(pi.x, pi.y, pi.z, pj, r2))
                     {
                         cnt[0]++; mn[0] = min(mn[0], j); mx[0] = max(mx[0], j);
                         tiny = tiny || tinyDist2(dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z));
+                        if (q < 32u) { rowMask |= 1u << q; } else if (row < kMaskRows) { uncovered |= 1u << row; }
                     }
                 },
                 [&](uint32_t j, const float4& pj)
@@ -636,10 +671,17 @@ namespace tsp
                 [&](uint32_t j, const float4& pj)
                 {
                     if (within(pi.x, pi.y, pi.z, pj, r2)) { cnt[2]++; mn[2] = min(mn[2], j); mx[2] = max(mx[2], j); }
+                },
+                [&](uint32_t, const uint2 (&)[3]) { return false; },
+                [&](uint32_t row)
+                {
+                    if (masks && row < kMaskRows) { d.ffMask[static_cast<size_t>(row) * d.nF + i] = rowMask; }
+                    rowMask = 0;
                 });
         }
         if (i < d.nF)
         {
+            d.ffMask[static_cast<size_t>(kMaskRows) * d.nF + i] = uncovered;
             const bool wide = cnt[0] > 0 && mx[0] - mn[0] > 0xffffu;
             d.cnt[REL_FF][i] = static_cast<uint16_t>(cnt[0]);
             d.cnt[REL_FB][i] = static_cast<uint16_t>(cnt[1]);
@@ -745,15 +787,17 @@ namespace tsp
         {
             const float4 pi = d.pos[i];
             uint32_t kF = 0, kB = 0, kR = 0;
+            const uint32_t uncovered = d.ffMask[static_cast<size_t>(kMaskRows) * d.nF + i];
+            const auto ffHit = [&](uint32_t j)
+            {
+                if (staged && kF < kStage) { stage[kF][t] = static_cast<uint16_t>(j - mF.base); }
+                else { sellWrite(d.pool[REL_FF], mF, lane, kF, j); }
+                kF++;
+            };
             forStencil(d, f, s, !wide,
-                [&](uint32_t j, const float4& pj)
+                [&](uint32_t j, const float4& pj, uint32_t, uint32_t)
                 {
-                    if (within(pi.x, pi.y, pi.z, pj, r2))
-                    {
-                        if (staged && kF < kStage) { stage[kF][t] = static_cast<uint16_t>(j - mF.base); }
-                        else { sellWrite(d.pool[REL_FF], mF, lane, kF, j); }
-                        kF++;
-                    }
+                    if (within(pi.x, pi.y, pi.z, pj, r2)) { ffHit(j); }
                 },
                 [&](uint32_t j, const float4& pj)
                 {
@@ -762,7 +806,21 @@ namespace tsp
                 [&](uint32_t j, const float4& pj)
                 {
                     if (within(pi.x, pi.y, pi.z, pj, r2)) { sellWrite(d.pool[REL_FR], mR, lane, kR++, j); }
-                });
+                },
+                // a row the count pass covered: its hits, in stream order, without loading a position
+                [&](uint32_t row, const uint2 (&rf)[3])
+                {
+                    if (row >= kMaskRows || ((uncovered >> row) & 1u)) { return false; }
+                    uint32_t m = d.ffMask[static_cast<size_t>(row) * d.nF + i];
+                    while (m)
+                    {
+                        const uint32_t q = static_cast<uint32_t>(__ffs(static_cast<int>(m))) - 1u;
+                        m &= m - 1u;
+                        ffHit(rowCandidate(rf, q));
+                    }
+                    return true;
+                },
+                [&](uint32_t) {});
             if (wide)
             {
                 // DictDelta16x2 split of a per-particle dict (neighbor_lists.cpp:838-858): the neighbors
@@ -1252,10 +1310,10 @@ namespace tsp
         }
         const uint32_t nTiles = (n + kTile - 1) / kTile;
         uint32_t* kIn = key; uint32_t* iIn = idx; uint32_t* kOut = key2; uint32_t* iOut = idx2;
-        for (uint32_t shift = 0; shift < bits; shift += 8)
+        for (uint32_t shift = 0; shift < bits; shift += kRadixBits)
         {
             kRadixHist<<<nTiles, kBlock, 0, s>>>(kIn, n, shift, nTiles, hist);
-            launchScan(s, hist, 256u * nTiles, tmp);
+            launchScan(s, hist, kRadixBins * nTiles, tmp);
             kRadixScatter<<<nTiles, kBlock, 0, s>>>(kIn, iIn, kOut, iOut, n, shift, nTiles, hist);
             uint32_t* t = kIn; kIn = kOut; kOut = t;
             t = iIn; iIn = iOut; iOut = t;

@@ -837,6 +837,7 @@ struct tsp_ctx
             d.scanTmp = alloc<uint32_t>((m + 2047ull) / 2048ull + 1ull, work);
         }
         d.ffWideMin = alloc<uint32_t>(nF, work, MEM_LF);
+        d.ffMask = alloc<uint32_t>(static_cast<size_t>(kMaskWords) * nF, work, MEM_SF);
         {
             const uint64_t nBlocks = nF / kBlock;
             d.haloIdx = alloc<uint32_t>(nBlocks * kHaloMax, work, MEM_LF);

@@ -31,6 +31,8 @@
 //                          (register budget of the sweep kernels); 0 leaves the choice to it
 //   TSP_TUNE_PERSISTENT    the solver sweeps run as persistent kernels: a resident grid whose
 //                          blocks walk the 256-particle tiles, instead of one block per tile
+//   TSP_TUNE_SLICE         SELL slice height: 32 (a wave covers two slices) or 64 (one slice per wave)
+//   TSP_TUNE_RADIX_BITS    digit width of the LSD radix sorts (8: 256 buckets, 6, 4)
 //   TSP_TUNE_SEARCH_ROW    the neighbor query walks the stencil by rows of three cells, loading
 //                          the rows' ranges together and TSP_TUNE_SEARCH_BATCH candidates at a time
 //                          (1: the row's candidates as one stream, 2: cell by cell; 0: the cell walk)
@@ -68,6 +70,12 @@
 #ifndef TSP_TUNE_SEARCH_BATCH
 #define TSP_TUNE_SEARCH_BATCH 4
 #endif
+#ifndef TSP_TUNE_SLICE
+#define TSP_TUNE_SLICE 32
+#endif
+#ifndef TSP_TUNE_RADIX_BITS
+#define TSP_TUNE_RADIX_BITS 8
+#endif
 #define TSP_WAVES_ATTR_0
 #define TSP_WAVES_ATTR_1 __attribute__((amdgpu_waves_per_eu(1)))
 #define TSP_WAVES_ATTR_2 __attribute__((amdgpu_waves_per_eu(2)))
@@ -83,7 +91,11 @@
 namespace tsp
 {
     constexpr int kBlock = 256;          // threads per block: 4 waves of 64
-    constexpr uint32_t kSlice = 32;      // SELL slice height (a wave covers two slices)
+    constexpr uint32_t kSlice = TSP_TUNE_SLICE;   // SELL slice height (32: a wave covers two slices)
+    static_assert(kSlice == 32 || kSlice == 64, "slice height");
+    constexpr uint32_t kRadixBits = TSP_TUNE_RADIX_BITS;
+    constexpr uint32_t kRadixBins = 1u << kRadixBits;
+    static_assert(kRadixBits >= 1 && kRadixBits <= 8, "radix digit width");
     constexpr uint32_t kEmpty32 = 0xffffffffu;
     constexpr uint32_t kFmtDelta16 = 0;
     constexpr uint32_t kFmtAbs32 = 1;
@@ -91,6 +103,12 @@ namespace tsp
     // formulation: its per-particle chunks become dict chunks only up to 256 distinct neighbors
     // (N_SELL_DICT_SIZE_MAX, neighbor_lists.cpp:326-332)
     constexpr uint32_t kDictMax = 256;
+
+    // FF hit masks of the count pass (one word per stencil row of three cells, reach 1): the fill
+    // takes the hits from them and loads no candidate position. A row with more than 32 fluid
+    // candidates is marked in the last word and tested again by the fill.
+    constexpr uint32_t kMaskRows = 9;
+    constexpr uint32_t kMaskWords = kMaskRows + 1;
     // FF neighbor halo of a 256-particle block: the distinct FF neighbors of its particles,
     // ascending; the FF entries of the block then hold halo slots instead of particle indices
     constexpr uint32_t kHaloMax = TSP_TUNE_HALO_MAX; // 24 B x 1000 records in kJacobi: six blocks per CU
@@ -297,6 +315,8 @@ namespace tsp
         uint64_t poolCap[3];     // u16 units
         float *kgFF;             // per FF entry (by its u16 position): gradient factor of the step (build variant)
         uint32_t *ffWideMin;     // per particle: min FF index of a list spanning more than 0xffff, else kEmpty32
+        uint32_t *ffMask;        // kMaskWords x nF: per stencil row, bit q = the row's q-th fluid candidate
+                                 // is an FF neighbor; then the rows the masks do not cover
         uint32_t *haloIdx;       // per block: kHaloMax particle indices, ascending
         uint32_t *haloCount;     // per block: halo size, or kNoHalo
         // rigid-source CSR (RF), counts of RB and RR

@@ -15,12 +15,19 @@ A request's scene is either a host state or a scene spec; a spec is generated on
 the request is admitted (tsp_batch_update_specs / tsp_pull_spec), so no host state is built or
 copied for it.
 
+Scenes of very different sizes are paced by the largest when they share a context: every step
+of the batch waits for the slowest scene's sweeps. serve_batched therefore sorts requests into
+size buckets (by fluid count), each with its own context, stream and host thread, so that the
+small scenes' steps overlap with the large ones' on the GPU instead of waiting for them; the
+buckets share the HBM budget.
+
 Besides HBM, a context's 32-bit key space is an admission resource: every resident scene holds
 a key range sized by its cell frame (2^(3 bits) for frames of up to 2^bits cells per axis). A
 context refuses scenes whose frames do not fit (TSP_ERR_CAPACITY) and keeps its resident scenes;
 the loop then admits fewer of the waiting requests, or holds them until a resident scene leaves.
 """
 import collections
+import threading
 import time
 
 from . import _capi
@@ -77,29 +84,44 @@ def _admit(batch, admit):
     return hosted + specced, ids
 
 
-def serve_batched(requests, hbm_budget_bytes, chunk_steps=10, device=0, keep_results=True):
-    """Serves `requests` (arrival times relative to the call) with continuous batching; returns
-    the wall time from the start to the last completion. Fills admitted_s, done_s and result."""
+class _Budget:
+    """The HBM budget the size buckets share."""
+
+    def __init__(self, total):
+        self.total = total
+        self.used = 0
+        self.lock = threading.Lock()
+
+    def take(self, n):
+        with self.lock:
+            if self.used + n > self.total:
+                return False
+            self.used += n
+            return True
+
+    def give(self, n):
+        with self.lock:
+            self.used -= n
+
+
+def _serve_bucket(requests, budget, chunk_steps, device, keep_results, t0):
+    """Continuous batching of one size bucket in its own context (arrival-order admission)."""
     batch = HipBatch(device)
     pending = collections.deque(sorted(requests, key=lambda r: r.arrival_s))
     waiting = collections.deque()
     active = {}            # scene id -> [request, steps left]
-    used = 0
     hold = False           # the key space refused the head of the queue: wait for a completion
-    t0 = time.perf_counter()
     try:
         while pending or waiting or active:
             now = time.perf_counter() - t0
             while pending and pending[0].arrival_s <= now:
                 waiting.append(pending.popleft())
             admit = []
-            while not hold and waiting and used + waiting[0].footprint <= hbm_budget_bytes:
-                r = waiting.popleft()
-                used += r.footprint
-                admit.append(r)
-            if waiting and not active and not admit:
+            while not hold and waiting and budget.take(waiting[0].footprint):
+                admit.append(waiting.popleft())
+            if waiting and not active and not admit and waiting[0].footprint > budget.total:
                 r = waiting[0]
-                raise RuntimeError(f"request {r.rid} needs {r.footprint} B, more than the {hbm_budget_bytes} B budget")
+                raise RuntimeError(f"request {r.rid} needs {r.footprint} B, more than the {budget.total} B budget")
             while admit:
                 try:
                     admitted, ids = _admit(batch, admit)
@@ -110,14 +132,15 @@ def serve_batched(requests, hbm_budget_bytes, chunk_steps=10, device=0, keep_res
                     # alone, and hold the queue when even that one does not fit
                     for r in reversed(admit):
                         waiting.appendleft(r)
-                        used -= r.footprint
+                        budget.give(r.footprint)
                     if len(admit) == 1:
                         hold = True
                         admit = []
                     else:
                         r = waiting.popleft()
-                        used += r.footprint
-                        admit = [r]
+                        admit = [r] if budget.take(r.footprint) else []
+                        if not admit:
+                            waiting.appendleft(r)
                     continue
                 now = time.perf_counter() - t0
                 for sid, r in zip(ids, admitted):
@@ -125,8 +148,13 @@ def serve_batched(requests, hbm_budget_bytes, chunk_steps=10, device=0, keep_res
                     active[sid] = [r, r.steps]
                 admit = []
             if not active:
-                if pending:
-                    _wait_until(t0, pending[0].arrival_s)
+                # nothing resident: wait for the next arrival, or for another bucket to free memory
+                if pending and not waiting:
+                    dt = pending[0].arrival_s - (time.perf_counter() - t0)
+                    if dt > 0:
+                        time.sleep(dt)
+                else:
+                    time.sleep(0.001)
                 continue
             n = min(chunk_steps, min(left for _, left in active.values()))
             batch.step(n)
@@ -145,11 +173,46 @@ def serve_batched(requests, hbm_budget_bytes, chunk_steps=10, device=0, keep_res
                 for sid in done:
                     r = active.pop(sid)[0]
                     r.done_s = now
-                    used -= r.footprint
+                    budget.give(r.footprint)
                 hold = False
-        return time.perf_counter() - t0
     finally:
         batch.close()
+
+
+# fluid counts that separate the size buckets of serve_batched
+DEFAULT_BUCKETS = (60_000, 400_000)
+
+
+def serve_batched(requests, hbm_budget_bytes, chunk_steps=10, device=0, keep_results=True, buckets=DEFAULT_BUCKETS):
+    """Serves `requests` (arrival times relative to the call) with continuous batching, one
+    context per size bucket (fluid counts split at `buckets`), the buckets' host threads running
+    side by side; returns the wall time from the start to the last completion. Fills admitted_s,
+    done_s and result."""
+    edges = sorted(buckets)
+    groups = [[] for _ in range(len(edges) + 1)]
+    for r in requests:
+        groups[sum(r.n[0] >= e for e in edges)].append(r)
+    budget = _Budget(hbm_budget_bytes)
+    for r in requests:
+        if r.footprint > hbm_budget_bytes:
+            raise RuntimeError(f"request {r.rid} needs {r.footprint} B, more than the {hbm_budget_bytes} B budget")
+    errors = []
+    t0 = time.perf_counter()
+
+    def run(g):
+        try:
+            _serve_bucket(g, budget, chunk_steps, device, keep_results, t0)
+        except BaseException as e:   # re-raised in the caller
+            errors.append(e)
+
+    threads = [threading.Thread(target=run, args=(g,)) for g in groups if g]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
+    return time.perf_counter() - t0
 
 
 def serve_plain(requests, device=0, keep_results=True):

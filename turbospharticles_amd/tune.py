@@ -1,10 +1,11 @@
 """Autotuner of the build-time variants of the step (the TSP_TUNE_* switches in csrc/tsp_device.h).
 
-Every variant gives the same bits; they differ in how the work maps onto the GPU: prefetched list
-entries, 24- or 32-byte Jacobi halo records, stored or recomputed gradient factors, the halo cap
-(LDS per sweep block), register budgets of the sweep kernels, sweep groups as HIP graphs, the
-neighbor query's walk of the stencil (cell by cell, or rows of three cells) and its candidate loads
-in flight. The
+Every variant gives the same bits; they differ in how the work maps onto the GPU: the SELL slice
+height (32 or 64 particles per slice), the radix sort's digit width (8, 6 or 4 bits: 4 to 8 passes
+over 28-bit keys), prefetched list entries, 24- or 32-byte Jacobi halo records, stored or
+recomputed gradient factors, the halo cap (LDS per sweep block), register budgets of the sweep
+kernels, sweep groups as HIP graphs, the neighbor query's walk of the stencil (cell by cell, or
+rows of three cells) and its candidate loads in flight. The
 tuner walks the switches one at a time from the current settings (coordinate descent), keeping a
 change when it shortens the measured step.
 
@@ -34,6 +35,8 @@ TUNED_H = os.path.join(HERE, "csrc", "tsp_tuned.h")
 
 # switch -> values tried (the first is the default of tsp_device.h)
 SPACE = {
+    "TSP_TUNE_SLICE": [32, 64],
+    "TSP_TUNE_RADIX_BITS": [8, 6, 4],
     "TSP_TUNE_PREFETCH": [1, 0],
     "TSP_TUNE_JACOBI_REC24": [1, 0],
     "TSP_TUNE_KG_STORE": [0, 1],
