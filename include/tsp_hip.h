@@ -27,8 +27,8 @@ typedef enum tsp_status {
     TSP_ERR_INVALID = -1,   /* bad argument, null handle, size mismatch */
     TSP_ERR_HIP = -2,       /* a HIP runtime call failed (GPU::Check, GPU/gpu_utils.h:24-57) */
     TSP_ERR_STATE = -3,     /* call out of order (e.g. step before pull) */
-    TSP_ERR_CAPACITY = -4,  /* a count exceeds a fixed limit (u16 neighbor counts, 1024 cells per
-                               frame axis for the 32-bit Morton ids) */
+    TSP_ERR_CAPACITY = -4,  /* a count exceeds a limit (u16 neighbor counts, 2^21 cells per frame
+                               axis, the context's cell limit) */
     TSP_ERR_NUMERIC = -5    /* degenerate rigid orientation (cpu_backend.cpp:46-57) */
 } tsp_status;
 
@@ -137,6 +137,13 @@ const char *tsp_name(void);
 int tsp_set_params(tsp_ctx *ctx, const tsp_params *params);
 /* I_SimBackend::SetSimDomainBounds (core/backend_sim.h:204) */
 int tsp_set_domain(tsp_ctx *ctx, const float mn[3], const float mx[3]);
+/* the most cells the cell frames of the context's scenes may hold together (the dense cell
+   tables take 24 bytes per cell); default and maximum 2^32 - 1. A layout beyond it (a pull, an
+   admission or a frame rebuild) fails with TSP_ERR_CAPACITY and leaves the resident scenes as
+   they were. No reference counterpart: the reference keeps compact tables of the occupied cells
+   only (CPU/backend/neighbor_search.h:25-212); callers that budget device memory per context
+   use it. */
+int tsp_set_cell_limit(tsp_ctx *ctx, uint64_t max_cells);
 /* I_SimBackend::PullFromUniversalSimState (core/backend_sim.h:207; gpu_backend.cpp:96-240) */
 int tsp_pull(tsp_ctx *ctx, const tsp_state_view *src);
 /* I_SimBackend::PushToUniversalSimState (core/backend_sim.h:211; gpu_backend.cpp:242-344) */

@@ -162,9 +162,14 @@ def test_served_stream_of_scene_specs_matches_host_built_states():
         _same_state(e.result, c.result, f"request {e.rid} (plain, from its spec)")
 
 
+# cells of three far-pair frames and the small dam breaks, not of four far pairs (the frames of
+# _far_pair(0..8) hold 360,639 to 528,471 cells, the dam breaks 16,128 to 31,500)
+CELL_LIMIT = 1_500_000
+
+
 def _far_pair(q):
-    """Two fluid particles 19.6 m apart along x: a cell frame of 998 x 19 x 19 cells at h 0.01,
-    whose Morton ids take 30 bits; a context's 32-bit key space holds three such scenes."""
+    """Two fluid particles 19.6 m apart along x: a cell frame of 999 x (19..23) x (19..23) cells
+    at h 0.01; a context limited to CELL_LIMIT cells holds three such scenes."""
     st = _capi.HostState(2, 0, 0, 0)
     st.fluid_id[:] = [2 * q, 2 * q + 1]
     st.fluid["x"][:] = [0.0, 19.6]
@@ -173,12 +178,12 @@ def _far_pair(q):
     return st
 
 
-def test_served_stream_waits_when_the_key_space_is_full():
-    """Six requests of scenes with large cell frames arrive together; the context can hold three
-    of them at a time, so admissions are refused (TSP_ERR_CAPACITY) and requeued until scenes
-    leave. Dam breaks in between are admitted as usual; every result equals its plain run."""
+def test_served_stream_waits_when_the_cell_tables_are_full():
+    """Six requests of scenes with large cell frames arrive together; the context's cell limit
+    holds three of them at a time, so admissions are refused (TSP_ERR_CAPACITY) and requeued until
+    scenes leave. Dam breaks in between are admitted as usual; every result equals its plain run."""
     from turbospharticles_amd import serving
-    print("case: served stream, key space for three large-frame scenes", flush=True)
+    print("case: served stream, cell limit for three large-frame scenes", flush=True)
     p = scenes.make_params(0.01, termination_cond=scenes.TERM_POS)
 
     def make():
@@ -190,7 +195,7 @@ def test_served_stream_waits_when_the_key_space_is_full():
         return reqs
 
     rb = make()
-    serving.serve_batched(rb, 200e9, chunk_steps=2)
+    serving.serve_batched(rb, 200e9, chunk_steps=2, cell_limit=CELL_LIMIT)
     rp = make()
     serving.serve_plain(rp)
     far = [r for r in rb if r.rid % 2 == 0]
@@ -201,13 +206,14 @@ def test_served_stream_waits_when_the_key_space_is_full():
 
 
 def test_refused_admission_keeps_the_resident_scenes_and_drops_removed_ids():
-    """A batch update that removes one scene and adds one that does not fit the key space: the
-    context refuses the new scene, the removal takes effect, the remaining scenes step on as if
+    """A batch update that removes one scene and adds two that do not fit the cell limit: the
+    context refuses the new scenes, the removal takes effect, the remaining scenes step on as if
     nothing happened, and the wrapper's scene table follows the context."""
     from turbospharticles_amd.backend import TspError
     print("case: refused admission with a removal", flush=True)
     p = scenes.make_params(0.01, termination_cond=scenes.TERM_POS)
     b = HipBatch(0)
+    b.set_cell_limit(CELL_LIMIT)
     ids = b.update(add=[(p, _far_pair(q)) for q in range(3)])
     dam = scenes.build_scene(scenes.dam_break_spec(8, 0.01, tank=(0.3, 0.2, 0.2)))
     (kept,) = b.update(add=[(p, dam)])

@@ -156,6 +156,7 @@ def _declare_hip(lib):
         "tsp_name": (ctypes.c_char_p, []),
         "tsp_set_params": (i, [vp, ctypes.POINTER(TspParams)]),
         "tsp_set_domain": (i, [vp, _fp, _fp]),
+        "tsp_set_cell_limit": (i, [vp, ctypes.c_uint64]),
         "tsp_pull": (i, [vp, ctypes.POINTER(TspStateView)]),
         "tsp_push": (i, [vp, ctypes.POINTER(TspStateView)]),
         "tsp_neighbor_search": (i, [vp]),

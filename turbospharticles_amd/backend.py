@@ -14,7 +14,7 @@ from . import _capi
 
 class TspError(RuntimeError):
     """A failed tsp_* call: `code` is its tsp_status (TSP_ERR_CAPACITY = -4 for a count beyond
-    a fixed limit, e.g. a context whose scenes' cell frames do not fit its key space)."""
+    limit, e.g. a context whose scenes' cell frames exceed its cell limit)."""
 
     def __init__(self, what, code, msg):
         super().__init__(f"{what} failed ({code}): {msg}")
@@ -52,6 +52,10 @@ class HipBackend:
 
     def set_params(self, params):
         self._check(self.lib.tsp_set_params(self.h, ctypes.byref(params)), "tsp_set_params")
+
+    def set_cell_limit(self, max_cells):
+        """The most cells the context's cell frames may hold (tsp_set_cell_limit)."""
+        self._check(self.lib.tsp_set_cell_limit(self.h, int(max_cells)), "tsp_set_cell_limit")
 
     # I_SimBackend::SetSimDomainBounds
     def set_domain(self, mn, mx):
@@ -205,6 +209,11 @@ class HipBatch:
     def _check(self, rc, what):
         if rc != 0:
             raise TspError(what, rc, self.lib.tsp_last_error(self.h).decode())
+
+    def set_cell_limit(self, max_cells):
+        """The most cells the cell frames of all resident scenes may hold together
+        (tsp_set_cell_limit); an admission beyond it is refused with TSP_ERR_CAPACITY."""
+        self._check(self.lib.tsp_set_cell_limit(self.h, int(max_cells)), "tsp_set_cell_limit")
 
     def update(self, add=(), remove=()):
         """Removes the scenes `remove` (ids), then appends `add`, a sequence of (params, state)

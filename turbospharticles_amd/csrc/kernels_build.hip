@@ -173,7 +173,7 @@ namespace tsp
         bvol[i] = vol;
     }
 
-    void launchRadixSort(hipStream_t, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*);
+    void launchRadixSort(hipStream_t, bool, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*);
 
     void launchBuildFluid(hipStream_t s, const Dev& d, uint32_t off, uint32_t n, const BuildFluid& b)
     {
@@ -188,7 +188,7 @@ namespace tsp
     {
         if (!n) { return; }
         kBuildBoundaryPos<<<blocksN(n), kBlock, 0, s>>>(c, n, xyz, ids, key, idx);
-        launchRadixSort(s, key, idx, key2, idx2, n, c.bits[0] + c.bits[1] + c.bits[2], hist, tmp);
+        launchRadixSort(s, false, key, idx, key2, idx2, n, c.bits[0] + c.bits[1] + c.bits[2], hist, tmp);
         kBuildBoundaryCoeff<<<blocksN(n), kBlock, 0, s>>>(c, n, xyz, key, idx, bpos, bvol);
     }
 }

@@ -126,7 +126,7 @@ namespace tsp
 
     // cell id of one position in its scene's key range; sets the abort flags when an in-domain
     // particle left the inner frame
-    __device__ __forceinline__ uint32_t cellKey(const Frame& f, float x, float y, float z, bool check, Status* st, Solve* so)
+    __device__ __forceinline__ uint64_t cellKey(const Frame& f, float x, float y, float z, bool check, Status* st, Solve* so)
     {
         const int cx = floorToInt(x * f.cellSizeInv);
         const int cy = floorToInt(y * f.cellSizeInv);
@@ -146,7 +146,7 @@ namespace tsp
         {
             return f.emptyKey;
         }
-        return f.keyBase + morton(static_cast<uint32_t>(lx), static_cast<uint32_t>(ly), static_cast<uint32_t>(lz));
+        return f.keyBase + morton64(static_cast<uint32_t>(lx), static_cast<uint32_t>(ly), static_cast<uint32_t>(lz));
     }
 
     enum Pop { POP_F = 0, POP_B = 1, POP_R = 2 };
@@ -159,7 +159,7 @@ namespace tsp
     }
 
     // key of particle i of a population; the padding slots of a fluid segment take the dead key
-    __device__ __forceinline__ uint32_t particleKey(const Dev& d, int pop, const float4* p, uint32_t i, bool check)
+    __device__ __forceinline__ uint64_t particleKey(const Dev& d, int pop, const float4* p, uint32_t i, bool check)
     {
         const uint32_t s = sceneOfParticle(d, pop, i);
         const Frame& f = d.frames[s];
@@ -168,23 +168,25 @@ namespace tsp
         return cellKey(f, q.x, q.y, q.z, check, d.status, &d.solve[s]);
     }
 
+    template <typename K>
     __global__ void __launch_bounds__(kBlock) kKeys(Dev d, int pop, const float4* __restrict__ p, uint32_t n, bool check,
-        uint32_t* __restrict__ key, uint32_t* __restrict__ idx)
+        K* __restrict__ key, uint32_t* __restrict__ idx)
     {
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) { return; }
-        key[i] = particleKey(d, pop, p, i, check);
+        key[i] = static_cast<K>(particleKey(d, pop, p, i, check));
         idx[i] = i;
     }
 
     // boundary keys; flags a descent so the host knows whether the order needs a sort
+    template <typename K>
     __global__ void __launch_bounds__(kBlock) kKeysSorted(Dev d, const float4* __restrict__ p, uint32_t n,
-        uint32_t* __restrict__ key, uint32_t* __restrict__ idx)
+        K* __restrict__ key, uint32_t* __restrict__ idx)
     {
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) { return; }
-        const uint32_t k = particleKey(d, POP_B, p, i, false);
-        key[i] = k;
+        const uint64_t k = particleKey(d, POP_B, p, i, false);
+        key[i] = static_cast<K>(k);
         idx[i] = i;
         if (i > 0 && particleKey(d, POP_B, p, i - 1, false) > k) { atomicOr(&d.status->bUnsorted, 1u); }
     }
@@ -195,7 +197,8 @@ namespace tsp
     constexpr uint32_t kTile = 2048;
     constexpr uint32_t kRounds = kTile / kBlock;
 
-    __global__ void __launch_bounds__(kBlock) kRadixHist(const uint32_t* __restrict__ key, uint32_t n, uint32_t shift,
+    template <typename K>
+    __global__ void __launch_bounds__(kBlock) kRadixHist(const K* __restrict__ key, uint32_t n, uint32_t shift,
         uint32_t nTiles, uint32_t* __restrict__ hist)
     {
         __shared__ uint32_t cnt[kRadixBins];
@@ -205,14 +208,15 @@ namespace tsp
         for (uint32_t r = 0; r < kRounds; r++)
         {
             const uint32_t i = base + r * kBlock + threadIdx.x;
-            if (i < n) { atomicAdd(&cnt[(key[i] >> shift) & (kRadixBins - 1u)], 1u); }
+            if (i < n) { atomicAdd(&cnt[static_cast<uint32_t>(key[i] >> shift) & (kRadixBins - 1u)], 1u); }
         }
         __syncthreads();
         if (threadIdx.x < kRadixBins) { hist[threadIdx.x * nTiles + blockIdx.x] = cnt[threadIdx.x]; }
     }
 
-    __global__ void __launch_bounds__(kBlock) kRadixScatter(const uint32_t* __restrict__ keyIn,
-        const uint32_t* __restrict__ idxIn, uint32_t* __restrict__ keyOut, uint32_t* __restrict__ idxOut, uint32_t n,
+    template <typename K>
+    __global__ void __launch_bounds__(kBlock) kRadixScatter(const K* __restrict__ keyIn,
+        const uint32_t* __restrict__ idxIn, K* __restrict__ keyOut, uint32_t* __restrict__ idxOut, uint32_t n,
         uint32_t shift, uint32_t nTiles, const uint32_t* __restrict__ offsets)
     {
         __shared__ uint32_t run[kRadixBins];
@@ -231,8 +235,8 @@ namespace tsp
         {
             const uint32_t i = base + r * kBlock + t;
             const bool valid = i < n;
-            const uint32_t k = valid ? keyIn[i] : 0u;
-            const uint32_t d = (k >> shift) & (kRadixBins - 1u);
+            const K k = valid ? keyIn[i] : K(0);
+            const uint32_t d = static_cast<uint32_t>(k >> shift) & (kRadixBins - 1u);
             // lanes of this wave holding the same digit
             uint64_t match = __ballot(valid);
             for (uint32_t bit = 0; bit < kRadixBits; bit++)
@@ -277,8 +281,9 @@ namespace tsp
     constexpr uint32_t kSmallPer = 16;
     constexpr uint32_t kSmallSort = kSmallThreads * kSmallPer;
 
-    __global__ void __launch_bounds__(kSmallThreads) kSortSmall(uint32_t* __restrict__ key, uint32_t* __restrict__ idx,
-        uint32_t* __restrict__ key2, uint32_t* __restrict__ idx2, uint32_t n, uint32_t bits)
+    template <typename K>
+    __global__ void __launch_bounds__(kSmallThreads) kSortSmall(K* __restrict__ key, uint32_t* __restrict__ idx,
+        K* __restrict__ key2, uint32_t* __restrict__ idx2, uint32_t n, uint32_t bits)
     {
         constexpr uint32_t kWaves = kSmallThreads / 64;
         __shared__ uint32_t run[kRadixBins];
@@ -289,15 +294,16 @@ namespace tsp
         const uint32_t wave = t >> 6, lane = t & 63u;
         const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
         for (uint32_t w = 0; w < kWaves; w++) { if (t < kRadixBins) { cnt[w][t] = 0; } }
-        uint32_t* kIn = key; uint32_t* iIn = idx; uint32_t* kOut = key2; uint32_t* iOut = idx2;
+        K* kIn = key; uint32_t* iIn = idx; K* kOut = key2; uint32_t* iOut = idx2;
         for (uint32_t shift = 0; shift < bits; shift += kRadixBits)
         {
-            uint32_t kr[kSmallPer], ir[kSmallPer];
+            K kr[kSmallPer];
+            uint32_t ir[kSmallPer];
 #pragma unroll
             for (uint32_t c = 0; c < kSmallPer; c++)
             {
                 const uint32_t i = c * kSmallThreads + t;
-                kr[c] = i < n ? kIn[i] : 0u;
+                kr[c] = i < n ? kIn[i] : K(0);
                 ir[c] = i < n ? iIn[i] : 0u;
             }
             if (t < kRadixBins) { run[t] = 0; }
@@ -305,7 +311,7 @@ namespace tsp
 #pragma unroll
             for (uint32_t c = 0; c < kSmallPer; c++)
             {
-                if (c * kSmallThreads + t < n) { atomicAdd(&run[(kr[c] >> shift) & (kRadixBins - 1u)], 1u); }
+                if (c * kSmallThreads + t < n) { atomicAdd(&run[static_cast<uint32_t>(kr[c] >> shift) & (kRadixBins - 1u)], 1u); }
             }
             __syncthreads();
             // exclusive scan of the 256 digit counts: inclusive scan per wave, then the wave totals
@@ -335,8 +341,8 @@ namespace tsp
                 if (c * kSmallThreads >= n) { break; }
                 const uint32_t i = c * kSmallThreads + t;
                 const bool valid = i < n;
-                const uint32_t k = kr[c];
-                const uint32_t dg = (k >> shift) & (kRadixBins - 1u);
+                const K k = kr[c];
+                const uint32_t dg = static_cast<uint32_t>(k >> shift) & (kRadixBins - 1u);
                 uint64_t match = __ballot(valid);
                 for (uint32_t bit = 0; bit < kRadixBits; bit++)
                 {
@@ -367,7 +373,7 @@ namespace tsp
                 }
             }
             __syncthreads();
-            uint32_t* tk = kIn; kIn = kOut; kOut = tk;
+            K* tk = kIn; kIn = kOut; kOut = tk;
             uint32_t* ti = iIn; iIn = iOut; iOut = ti;
         }
         if (kIn != key)
@@ -393,10 +399,10 @@ namespace tsp
     // cell tables
 
     // cell-table index of an in-frame key (local Morton id = key - keyBase)
-    __device__ __forceinline__ uint32_t linearOfKey(const Frame& f, uint32_t k)
+    __device__ __forceinline__ uint32_t linearOfKey(const Frame& f, uint64_t k)
     {
-        const uint32_t m = k - f.keyBase;
-        return linearCell(f, compact10(m), compact10(m >> 1), compact10(m >> 2));
+        const uint64_t m = k - f.keyBase;
+        return linearCell(f, compact21(m), compact21(m >> 1), compact21(m >> 2));
     }
 
     // clears the cells of the runs of the previous sorted keys
@@ -405,10 +411,10 @@ namespace tsp
     {
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) { return; }
-        const uint32_t k = key[i];
+        const uint64_t k = keyAt(d, key, i);
         const Frame& f = d.frames[sceneOfKey(d, k)];
         if (k >= f.emptyKey) { return; }
-        if (i == 0 || key[i - 1] != k) { cell[linearOfKey(f, k)] = make_uint2(0u, 0u); }
+        if (i == 0 || keyAt(d, key, i - 1) != k) { cell[linearOfKey(f, k)] = make_uint2(0u, 0u); }
     }
 
     __global__ void __launch_bounds__(kBlock) kClearAll(uint2* __restrict__ cell, uint64_t n)
@@ -422,11 +428,11 @@ namespace tsp
     {
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) { return; }
-        const uint32_t k = key[i];
+        const uint64_t k = keyAt(d, key, i);
         const Frame& f = d.frames[sceneOfKey(d, k)];
         if (k >= f.emptyKey) { return; }
-        const bool first = i == 0 || key[i - 1] != k;
-        const bool last = i + 1 == n || key[i + 1] != k;
+        const bool first = i == 0 || keyAt(d, key, i - 1) != k;
+        const bool last = i + 1 == n || keyAt(d, key, i + 1) != k;
         if (!first && !last) { return; }
         const uint32_t lin = linearOfKey(f, k);
         if (first) { cell[lin].x = i; }
@@ -441,12 +447,12 @@ namespace tsp
         int lo[3], hi[3];
     };
 
-    __device__ __forceinline__ bool stencilOf(const Frame& f, uint32_t k, Stencil& s)
+    __device__ __forceinline__ bool stencilOf(const Frame& f, uint64_t k, Stencil& s)
     {
         if (k >= f.emptyKey) { return false; }
-        const uint32_t m = k - f.keyBase;
-        const int l[3] = { static_cast<int>(compact10(m)), static_cast<int>(compact10(m >> 1)),
-                           static_cast<int>(compact10(m >> 2)) };
+        const uint64_t m = k - f.keyBase;
+        const int l[3] = { static_cast<int>(compact21(m)), static_cast<int>(compact21(m >> 1)),
+                           static_cast<int>(compact21(m >> 2)) };
         for (int a = 0; a < 3; a++)
         {
             s.lo[a] = max(l[a] - f.reach, 0);
@@ -479,6 +485,13 @@ namespace tsp
     {
         for (int o = kSlice / 2; o > 0; o >>= 1) { v += static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o, kSlice)); }
         return v;
+    }
+
+    // the particle index of the q-th fluid candidate of a row stream (the three ranges in order)
+    __device__ __forceinline__ uint32_t rowCandidate(const uint2 (&r)[3], uint32_t q)
+    {
+        const uint32_t n0 = r[0].y - r[0].x, n01 = n0 + (r[1].y - r[1].x);
+        return q < n0 ? r[0].x + q : (q < n01 ? r[1].x + (q - n0) : r[2].x + (q - n01));
     }
 
     // The candidates of the stencil, per relation in list order: cells in z-y-x raster order,
@@ -544,13 +557,6 @@ namespace tsp
             }
         }
 #endif
-    }
-
-    // the particle index of the q-th fluid candidate of a row stream (the three ranges in order)
-    __device__ __forceinline__ uint32_t rowCandidate(const uint2 (&r)[3], uint32_t q)
-    {
-        const uint32_t n0 = r[0].y - r[0].x, n01 = n0 + (r[1].y - r[1].x);
-        return q < n0 ? r[0].x + q : (q < n01 ? r[1].x + (q - n0) : r[2].x + (q - n01));
     }
 
     // preF(row, rf) may take a row's fluid candidates itself (returns true) instead of the stream;
@@ -645,7 +651,7 @@ namespace tsp
         Stencil s;
         // rows the masks do not cover (all of them unless the row form walks a reach-1 stencil)
         uint32_t uncovered = (1u << kMaskRows) - 1u, rowMask = 0;
-        if (i < d.nF && stencilOf(f, d.keyF[i], s))
+        if (i < d.nF && stencilOf(f, keyAt(d, d.keyF, i), s))
         {
             const float4 pi = d.pos[i];
             const bool masks = TSP_TUNE_SEARCH_ROW == 1 && f.reach == 1;
@@ -777,7 +783,7 @@ namespace tsp
         const float r2 = d.consts[sc].r2;
         const uint32_t lane = i & (kSlice - 1), slice = i / kSlice;
         Stencil s;
-        const bool active = i < d.nF && stencilOf(f, d.keyF[i], s);
+        const bool active = i < d.nF && stencilOf(f, keyAt(d, d.keyF, i), s);
         const uint32_t cF = i < d.nF ? d.cnt[REL_FF][i] : 0u;
         const bool wide = i < d.nF && d.ffWideMin[i] != kEmpty32;
         SliceMeta mF{}, mB{}, mR{};
@@ -1287,42 +1293,58 @@ namespace tsp
 
     void launchResetSearch(hipStream_t s, const Dev& d) { kResetSearch<<<1, kBlock, 0, s>>>(d); }
 
+    // key arrays hold u32 keys, or u64 keys (two words each) when d.key64
     void launchKeys(hipStream_t s, const Dev& d, int pop, const float4* p, uint32_t n, bool check, uint32_t* key,
                     uint32_t* idx)
     {
-        if (n) { kKeys<<<blocksFor(n), kBlock, 0, s>>>(d, pop, p, n, check, key, idx); }
+        if (!n) { return; }
+        if (d.key64) { kKeys<<<blocksFor(n), kBlock, 0, s>>>(d, pop, p, n, check, reinterpret_cast<uint64_t*>(key), idx); }
+        else { kKeys<<<blocksFor(n), kBlock, 0, s>>>(d, pop, p, n, check, key, idx); }
     }
 
     void launchKeysSorted(hipStream_t s, const Dev& d, const float4* p, uint32_t n, uint32_t* key, uint32_t* idx)
     {
-        if (n) { kKeysSorted<<<blocksFor(n), kBlock, 0, s>>>(d, p, n, key, idx); }
+        if (!n) { return; }
+        if (d.key64) { kKeysSorted<<<blocksFor(n), kBlock, 0, s>>>(d, p, n, reinterpret_cast<uint64_t*>(key), idx); }
+        else { kKeysSorted<<<blocksFor(n), kBlock, 0, s>>>(d, p, n, key, idx); }
     }
 
-    // sorts (key, idx) stably by the low `bits` bits; the result ends in (key, idx)
-    void launchRadixSort(hipStream_t s, uint32_t* key, uint32_t* idx, uint32_t* key2, uint32_t* idx2, uint32_t n,
-                         uint32_t bits, uint32_t* hist, uint32_t* tmp)
+    template <typename K>
+    static void radixSort(hipStream_t s, K* key, uint32_t* idx, K* key2, uint32_t* idx2, uint32_t n, uint32_t bits,
+                          uint32_t* hist, uint32_t* tmp)
     {
-        if (n < 2) { return; }
         if (n <= kSmallSort)
         {
-            kSortSmall<<<1, kSmallThreads, 0, s>>>(key, idx, key2, idx2, n, bits);
+            kSortSmall<K><<<1, kSmallThreads, 0, s>>>(key, idx, key2, idx2, n, bits);
             return;
         }
         const uint32_t nTiles = (n + kTile - 1) / kTile;
-        uint32_t* kIn = key; uint32_t* iIn = idx; uint32_t* kOut = key2; uint32_t* iOut = idx2;
+        K* kIn = key; uint32_t* iIn = idx; K* kOut = key2; uint32_t* iOut = idx2;
         for (uint32_t shift = 0; shift < bits; shift += kRadixBits)
         {
-            kRadixHist<<<nTiles, kBlock, 0, s>>>(kIn, n, shift, nTiles, hist);
+            kRadixHist<K><<<nTiles, kBlock, 0, s>>>(kIn, n, shift, nTiles, hist);
             launchScan(s, hist, kRadixBins * nTiles, tmp);
-            kRadixScatter<<<nTiles, kBlock, 0, s>>>(kIn, iIn, kOut, iOut, n, shift, nTiles, hist);
-            uint32_t* t = kIn; kIn = kOut; kOut = t;
-            t = iIn; iIn = iOut; iOut = t;
+            kRadixScatter<K><<<nTiles, kBlock, 0, s>>>(kIn, iIn, kOut, iOut, n, shift, nTiles, hist);
+            K* t = kIn; kIn = kOut; kOut = t;
+            uint32_t* ti = iIn; iIn = iOut; iOut = ti;
         }
         if (kIn != key)
         {
-            (void)hipMemcpyAsync(key, kIn, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s);
+            (void)hipMemcpyAsync(key, kIn, sizeof(K) * n, hipMemcpyDeviceToDevice, s);
             (void)hipMemcpyAsync(idx, iIn, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s);
         }
+    }
+
+    // sorts (key, idx) stably by the low `bits` bits; the result ends in (key, idx)
+    void launchRadixSort(hipStream_t s, bool key64, uint32_t* key, uint32_t* idx, uint32_t* key2, uint32_t* idx2,
+                         uint32_t n, uint32_t bits, uint32_t* hist, uint32_t* tmp)
+    {
+        if (n < 2) { return; }
+        if (key64)
+        {
+            radixSort(s, reinterpret_cast<uint64_t*>(key), idx, reinterpret_cast<uint64_t*>(key2), idx2, n, bits, hist, tmp);
+        }
+        else { radixSort(s, key, idx, key2, idx2, n, bits, hist, tmp); }
     }
 
     void launchPermute(hipStream_t s, const Gather& g, const uint32_t* idx)

@@ -28,7 +28,7 @@ namespace tsp
         if (hc <= kHaloMax)
         {
             const uint32_t* idx = d.haloIdx + static_cast<size_t>(blk) * kHaloMax;
-            for (uint32_t q = threadIdx.x; q < hc; q += kBlock) { load(q, idx[q]); }
+            for (uint32_t q = threadIdx.x; q < hc; q += blockDim.x) { load(q, idx[q]); }
         }
         __syncthreads();
     }
@@ -283,12 +283,17 @@ namespace tsp
         return v;
     }
 
+    // threads of a solver sweep block (one 256-particle tile per block)
+    constexpr uint32_t kSweepThreads = TSP_TUNE_SWEEP_THREADS;
+    static_assert(kBlock % kSweepThreads == 0 && kSweepThreads % 64 == 0, "sweep block size");
+
     // the bookkeeping of one scene after a Jacobi sweep (sph_solver.cpp:207-261): the scene's block
     // partials summed in a fixed order, the averages, TerminationConditionSatisfied and the error
-    // decision of the next sweep. Run by one whole block; nTot is the grid size of the sweep.
+    // decision of the next sweep. Run by one whole block of kSweepThreads; nTot is the grid size of
+    // the sweep.
     __device__ void finalizeSweep(const Dev& d, uint32_t sc, uint32_t nTot)
     {
-        __shared__ double red[3][kBlock / 64];
+        __shared__ double red[3][kSweepThreads / 64];
         __shared__ uint32_t flags[2];
         Solve& s = d.solve[sc];
         const Consts& k = d.consts[sc];
@@ -307,12 +312,13 @@ namespace tsp
                 const double* src = d.errPartial + static_cast<size_t>(r) * nTot + b0;
                 double a = 0.0;
                 uint32_t b = threadIdx.x;
-                for (; b + 3 * kBlock < nBlocks; b += 4 * kBlock)
+                constexpr uint32_t kT = kSweepThreads;
+                for (; b + 3 * kT < nBlocks; b += 4 * kT)
                 {
-                    const double v0 = src[b], v1 = src[b + kBlock], v2 = src[b + 2 * kBlock], v3 = src[b + 3 * kBlock];
+                    const double v0 = src[b], v1 = src[b + kT], v2 = src[b + 2 * kT], v3 = src[b + 3 * kT];
                     a += v0; a += v1; a += v2; a += v3;
                 }
-                for (; b < nBlocks; b += kBlock) { a += src[b]; }
+                for (; b < nBlocks; b += kT) { a += src[b]; }
                 a = waveSum(a);
                 if ((threadIdx.x & 63u) == 0) { red[r][threadIdx.x >> 6] = a; }
             }
@@ -322,7 +328,7 @@ namespace tsp
                 for (int r = 0; r < 3; r++)
                 {
                     double a = red[r][0];
-                    for (uint32_t w = 1; w < kBlock / 64; w++) { a += red[r][w]; }
+                    for (uint32_t w = 1; w < kSweepThreads / 64; w++) { a += red[r][w]; }
                     red[r][0] = a;
                 }
             }
@@ -400,7 +406,6 @@ namespace tsp
     {
         __shared__ float4 hp[kHaloMax];
         __shared__ uint32_t skip;
-        const uint32_t i = blk * kBlock + threadIdx.x;
         const uint32_t sc = d.blockScene[blk];
         const Consts& k = d.consts[sc];
         if (sweep)
@@ -412,7 +417,10 @@ namespace tsp
         }
         const uint32_t hc = d.haloCount[blk];
         stageHalo(d, blk, hc, [&](uint32_t q, uint32_t j) { hp[q] = d.pos[j]; });
-        if (i - d.scenes[sc].offF >= d.scenes[sc].nF) { return; }
+        for (uint32_t r = 0; r < kBlock / kSweepThreads; r++)
+        {
+        const uint32_t i = blk * kBlock + r * kSweepThreads + threadIdx.x;
+        if (i - d.scenes[sc].offF >= d.scenes[sc].nF) { continue; }
         const float4 p4 = d.pos[i];
         const float xi = p4.x, yi = p4.y, zi = p4.z, pi = p4.w;
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
@@ -461,10 +469,11 @@ namespace tsp
             az = __builtin_fmaf(coeff, f * dz, az);
         });
         d.ap[i] = make_float4(0.0f - ax, 0.0f - ay, 0.0f - az, 0.0f);
+        }
     }
 
     template <bool kFast>
-    __global__ void __launch_bounds__(kBlock) TSP_WAVES_ATTR(TSP_TUNE_PA_WAVES) kPressureAcc(Dev d, bool sweep, uint32_t nTiles)
+    __global__ void __launch_bounds__(kSweepThreads) TSP_WAVES_ATTR(TSP_TUNE_PA_WAVES) kPressureAcc(Dev d, bool sweep, uint32_t nTiles)
     {
         // block b of the launch finishes the previous sweep of scenes b, b + grid, ...
         if (sweep)
@@ -479,7 +488,7 @@ namespace tsp
     template <bool kFast>
     __device__ __forceinline__ void jacobiTile(const Dev& d, uint32_t blk, uint32_t nTot)
     {
-        __shared__ double red[3][kBlock / 64];
+        __shared__ double red[3][kSweepThreads / 64];
 #if TSP_TUNE_JACOBI_REC24
         // 24-byte halo records (x, y, z, a^p x | a^p y, a^p z): 24 KB per block, six blocks per CU
         __shared__ float4 hp[kHaloMax];
@@ -487,7 +496,6 @@ namespace tsp
 #else
         __shared__ float4 hp[kHaloMax], ha[kHaloMax];
 #endif
-        const uint32_t i = blk * kBlock + threadIdx.x;
         const uint32_t sc = d.blockScene[blk];
         const Consts& k = d.consts[sc];
         const Scene& S = d.scenes[sc];
@@ -508,6 +516,9 @@ namespace tsp
         stageHalo(d, blk, hc, [&](uint32_t q, uint32_t j) { hp[q] = d.pos[j]; ha[q] = d.ap[j]; });
 #endif
         double eRaw = 0.0, ePos = 0.0, eAbs = 0.0;
+        for (uint32_t r = 0; r < kBlock / kSweepThreads; r++)
+        {
+        const uint32_t i = blk * kBlock + r * kSweepThreads + threadIdx.x;
         if (i - S.offF < S.nF)
         {
             const float4 pi = d.pos[i], ai = d.ap[i];
@@ -575,10 +586,11 @@ namespace tsp
             if (compute && valid)
             {
                 const double e = static_cast<double>(pred / k.rho0);
-                eRaw = e;
-                ePos = 0.0 < e ? e : 0.0;
-                eAbs = __builtin_fabs(e);
+                eRaw += e;
+                ePos += 0.0 < e ? e : 0.0;
+                eAbs += __builtin_fabs(e);
             }
+        }
         }
         if (!compute) { return; }
         eRaw = waveSum(eRaw);
@@ -592,21 +604,21 @@ namespace tsp
             for (int r = 0; r < 3; r++)
             {
                 double s = 0.0;
-                for (uint32_t q = 0; q < kBlock / 64; q++) { s += red[r][q]; }
+                for (uint32_t q = 0; q < kSweepThreads / 64; q++) { s += red[r][q]; }
                 d.errPartial[static_cast<size_t>(r) * nTot + blk] = s;
             }
         }
     }
 
     template <bool kFast>
-    __global__ void __launch_bounds__(kBlock) TSP_WAVES_ATTR(TSP_TUNE_JACOBI_WAVES) kJacobi(Dev d, uint32_t nTiles)
+    __global__ void __launch_bounds__(kSweepThreads) TSP_WAVES_ATTR(TSP_TUNE_JACOBI_WAVES) kJacobi(Dev d, uint32_t nTiles)
     {
         forTiles(nTiles, [&](uint32_t blk) { jacobiTile<kFast>(d, blk, nTiles); });
     }
 
     // the bookkeeping of the last sweep of a group, one block per scene, before the host reads
     // the solver states; nTot is the grid size of the sweeps
-    __global__ void __launch_bounds__(kBlock) kFinalize(Dev d, uint32_t nTot)
+    __global__ void __launch_bounds__(kSweepThreads) kFinalize(Dev d, uint32_t nTot)
     {
         finalizeSweep(d, blockIdx.x, nTot);
     }
@@ -838,19 +850,19 @@ namespace tsp
     {
         if (!d.nF) { return; }
         const uint32_t n = blocksOf(d.nF);
-        if (d.fastPairs) { kPressureAcc<true><<<sweepGrid(n), kBlock, 0, s>>>(d, sweep, n); }
-        else { kPressureAcc<false><<<sweepGrid(n), kBlock, 0, s>>>(d, sweep, n); }
+        if (d.fastPairs) { kPressureAcc<true><<<sweepGrid(n), kSweepThreads, 0, s>>>(d, sweep, n); }
+        else { kPressureAcc<false><<<sweepGrid(n), kSweepThreads, 0, s>>>(d, sweep, n); }
     }
     void launchJacobi(hipStream_t s, const Dev& d)
     {
         if (!d.nF) { return; }
         const uint32_t n = blocksOf(d.nF);
-        if (d.fastPairs) { kJacobi<true><<<sweepGrid(n), kBlock, 0, s>>>(d, n); }
-        else { kJacobi<false><<<sweepGrid(n), kBlock, 0, s>>>(d, n); }
+        if (d.fastPairs) { kJacobi<true><<<sweepGrid(n), kSweepThreads, 0, s>>>(d, n); }
+        else { kJacobi<false><<<sweepGrid(n), kSweepThreads, 0, s>>>(d, n); }
     }
     void launchFinalize(hipStream_t s, const Dev& d)
     {
-        if (d.nF && d.nScenes) { kFinalize<<<d.nScenes, kBlock, 0, s>>>(d, blocksOf(d.nF)); }
+        if (d.nF && d.nScenes) { kFinalize<<<d.nScenes, kSweepThreads, 0, s>>>(d, blocksOf(d.nF)); }
     }
     void launchIntegration(hipStream_t s, const Dev& d) { if (d.nF) { kIntegration<<<blocksOf(d.nF), kBlock, 0, s>>>(d); } }
 

@@ -31,7 +31,7 @@ namespace tsp
     void launchResetSearch(hipStream_t, const Dev&);
     void launchKeys(hipStream_t, const Dev&, int, const float4*, uint32_t, bool, uint32_t*, uint32_t*);
     void launchKeysSorted(hipStream_t, const Dev&, const float4*, uint32_t, uint32_t*, uint32_t*);
-    void launchRadixSort(hipStream_t, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*);
+    void launchRadixSort(hipStream_t, bool, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*);
     void launchPermute(hipStream_t, const Gather&, const uint32_t*);
     void launchClearCells(hipStream_t, const Dev&, const uint32_t*, uint32_t, uint2*);
     void launchClearAll(hipStream_t, uint2*, uint64_t);
@@ -301,6 +301,7 @@ struct tsp_ctx
     std::vector<void*> poolAllocs;  // SELL-32 pools and the RF list
     std::vector<void*> contactAllocs; // RB/RR lists and the contact boundary arrays (grown on demand)
     uint64_t cellCap = 0;
+    uint64_t cellLimit = 0xffffffffull;   // most cells of all scenes' frames (tsp_set_cell_limit)
 
     // rigid contacts of the last search
     bool contactLists = false;      // RB or RR entries exist: the lists are stored
@@ -812,12 +813,13 @@ struct tsp_ctx
         // buffers sized by the layout
         releaseList(work);
         const uint32_t nSort = std::max(nF, std::max(nB, nR));
-        d.keyF = alloc<uint32_t>(nSort, work, MEM_SF);
-        d.keyF2 = alloc<uint32_t>(nSort, work, MEM_SF);
+        // key arrays: room for u64 keys (two words each), which frames beyond 1024 cells per axis need
+        d.keyF = alloc<uint32_t>(2ull * nSort, work, MEM_SF);
+        d.keyF2 = alloc<uint32_t>(2ull * nSort, work, MEM_SF);
         d.idxF = alloc<uint32_t>(nSort, work, MEM_SF);
         d.idxF2 = alloc<uint32_t>(nSort, work, MEM_SF);
-        d.keyR = alloc<uint32_t>(nR, work, MEM_SR);
-        d.keyR2 = alloc<uint32_t>(nR, work, MEM_SR);
+        d.keyR = alloc<uint32_t>(2ull * nR, work, MEM_SR);
+        d.keyR2 = alloc<uint32_t>(2ull * nR, work, MEM_SR);
         d.idxR = alloc<uint32_t>(nR, work, MEM_SR);
         d.idxR2 = alloc<uint32_t>(nR, work, MEM_SR);
         const uint64_t nTiles = (nSort + 2047ull) / 2048ull;
@@ -878,7 +880,7 @@ struct tsp_ctx
         d.scenes = alloc<Scene>(S, work);
         d.frames = alloc<Frame>(S, work);
         d.consts = alloc<Consts>(S, work);
-        d.sceneKey0 = alloc<uint32_t>(S + 1ull, work);
+        d.sceneKey0 = alloc<uint64_t>(S + 1ull, work);
         {
             std::vector<Scene> t(S);
             for (uint32_t s = 0; s < S; s++)
@@ -1018,9 +1020,10 @@ struct tsp_ctx
                 {
                     const int64_t o = static_cast<int64_t>(floorToIntHost(mn[a] * inv)) - margin;
                     const int64_t e = static_cast<int64_t>(floorToIntHost(mx[a] * inv)) + margin;
-                    if (o > e || e - o + 1 > 1024)
+                    // TryBuildCellCoordFrame (core/cell_frame.h:32-66): 21 Morton bits per axis
+                    if (o > e || o < INT32_MIN || e > INT32_MAX || e - o + 1 > (1ll << 21))
                     {
-                        throw Error{ TSP_ERR_CAPACITY, "cell frame exceeds 1024 cells per axis (32-bit Morton keys)" };
+                        throw Error{ TSP_ERR_CAPACITY, "cell frame exceeds the supported range (2^21 cells per axis)" };
                     }
                     f.orig[a] = static_cast<int32_t>(o);
                     f.dim[a] = static_cast<uint32_t>(e - o + 1);
@@ -1039,8 +1042,10 @@ struct tsp_ctx
         }
         // key range of each scene: its Morton ids, then its empty key (outside the frame) and its
         // dead key (padding slots), so a sort of all scenes keeps every scene's segment in place
+        // The keys are 32-bit while the whole key space fits (every frame within 1024 cells per
+        // axis, and scenes together), else 64-bit: the sort then moves 8-byte keys.
         uint64_t key = 0, cells = 0;
-        std::vector<uint32_t> key0(S + 1ull);
+        std::vector<uint64_t> key0(S + 1ull);
         std::vector<Frame> frames(S);
         for (uint32_t s = 0; s < S; s++)
         {
@@ -1048,24 +1053,30 @@ struct tsp_ctx
             const uint32_t dimMax = std::max(f.dim[0], std::max(f.dim[1], f.dim[2]));
             uint32_t axisBits = 1;
             if (dimMax >= 2) { axisBits = 0; for (uint32_t v = dimMax - 1; v; v >>= 1) { axisBits++; } }
-            f.keyBase = static_cast<uint32_t>(key);
-            f.emptyKey = static_cast<uint32_t>(key + (1ull << (3 * axisBits)));
+            const uint64_t range = 1ull << (3 * axisBits);
+            if (range + 2 > ~0ull - key)
+            {
+                throw Error{ TSP_ERR_CAPACITY, "the scenes' cell frames exceed the 64-bit key space of one context" };
+            }
+            f.keyBase = key;
+            f.emptyKey = key + range;
             f.deadKey = f.emptyKey + 1;
             f.cellOff = static_cast<uint32_t>(cells);
             key0[s] = f.keyBase;
-            key += (1ull << (3 * axisBits)) + 2;
+            key += range + 2;
             cells += static_cast<uint64_t>(f.dim[0]) * f.dim[1] * f.dim[2];
-            if (key > 0xffffffffull || cells > 0xffffffffull)
+            if (cells > cellLimit)
             {
-                throw Error{ TSP_ERR_CAPACITY, "the scenes' cell frames exceed the 32-bit key space of one context" };
+                throw Error{ TSP_ERR_CAPACITY, "the scenes' cell frames exceed the context's cell limit" };
             }
             frames[s] = f;
         }
-        key0[S] = static_cast<uint32_t>(key);
+        key0[S] = key;
         keyBits = 1;
         for (uint64_t v = key > 0 ? key - 1 : 0; v >> 1; v >>= 1) { keyBits++; }
+        d.key64 = keyBits > 32 ? 1u : 0u;
         h2d(d.frames, frames.data(), sizeof(Frame) * S);
-        h2d(d.sceneKey0, key0.data(), sizeof(uint32_t) * key0.size());
+        h2d(d.sceneKey0, key0.data(), sizeof(uint64_t) * key0.size());
         if (cells > cellCap || frameAllocs.empty())
         {
             releaseList(frameAllocs);
@@ -1089,7 +1100,7 @@ struct tsp_ctx
             sync();
             if (unsorted)
             {
-                launchRadixSort(stream, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nB, keyBits, radixHist, d.scanTmp);
+                launchRadixSort(stream, d.key64, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nB, keyBits, radixHist, d.scanTmp);
                 Gather g{};
                 g.s4[0] = d.bpos; g.d4[0] = bpos2; g.n4 = 1;
                 g.s1[0] = d.bvol; g.d1[0] = bvol2; g.n1 = 1;
@@ -1128,12 +1139,12 @@ struct tsp_ctx
             {
                 Scope sc(prof, stream, "keys_sort_rigid");
                 launchKeys(stream, d, kPopR, d.rpos, d.nR, true, d.keyR, d.idxR);
-                launchRadixSort(stream, d.keyR, d.idxR, d.keyR2, d.idxR2, d.nR, keyBits, radixHist, d.scanTmp);
+                launchRadixSort(stream, d.key64, d.keyR, d.idxR, d.keyR2, d.idxR2, d.nR, keyBits, radixHist, d.scanTmp);
             }
             {
                 Scope sc(prof, stream, "keys_sort");
                 launchKeys(stream, d, kPopF, d.pos, d.nF, true, d.keyF, d.idxF);
-                launchRadixSort(stream, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nF, keyBits, radixHist, d.scanTmp);
+                launchRadixSort(stream, d.key64, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nF, keyBits, radixHist, d.scanTmp);
             }
             // the step re-orders positions, velocities (with density) and pressure (pos.w, the
             // warm start); the standalone search re-orders every field
@@ -1852,6 +1863,15 @@ int tsp_set_params(tsp_ctx* ctx, const tsp_params* p)
     });
 }
 
+int tsp_set_cell_limit(tsp_ctx* ctx, uint64_t max_cells)
+{
+    return guarded(ctx, [&]() {
+        if (max_cells == 0 || max_cells > 0xffffffffull) { throw Error{ TSP_ERR_INVALID, "cell limit outside 1..2^32-1" }; }
+        ctx->cellLimit = max_cells;
+        return TSP_OK;
+    });
+}
+
 int tsp_set_domain(tsp_ctx* ctx, const float mn[3], const float mx[3])
 {
     return guarded(ctx, [&]() {
@@ -2061,9 +2081,20 @@ int tsp_get_sort(tsp_ctx* ctx, uint32_t* orderF, uint64_t* keysF, uint32_t* orde
         const SceneHost& h = ctx->scenes[0];
         ctx->sync();
         // scene-local Morton ids; out-of-frame ids as all ones
-        const auto widen = [&](uint64_t* dst, const uint32_t* src, uint32_t n) {
-            std::vector<uint32_t> k(n);
-            if (n) { hipCheck(hipMemcpy(k.data(), src, sizeof(uint32_t) * n, hipMemcpyDeviceToHost), "D2H"); }
+        // key i of a key array (u32, or u64 in two words)
+        const auto widen = [&](uint64_t* dst, const uint32_t* src, uint32_t off, uint32_t n) {
+            std::vector<uint64_t> k(n);
+            if (n && d.key64)
+            {
+                hipCheck(hipMemcpy(k.data(), reinterpret_cast<const uint64_t*>(src) + off, sizeof(uint64_t) * n,
+                                   hipMemcpyDeviceToHost), "D2H");
+            }
+            else if (n)
+            {
+                std::vector<uint32_t> k32(n);
+                hipCheck(hipMemcpy(k32.data(), src + off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost), "D2H");
+                for (uint32_t i = 0; i < n; i++) { k[i] = k32[i]; }
+            }
             for (uint32_t i = 0; i < n; i++) { dst[i] = k[i] >= h.frame.emptyKey ? ~0ull : k[i] - h.frame.keyBase; }
         };
         if (orderF && h.nF)
@@ -2071,13 +2102,13 @@ int tsp_get_sort(tsp_ctx* ctx, uint32_t* orderF, uint64_t* keysF, uint32_t* orde
             hipCheck(hipMemcpy(orderF, d.idxF + h.offF, sizeof(uint32_t) * h.nF, hipMemcpyDeviceToHost), "D2H");
             for (uint32_t i = 0; i < h.nF; i++) { orderF[i] -= h.offF; }
         }
-        if (keysF) { widen(keysF, d.keyF + h.offF, h.nF); }
+        if (keysF) { widen(keysF, d.keyF, h.offF, h.nF); }
         if ((orderB || keysB) && h.nB)
         {
             // the boundary order relative to the pulled order, from the ids (sequence ids on pull)
             launchKeys(ctx->stream, d, kPopB, d.bpos, d.nB, false, d.keyF2, d.idxF2);
             ctx->sync();
-            if (keysB) { widen(keysB, d.keyF2 + h.offB, h.nB); }
+            if (keysB) { widen(keysB, d.keyF2, h.offB, h.nB); }
             if (orderB)
             {
                 std::vector<uint64_t> idb(h.nB);

@@ -33,6 +33,8 @@
 //                          blocks walk the 256-particle tiles, instead of one block per tile
 //   TSP_TUNE_SLICE         SELL slice height: 32 (a wave covers two slices) or 64 (one slice per wave)
 //   TSP_TUNE_RADIX_BITS    digit width of the LSD radix sorts (8: 256 buckets, 6, 4)
+//   TSP_TUNE_SWEEP_THREADS threads per 256-particle tile of the solver sweeps (256, or 128: two
+//                          particles per thread)
 //   TSP_TUNE_SEARCH_ROW    the neighbor query walks the stencil by rows of three cells, loading
 //                          the rows' ranges together and TSP_TUNE_SEARCH_BATCH candidates at a time
 //                          (1: the row's candidates as one stream, 2: cell by cell; 0: the cell walk)
@@ -75,6 +77,9 @@
 #endif
 #ifndef TSP_TUNE_RADIX_BITS
 #define TSP_TUNE_RADIX_BITS 8
+#endif
+#ifndef TSP_TUNE_SWEEP_THREADS
+#define TSP_TUNE_SWEEP_THREADS 256
 #endif
 #define TSP_WAVES_ATTR_0
 #define TSP_WAVES_ATTR_1 __attribute__((amdgpu_waves_per_eu(1)))
@@ -185,9 +190,9 @@ namespace tsp
         uint32_t dim[3];
         float cellSizeInv;
         int32_t reach;       // stencil half width in cells
-        uint32_t keyBase;    // the scene's Morton ids are keyBase + morton(local cell)
-        uint32_t emptyKey;   // keyBase + (1 << 3 axisBits): particles outside the frame sort last
-        uint32_t deadKey;    // emptyKey + 1: the padding slots of the fluid segment
+        uint64_t keyBase;    // the scene's Morton ids are keyBase + morton(local cell)
+        uint64_t emptyKey;   // keyBase + (1 << 3 axisBits): particles outside the frame sort last
+        uint64_t deadKey;    // emptyKey + 1: the padding slots of the fluid segment
         uint32_t cellOff;    // the scene's first cell in the cell tables
         int32_t innerLo[3], innerHi[3];
         int32_t domainSet;
@@ -283,7 +288,8 @@ namespace tsp
         Frame *frames;
         Consts *consts;
         Solve *solve;
-        uint32_t *sceneKey0;         // nScenes + 1 ascending key bases (the last one ends the key space)
+        uint32_t key64;              // the keys are 64-bit (some frame needs more than 32 bits of key space)
+        uint64_t *sceneKey0;         // nScenes + 1 ascending key bases (the last one ends the key space)
         uint32_t *blockScene;        // scene of each fluid thread block
         uint16_t *sceneB;            // scene of each boundary particle
         uint32_t *bodyScene;         // scene of each rigid body
@@ -305,7 +311,7 @@ namespace tsp
         float *rbForce;          // nRB x 6: force, torque (float, like the reference members)
         uint32_t *byBodyOff, *byBodyIdx;
         // neighbor search
-        uint32_t *keyF, *keyF2, *idxF, *idxF2;
+        uint32_t *keyF, *keyF2, *idxF, *idxF2;   // keys: u32, or u64 (two words each) when key64
         uint32_t *keyR, *keyR2, *idxR, *idxR2;
         uint2 *cellF, *cellB, *cellR;
         // SELL-32
@@ -346,8 +352,14 @@ namespace tsp
 
     __device__ __forceinline__ float rbv(const Dev& d, int f, uint32_t b) { return d.rb[f * d.nRB + b]; }
 
+    // key i of a key array (u32, or u64 when the context's keys are 64-bit)
+    __device__ __forceinline__ uint64_t keyAt(const Dev& d, const uint32_t* keys, uint32_t i)
+    {
+        return d.key64 ? reinterpret_cast<const uint64_t*>(keys)[i] : static_cast<uint64_t>(keys[i]);
+    }
+
     // scene of a key: the last scene whose key base is not above it
-    __device__ __forceinline__ uint32_t sceneOfKey(const Dev& d, uint32_t key)
+    __device__ __forceinline__ uint32_t sceneOfKey(const Dev& d, uint64_t key)
     {
         uint32_t lo = 0, hi = d.nScenes;
         while (hi - lo > 1)
@@ -367,6 +379,29 @@ namespace tsp
     {
         const int i = static_cast<int>(v);
         return (v < static_cast<float>(i)) ? i - 1 : i;
+    }
+
+    // SpreadBitsWithGaps_21to63 (neighbor_search.cpp:2057-2070): bit b of v to bit 3b
+    __device__ __forceinline__ uint64_t spread21(uint32_t v)
+    {
+        uint64_t x = v & 0x1fffffu;
+        x = (x | (x << 32)) & 0x1f00000000ffffull;
+        x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+        x = (x | (x << 8)) & 0x100f00f00f00f00full;
+        x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+        x = (x | (x << 2)) & 0x1249249249249249ull;
+        return x;
+    }
+
+    __device__ __forceinline__ uint32_t compact21(uint64_t x)
+    {
+        x &= 0x1249249249249249ull;
+        x = (x | (x >> 2)) & 0x10c30c30c30c30c3ull;
+        x = (x | (x >> 4)) & 0x100f00f00f00f00full;
+        x = (x | (x >> 8)) & 0x1f0000ff0000ffull;
+        x = (x | (x >> 16)) & 0x1f00000000ffffull;
+        x = (x | (x >> 32)) & 0x1fffffull;
+        return static_cast<uint32_t>(x);
     }
 
     __device__ __forceinline__ uint32_t spread10(uint32_t v)
@@ -390,10 +425,16 @@ namespace tsp
     }
 
     // Morton id of local cell coordinates (x bits at 0,3,6..., y at 1,4,..., z at 2,5,...),
-    // MapToCellId_Morton, neighbor_search.cpp:2032-2055, for frames of up to 1024 cells per axis
+    // MapToCellId_Morton, neighbor_search.cpp:2032-2055: 32 bits for frames of up to 1024 cells
+    // per axis, 63 bits for frames of up to 2^21
     __device__ __forceinline__ uint32_t morton(uint32_t lx, uint32_t ly, uint32_t lz)
     {
         return spread10(lx) | (spread10(ly) << 1) | (spread10(lz) << 2);
+    }
+
+    __device__ __forceinline__ uint64_t morton64(uint32_t lx, uint32_t ly, uint32_t lz)
+    {
+        return spread21(lx) | (spread21(ly) << 1) | (spread21(lz) << 2);
     }
 
     // index of a local cell of a scene's frame in the cell tables

@@ -21,10 +21,11 @@ size buckets (by fluid count), each with its own context, stream and host thread
 small scenes' steps overlap with the large ones' on the GPU instead of waiting for them; the
 buckets share the HBM budget.
 
-Besides HBM, a context's 32-bit key space is an admission resource: every resident scene holds
-a key range sized by its cell frame (2^(3 bits) for frames of up to 2^bits cells per axis). A
-context refuses scenes whose frames do not fit (TSP_ERR_CAPACITY) and keeps its resident scenes;
-the loop then admits fewer of the waiting requests, or holds them until a resident scene leaves.
+Besides HBM, a context's cell tables are an admission resource: every resident scene holds the
+cells of its cell frame (24 bytes each, dense over the frame's volume), and a context takes at
+most its cell limit of them (tsp_set_cell_limit; 2^32 - 1 by default). A context refuses scenes
+whose frames do not fit (TSP_ERR_CAPACITY) and keeps its resident scenes; the loop then admits
+fewer of the waiting requests, or holds them until a resident scene leaves.
 """
 import collections
 import threading
@@ -104,13 +105,15 @@ class _Budget:
             self.used -= n
 
 
-def _serve_bucket(requests, budget, chunk_steps, device, keep_results, t0):
+def _serve_bucket(requests, budget, chunk_steps, device, keep_results, t0, cell_limit=None):
     """Continuous batching of one size bucket in its own context (arrival-order admission)."""
     batch = HipBatch(device)
+    if cell_limit is not None:
+        batch.set_cell_limit(cell_limit)
     pending = collections.deque(sorted(requests, key=lambda r: r.arrival_s))
     waiting = collections.deque()
     active = {}            # scene id -> [request, steps left]
-    hold = False           # the key space refused the head of the queue: wait for a completion
+    hold = False           # the context refused the head of the queue: wait for a completion
     try:
         while pending or waiting or active:
             now = time.perf_counter() - t0
@@ -183,11 +186,12 @@ def _serve_bucket(requests, budget, chunk_steps, device, keep_results, t0):
 DEFAULT_BUCKETS = (60_000, 400_000)
 
 
-def serve_batched(requests, hbm_budget_bytes, chunk_steps=10, device=0, keep_results=True, buckets=DEFAULT_BUCKETS):
+def serve_batched(requests, hbm_budget_bytes, chunk_steps=10, device=0, keep_results=True, buckets=DEFAULT_BUCKETS,
+                  cell_limit=None):
     """Serves `requests` (arrival times relative to the call) with continuous batching, one
     context per size bucket (fluid counts split at `buckets`), the buckets' host threads running
     side by side; returns the wall time from the start to the last completion. Fills admitted_s,
-    done_s and result."""
+    done_s and result. `cell_limit` caps each context's cell tables (tsp_set_cell_limit)."""
     edges = sorted(buckets)
     groups = [[] for _ in range(len(edges) + 1)]
     for r in requests:
@@ -201,7 +205,7 @@ def serve_batched(requests, hbm_budget_bytes, chunk_steps=10, device=0, keep_res
 
     def run(g):
         try:
-            _serve_bucket(g, budget, chunk_steps, device, keep_results, t0)
+            _serve_bucket(g, budget, chunk_steps, device, keep_results, t0, cell_limit)
         except BaseException as e:   # re-raised in the caller
             errors.append(e)
 

@@ -2,7 +2,7 @@
 
 Every variant gives the same bits; they differ in how the work maps onto the GPU: the SELL slice
 height (32 or 64 particles per slice), the radix sort's digit width (8, 6 or 4 bits: 4 to 8 passes
-over 28-bit keys), prefetched list entries, 24- or 32-byte Jacobi halo records, stored or
+over 28-bit keys), the threads of a solver sweep tile (256, or 128 with two particles each), prefetched list entries, 24- or 32-byte Jacobi halo records, stored or
 recomputed gradient factors, the halo cap (LDS per sweep block), register budgets of the sweep
 kernels, sweep groups as HIP graphs, the neighbor query's walk of the stencil (cell by cell, or
 rows of three cells) and its candidate loads in flight. The
@@ -37,6 +37,7 @@ TUNED_H = os.path.join(HERE, "csrc", "tsp_tuned.h")
 SPACE = {
     "TSP_TUNE_SLICE": [32, 64],
     "TSP_TUNE_RADIX_BITS": [8, 6, 4],
+    "TSP_TUNE_SWEEP_THREADS": [256, 128],
     "TSP_TUNE_PREFETCH": [1, 0],
     "TSP_TUNE_JACOBI_REC24": [1, 0],
     "TSP_TUNE_KG_STORE": [0, 1],
