@@ -1,5 +1,5 @@
 // tsp_tuned.h -- build-time variant chosen by turbospharticles_amd/tune.py on MI355X
-// (1M rigid cube drop: 4.082 ms per step; every candidate checked bit-identical)
+// (1M rigid cube drop: 3.79 ms per step; every candidate checked bit-identical)
 #ifndef TSP_TUNE_GRAPHS
 #define TSP_TUNE_GRAPHS 1
 #endif
@@ -24,9 +24,18 @@
 #ifndef TSP_TUNE_PREFETCH
 #define TSP_TUNE_PREFETCH 1
 #endif
+#ifndef TSP_TUNE_RADIX_BITS
+#define TSP_TUNE_RADIX_BITS 8
+#endif
 #ifndef TSP_TUNE_SEARCH_BATCH
 #define TSP_TUNE_SEARCH_BATCH 4
 #endif
 #ifndef TSP_TUNE_SEARCH_ROW
 #define TSP_TUNE_SEARCH_ROW 1
+#endif
+#ifndef TSP_TUNE_SLICE
+#define TSP_TUNE_SLICE 64
+#endif
+#ifndef TSP_TUNE_SWEEP_THREADS
+#define TSP_TUNE_SWEEP_THREADS 256
 #endif
