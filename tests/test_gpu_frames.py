@@ -1,10 +1,11 @@
-"""Cell frames beyond 1024 cells per axis: 64-bit Morton keys on the HIP path.
+"""Cell frames beyond 1024 cells per axis, and 64-bit sort keys on the HIP path.
 
-A 21 m channel (1.0 x 0.1 x 0.1 cross-section scaled: 21 x 0.1 x 0.1 m at h = 0.01) with a small
-fluid block at one end: its walls give a cell frame of 1070 x 25 x 25 cells, whose Morton ids need
-33 bits of key space (the reference's frames take up to 2^21 cells per axis with 63-bit ids,
-core/cell_frame.h:29, neighbor_search.cpp:2032-2070). Morton order, every list and three steps must
-equal the oracle bit for bit; the same scene in a batch with an ordinary dam break equals both alone.
+A 21 m channel (21 x 0.1 x 0.1 m at h = 0.01) with a small fluid block at one end: its walls give
+a cell frame of 1070 x 25 x 25 cells, whose Morton ids reach bit 30 (the reference's frames take
+up to 2^21 cells per axis with 63-bit ids, core/cell_frame.h:29, neighbor_search.cpp:2032-2070).
+A 42 m channel (2,120 cells along x) has Morton ids beyond 2^32, so the context sorts 64-bit keys.
+Morton order, every list and three steps must equal the oracle bit for bit; the channels in a
+batch with an ordinary dam break (64-bit keys for all three) equal each scene alone.
 """
 import numpy as np
 import pytest
@@ -20,13 +21,14 @@ pytestmark = pytest.mark.gpu
 H = 0.01
 
 
-def channel():
-    return scenes.build_scene(scenes.dam_break_spec(0, H, counts=(20, 8, 8), tank=(21.0, 0.1, 0.1)))
+def channel(length=21.0):
+    return scenes.build_scene(scenes.dam_break_spec(0, H, counts=(20, 8, 8), tank=(length, 0.1, 0.1)))
 
 
-def test_frame_beyond_1024_cells_per_axis_matches_the_oracle():
-    print("case: 21 m channel, frame 1070 x 25 x 25 cells (64-bit keys)", flush=True)
-    st = channel()
+@pytest.mark.parametrize("length,key_bits", [(21.0, 30), (42.0, 33)])
+def test_frame_beyond_1024_cells_per_axis_matches_the_oracle(length, key_bits):
+    print(f"case: {length} m channel, Morton ids beyond {key_bits} bits", flush=True)
+    st = channel(length)
     p = scenes.make_params(H, termination_cond=scenes.TERM_POS)
     g = HipBackend(0)
     g.set_params(p)
@@ -45,7 +47,7 @@ def test_frame_beyond_1024_cells_per_axis_matches_the_oracle():
     assert np.array_equal(kf_g, kf_o)
     # the boundary's cell ids as a multiset, and ids beyond 30 bits (x cells past 1024)
     assert np.array_equal(np.sort(kb_g), np.sort(kb_o))
-    assert int(kb_o.max()).bit_length() > 30
+    assert int(kb_o.max()).bit_length() > key_bits
     for rel in (O.FF, O.FB):
         cg, ig = g.neighbors(rel)
         co, io, _ = o.lists(rel)
@@ -62,9 +64,9 @@ def test_frame_beyond_1024_cells_per_axis_matches_the_oracle():
 
 
 def test_large_frame_scene_in_a_batch_matches_each_scene_alone():
-    print("case: the channel and a dam break in one context, 4 steps", flush=True)
+    print("case: both channels and a dam break in one context, 4 steps", flush=True)
     p = scenes.make_params(H, termination_cond=scenes.TERM_POS)
-    states = [channel(), scenes.build_scene(scenes.dam_break_spec(12, H, tank=(0.4, 0.3, 0.3)))]
+    states = [channel(), scenes.build_scene(scenes.dam_break_spec(12, H, tank=(0.4, 0.3, 0.3))), channel(42.0)]
     alone = []
     for st in states:
         g = HipBackend(0)

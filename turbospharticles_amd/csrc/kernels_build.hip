@@ -173,7 +173,8 @@ namespace tsp
         bvol[i] = vol;
     }
 
-    void launchRadixSort(hipStream_t, bool, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*);
+    void launchRadixSort(hipStream_t, bool, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*,
+                         uint32_t*);
 
     void launchBuildFluid(hipStream_t s, const Dev& d, uint32_t off, uint32_t n, const BuildFluid& b)
     {
@@ -181,14 +182,14 @@ namespace tsp
     }
 
     // scratch: xyz 3n floats; key, idx, key2, idx2 n words each; hist 256 per 2048-key tile;
-    // tmp the scan's block sums
+    // tmp a zeroed scan state for the digit counts (its word 2 is set if a scan gave up)
     void launchBuildBoundary(hipStream_t s, const BuildBoundary& c, uint32_t n, float4* bpos, float* bvol, uint64_t* ids,
                              float* xyz, uint32_t* key, uint32_t* idx, uint32_t* key2, uint32_t* idx2, uint32_t* hist,
                              uint32_t* tmp)
     {
         if (!n) { return; }
         kBuildBoundaryPos<<<blocksN(n), kBlock, 0, s>>>(c, n, xyz, ids, key, idx);
-        launchRadixSort(s, false, key, idx, key2, idx2, n, c.bits[0] + c.bits[1] + c.bits[2], hist, tmp);
+        launchRadixSort(s, false, key, idx, key2, idx2, n, c.bits[0] + c.bits[1] + c.bits[2], hist, tmp, tmp + 2);
         kBuildBoundaryCoeff<<<blocksN(n), kBlock, 0, s>>>(c, n, xyz, key, idx, bpos, bvol);
     }
 }

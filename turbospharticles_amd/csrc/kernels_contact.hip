@@ -555,14 +555,14 @@ namespace tsp
     // ----------------------------------------------------------------------------------------
     // launchers
 
-    void launchScan(hipStream_t s, uint32_t* v, uint32_t m, uint32_t* tmp);
+    void launchScan(hipStream_t s, uint32_t* v, uint32_t m, uint32_t* tmp, uint32_t* fail);
 
     // RB/RR offsets from the search counts (before the contact fill of the rigid query)
     void launchContactOffsets(hipStream_t s, const Dev& d, uint32_t* tmp)
     {
         kContactOffsets<<<blocksN(d.nR + 1ull), kBlock, 0, s>>>(d);
-        launchScan(s, d.rbOff, d.nR + 1, tmp);
-        launchScan(s, d.rrOff, d.nR + 1, tmp);
+        launchScan(s, d.rbOff, d.nR + 1, tmp, &d.status->scanFail);
+        launchScan(s, d.rrOff, d.nR + 1, tmp, &d.status->scanFail);
     }
 
     // the contact boundary particles and their BR lists for every scene, then one block per scene
@@ -572,10 +572,10 @@ namespace tsp
         (void)hipMemsetAsync(d.cbScan, 0, sizeof(uint32_t) * (d.nB + 1ull), s);
         (void)hipMemsetAsync(d.brOff, 0, sizeof(uint32_t) * (nCBMax + 1ull), s);
         kContactMark<<<blocksN(d.nR), kBlock, 0, s>>>(d);
-        launchScan(s, d.cbScan, d.nB + 1, tmp);
+        launchScan(s, d.cbScan, d.nB + 1, tmp, &d.status->scanFail);
         if (d.nB) { kContactLocal<<<blocksN(d.nB), kBlock, 0, s>>>(d); }
         kContactBrCount<<<blocksN(d.nR), kBlock, 0, s>>>(d);
-        launchScan(s, d.brOff, nCBMax + 1, tmp);
+        launchScan(s, d.brOff, nCBMax + 1, tmp, &d.status->scanFail);
         (void)hipMemcpyAsync(d.brPos, d.brOff, sizeof(uint32_t) * (nCBMax + 1ull), hipMemcpyDeviceToDevice, s);
         kContactBrFill<<<blocksN(d.nR), kBlock, 0, s>>>(d);
         if (nCBMax) { kContactBrSort<<<blocksN(nCBMax), kBlock, 0, s>>>(d); }

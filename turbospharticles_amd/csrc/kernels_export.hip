@@ -48,7 +48,7 @@ namespace tsp
         speed[o] = __builtin_sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
     }
 
-    void launchScan(hipStream_t s, uint32_t* v, uint32_t m, uint32_t* tmp);
+    void launchScan(hipStream_t s, uint32_t* v, uint32_t m, uint32_t* tmp, uint32_t* fail);
 
     // the scene's exported fluid particles: flags (n + 1 words) end as their exclusive scan,
     // flags[n] the count; xyz and speed receive them in particle order
@@ -57,7 +57,7 @@ namespace tsp
     {
         (void)hipMemsetAsync(flags, 0, sizeof(uint32_t) * (n + 1ull), s);
         if (n) { kSurfaceMark<<<blocksN(n), kBlock, 0, s>>>(d, off, n, threshold, flags); }
-        launchScan(s, flags, n + 1, tmp);
+        launchScan(s, flags, n + 1, tmp, &d.status->scanFail);
         if (n) { kSurfaceCompact<<<blocksN(n), kBlock, 0, s>>>(d, off, n, flags, xyz, speed); }
     }
 }

@@ -49,39 +49,72 @@ namespace tsp
         return r;
     }
 
-    __global__ void __launch_bounds__(kBlock) kScanSums(const uint32_t* __restrict__ v, uint32_t m, uint32_t* __restrict__ sums)
-    {
-        const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 8u;
-        uint32_t s = 0;
-        for (uint32_t q = 0; q < 8; q++) { if (base + q < m) { s += v[base + q]; } }
-        uint32_t total;
-        blockExclusiveScan(s, total);
-        if (threadIdx.x == 0) { sums[blockIdx.x] = total; }
-    }
+    // One-pass exclusive scan (decoupled look-back, Merrill & Garland 2016): each block takes the
+    // next tile number from a counter, so a tile waits only on tiles whose blocks started before
+    // it; it publishes its sum, adds its predecessors' sums walking back to the first inclusive
+    // prefix, publishes that prefix and writes its values. Status words carry the scan's epoch
+    // (bit 33 up), a prefix flag (bit 32) and the value, so no clearing pass is needed: the block
+    // of the last tile resets the counter and counts the scan in the epoch word. A scan's epoch is
+    // 1 + (scans before it mod 2^31 - 1), never 0, so the zeroed state of a new buffer matches no
+    // scan. A wait longer than kScanSpins polls gives up and sets the fail word (never expected;
+    // the host reports it).
+    constexpr uint32_t kScanSpins = 1u << 20;
 
-    // one block: exclusive scan of the tile sums (any count)
-    __global__ void __launch_bounds__(kBlock) kScanTop(uint32_t* __restrict__ sums, uint32_t nb)
+    __global__ void __launch_bounds__(kBlock) kScanOnePass(uint32_t* __restrict__ v, uint32_t m, uint32_t* __restrict__ tmp,
+        uint32_t* __restrict__ fail)
     {
-        uint32_t carry = 0;
-        for (uint32_t b0 = 0; b0 < nb; b0 += kBlock)
+        __shared__ uint32_t sTile, sEpoch, sPrefix;
+        uint32_t* counter = tmp;
+        uint32_t* epochWord = tmp + 1;
+        unsigned long long* status = reinterpret_cast<unsigned long long*>(tmp + kScanHeaderWords);
+        const uint32_t nb = (m + kScanTile - 1) / kScanTile;
+        if (threadIdx.x == 0)
         {
-            const uint32_t b = b0 + threadIdx.x;
-            const uint32_t v = b < nb ? sums[b] : 0u;
-            uint32_t total;
-            const uint32_t r = blockExclusiveScan(v, total);
-            if (b < nb) { sums[b] = carry + r; }
-            carry += total;
+            sEpoch = __hip_atomic_load(epochWord, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            sTile = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         }
-    }
-
-    __global__ void __launch_bounds__(kBlock) kScanApply(uint32_t* __restrict__ v, uint32_t m, const uint32_t* __restrict__ sums)
-    {
-        const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 8u;
+        __syncthreads();
+        const uint32_t tile = sTile;
+        const unsigned long long tag = static_cast<unsigned long long>(sEpoch % 0x7fffffffu + 1u) << 33;
+        const uint32_t base = tile * kScanTile + threadIdx.x * 8u;
         uint32_t loc[8];
-        uint32_t s = 0;
-        for (uint32_t q = 0; q < 8; q++) { loc[q] = base + q < m ? v[base + q] : 0u; s += loc[q]; }
+        uint32_t sum = 0;
+        for (uint32_t q = 0; q < 8; q++) { loc[q] = base + q < m ? v[base + q] : 0u; sum += loc[q]; }
         uint32_t total;
-        uint32_t run = sums[blockIdx.x] + blockExclusiveScan(s, total);
+        const uint32_t r = blockExclusiveScan(sum, total);
+        if (threadIdx.x == 0)
+        {
+            constexpr unsigned long long kPrefix = 1ull << 32;
+            uint32_t prefix = 0;
+            if (tile > 0)
+            {
+                __hip_atomic_store(&status[tile], tag | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                uint32_t j = tile - 1, spins = 0;
+                for (;;)
+                {
+                    const unsigned long long w = __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                    if ((w >> 33) != (tag >> 33))
+                    {
+                        if (++spins > kScanSpins) { atomicOr(fail, 1u); break; }
+                        __builtin_amdgcn_s_sleep(2);
+                        continue;
+                    }
+                    prefix += static_cast<uint32_t>(w);
+                    if ((w & kPrefix) || j == 0) { break; }
+                    j--;
+                }
+            }
+            __hip_atomic_store(&status[tile], tag | kPrefix | (prefix + total), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            sPrefix = prefix;
+            if (tile == nb - 1)
+            {
+                // every block has drawn its tile: ready the counter and epoch for the next scan
+                __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(epochWord, sEpoch + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        __syncthreads();
+        uint32_t run = sPrefix + r;
         for (uint32_t q = 0; q < 8; q++)
         {
             if (base + q < m) { v[base + q] = run; }
@@ -89,14 +122,13 @@ namespace tsp
         }
     }
 
-    // exclusive scan of m values in place; tmp holds ceil(m / 2048) block sums
-    void launchScan(hipStream_t s, uint32_t* v, uint32_t m, uint32_t* tmp)
+    // exclusive scan of m values in place; tmp is a scan state of scanTmpWords(m) words, zeroed
+    // when allocated (fail: set if a look-back gave up)
+    void launchScan(hipStream_t s, uint32_t* v, uint32_t m, uint32_t* tmp, uint32_t* fail)
     {
         if (m == 0) { return; }
         const uint32_t nb = (m + kScanTile - 1) / kScanTile;
-        kScanSums<<<nb, kBlock, 0, s>>>(v, m, tmp);
-        kScanTop<<<1, kBlock, 0, s>>>(tmp, nb);
-        kScanApply<<<nb, kBlock, 0, s>>>(v, m, tmp);
+        kScanOnePass<<<nb, kBlock, 0, s>>>(v, m, tmp, fail);
     }
 
     // clears the search status block and the scenes' frame-abort flags
@@ -104,7 +136,8 @@ namespace tsp
     {
         constexpr uint32_t words = static_cast<uint32_t>(sizeof(Status) / sizeof(uint32_t));
         static_assert(words <= kBlock, "the reset covers one block");
-        if (threadIdx.x < words) { reinterpret_cast<uint32_t*>(d.status)[threadIdx.x] = 0u; }
+        constexpr uint32_t keep = static_cast<uint32_t>(offsetof(Status, scanFail) / sizeof(uint32_t));
+        if (threadIdx.x < words && threadIdx.x != keep) { reinterpret_cast<uint32_t*>(d.status)[threadIdx.x] = 0u; }
         for (uint32_t s = threadIdx.x; s < d.nScenes; s += kBlock)
         {
             Solve& so = d.solve[s];
@@ -497,7 +530,8 @@ namespace tsp
     // The candidates of the stencil, per relation in list order: cells in z-y-x raster order,
     // ascending index inside a cell. onF(j, pos), onB(j, bpos), onR(j, rpos) see each candidate of
     // their relation once; the relations are separate lists, so only the order inside each one
-    // matters. withF = false skips the fluid candidates.
+    // matters. withF / withB / withR = false skip a relation's candidates (and its cell ranges);
+    // needF(row) = false skips a row's fluid candidates.
 #if TSP_TUNE_SEARCH_ROW
     // Row form: the ranges of up to three x-adjacent cells load together, and each relation's
     // candidates along the row form one stream whose loads issue TSP_TUNE_SEARCH_BATCH at a time
@@ -561,9 +595,9 @@ namespace tsp
 
     // preF(row, rf) may take a row's fluid candidates itself (returns true) instead of the stream;
     // postF(row) follows the row's fluid part. row counts the rows of three cells in stencil order.
-    template <typename OnF, typename OnB, typename OnR, typename PreF, typename PostF>
-    __device__ __forceinline__ void forStencil(const Dev& d, const Frame& f, const Stencil& s, bool withF, OnF onF,
-                                               OnB onB, OnR onR, PreF preF, PostF postF)
+    template <typename OnF, typename OnB, typename OnR, typename NeedF, typename PreF, typename PostF>
+    __device__ __forceinline__ void forStencil(const Dev& d, const Frame& f, const Stencil& s, bool withF, bool withB,
+                                               bool withR, OnF onF, OnB onB, OnR onR, NeedF needF, PreF preF, PostF postF)
     {
         const auto same = [](uint32_t e) { return e; };
         uint32_t row = 0;
@@ -575,16 +609,17 @@ namespace tsp
                 {
                     const int nx = min(3, s.hi[0] - cx + 1);
                     const uint32_t lin = linearCell(f, cx, cy, cz);
+                    const bool rowF = withF && needF(row);
                     uint2 rf[3], rb[3], rr[3];
 #pragma unroll
                     for (int c = 0; c < 3; c++)
                     {
                         const bool in = c < nx;
-                        rf[c] = (in && withF) ? d.cellF[lin + c] : make_uint2(0u, 0u);
-                        rb[c] = in ? d.cellB[lin + c] : make_uint2(0u, 0u);
-                        rr[c] = (in && d.nR) ? d.cellR[lin + c] : make_uint2(0u, 0u);
+                        rf[c] = (in && rowF) ? d.cellF[lin + c] : make_uint2(0u, 0u);
+                        rb[c] = (in && withB) ? d.cellB[lin + c] : make_uint2(0u, 0u);
+                        rr[c] = (in && withR && d.nR) ? d.cellR[lin + c] : make_uint2(0u, 0u);
                     }
-                    if (withF)
+                    if (rowF)
                     {
                         if (!preF(row, rf))
                         {
@@ -593,10 +628,16 @@ namespace tsp
                         }
                         postF(row);
                     }
-                    rowStream(rb, same, [&](uint32_t j) { return d.bpos[j]; },
-                              [&](uint32_t j, const float4& p, uint32_t) { onB(j, p); });
-                    rowStream(rr, [&](uint32_t e) { return d.idxR[e]; }, [&](uint32_t j) { return d.rpos[j]; },
-                              [&](uint32_t j, const float4& p, uint32_t) { onR(j, p); });
+                    if (withB)
+                    {
+                        rowStream(rb, same, [&](uint32_t j) { return d.bpos[j]; },
+                                  [&](uint32_t j, const float4& p, uint32_t) { onB(j, p); });
+                    }
+                    if (withR)
+                    {
+                        rowStream(rr, [&](uint32_t e) { return d.idxR[e]; }, [&](uint32_t j) { return d.rpos[j]; },
+                                  [&](uint32_t j, const float4& p, uint32_t) { onR(j, p); });
+                    }
                 }
             }
         }
@@ -604,9 +645,9 @@ namespace tsp
 #else
     // Cell form: one cell at a time, its candidates one at a time (no hit masks: every row is
     // tested, row numbers are not passed)
-    template <typename OnF, typename OnB, typename OnR, typename PreF, typename PostF>
-    __device__ __forceinline__ void forStencil(const Dev& d, const Frame& f, const Stencil& s, bool withF, OnF onF,
-                                               OnB onB, OnR onR, PreF, PostF)
+    template <typename OnF, typename OnB, typename OnR, typename NeedF, typename PreF, typename PostF>
+    __device__ __forceinline__ void forStencil(const Dev& d, const Frame& f, const Stencil& s, bool withF, bool withB,
+                                               bool withR, OnF onF, OnB onB, OnR onR, NeedF, PreF, PostF)
     {
         for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
         {
@@ -620,9 +661,12 @@ namespace tsp
                         const uint2 rf = d.cellF[lin];
                         for (uint32_t j = rf.x; j < rf.y; j++) { onF(j, d.pos[j], kMaskRows, 0u); }
                     }
-                    const uint2 rb = d.cellB[lin];
-                    for (uint32_t j = rb.x; j < rb.y; j++) { onB(j, d.bpos[j]); }
-                    if (d.nR)
+                    if (withB)
+                    {
+                        const uint2 rb = d.cellB[lin];
+                        for (uint32_t j = rb.x; j < rb.y; j++) { onB(j, d.bpos[j]); }
+                    }
+                    if (withR && d.nR)
                     {
                         const uint2 rr = d.cellR[lin];
                         for (uint32_t q = rr.x; q < rr.y; q++)
@@ -656,7 +700,7 @@ namespace tsp
             const float4 pi = d.pos[i];
             const bool masks = TSP_TUNE_SEARCH_ROW == 1 && f.reach == 1;
             if (masks) { uncovered = 0; }
-            forStencil(d, f, s, true,
+            forStencil(d, f, s, true, true, true,
                 [&](uint32_t j, const float4& pj, uint32_t row, uint32_t q)
                 {
                     if (within(pi.x, pi.y, pi.z, pj, r2))
@@ -678,6 +722,7 @@ namespace tsp
                 {
                     if (within(pi.x, pi.y, pi.z, pj, r2)) { cnt[2]++; mn[2] = min(mn[2], j); mx[2] = max(mx[2], j); }
                 },
+                [&](uint32_t) { return true; },
                 [&](uint32_t, const uint2 (&)[3]) { return false; },
                 [&](uint32_t row)
                 {
@@ -794,13 +839,16 @@ namespace tsp
             const float4 pi = d.pos[i];
             uint32_t kF = 0, kB = 0, kR = 0;
             const uint32_t uncovered = d.ffMask[static_cast<size_t>(kMaskRows) * d.nF + i];
+            // the row's hit mask, read before its cell ranges: a covered row without hits loads
+            // no range; relations without entries are not walked at all
+            uint32_t rowMask = 0;
             const auto ffHit = [&](uint32_t j)
             {
                 if (staged && kF < kStage) { stage[kF][t] = static_cast<uint16_t>(j - mF.base); }
                 else { sellWrite(d.pool[REL_FF], mF, lane, kF, j); }
                 kF++;
             };
-            forStencil(d, f, s, !wide,
+            forStencil(d, f, s, !wide, d.cnt[REL_FB][i] != 0, d.cnt[REL_FR][i] != 0,
                 [&](uint32_t j, const float4& pj, uint32_t, uint32_t)
                 {
                     if (within(pi.x, pi.y, pi.z, pj, r2)) { ffHit(j); }
@@ -813,11 +861,17 @@ namespace tsp
                 {
                     if (within(pi.x, pi.y, pi.z, pj, r2)) { sellWrite(d.pool[REL_FR], mR, lane, kR++, j); }
                 },
+                [&](uint32_t row)
+                {
+                    if (row >= kMaskRows || ((uncovered >> row) & 1u)) { return true; }
+                    rowMask = d.ffMask[static_cast<size_t>(row) * d.nF + i];
+                    return rowMask != 0u;
+                },
                 // a row the count pass covered: its hits, in stream order, without loading a position
                 [&](uint32_t row, const uint2 (&rf)[3])
                 {
                     if (row >= kMaskRows || ((uncovered >> row) & 1u)) { return false; }
-                    uint32_t m = d.ffMask[static_cast<size_t>(row) * d.nF + i];
+                    uint32_t m = rowMask;
                     while (m)
                     {
                         const uint32_t q = static_cast<uint32_t>(__ffs(static_cast<int>(m))) - 1u;
@@ -1311,7 +1365,7 @@ namespace tsp
 
     template <typename K>
     static void radixSort(hipStream_t s, K* key, uint32_t* idx, K* key2, uint32_t* idx2, uint32_t n, uint32_t bits,
-                          uint32_t* hist, uint32_t* tmp)
+                          uint32_t* hist, uint32_t* tmp, uint32_t* fail)
     {
         if (n <= kSmallSort)
         {
@@ -1323,7 +1377,7 @@ namespace tsp
         for (uint32_t shift = 0; shift < bits; shift += kRadixBits)
         {
             kRadixHist<K><<<nTiles, kBlock, 0, s>>>(kIn, n, shift, nTiles, hist);
-            launchScan(s, hist, kRadixBins * nTiles, tmp);
+            launchScan(s, hist, kRadixBins * nTiles, tmp, fail);
             kRadixScatter<K><<<nTiles, kBlock, 0, s>>>(kIn, iIn, kOut, iOut, n, shift, nTiles, hist);
             K* t = kIn; kIn = kOut; kOut = t;
             uint32_t* ti = iIn; iIn = iOut; iOut = ti;
@@ -1335,16 +1389,18 @@ namespace tsp
         }
     }
 
-    // sorts (key, idx) stably by the low `bits` bits; the result ends in (key, idx)
+    // sorts (key, idx) stably by the low `bits` bits; the result ends in (key, idx); tmp is a scan
+    // state for 256 x ceil(n / 2048) values, fail its give-up word
     void launchRadixSort(hipStream_t s, bool key64, uint32_t* key, uint32_t* idx, uint32_t* key2, uint32_t* idx2,
-                         uint32_t n, uint32_t bits, uint32_t* hist, uint32_t* tmp)
+                         uint32_t n, uint32_t bits, uint32_t* hist, uint32_t* tmp, uint32_t* fail)
     {
         if (n < 2) { return; }
         if (key64)
         {
-            radixSort(s, reinterpret_cast<uint64_t*>(key), idx, reinterpret_cast<uint64_t*>(key2), idx2, n, bits, hist, tmp);
+            radixSort(s, reinterpret_cast<uint64_t*>(key), idx, reinterpret_cast<uint64_t*>(key2), idx2, n, bits, hist, tmp,
+                      fail);
         }
-        else { radixSort(s, key, idx, key2, idx2, n, bits, hist, tmp); }
+        else { radixSort(s, key, idx, key2, idx2, n, bits, hist, tmp, fail); }
     }
 
     void launchPermute(hipStream_t s, const Gather& g, const uint32_t* idx)
@@ -1379,7 +1435,7 @@ namespace tsp
         const uint32_t nSlices = (d.nF + kSlice - 1) / kSlice;
         const dim3 g(blocksFor(nSlices + 1ull), 3);
         kSliceUnits<<<g, kBlock, 0, s>>>(d, nSlices, units);
-        for (uint32_t r = 0; r < 3; r++) { launchScan(s, units + r * (nSlices + 1), nSlices + 1, tmp); }
+        for (uint32_t r = 0; r < 3; r++) { launchScan(s, units + r * (nSlices + 1), nSlices + 1, tmp, &d.status->scanFail); }
         kSliceApply<<<g, kBlock, 0, s>>>(d, nSlices, units);
     }
 
@@ -1399,7 +1455,7 @@ namespace tsp
         kQueryRigid<<<blocksFor(64ull * d.nR), kBlock, 0, s>>>(d, mode);
         if (mode == 0)
         {
-            launchScan(s, d.rfOff, d.nR + 1, tmp);
+            launchScan(s, d.rfOff, d.nR + 1, tmp, &d.status->scanFail);
             kRigidTotals<<<blocksFor(d.nR), kBlock, 0, s>>>(d);
         }
     }

@@ -443,8 +443,8 @@ namespace tsp
             sy = __builtin_fmaf(coeff, dy, sy);
             sz = __builtin_fmaf(coeff, dz, sz);
         };
-        visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t e) { ffTerm(hp[q], e); },
-                [&](uint32_t j, uint32_t e) { ffTerm(d.pos[j], e); });
+        visitFFSplit(ff, hc <= kHaloMax, [&](uint32_t q) { return hp[q]; }, ffTerm,
+                     [&](uint32_t j, uint32_t e) { ffTerm(d.pos[j], e); });
         float ax = dict ? sx * k.presOuterFF : sx, ay = dict ? sy * k.presOuterFF : sy, az = dict ? sz * k.presOuterFF : sz;
         const List fb = listOf(d, REL_FB, i);
         visit(fb, [&](uint32_t j, uint32_t)
@@ -537,26 +537,32 @@ namespace tsp
                 const float sv = __builtin_fmaf(axi - axj, dx, __builtin_fmaf(ayi - ayj, dy, (azi - azj) * dz));
                 sf = __builtin_fmaf(kg, sv, sf);
             };
+            // a halo record: position and a^p of one neighbor
+            struct Rec
+            {
+                float x, y, z, ax, ay, az;
+            };
 #if TSP_TUNE_JACOBI_REC24
-            const auto fromHalo = [&](uint32_t q, uint32_t e)
+            const auto haloLoad = [&](uint32_t q)
             {
                 const float4 pj = hp[q];
                 const float2 aj = ha[q];
-                ffTerm(pj.x, pj.y, pj.z, pj.w, aj.x, aj.y, e);
+                return Rec{ pj.x, pj.y, pj.z, pj.w, aj.x, aj.y };
             };
 #else
-            const auto fromHalo = [&](uint32_t q, uint32_t e)
+            const auto haloLoad = [&](uint32_t q)
             {
                 const float4 pj = hp[q], aj = ha[q];
-                ffTerm(pj.x, pj.y, pj.z, aj.x, aj.y, aj.z, e);
+                return Rec{ pj.x, pj.y, pj.z, aj.x, aj.y, aj.z };
             };
 #endif
-            visitFF(ff, hc <= kHaloMax, fromHalo,
-                    [&](uint32_t j, uint32_t e)
-                    {
-                        const float4 pj = d.pos[j], aj = d.ap[j];
-                        ffTerm(pj.x, pj.y, pj.z, aj.x, aj.y, aj.z, e);
-                    });
+            visitFFSplit(ff, hc <= kHaloMax, haloLoad,
+                         [&](const Rec& r, uint32_t e) { ffTerm(r.x, r.y, r.z, r.ax, r.ay, r.az, e); },
+                         [&](uint32_t j, uint32_t e)
+                         {
+                             const float4 pj = d.pos[j], aj = d.ap[j];
+                             ffTerm(pj.x, pj.y, pj.z, aj.x, aj.y, aj.z, e);
+                         });
             sf *= k.mass;
             float sb = 0.0f;
             const List fb = listOf(d, REL_FB, i);

@@ -31,7 +31,8 @@ namespace tsp
     void launchResetSearch(hipStream_t, const Dev&);
     void launchKeys(hipStream_t, const Dev&, int, const float4*, uint32_t, bool, uint32_t*, uint32_t*);
     void launchKeysSorted(hipStream_t, const Dev&, const float4*, uint32_t, uint32_t*, uint32_t*);
-    void launchRadixSort(hipStream_t, bool, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*);
+    void launchRadixSort(hipStream_t, bool, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*,
+                         uint32_t*);
     void launchPermute(hipStream_t, const Gather&, const uint32_t*);
     void launchClearCells(hipStream_t, const Dev&, const uint32_t*, uint32_t, uint2*);
     void launchClearAll(hipStream_t, uint2*, uint64_t);
@@ -423,6 +424,7 @@ struct tsp_ctx
         sync();
         prof.resolve();
         last = *hostStatus;
+        if (last.scanFail) { throw Error{ TSP_ERR_STATE, "a one-pass scan gave up waiting for a predecessor tile" }; }
     }
 
     void uploadConsts()
@@ -524,11 +526,14 @@ struct tsp_ctx
             uint32_t* key2 = alloc<uint32_t>(nB, tmp);
             uint32_t* idx2 = alloc<uint32_t>(nB, tmp);
             uint32_t* hist = alloc<uint32_t>(256ull * nTiles, tmp);
-            uint32_t* sums = alloc<uint32_t>((256ull * nTiles + 2047ull) / 2048ull + 1ull, tmp);
+            uint32_t* sums = alloc<uint32_t>(scanTmpWords(256ull * nTiles), tmp);
+            zero(sums, sizeof(uint32_t) * scanTmpWords(256ull * nTiles));
             launchBuildBoundary(stream, bb, nB, n.bpos + h.offB, n.bvol + h.offB, nidB + h.offB, xyz, key, idx, key2, idx2,
                                 hist, sums);
             hipCheck(hipGetLastError(), "device builder launch");
-            sync();
+            uint32_t scanFailed = 0;
+            d2h(&scanFailed, sums + 2, sizeof(uint32_t));
+            if (scanFailed) { throw Error{ TSP_ERR_STATE, "a one-pass scan gave up waiting for a predecessor tile" }; }
         }
         catch (...)
         {
@@ -563,7 +568,8 @@ struct tsp_ctx
         d.cErrB = alloc<double>(rbCap, contactAllocs);
         // block sums of the contact scans (boundary flags, BR offsets)
         const uint64_t m = std::max<uint64_t>(d.nB + 1ull, rbCap + 1);
-        contactScanTmp = alloc<uint32_t>((m + 2047ull) / 2048ull + 1ull, contactAllocs);
+        contactScanTmp = alloc<uint32_t>(scanTmpWords(m), contactAllocs);
+        zero(contactScanTmp, sizeof(uint32_t) * scanTmpWords(m));
     }
 
     // SolveRigidContacts (CPU/backend/rigid_solver.cpp:602-799) for every scene whose rigid
@@ -836,7 +842,8 @@ struct tsp_ctx
         {
             // block sums of the scans: radix digit offsets, slice units, RF offsets
             const uint64_t m = std::max<uint64_t>(256ull * nTiles, std::max<uint64_t>(nSlices + 1ull, nR + 1ull));
-            d.scanTmp = alloc<uint32_t>((m + 2047ull) / 2048ull + 1ull, work);
+            d.scanTmp = alloc<uint32_t>(scanTmpWords(m), work);
+            zero(d.scanTmp, sizeof(uint32_t) * scanTmpWords(m));
         }
         d.ffWideMin = alloc<uint32_t>(nF, work, MEM_LF);
         d.ffMask = alloc<uint32_t>(static_cast<size_t>(kMaskWords) * nF, work, MEM_SF);
@@ -1050,10 +1057,15 @@ struct tsp_ctx
         for (uint32_t s = 0; s < S; s++)
         {
             Frame& f = scenes[s].frame;
-            const uint32_t dimMax = std::max(f.dim[0], std::max(f.dim[1], f.dim[2]));
-            uint32_t axisBits = 1;
-            if (dimMax >= 2) { axisBits = 0; for (uint32_t v = dimMax - 1; v; v >>= 1) { axisBits++; } }
-            const uint64_t range = 1ull << (3 * axisBits);
+            // the scene's Morton ids lie below morton(dim - 1) + 1: the interleave is monotone in
+            // each coordinate (a range no wider than the frame needs, so the sort runs fewer passes)
+            const auto spread = [](uint64_t v)
+            {
+                uint64_t r = 0;
+                for (uint32_t b = 0; b < 21; b++) { r |= ((v >> b) & 1ull) << (3 * b); }
+                return r;
+            };
+            const uint64_t range = (spread(f.dim[0] - 1) | (spread(f.dim[1] - 1) << 1) | (spread(f.dim[2] - 1) << 2)) + 1;
             if (range + 2 > ~0ull - key)
             {
                 throw Error{ TSP_ERR_CAPACITY, "the scenes' cell frames exceed the 64-bit key space of one context" };
@@ -1100,7 +1112,8 @@ struct tsp_ctx
             sync();
             if (unsorted)
             {
-                launchRadixSort(stream, d.key64, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nB, keyBits, radixHist, d.scanTmp);
+                launchRadixSort(stream, d.key64, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nB, keyBits, radixHist, d.scanTmp,
+                                &d.status->scanFail);
                 Gather g{};
                 g.s4[0] = d.bpos; g.d4[0] = bpos2; g.n4 = 1;
                 g.s1[0] = d.bvol; g.d1[0] = bvol2; g.n1 = 1;
@@ -1139,12 +1152,14 @@ struct tsp_ctx
             {
                 Scope sc(prof, stream, "keys_sort_rigid");
                 launchKeys(stream, d, kPopR, d.rpos, d.nR, true, d.keyR, d.idxR);
-                launchRadixSort(stream, d.key64, d.keyR, d.idxR, d.keyR2, d.idxR2, d.nR, keyBits, radixHist, d.scanTmp);
+                launchRadixSort(stream, d.key64, d.keyR, d.idxR, d.keyR2, d.idxR2, d.nR, keyBits, radixHist, d.scanTmp,
+                                &d.status->scanFail);
             }
             {
                 Scope sc(prof, stream, "keys_sort");
                 launchKeys(stream, d, kPopF, d.pos, d.nF, true, d.keyF, d.idxF);
-                launchRadixSort(stream, d.key64, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nF, keyBits, radixHist, d.scanTmp);
+                launchRadixSort(stream, d.key64, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nF, keyBits, radixHist, d.scanTmp,
+                                &d.status->scanFail);
             }
             // the step re-orders positions, velocities (with density) and pressure (pos.w, the
             // warm start); the standalone search re-orders every field
@@ -1350,7 +1365,10 @@ struct tsp_ctx
             lastMax = std::max(lastMax, h.lastSweeps);
         }
         const uint32_t maxSweeps = maxIters > 1 ? maxIters - 1 : 1;
-        uint32_t group = std::max<uint32_t>(1u, lastMax > 1 ? lastMax - 1 : 1u);
+        // the first group: the previous step's sweeps of the slowest scene plus TSP_TUNE_SWEEP_AHEAD
+        // (sweeps queued past convergence return at once; one short of it costs a status read)
+        const int64_t guess = static_cast<int64_t>(lastMax) + TSP_TUNE_SWEEP_AHEAD;
+        uint32_t group = static_cast<uint32_t>(std::max<int64_t>(1, std::min<int64_t>(guess, maxSweeps)));
         uint32_t queued = 0;
         while (anyFluid)
         {
@@ -1657,14 +1675,17 @@ struct tsp_ctx
             releaseList(expAllocs);
             expCap = std::max<uint64_t>(h.nF + 1ull, 1024);
             expFlags = alloc<uint32_t>(expCap, expAllocs);
-            expTmp = alloc<uint32_t>((expCap + 2047) / 2048 + 1, expAllocs);
+            expTmp = alloc<uint32_t>(scanTmpWords(expCap), expAllocs);
+            zero(expTmp, sizeof(uint32_t) * scanTmpWords(expCap));
             expXyz = alloc<float>(3 * expCap, expAllocs);
             expSpeed = alloc<float>(expCap, expAllocs);
         }
         launchExportSurface(stream, d, h.offF, h.nF, threshold, expFlags, expTmp, expXyz, expSpeed);
-        uint32_t count = 0;
+        uint32_t count = 0, scanFailed = 0;
         hipCheck(hipMemcpyAsync(&count, expFlags + h.nF, sizeof(uint32_t), hipMemcpyDeviceToHost, stream), "D2H");
+        hipCheck(hipMemcpyAsync(&scanFailed, &d.status->scanFail, sizeof(uint32_t), hipMemcpyDeviceToHost, stream), "D2H");
         sync();
+        if (scanFailed) { throw Error{ TSP_ERR_STATE, "a one-pass scan gave up waiting for a predecessor tile" }; }
         v->n_fluid = count;
         if (count > v->cap_fluid) { throw Error{ TSP_ERR_CAPACITY, "the fluid export arrays hold fewer particles than the surface set" }; }
         if (count && v->fluid_xyz) { d2h(v->fluid_xyz, expXyz, sizeof(float) * 3ull * count); }

@@ -23,6 +23,8 @@
 
 // build-time variants of the solver sweeps (all give the same bits; the build picks one):
 //   TSP_TUNE_PREFETCH      the next four list entries load while the current four are summed
+//   TSP_TUNE_GATHER4       the sweeps read the LDS halo records of four entries before summing them
+//   TSP_TUNE_SWEEP_AHEAD   the first sweep group of a step: the previous step's sweeps plus this
 //   TSP_TUNE_JACOBI_REC24  kJacobi's halo records take 24 bytes instead of 32 (six blocks per CU)
 //   TSP_TUNE_KG_STORE      kSourceDiag stores each FF pair's gradient factor; the sweeps read it
 //   TSP_TUNE_HALO_MAX      FF halo records per 256-particle block (LDS per sweep block)
@@ -41,6 +43,12 @@
 // the settings chosen by turbospharticles_amd/tune.py, when present, come first
 #if __has_include("tsp_tuned.h")
 #include "tsp_tuned.h"
+#endif
+#ifndef TSP_TUNE_SWEEP_AHEAD
+#define TSP_TUNE_SWEEP_AHEAD -1
+#endif
+#ifndef TSP_TUNE_GATHER4
+#define TSP_TUNE_GATHER4 0
 #endif
 #ifndef TSP_TUNE_PREFETCH
 #define TSP_TUNE_PREFETCH 1
@@ -109,6 +117,11 @@ namespace tsp
     // (N_SELL_DICT_SIZE_MAX, neighbor_lists.cpp:326-332)
     constexpr uint32_t kDictMax = 256;
 
+    // state of a one-pass scan of m values: counter, epoch, two spare words, then one u64 status
+    // word per 2048-value tile; zeroed when allocated
+    constexpr uint32_t kScanHeaderWords = 4;
+    inline constexpr uint64_t scanTmpWords(uint64_t m) { return kScanHeaderWords + 2ull * ((m + 2047ull) / 2048ull); }
+
     // FF hit masks of the count pass (one word per stencil row of three cells, reach 1): the fill
     // takes the hits from them and loads no candidate position. A row with more than 32 fluid
     // candidates is marked in the last word and tested again by the fill.
@@ -137,7 +150,8 @@ namespace tsp
         uint32_t overflow;       // a SELL pool is too small
         uint32_t bUnsorted;      // boundary ids are not non-decreasing under the frames
         uint32_t pairTiny;       // some FF or FB pair has 0 < d2 < 2^-96: the kernels take the general sqrt
-        uint32_t pad0[2];
+        uint32_t scanFail;       // sticky: a one-pass scan gave up waiting (kResetSearch keeps it)
+        uint32_t pad0;
         uint64_t poolUnits[3];   // u16 units used by FF, FB, FR
         uint64_t entries[3];     // entries (without padding) of FF, FB, FR
         uint64_t rfTotal, rbTotal, rrTotal;
@@ -191,7 +205,7 @@ namespace tsp
         float cellSizeInv;
         int32_t reach;       // stencil half width in cells
         uint64_t keyBase;    // the scene's Morton ids are keyBase + morton(local cell)
-        uint64_t emptyKey;   // keyBase + (1 << 3 axisBits): particles outside the frame sort last
+        uint64_t emptyKey;   // keyBase + morton(dim - 1) + 1: particles outside the frame sort last
         uint64_t deadKey;    // emptyKey + 1: the padding slots of the fluid segment
         uint32_t cellOff;    // the scene's first cell in the cell tables
         int32_t innerLo[3], innerHi[3];
@@ -595,14 +609,23 @@ namespace tsp
         return l;
     }
 
-    // visits the entries of one list in storage order, as fn(j, e): j the neighbor (or its halo
-    // slot when kSlots: the stored value without the slice base) and e the entry's position in
-    // u16 units of the pool, which also indexes the per-entry gradient factors. The index loads of
-    // four entries are issued together so that their gathers overlap; the sums still run entry
-    // by entry.
-    template <bool kSlots, typename Fn>
-    __device__ __forceinline__ void visitEntries(const List& l, Fn&& fn)
+    // visits the entries of one list in storage order, as use(load(j), e): j the neighbor (or its
+    // halo slot when kSlots: the stored value without the slice base) and e the entry's position
+    // in u16 units of the pool, which also indexes the per-entry gradient factors. The index loads
+    // of four entries are issued together, then the four gathers (load), then the four sums (use)
+    // in entry order.
+    template <bool kSlots, typename Load, typename Use>
+    __device__ __forceinline__ void visitSplit(const List& l, Load&& load, Use&& use)
     {
+        const auto fn4 = [&](uint32_t j0, uint32_t j1, uint32_t j2, uint32_t j3, uint32_t e, uint32_t step)
+        {
+            const auto r0 = load(j0);
+            const auto r1 = load(j1);
+            const auto r2 = load(j2);
+            const auto r3 = load(j3);
+            use(r0, e); use(r1, e + step); use(r2, e + 2u * step); use(r3, e + 3u * step);
+        };
+        const auto fn = [&](uint32_t j, uint32_t e) { use(load(j), e); };
         uint32_t q = 0;
         if (l.m.format == kFmtDelta16)
         {
@@ -621,8 +644,7 @@ namespace tsp
                     n0 = p[(q + 4) * kSlice]; n1 = p[(q + 5) * kSlice];
                     n2 = p[(q + 6) * kSlice]; n3 = p[(q + 7) * kSlice];
                 }
-                fn(base + j0, e0 + (q + 0) * kSlice); fn(base + j1, e0 + (q + 1) * kSlice);
-                fn(base + j2, e0 + (q + 2) * kSlice); fn(base + j3, e0 + (q + 3) * kSlice);
+                fn4(base + j0, base + j1, base + j2, base + j3, e0 + q * kSlice, kSlice);
                 j0 = n0; j1 = n1; j2 = n2; j3 = n3;
             }
 #else
@@ -630,8 +652,7 @@ namespace tsp
             {
                 const uint32_t j0 = p[(q + 0) * kSlice], j1 = p[(q + 1) * kSlice];
                 const uint32_t j2 = p[(q + 2) * kSlice], j3 = p[(q + 3) * kSlice];
-                fn(base + j0, e0 + (q + 0) * kSlice); fn(base + j1, e0 + (q + 1) * kSlice);
-                fn(base + j2, e0 + (q + 2) * kSlice); fn(base + j3, e0 + (q + 3) * kSlice);
+                fn4(base + j0, base + j1, base + j2, base + j3, e0 + q * kSlice, kSlice);
             }
 #endif
             for (; q < l.n; q++) { fn(base + p[q * kSlice], e0 + q * kSlice); }
@@ -644,11 +665,17 @@ namespace tsp
             {
                 const uint32_t j0 = p[(q + 0) * kSlice], j1 = p[(q + 1) * kSlice];
                 const uint32_t j2 = p[(q + 2) * kSlice], j3 = p[(q + 3) * kSlice];
-                fn(j0, e0 + 2u * (q + 0) * kSlice); fn(j1, e0 + 2u * (q + 1) * kSlice);
-                fn(j2, e0 + 2u * (q + 2) * kSlice); fn(j3, e0 + 2u * (q + 3) * kSlice);
+                fn4(j0, j1, j2, j3, e0 + 2u * q * kSlice, 2u * kSlice);
             }
             for (; q < l.n; q++) { fn(p[q * kSlice], e0 + 2u * q * kSlice); }
         }
+    }
+
+    // the same with the neighbor passed through: fn(j, e)
+    template <bool kSlots, typename Fn>
+    __device__ __forceinline__ void visitEntries(const List& l, Fn&& fn)
+    {
+        visitSplit<kSlots>(l, [](uint32_t j) { return j; }, fn);
     }
 
     template <typename Fn>
@@ -659,6 +686,19 @@ namespace tsp
     __device__ __forceinline__ void visitFF(const List& l, bool halo, FromHalo&& fh, FromGlobal&& fg)
     {
         if (halo) { visitEntries<true>(l, fh); }
+        else { visitEntries<false>(l, fg); }
+    }
+
+    // the same with the halo records read as load(slot) and summed as use(record, e): with
+    // TSP_TUNE_GATHER4 the four LDS reads of a group issue before its four sums
+    template <typename HLoad, typename HUse, typename FromGlobal>
+    __device__ __forceinline__ void visitFFSplit(const List& l, bool halo, HLoad&& hl, HUse&& hu, FromGlobal&& fg)
+    {
+#if TSP_TUNE_GATHER4
+        if (halo) { visitSplit<true>(l, hl, hu); }
+#else
+        if (halo) { visitEntries<true>(l, [&](uint32_t q, uint32_t e) { hu(hl(q), e); }); }
+#endif
         else { visitEntries<false>(l, fg); }
     }
 
