@@ -69,15 +69,6 @@ int tsp_state_file_read(const char *path, tsp_state_view *dst);
    written. tsp_state_file_read refuses the same contents. */
 int tsp_state_file_write(const char *path, const tsp_state_view *src);
 
-/* The reference's export files (TSPSNP4, eval/serial/snapshot_format.h:29-60,
-   serialization.cpp:1226-1396: SaveBoundaryBin, SaveExportSnapshot) from an export view: the
-   particle data file (a 64-byte header, fluid position triples and speeds, rigid position
-   triples, volumes and body indices; n_boundary is recorded in the header only) and the static
-   boundary file (boundary position triples, volumes, one mesh flag byte each; a null flag array
-   writes zeros). Both are written through a ".tmp" file renamed over the destination. */
-int tsp_snapshot_file_write(const char *path, const tsp_export_view *snap);
-int tsp_boundary_file_write(const char *path, const tsp_export_view *snap);
-
 #ifdef __cplusplus
 }
 #endif

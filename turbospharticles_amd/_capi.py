@@ -213,8 +213,6 @@ def _declare_host(lib):
         "tsp_state_file_info": (i, [ctypes.c_char_p, _u32p, _u64p, ctypes.POINTER(ctypes.c_double)]),
         "tsp_state_file_read": (i, [ctypes.c_char_p, ctypes.POINTER(TspStateView)]),
         "tsp_state_file_write": (i, [ctypes.c_char_p, ctypes.POINTER(TspStateView)]),
-        "tsp_snapshot_file_write": (i, [ctypes.c_char_p, ctypes.POINTER(TspExportView)]),
-        "tsp_boundary_file_write": (i, [ctypes.c_char_p, ctypes.POINTER(TspExportView)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -396,17 +394,3 @@ def export_from(call, n_fluid_cap, n_boundary, n_rigid, include_boundary):
     return TSP_ERR_CAPACITY, None
 
 
-def save_snapshot(path, snap):
-    """Writes the reference's particle data file (TSPSNP4) of an export snapshot."""
-    v = snap.view()
-    rc = host_lib().tsp_snapshot_file_write(os.fsencode(path), ctypes.byref(v))
-    if rc != TSP_OK:
-        raise RuntimeError(f"tsp_snapshot_file_write failed ({rc}): {path}")
-
-
-def save_boundary_file(path, snap):
-    """Writes the reference's static boundary file (TSPSNP4) of an export snapshot."""
-    v = snap.view()
-    rc = host_lib().tsp_boundary_file_write(os.fsencode(path), ctypes.byref(v))
-    if rc != TSP_OK:
-        raise RuntimeError(f"tsp_boundary_file_write failed ({rc}): {path}")

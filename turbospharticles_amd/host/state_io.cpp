@@ -209,89 +209,3 @@ extern "C" int tsp_state_file_write(const char* path, const tsp_state_view* v)
     std::remove(tmp.c_str());
     return TSP_ERR_INVALID;
 }
-
-// --- export snapshots (TSPSNP4): eval/serial/snapshot_format.h:29-60, serialization.cpp:301-367,
-//     :1226-1396. A 64-byte header {magic "TSPSNP4\0", endian tag, 64, step, time, N_F, N_B, N_R,
-//     N_rankF}, then the particle data file's arrays posF (x, y, z triples), velMagF, posR
-//     (triples), volR, bodyIdR -- N_B there is informational -- or the static boundary file's
-//     posB (triples), volB, isMeshB (one byte each) with N_F = N_R = 0. ---
-namespace
-{
-#pragma pack(push, 1)
-    struct SnapHeader
-    {
-        char magic[8];
-        uint32_t endianTag;
-        uint32_t headerBytes;
-        uint64_t step;
-        double time;
-        uint64_t nF, nB, nR, nRankF;
-    };
-#pragma pack(pop)
-    static_assert(sizeof(SnapHeader) == 64, "TSPSNP4 header");
-
-    constexpr char kSnapMagic[8] = { 'T', 'S', 'P', 'S', 'N', 'P', '4', '\0' };
-
-    // the file written beside the destination and renamed over it
-    template <typename Body>
-    int writeAtomically(const char* path, Body&& body)
-    {
-        const std::string tmp = std::string(path) + ".tmp";
-        bool ok = false;
-        {
-            File fh;
-            fh.f = std::fopen(tmp.c_str(), "wb");
-            if (!fh.f) { return TSP_ERR_INVALID; }
-            ok = body(fh.f) && std::fflush(fh.f) == 0;
-            ok = (std::fclose(fh.f) == 0) && ok;
-            fh.f = nullptr;
-        }
-        if (ok && std::rename(tmp.c_str(), path) == 0) { return TSP_OK; }
-        std::remove(tmp.c_str());
-        return TSP_ERR_INVALID;
-    }
-}
-
-extern "C" int tsp_snapshot_file_write(const char* path, const tsp_export_view* v)
-{
-    if (!path || !v || !std::isfinite(v->time)) { return TSP_ERR_INVALID; }
-    if ((v->n_fluid && (!v->fluid_xyz || !v->fluid_vel_mag)) || (v->n_rigid && (!v->rigid_xyz || !v->rigid_vol || !v->rigid_body)))
-    {
-        return TSP_ERR_INVALID;
-    }
-    return writeAtomically(path, [&](std::FILE* f) {
-        SnapHeader h{};
-        std::memcpy(h.magic, kSnapMagic, 8);
-        h.endianTag = kEndian;
-        h.headerBytes = sizeof(SnapHeader);
-        h.step = v->step;
-        h.time = v->time;
-        h.nF = v->n_fluid; h.nB = v->n_boundary; h.nR = v->n_rigid; h.nRankF = 0;
-        bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
-        ok = ok && writeArray(f, v->fluid_xyz, 12ull * v->n_fluid);
-        ok = ok && writeArray(f, v->fluid_vel_mag, 4ull * v->n_fluid);
-        ok = ok && writeArray(f, v->rigid_xyz, 12ull * v->n_rigid);
-        ok = ok && writeArray(f, v->rigid_vol, 4ull * v->n_rigid);
-        ok = ok && writeArray(f, v->rigid_body, 4ull * v->n_rigid);
-        return ok;
-    });
-}
-
-extern "C" int tsp_boundary_file_write(const char* path, const tsp_export_view* v)
-{
-    if (!path || !v || !std::isfinite(v->time) || !v->n_boundary || !v->boundary_xyz || !v->boundary_vol) { return TSP_ERR_INVALID; }
-    return writeAtomically(path, [&](std::FILE* f) {
-        SnapHeader h{};
-        std::memcpy(h.magic, kSnapMagic, 8);
-        h.endianTag = kEndian;
-        h.headerBytes = sizeof(SnapHeader);
-        h.step = v->step;
-        h.time = v->time;
-        h.nB = v->n_boundary;
-        bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
-        ok = ok && writeArray(f, v->boundary_xyz, 12ull * v->n_boundary);
-        ok = ok && writeArray(f, v->boundary_vol, 4ull * v->n_boundary);
-        ok = ok && writeArray(f, v->boundary_is_mesh, 1ull * v->n_boundary);
-        return ok;
-    });
-}

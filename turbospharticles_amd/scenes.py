@@ -123,18 +123,6 @@ def load_state(path):
     return _capi.load_state(path)
 
 
-def save_snapshot(path, snap):
-    """Writes an export snapshot as the reference's particle data file (TSPSNP4,
-    SaveExportSnapshot, eval/serial/serialization.cpp:1285-1396)."""
-    _capi.save_snapshot(path, snap)
-
-
-def save_boundary_file(path, snap):
-    """Writes the boundary of an export snapshot as the reference's static boundary file
-    (SaveBoundaryBin, serialization.cpp:1226-1283)."""
-    _capi.save_boundary_file(path, snap)
-
-
 # --- mixed-size scene sets: each scene's tank scaled to its own fluid block ---------------------
 
 TANK_RATIO = (2.5, 1.25, 1.25)   # tank / fluid block edge, as the 0.4 m block in the 1.0 x 0.5 x 0.5 tank
