@@ -173,8 +173,8 @@ namespace tsp
         bvol[i] = vol;
     }
 
-    void launchRadixSort(hipStream_t, bool, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*,
-                         uint32_t*);
+    bool launchRadixSort(hipStream_t, bool, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*,
+                         uint32_t*, bool);
 
     void launchBuildFluid(hipStream_t s, const Dev& d, uint32_t off, uint32_t n, const BuildFluid& b)
     {
@@ -189,7 +189,7 @@ namespace tsp
     {
         if (!n) { return; }
         kBuildBoundaryPos<<<blocksN(n), kBlock, 0, s>>>(c, n, xyz, ids, key, idx);
-        launchRadixSort(s, false, key, idx, key2, idx2, n, c.bits[0] + c.bits[1] + c.bits[2], hist, tmp, tmp + 2);
+        launchRadixSort(s, false, key, idx, key2, idx2, n, c.bits[0] + c.bits[1] + c.bits[2], hist, tmp, tmp + 2, true);
         kBuildBoundaryCoeff<<<blocksN(n), kBlock, 0, s>>>(c, n, xyz, key, idx, bpos, bvol);
     }
 }

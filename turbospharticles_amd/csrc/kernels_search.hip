@@ -475,9 +475,17 @@ namespace tsp
     // ----------------------------------------------------------------------------------------
     // query: one thread per fluid particle (sorted order), 27-cell stencil in z-y-x raster order
 
+    // the stencil's cell box, and the frame's cell-table layout kept in registers (the walk
+    // stores lists, so the compiler would otherwise reload the frame at every row)
     struct Stencil
     {
         int lo[3], hi[3];
+        uint32_t off, dx, dxy;   // first cell of the scene's tables, cells per y step, per z step
+
+        __device__ __forceinline__ uint32_t cell(int x, int y, int z) const
+        {
+            return off + static_cast<uint32_t>(x) + dx * static_cast<uint32_t>(y) + dxy * static_cast<uint32_t>(z);
+        }
     };
 
     __device__ __forceinline__ bool stencilOf(const Frame& f, uint64_t k, Stencil& s)
@@ -491,6 +499,9 @@ namespace tsp
             s.lo[a] = max(l[a] - f.reach, 0);
             s.hi[a] = min(l[a] + f.reach, static_cast<int>(f.dim[a]) - 1);
         }
+        s.off = f.cellOff;
+        s.dx = f.dim[0];
+        s.dxy = f.dim[0] * f.dim[1];
         return true;
     }
 
@@ -608,7 +619,7 @@ namespace tsp
                 for (int cx = s.lo[0]; cx <= s.hi[0]; cx += 3, row++)
                 {
                     const int nx = min(3, s.hi[0] - cx + 1);
-                    const uint32_t lin = linearCell(f, cx, cy, cz);
+                    const uint32_t lin = s.cell(cx, cy, cz);
                     const bool rowF = withF && needF(row);
                     uint2 rf[3], rb[3], rr[3];
 #pragma unroll
@@ -655,7 +666,7 @@ namespace tsp
             {
                 for (int cx = s.lo[0]; cx <= s.hi[0]; cx++)
                 {
-                    const uint32_t lin = linearCell(f, cx, cy, cz);
+                    const uint32_t lin = s.cell(cx, cy, cz);
                     if (withF)
                     {
                         const uint2 rf = d.cellF[lin];
@@ -839,9 +850,9 @@ namespace tsp
             const float4 pi = d.pos[i];
             uint32_t kF = 0, kB = 0, kR = 0;
             const uint32_t uncovered = d.ffMask[static_cast<size_t>(kMaskRows) * d.nF + i];
-            // the row's hit mask, read before its cell ranges: a covered row without hits loads
-            // no range; relations without entries are not walked at all
-            uint32_t rowMask = 0;
+            // the row's hit mask, read before its cell ranges (and the next row's a row ahead): a
+            // covered row without hits loads no range; relations without entries are not walked
+            uint32_t rowMask = 0, nextMask = d.ffMask[i];
             const auto ffHit = [&](uint32_t j)
             {
                 if (staged && kF < kStage) { stage[kF][t] = static_cast<uint16_t>(j - mF.base); }
@@ -863,8 +874,10 @@ namespace tsp
                 },
                 [&](uint32_t row)
                 {
+                    const uint32_t cur = nextMask;
+                    if (row + 1u < kMaskRows) { nextMask = d.ffMask[static_cast<size_t>(row + 1u) * d.nF + i]; }
                     if (row >= kMaskRows || ((uncovered >> row) & 1u)) { return true; }
-                    rowMask = d.ffMask[static_cast<size_t>(row) * d.nF + i];
+                    rowMask = cur;
                     return rowMask != 0u;
                 },
                 // a row the count pass covered: its hits, in stream order, without loading a position
@@ -899,7 +912,7 @@ namespace tsp
                         {
                             for (int cx = s.lo[0]; cx <= s.hi[0]; cx++)
                             {
-                                const uint2 rf = d.cellF[linearCell(f, cx, cy, cz)];
+                                const uint2 rf = d.cellF[s.cell(cx, cy, cz)];
                                 for (uint32_t j = rf.x; j < rf.y; j++)
                                 {
                                     if (!within(pi.x, pi.y, pi.z, d.pos[j], r2)) { continue; }
@@ -1364,13 +1377,13 @@ namespace tsp
     }
 
     template <typename K>
-    static void radixSort(hipStream_t s, K* key, uint32_t* idx, K* key2, uint32_t* idx2, uint32_t n, uint32_t bits,
-                          uint32_t* hist, uint32_t* tmp, uint32_t* fail)
+    static bool radixSort(hipStream_t s, K* key, uint32_t* idx, K* key2, uint32_t* idx2, uint32_t n, uint32_t bits,
+                          uint32_t* hist, uint32_t* tmp, uint32_t* fail, bool copyBack)
     {
         if (n <= kSmallSort)
         {
             kSortSmall<K><<<1, kSmallThreads, 0, s>>>(key, idx, key2, idx2, n, bits);
-            return;
+            return false;
         }
         const uint32_t nTiles = (n + kTile - 1) / kTile;
         K* kIn = key; uint32_t* iIn = idx; K* kOut = key2; uint32_t* iOut = idx2;
@@ -1382,25 +1395,26 @@ namespace tsp
             K* t = kIn; kIn = kOut; kOut = t;
             uint32_t* ti = iIn; iIn = iOut; iOut = ti;
         }
-        if (kIn != key)
-        {
-            (void)hipMemcpyAsync(key, kIn, sizeof(K) * n, hipMemcpyDeviceToDevice, s);
-            (void)hipMemcpyAsync(idx, iIn, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s);
-        }
+        if (kIn == key) { return false; }
+        if (!copyBack) { return true; }
+        (void)hipMemcpyAsync(key, kIn, sizeof(K) * n, hipMemcpyDeviceToDevice, s);
+        (void)hipMemcpyAsync(idx, iIn, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s);
+        return false;
     }
 
-    // sorts (key, idx) stably by the low `bits` bits; the result ends in (key, idx); tmp is a scan
-    // state for 256 x ceil(n / 2048) values, fail its give-up word
-    void launchRadixSort(hipStream_t s, bool key64, uint32_t* key, uint32_t* idx, uint32_t* key2, uint32_t* idx2,
-                         uint32_t n, uint32_t bits, uint32_t* hist, uint32_t* tmp, uint32_t* fail)
+    // sorts (key, idx) stably by the low `bits` bits; tmp is a scan state for 256 x ceil(n / 2048)
+    // values, fail its give-up word. The result ends in (key, idx), or, without copyBack, in
+    // (key2, idx2) after an odd number of passes: the return value says which.
+    bool launchRadixSort(hipStream_t s, bool key64, uint32_t* key, uint32_t* idx, uint32_t* key2, uint32_t* idx2,
+                         uint32_t n, uint32_t bits, uint32_t* hist, uint32_t* tmp, uint32_t* fail, bool copyBack)
     {
-        if (n < 2) { return; }
+        if (n < 2) { return false; }
         if (key64)
         {
-            radixSort(s, reinterpret_cast<uint64_t*>(key), idx, reinterpret_cast<uint64_t*>(key2), idx2, n, bits, hist, tmp,
-                      fail);
+            return radixSort(s, reinterpret_cast<uint64_t*>(key), idx, reinterpret_cast<uint64_t*>(key2), idx2, n, bits, hist,
+                             tmp, fail, copyBack);
         }
-        else { radixSort(s, key, idx, key2, idx2, n, bits, hist, tmp, fail); }
+        return radixSort(s, key, idx, key2, idx2, n, bits, hist, tmp, fail, copyBack);
     }
 
     void launchPermute(hipStream_t s, const Gather& g, const uint32_t* idx)

@@ -31,8 +31,8 @@ namespace tsp
     void launchResetSearch(hipStream_t, const Dev&);
     void launchKeys(hipStream_t, const Dev&, int, const float4*, uint32_t, bool, uint32_t*, uint32_t*);
     void launchKeysSorted(hipStream_t, const Dev&, const float4*, uint32_t, uint32_t*, uint32_t*);
-    void launchRadixSort(hipStream_t, bool, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*,
-                         uint32_t*);
+    bool launchRadixSort(hipStream_t, bool, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*,
+                         uint32_t*, bool);
     void launchPermute(hipStream_t, const Gather&, const uint32_t*);
     void launchClearCells(hipStream_t, const Dev&, const uint32_t*, uint32_t, uint2*);
     void launchClearAll(hipStream_t, uint2*, uint64_t);
@@ -1112,8 +1112,12 @@ struct tsp_ctx
             sync();
             if (unsorted)
             {
-                launchRadixSort(stream, d.key64, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nB, keyBits, radixHist, d.scanTmp,
-                                &d.status->scanFail);
+                if (launchRadixSort(stream, d.key64, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nB, keyBits, radixHist, d.scanTmp,
+                                    &d.status->scanFail, false))
+                {
+                    std::swap(d.keyF, d.keyF2);
+                    std::swap(d.idxF, d.idxF2);
+                }
                 Gather g{};
                 g.s4[0] = d.bpos; g.d4[0] = bpos2; g.n4 = 1;
                 g.s1[0] = d.bvol; g.d1[0] = bvol2; g.n1 = 1;
@@ -1152,14 +1156,22 @@ struct tsp_ctx
             {
                 Scope sc(prof, stream, "keys_sort_rigid");
                 launchKeys(stream, d, kPopR, d.rpos, d.nR, true, d.keyR, d.idxR);
-                launchRadixSort(stream, d.key64, d.keyR, d.idxR, d.keyR2, d.idxR2, d.nR, keyBits, radixHist, d.scanTmp,
-                                &d.status->scanFail);
+                if (launchRadixSort(stream, d.key64, d.keyR, d.idxR, d.keyR2, d.idxR2, d.nR, keyBits, radixHist, d.scanTmp,
+                                    &d.status->scanFail, false))
+                {
+                    std::swap(d.keyR, d.keyR2);
+                    std::swap(d.idxR, d.idxR2);
+                }
             }
             {
                 Scope sc(prof, stream, "keys_sort");
                 launchKeys(stream, d, kPopF, d.pos, d.nF, true, d.keyF, d.idxF);
-                launchRadixSort(stream, d.key64, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nF, keyBits, radixHist, d.scanTmp,
-                                &d.status->scanFail);
+                if (launchRadixSort(stream, d.key64, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nF, keyBits, radixHist, d.scanTmp,
+                                    &d.status->scanFail, false))
+                {
+                    std::swap(d.keyF, d.keyF2);
+                    std::swap(d.idxF, d.idxF2);
+                }
             }
             // the step re-orders positions, velocities (with density) and pressure (pos.w, the
             // warm start); the standalone search re-orders every field
@@ -1300,7 +1312,12 @@ struct tsp_ctx
             sweepLaunches(n);
             return;
         }
-        std::string key(reinterpret_cast<const char*>(&d), sizeof(Dev));
+        // keyed by the kernel arguments the sweeps read: the sort buffers, which trade places
+        // from search to search, are left out
+        Dev k = d;
+        k.keyF = k.keyF2 = k.idxF = k.idxF2 = nullptr;
+        k.keyR = k.keyR2 = k.idxR = k.idxR2 = nullptr;
+        std::string key(reinterpret_cast<const char*>(&k), sizeof(Dev));
         key.append(reinterpret_cast<const char*>(&n), sizeof(n));
         auto it = graphs.find(key);
         if (it == graphs.end())
