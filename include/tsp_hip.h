@@ -96,7 +96,9 @@ enum {
     TSP_FLAG_FRAME_REBUILT = 1u << 1,  /* the cell frame was rebuilt during the call */
     TSP_FLAG_RIGID_CONTACT = 1u << 2,  /* rigid particles had boundary neighbors or neighbors of
                                           other bodies: the rigid contact solve ran */
-    TSP_FLAG_LIST_REGROWN = 1u << 3    /* a neighbor list pool was regrown and the query re-run */
+    TSP_FLAG_LIST_REGROWN = 1u << 3,   /* a neighbor list pool was regrown and the query re-run */
+    TSP_FLAG_STEP_GRAPH = 1u << 4      /* a step of the call ran as one captured graph launch (no
+                                          reference counterpart: the reference launches each kernel) */
 };
 
 typedef struct tsp_step_info {
@@ -144,6 +146,10 @@ int tsp_set_domain(tsp_ctx *ctx, const float mn[3], const float mx[3]);
    only (CPU/backend/neighbor_search.h:25-212); callers that budget device memory per context
    use it. */
 int tsp_set_cell_limit(tsp_ctx *ctx, uint64_t max_cells);
+/* steps as captured graphs (1) or launch by launch (0); the default is the build's
+   TSP_TUNE_STEP_GRAPH. Both give the same bits. No reference counterpart (the reference launches
+   each kernel from the host): for callers that compare the two forms or replay steps cheaply. */
+int tsp_set_step_graph(tsp_ctx *ctx, int enable);
 /* I_SimBackend::PullFromUniversalSimState (core/backend_sim.h:207; gpu_backend.cpp:96-240) */
 int tsp_pull(tsp_ctx *ctx, const tsp_state_view *src);
 /* I_SimBackend::PushToUniversalSimState (core/backend_sim.h:211; gpu_backend.cpp:242-344) */

@@ -20,7 +20,7 @@ import os
 import numpy as np
 import pytest
 
-from turbospharticles_amd import scenes, serving
+from turbospharticles_amd import _capi, scenes, serving
 from turbospharticles_amd.backend import HipBackend, HipBatch
 
 import oracle_lib as O
@@ -49,14 +49,16 @@ def _steps_against_the_oracle(spec, steps, body_tol):
     p = scenes.make_params(0.004, termination_cond=scenes.TERM_POS)
     g = HipBackend(0)
     g.set_params(p)
+    g.set_step_graph(True)
     g.pull(st)
     o = O.Oracle(p, chunk_width=1, threads=THREADS)
     o.pull(st)
-    contact = 0
+    contact = graph = 0
     for s in range(steps):
         g.step(1)
         o.step()
         gi, oi = g.info(), o.info()
+        graph += int(gi.flags & _capi.FLAG_STEP_GRAPH != 0)
         print(f"  step {s + 1}: iterations {gi.solver_iters} / {oi.solver_iters}, contact iterations "
               f"{gi.contact_iters} / {oi.contact_iters}, contact boundary particles {gi.contact_boundary} / "
               f"{oi.contact_boundary}", flush=True)
@@ -68,6 +70,8 @@ def _steps_against_the_oracle(spec, steps, body_tol):
         # the contact solve ran (a resting body may converge in its first evaluation)
         assert bool(gi.flags & 4) == (oi.n_rb + oi.n_rr > 0), s
         contact += int(gi.flags & 4 != 0)
+    # steps after the first run as one captured graph unless the contact solve runs
+    assert graph == (0 if contact else steps - 1)
     sg, so = g.push(), o.push()
     g.close()
     assert np.array_equal(sg.fluid_id, so.fluid_id)

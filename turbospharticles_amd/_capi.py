@@ -30,6 +30,7 @@ FLAG_MAX_ITERS = 1
 FLAG_FRAME_REBUILT = 2
 FLAG_RIGID_CONTACT = 4
 FLAG_LIST_REGROWN = 8
+FLAG_STEP_GRAPH = 16
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -157,6 +158,7 @@ def _declare_hip(lib):
         "tsp_set_params": (i, [vp, ctypes.POINTER(TspParams)]),
         "tsp_set_domain": (i, [vp, _fp, _fp]),
         "tsp_set_cell_limit": (i, [vp, ctypes.c_uint64]),
+        "tsp_set_step_graph": (i, [vp, i]),
         "tsp_pull": (i, [vp, ctypes.POINTER(TspStateView)]),
         "tsp_push": (i, [vp, ctypes.POINTER(TspStateView)]),
         "tsp_neighbor_search": (i, [vp]),

@@ -53,6 +53,10 @@ class HipBackend:
     def set_params(self, params):
         self._check(self.lib.tsp_set_params(self.h, ctypes.byref(params)), "tsp_set_params")
 
+    def set_step_graph(self, enable):
+        """Steps as captured graphs or launch by launch (tsp_set_step_graph)."""
+        self._check(self.lib.tsp_set_step_graph(self.h, 1 if enable else 0), "tsp_set_step_graph")
+
     def set_cell_limit(self, max_cells):
         """The most cells the context's cell frames may hold (tsp_set_cell_limit)."""
         self._check(self.lib.tsp_set_cell_limit(self.h, int(max_cells)), "tsp_set_cell_limit")
@@ -209,6 +213,10 @@ class HipBatch:
     def _check(self, rc, what):
         if rc != 0:
             raise TspError(what, rc, self.lib.tsp_last_error(self.h).decode())
+
+    def set_step_graph(self, enable):
+        """Steps as captured graphs or launch by launch (tsp_set_step_graph)."""
+        self._check(self.lib.tsp_set_step_graph(self.h, 1 if enable else 0), "tsp_set_step_graph")
 
     def set_cell_limit(self, max_cells):
         """The most cells the cell frames of all resident scenes may hold together

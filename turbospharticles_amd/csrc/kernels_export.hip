@@ -29,7 +29,7 @@ namespace tsp
         const uint32_t blk = i / kBlock;
         const uint32_t hc = d.haloCount[blk];
         const uint32_t* halo = d.haloIdx + static_cast<size_t>(blk) * kHaloMax;
-        visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { flags[halo[q] - off] = 1u; },
+        visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { flags[halo[q >> kSlotShift] - off] = 1u; },
                 [&](uint32_t j, uint32_t) { flags[j - off] = 1u; });
     }
 

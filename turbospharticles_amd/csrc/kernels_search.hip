@@ -830,6 +830,7 @@ namespace tsp
 
     __global__ void __launch_bounds__(kBlock) kQueryFill(Dev d)
     {
+        if (stopped(d)) { return; }
         __shared__ uint16_t stage[kStage][kBlock];
         const uint32_t blk = xcdBlock();
         const uint32_t t = threadIdx.x;
@@ -971,6 +972,7 @@ namespace tsp
 
     __global__ void __launch_bounds__(kBlock) kHaloBuild(Dev d)
     {
+        if (stopped(d)) { return; }
         __shared__ uint32_t table[kHashSlots];
         __shared__ uint16_t rank[kHashSlots];
         __shared__ uint32_t keys[kSortKeys];
@@ -1055,7 +1057,7 @@ namespace tsp
                     const uint32_t off = sellRead(d.pool[REL_FF], m, lane, k) - b0;
                     const uint32_t w = off >> 5;
                     const uint32_t slot = rank[w] + static_cast<uint32_t>(__popc(table[w] & ((1u << (off & 31u)) - 1u)));
-                    d.pool[REL_FF][m.offset + k * kSlice + lane] = static_cast<uint16_t>(slot);
+                    d.pool[REL_FF][m.offset + k * kSlice + lane] = static_cast<uint16_t>(slot << kSlotShift);
                 }
                 return;
             }
@@ -1128,7 +1130,7 @@ namespace tsp
         for (uint32_t k = 0; k < n; k++)
         {
             const uint32_t j = sellRead(d.pool[REL_FF], m, lane, k);
-            const uint32_t slot = rank[haloFind(table, j)];
+            const uint32_t slot = rank[haloFind(table, j)] << kSlotShift;
             if (m.format == kFmtDelta16) { d.pool[REL_FF][m.offset + k * kSlice + lane] = static_cast<uint16_t>(slot); }
             else { reinterpret_cast<uint32_t*>(d.pool[REL_FF] + m.offset)[k * kSlice + lane] = slot; }
         }
@@ -1142,6 +1144,7 @@ namespace tsp
 
     __global__ void __launch_bounds__(kBlock) kQueryRigid(Dev d, int mode)
     {
+        if (stopped(d)) { return; }
         const uint32_t i = (blockIdx.x * kBlock + threadIdx.x) >> 6;
         const uint32_t lane = threadIdx.x & 63u;
         if (i >= d.nR) { return; }
@@ -1274,6 +1277,36 @@ namespace tsp
             d.status->rfTotal = d.rfOff[d.nR];
             if (d.rfOff[d.nR] > d.rfCap) { atomicOr(&d.status->overflow, 4u); }
         }
+    }
+
+    // the decision point of a captured step, after the search's counts and scans: what the host
+    // would otherwise read to grow storage, rebuild a frame or take the contact solve calls off
+    // the rest of the captured step (status->stop bit 1); the host then runs the step anew
+    __global__ void kStepGate(Dev d)
+    {
+        if (threadIdx.x != 0) { return; }
+        const Status& st = *d.status;
+        const bool refuse = st.abort != 0u || st.overflow != 0u || st.rbTotal != 0ull || st.rrTotal != 0ull
+                            || (st.pairTiny != 0u && d.fastPairs != 0u);
+        if (refuse) { d.status->stop |= 1u; }
+    }
+
+    // after the captured sweeps: a scene with fluid that has not converged calls off the captured
+    // tail (bit 2); the host adds sweeps and runs the tail itself
+    __global__ void __launch_bounds__(kBlock) kSolveGate(Dev d)
+    {
+        if (stopped(d)) { return; }
+        bool open = false;
+        for (uint32_t sc = threadIdx.x; sc < d.nScenes; sc += kBlock)
+        {
+            open = open || (d.scenes[sc].nF != 0u && d.solve[sc].converged == 0u);
+        }
+        if (open) { atomicOr(&d.status->stop, 2u); }
+    }
+
+    __global__ void kClearStop(Dev d)
+    {
+        if (threadIdx.x == 0) { d.status->stop = 0u; }
     }
 
     // scene AABB of the positions of one particle class (ComputeSceneAABB_ParticleClass,
@@ -1473,6 +1506,10 @@ namespace tsp
             kRigidTotals<<<blocksFor(d.nR), kBlock, 0, s>>>(d);
         }
     }
+
+    void launchStepGate(hipStream_t s, const Dev& d) { kStepGate<<<1, 64, 0, s>>>(d); }
+    void launchSolveGate(hipStream_t s, const Dev& d) { kSolveGate<<<1, kBlock, 0, s>>>(d); }
+    void launchClearStop(hipStream_t s, const Dev& d) { kClearStop<<<1, 64, 0, s>>>(d); }
 
     void launchAabb(hipStream_t s, const Dev& d, int pop, const float4* p, uint32_t n, uint32_t* out)
     {

@@ -36,6 +36,7 @@ namespace tsp
     template <bool kFast>
     __global__ void __launch_bounds__(kBlock) kDensity(Dev d)
     {
+        if (stopped(d)) { return; }
         __shared__ float4 hp[kHaloMax];
         const uint32_t blk = xcdBlock();
         const uint32_t i = blk * kBlock + threadIdx.x;
@@ -53,7 +54,7 @@ namespace tsp
             // dict chunk: unscaled weights, scaled once (sph_solver.cpp:797-810)
             float wsum = 0.0f;
             const auto ffTerm = [&](const float4& pj) { wsum += kwUnscaled<kFast>(dist2(xi - pj.x, yi - pj.y, zi - pj.z), k); };
-            visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(hp[q]); }, [&](uint32_t j, uint32_t) { ffTerm(d.pos[j]); });
+            visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(recAt(hp, q)); }, [&](uint32_t j, uint32_t) { ffTerm(d.pos[j]); });
             dF = wsum * k.weightScale;
         }
         else
@@ -61,7 +62,7 @@ namespace tsp
             // normal chunk: the alpha-scaled weight per neighbor (sph_solver.cpp:826-831)
             float s = 0.0f;
             const auto ffTerm = [&](const float4& pj) { s = __builtin_fmaf(k.mass, kwNormal<kFast>(xi - pj.x, yi - pj.y, zi - pj.z, k), s); };
-            visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(hp[q]); }, [&](uint32_t j, uint32_t) { ffTerm(d.pos[j]); });
+            visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(recAt(hp, q)); }, [&](uint32_t j, uint32_t) { ffTerm(d.pos[j]); });
             dF = s;
         }
         float dB = 0.0f, dR = 0.0f;
@@ -83,6 +84,7 @@ namespace tsp
     template <bool kFast>
     __global__ void __launch_bounds__(kBlock) kNonPressure(Dev d)
     {
+        if (stopped(d)) { return; }
         __shared__ float4 hp[kHaloMax], hv[kHaloMax];
         const uint32_t blk = xcdBlock();
         const uint32_t i = blk * kBlock + threadIdx.x;
@@ -110,7 +112,7 @@ namespace tsp
             aFy = __builtin_fmaf(coeff, dy, aFy);
             aFz = __builtin_fmaf(coeff, dz, aFz);
         };
-        visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(hp[q], hv[q]); },
+        visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(recAt(hp, q), recAt(hv, q)); },
                 [&](uint32_t j, uint32_t) { ffTerm(d.pos[j], d.vel[j]); });
         if (dict) { aFx *= k.mass; aFy *= k.mass; aFz *= k.mass; }
         float aBx = 0.0f, aBy = 0.0f, aBz = 0.0f;
@@ -154,6 +156,7 @@ namespace tsp
     template <bool kFast>
     __global__ void __launch_bounds__(kBlock) kSourceDiag(Dev d)
     {
+        if (stopped(d)) { return; }
         __shared__ float4 hp[kHaloMax], hv[kHaloMax];
         const uint32_t blk = xcdBlock();
         const uint32_t i = blk * kBlock + threadIdx.x;
@@ -185,7 +188,7 @@ namespace tsp
             cFz = __builtin_fmaf(kg, dz, cFz);
             g2 = __builtin_fmaf(kg, kg * d2, g2);
         };
-        visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t e) { ffTerm(hp[q], hv[q], e); },
+        visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t e) { ffTerm(recAt(hp, q), recAt(hv, q), e); },
                 [&](uint32_t j, uint32_t e) { ffTerm(d.pos[j], d.vp[j], e); });
         float sB = 0.0f, cBx = 0.0f, cBy = 0.0f, cBz = 0.0f;
         const List fb = listOf(d, REL_FB, i);
@@ -260,6 +263,7 @@ namespace tsp
     // increment and error decision of the first sweep (sph_solver.cpp:207-235)
     __global__ void __launch_bounds__(kBlock) kSolverInit(Dev d)
     {
+        if (stopped(d)) { return; }
         for (uint32_t sc = threadIdx.x; sc < d.nScenes; sc += kBlock)
         {
             Solve& s = d.solve[sc];
@@ -443,7 +447,7 @@ namespace tsp
             sy = __builtin_fmaf(coeff, dy, sy);
             sz = __builtin_fmaf(coeff, dz, sz);
         };
-        visitFFSplit(ff, hc <= kHaloMax, [&](uint32_t q) { return hp[q]; }, ffTerm,
+        visitFFSplit(ff, hc <= kHaloMax, [&](uint32_t q) { return recAt(hp, q); }, ffTerm,
                      [&](uint32_t j, uint32_t e) { ffTerm(d.pos[j], e); });
         float ax = dict ? sx * k.presOuterFF : sx, ay = dict ? sy * k.presOuterFF : sy, az = dict ? sz * k.presOuterFF : sz;
         const List fb = listOf(d, REL_FB, i);
@@ -475,6 +479,7 @@ namespace tsp
     template <bool kFast>
     __global__ void __launch_bounds__(kSweepThreads) TSP_WAVES_ATTR(TSP_TUNE_PA_WAVES) kPressureAcc(Dev d, bool sweep, uint32_t nTiles)
     {
+        if (stopped(d)) { return; }
         // block b of the launch finishes the previous sweep of scenes b, b + grid, ...
         if (sweep)
         {
@@ -545,14 +550,14 @@ namespace tsp
 #if TSP_TUNE_JACOBI_REC24
             const auto haloLoad = [&](uint32_t q)
             {
-                const float4 pj = hp[q];
-                const float2 aj = ha[q];
+                const float4 pj = recAt(hp, q);
+                const float2 aj = recAt(ha, q);
                 return Rec{ pj.x, pj.y, pj.z, pj.w, aj.x, aj.y };
             };
 #else
             const auto haloLoad = [&](uint32_t q)
             {
-                const float4 pj = hp[q], aj = ha[q];
+                const float4 pj = recAt(hp, q), aj = recAt(ha, q);
                 return Rec{ pj.x, pj.y, pj.z, aj.x, aj.y, aj.z };
             };
 #endif
@@ -619,6 +624,7 @@ namespace tsp
     template <bool kFast>
     __global__ void __launch_bounds__(kSweepThreads) TSP_WAVES_ATTR(TSP_TUNE_JACOBI_WAVES) kJacobi(Dev d, uint32_t nTiles)
     {
+        if (stopped(d)) { return; }
         forTiles(nTiles, [&](uint32_t blk) { jacobiTile<kFast>(d, blk, nTiles); });
     }
 
@@ -626,11 +632,13 @@ namespace tsp
     // the solver states; nTot is the grid size of the sweeps
     __global__ void __launch_bounds__(kSweepThreads) kFinalize(Dev d, uint32_t nTot)
     {
+        if (stopped(d)) { return; }
         finalizeSweep(d, blockIdx.x, nTot);
     }
 
     __global__ void __launch_bounds__(kBlock) kIntegration(Dev d)
     {
+        if (stopped(d)) { return; }
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         const uint32_t sc = d.blockScene[blockIdx.x];
         const Consts& k = d.consts[sc];
@@ -649,6 +657,7 @@ namespace tsp
     // UpdateRigidParticles(predicted), cpu_backend.cpp:2009-2063
     __global__ void __launch_bounds__(kBlock) kRigidUpdate(Dev d, bool predicted)
     {
+        if (stopped(d)) { return; }
         const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
         if (i < d.nR) { rigidUpdateOne(d, i, predicted); }
     }
@@ -656,6 +665,7 @@ namespace tsp
     // ClearRigidForcesAndAddGravity, cpu_backend.cpp:2065-2087
     __global__ void kRigidClearGravity(Dev d)
     {
+        if (stopped(d)) { return; }
         const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
         if (b >= d.nRB) { return; }
         const Consts& k = d.consts[d.bodyScene[b]];
@@ -672,6 +682,7 @@ namespace tsp
     // :1717-1778): reaction of the fluid neighbors of each rigid particle
     __global__ void __launch_bounds__(kBlock) kRigidForces(Dev d, bool pressure)
     {
+        if (stopped(d)) { return; }
         // one wave per rigid particle: the lanes compute 64 pairs' coefficients and gradients at
         // once, then every lane folds them in list order (the single-thread sum, term by term)
         const uint32_t i = (blockIdx.x * kBlock + threadIdx.x) >> 6;
@@ -723,6 +734,7 @@ namespace tsp
     // ReduceRigidParticleForcesToBody, cpu_backend.cpp:2089-2172, with FP64 sums: one block per body
     __global__ void __launch_bounds__(kBlock) kRigidReduce(Dev d)
     {
+        if (stopped(d)) { return; }
         __shared__ double red[6][kBlock];
         const uint32_t b = blockIdx.x;
         const uint32_t nb = d.nRB;
@@ -759,6 +771,7 @@ namespace tsp
     // PredictRigidBodies, cpu_backend.cpp:2174-2235
     __global__ void kRigidPredict(Dev d)
     {
+        if (stopped(d)) { return; }
         const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
         if (b < d.nRB) { rigidPredictOne(d, b); }
     }
@@ -780,6 +793,7 @@ namespace tsp
     // IntegrateRigidBodies, cpu_backend.cpp:2237-2303
     __global__ void kRigidIntegrate(Dev d)
     {
+        if (stopped(d)) { return; }
         const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
         if (b >= d.nRB) { return; }
         const Consts& k = d.consts[d.bodyScene[b]];

@@ -41,6 +41,9 @@ namespace tsp
     void launchScanSlices(hipStream_t, const Dev&, uint32_t*, uint32_t*);
     void launchQueryFill(hipStream_t, const Dev&);
     void launchQueryRigid(hipStream_t, const Dev&, int, uint32_t*);
+    void launchStepGate(hipStream_t, const Dev&);
+    void launchSolveGate(hipStream_t, const Dev&);
+    void launchClearStop(hipStream_t, const Dev&);
     void launchContactOffsets(hipStream_t, const Dev&, uint32_t*);
     void launchContactSolve(hipStream_t, const Dev&, uint32_t, uint32_t*);
     void launchExportSurface(hipStream_t, const Dev&, uint32_t, uint32_t, uint32_t, uint32_t*, uint32_t*, float*, float*);
@@ -337,6 +340,7 @@ struct tsp_ctx
     ~tsp_ctx()
     {
         dropGraphs();
+        dropStepGraphs();
         releaseList(statAllocs);
         releaseList(poolAllocs);
         releaseList(contactAllocs);
@@ -1135,6 +1139,94 @@ struct tsp_ctx
     }
 
     // --- re-order, construct and query (DataReOrder + Construct + Query_Count/Fill), all scenes ---
+    // the fluid fields a search re-orders trade places with their second buffers: positions,
+    // velocities (with density) and pressure (pos.w, the warm start) in a step, every field in
+    // the standalone search
+    void swapFields(bool allFields)
+    {
+        std::swap(d.pos, pos2);
+        std::swap(d.vel, vel2);
+        std::swap(d.id, idF2);
+        if (allFields)
+        {
+            std::swap(d.vp, vp2);
+            std::swap(d.ap, ap2);
+            std::swap(d.source, source2);
+            std::swap(d.diag, diag2);
+        }
+    }
+
+    // the search up to its status: keys, sorts, re-order, cell tables, counts, list offsets
+    void searchFront(bool allFields)
+    {
+        {
+            Scope sc(prof, stream, "reset");
+            launchResetSearch(stream, d);
+            // the runs of the previous sorted keys are the only non-empty cells
+            if (!cellsClean)
+            {
+                launchClearCells(stream, d, d.keyF, d.nF, d.cellF);
+                launchClearCells(stream, d, d.keyR, d.nR, d.cellR);
+            }
+        }
+        if (d.nR)
+        {
+            Scope sc(prof, stream, "keys_sort_rigid");
+            launchKeys(stream, d, kPopR, d.rpos, d.nR, true, d.keyR, d.idxR);
+            if (launchRadixSort(stream, d.key64, d.keyR, d.idxR, d.keyR2, d.idxR2, d.nR, keyBits, radixHist, d.scanTmp,
+                                &d.status->scanFail, false))
+            {
+                std::swap(d.keyR, d.keyR2);
+                std::swap(d.idxR, d.idxR2);
+            }
+        }
+        {
+            Scope sc(prof, stream, "keys_sort");
+            launchKeys(stream, d, kPopF, d.pos, d.nF, true, d.keyF, d.idxF);
+            if (launchRadixSort(stream, d.key64, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nF, keyBits, radixHist, d.scanTmp,
+                                &d.status->scanFail, false))
+            {
+                std::swap(d.keyF, d.keyF2);
+                std::swap(d.idxF, d.idxF2);
+            }
+        }
+        {
+            Scope sc(prof, stream, "permute");
+            Gather g{};
+            g.s4[0] = d.pos; g.d4[0] = pos2;
+            g.s4[1] = d.vel; g.d4[1] = vel2;
+            g.n4 = 2;
+            if (allFields)
+            {
+                g.s4[2] = d.vp; g.d4[2] = vp2;
+                g.s4[3] = d.ap; g.d4[3] = ap2;
+                g.n4 = 4;
+                g.s1[0] = d.source; g.d1[0] = source2;
+                g.s1[1] = d.diag; g.d1[1] = diag2;
+                g.n1 = 2;
+            }
+            g.idS = d.id; g.idD = idF2;
+            g.n = d.nF;
+            launchPermute(stream, g, d.idxF);
+        }
+        swapFields(allFields);
+        {
+            Scope sc(prof, stream, "cells");
+            launchBuildCells(stream, d, d.keyF, d.nF, d.cellF);
+            launchBuildCells(stream, d, d.keyR, d.nR, d.cellR);
+            cellsClean = false;
+        }
+        {
+            Scope sc(prof, stream, "query_count");
+            launchQueryCount(stream, d);
+            launchQueryRigid(stream, d, 0, d.scanTmp);
+        }
+        {
+            Scope sc(prof, stream, "scan");
+            launchScanSlices(stream, d, sliceUnits, d.scanTmp);
+        }
+    }
+
     void neighborSearch(bool allFields)
     {
         bool rebuild = layoutDirty;
@@ -1142,93 +1234,13 @@ struct tsp_ctx
         if (rebuild) { layoutFrames(); }
         for (int attempt = 0; attempt < 4; attempt++)
         {
-            {
-                Scope sc(prof, stream, "reset");
-                launchResetSearch(stream, d);
-                // the runs of the previous sorted keys are the only non-empty cells
-                if (!cellsClean)
-                {
-                    launchClearCells(stream, d, d.keyF, d.nF, d.cellF);
-                    launchClearCells(stream, d, d.keyR, d.nR, d.cellR);
-                }
-            }
-            if (d.nR)
-            {
-                Scope sc(prof, stream, "keys_sort_rigid");
-                launchKeys(stream, d, kPopR, d.rpos, d.nR, true, d.keyR, d.idxR);
-                if (launchRadixSort(stream, d.key64, d.keyR, d.idxR, d.keyR2, d.idxR2, d.nR, keyBits, radixHist, d.scanTmp,
-                                    &d.status->scanFail, false))
-                {
-                    std::swap(d.keyR, d.keyR2);
-                    std::swap(d.idxR, d.idxR2);
-                }
-            }
-            {
-                Scope sc(prof, stream, "keys_sort");
-                launchKeys(stream, d, kPopF, d.pos, d.nF, true, d.keyF, d.idxF);
-                if (launchRadixSort(stream, d.key64, d.keyF, d.idxF, d.keyF2, d.idxF2, d.nF, keyBits, radixHist, d.scanTmp,
-                                    &d.status->scanFail, false))
-                {
-                    std::swap(d.keyF, d.keyF2);
-                    std::swap(d.idxF, d.idxF2);
-                }
-            }
-            // the step re-orders positions, velocities (with density) and pressure (pos.w, the
-            // warm start); the standalone search re-orders every field
-            const auto swapFields = [&]()
-            {
-                std::swap(d.pos, pos2);
-                std::swap(d.vel, vel2);
-                std::swap(d.id, idF2);
-                if (allFields)
-                {
-                    std::swap(d.vp, vp2);
-                    std::swap(d.ap, ap2);
-                    std::swap(d.source, source2);
-                    std::swap(d.diag, diag2);
-                }
-            };
-            {
-                Scope sc(prof, stream, "permute");
-                Gather g{};
-                g.s4[0] = d.pos; g.d4[0] = pos2;
-                g.s4[1] = d.vel; g.d4[1] = vel2;
-                g.n4 = 2;
-                if (allFields)
-                {
-                    g.s4[2] = d.vp; g.d4[2] = vp2;
-                    g.s4[3] = d.ap; g.d4[3] = ap2;
-                    g.n4 = 4;
-                    g.s1[0] = d.source; g.d1[0] = source2;
-                    g.s1[1] = d.diag; g.d1[1] = diag2;
-                    g.n1 = 2;
-                }
-                g.idS = d.id; g.idD = idF2;
-                g.n = d.nF;
-                launchPermute(stream, g, d.idxF);
-            }
-            swapFields();
-            {
-                Scope sc(prof, stream, "cells");
-                launchBuildCells(stream, d, d.keyF, d.nF, d.cellF);
-                launchBuildCells(stream, d, d.keyR, d.nR, d.cellR);
-                cellsClean = false;
-            }
-            {
-                Scope sc(prof, stream, "query_count");
-                launchQueryCount(stream, d);
-                launchQueryRigid(stream, d, 0, d.scanTmp);
-            }
-            {
-                Scope sc(prof, stream, "scan");
-                launchScanSlices(stream, d, sliceUnits, d.scanTmp);
-            }
+            searchFront(allFields);
             readStatus();
             if (last.abort)
             {
                 // a particle left its scene's inner frame: undo the permutation, rebuild that
                 // scene's frame first
-                swapFields();
+                swapFields(allFields);
                 for (uint32_t s = 0; s < nScenes(); s++)
                 {
                     if (hostSolve[s].abort) { scenes[s].frameValid = false; }
@@ -1242,7 +1254,7 @@ struct tsp_ctx
             // of this search are not written)
             const auto refuse = [&](const char* why)
             {
-                swapFields();
+                swapFields(allFields);
                 listsValid = false;
                 throw Error{ TSP_ERR_CAPACITY, why };
             };
@@ -1345,10 +1357,11 @@ struct tsp_ctx
         hipCheck(hipGraphLaunch(it->second, stream), "hipGraphLaunch");
     }
 
-    // --- one IISPH step of every scene (Step_IISPH, sph_solver.cpp:34-319) ---
-    void stepOnce()
+    // --- one IISPH step of every scene (Step_IISPH, sph_solver.cpp:34-319), in phases ---
+
+    // density, non-pressure forces, rigid prediction, source terms and diagonals
+    void predictLaunches()
     {
-        neighborSearch(false);
         {
             Scope sc(prof, stream, "density");
             launchDensity(stream, d);
@@ -1370,10 +1383,15 @@ struct tsp_ctx
             Scope sc(prof, stream, "source_diag");
             launchSourceDiag(stream, d);
         }
-        // relaxed Jacobi sweeps of all scenes; each group is followed by one read of the solver
-        // states; sweeps queued past a scene's convergence return at once for that scene
+    }
+
+    // the first sweep group: the previous step's sweeps of the slowest scene plus
+    // TSP_TUNE_SWEEP_AHEAD (sweeps queued past convergence return at once; one short of it costs
+    // a status read); maxSweeps bounds the solve, anyFluid says whether there is one
+    uint32_t firstSweepGroup(bool& anyFluid, uint32_t& maxSweeps) const
+    {
         uint32_t maxIters = 1, lastMax = 0;
-        bool anyFluid = false;
+        anyFluid = false;
         for (const SceneHost& h : scenes)
         {
             if (!h.nF) { continue; }
@@ -1381,25 +1399,35 @@ struct tsp_ctx
             maxIters = std::max(maxIters, h.params.max_solver_iters);
             lastMax = std::max(lastMax, h.lastSweeps);
         }
-        const uint32_t maxSweeps = maxIters > 1 ? maxIters - 1 : 1;
-        // the first group: the previous step's sweeps of the slowest scene plus TSP_TUNE_SWEEP_AHEAD
-        // (sweeps queued past convergence return at once; one short of it costs a status read)
+        maxSweeps = maxIters > 1 ? maxIters - 1 : 1;
         const int64_t guess = static_cast<int64_t>(lastMax) + TSP_TUNE_SWEEP_AHEAD;
-        uint32_t group = static_cast<uint32_t>(std::max<int64_t>(1, std::min<int64_t>(guess, maxSweeps)));
-        uint32_t queued = 0;
-        while (anyFluid)
+        return static_cast<uint32_t>(std::max<int64_t>(1, std::min<int64_t>(guess, maxSweeps)));
+    }
+
+    // relaxed Jacobi sweeps of all scenes until every scene has converged; each group is followed
+    // by one read of the solver states. queued sweeps have run already (their states read).
+    void solveLoop(uint32_t queued, uint32_t group, uint32_t maxSweeps)
+    {
+        for (;;)
         {
+            if (queued > 0)
+            {
+                bool all = true;
+                for (uint32_t s = 0; s < nScenes(); s++) { all = all && (scenes[s].nF == 0 || hostSolve[s].converged); }
+                if (all) { return; }
+            }
             const uint32_t n = std::min(group, maxSweeps - queued);
             if (n == 0) { throw Error{ TSP_ERR_STATE, "pressure solve did not stop at max_solver_iters" }; }
             launchSweeps(n);
             queued += n;
             readStatus();
-            bool all = true;
-            for (uint32_t s = 0; s < nScenes(); s++) { all = all && (scenes[s].nF == 0 || hostSolve[s].converged); }
-            if (all) { break; }
             group = 1;
         }
-        if (!anyFluid) { readStatus(); }
+    }
+
+    // final pressure accelerations, rigid coupling, integration and the contact solve
+    void tailLaunches()
+    {
         {
             Scope sc(prof, stream, "pressure_acc_final");
             launchPressureAcc(stream, d, false);
@@ -1423,6 +1451,11 @@ struct tsp_ctx
             launchRigidIntegrate(stream, d);
             launchRigidUpdate(stream, d, false);
         }
+    }
+
+    // the scenes' counters after a step, from the solver states read last
+    void stepBookkeeping()
+    {
         for (uint32_t s = 0; s < nScenes(); s++)
         {
             SceneHost& h = scenes[s];
@@ -1440,6 +1473,150 @@ struct tsp_ctx
             h.step++;
             h.time += static_cast<double>(p.time_step);
         }
+    }
+
+    // --- the step as captured graphs (TSP_TUNE_STEP_GRAPH, tsp_set_step_graph) ---
+    // The launches of a step are captured once per argument set and replayed: the search with
+    // the lists and the prediction as one graph, the first sweep group as the sweep-group graph,
+    // the tail as one graph, with one status read at the end. What the host decides from the
+    // search's status in the launch-by-launch form is decided on the device instead: the gate
+    // after the scans calls the rest of the step off (stop bit 1) when a frame must be rebuilt,
+    // storage must grow, a contact solve or the general square root is needed; the gate after the
+    // sweeps calls the tail off (bit 2) when a scene needs more sweeps. A called-off step is
+    // finished launch by launch: anew after bit 1 (the permutation undone), from the sweeps after
+    // bit 2. The order of every operation is the launch-by-launch form's, so the bits are too.
+    struct StepGraph
+    {
+        hipGraphExec_t exec = nullptr;
+        Dev after{};                       // the device arguments after the piece (buffers traded)
+        float4 *pos2 = nullptr, *vel2 = nullptr;
+        uint64_t* idF2 = nullptr;
+    };
+    std::map<std::string, StepGraph> stepGraphs;
+    bool stepGraphOn = TSP_TUNE_STEP_GRAPH != 0;
+
+    void dropStepGraphs()
+    {
+        for (auto& g : stepGraphs) { (void)hipGraphExecDestroy(g.second.exec); }
+        stepGraphs.clear();
+    }
+
+    // the captured form covers steps that need no host decision in the common case
+    bool stepGraphAllowed() const
+    {
+        if (!stepGraphOn || prof.on || layoutDirty || !listsValid || cellsClean || contactLists || !d.nF) { return false; }
+        if (d.fastPairs != d.fastDiv) { return false; }
+        for (const SceneHost& h : scenes) { if (!h.frameValid) { return false; } }
+        return true;
+    }
+
+    // launches piece `id` of the step as a graph (captured on first use for these arguments) and
+    // takes over the host state the launches leave (the traded buffers)
+    template <typename Fn>
+    void launchPiece(int id, Fn&& launches)
+    {
+        std::string key(reinterpret_cast<const char*>(&id), sizeof(id));
+        key.append(reinterpret_cast<const char*>(&d), sizeof(Dev));
+        const void* extra[5] = { pos2, vel2, idF2, sliceUnits, radixHist };
+        key.append(reinterpret_cast<const char*>(extra), sizeof(extra));
+        key.append(reinterpret_cast<const char*>(&keyBits), sizeof(keyBits));
+        auto it = stepGraphs.find(key);
+        if (it == stepGraphs.end())
+        {
+            if (stepGraphs.size() >= 32) { dropStepGraphs(); }
+            const Dev before = d;
+            float4* const p2 = pos2;
+            float4* const v2 = vel2;
+            uint64_t* const i2 = idF2;
+            hipCheck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            hipGraph_t g = nullptr;
+            try
+            {
+                launches();
+            }
+            catch (...)
+            {
+                (void)hipStreamEndCapture(stream, &g);
+                if (g) { (void)hipGraphDestroy(g); }
+                d = before; pos2 = p2; vel2 = v2; idF2 = i2;
+                throw;
+            }
+            hipCheck(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
+            StepGraph sg;
+            const hipError_t rc = hipGraphInstantiate(&sg.exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            sg.after = d; sg.pos2 = pos2; sg.vel2 = vel2; sg.idF2 = idF2;
+            d = before; pos2 = p2; vel2 = v2; idF2 = i2;
+            hipCheck(rc, "hipGraphInstantiate");
+            it = stepGraphs.emplace(std::move(key), sg).first;
+        }
+        hipCheck(hipGraphLaunch(it->second.exec, stream), "hipGraphLaunch");
+        d = it->second.after;
+        pos2 = it->second.pos2; vel2 = it->second.vel2; idF2 = it->second.idF2;
+    }
+
+    // 0: the step is done; 1: called off at the search (run the step launch by launch);
+    // 2: the first group did not finish the solve (continue with more sweeps and the tail)
+    int stepGraph(uint32_t group)
+    {
+        launchPiece(0, [&]()
+        {
+            searchFront(false);
+            launchStepGate(stream, d);
+            d.stopMask = 1u;
+            launchQueryFill(stream, d);
+            launchQueryRigid(stream, d, 1, d.scanTmp);
+            launchHaloBuild(stream, d);
+            predictLaunches();
+        });
+        // d.stopMask is 1 here (the state the piece leaves)
+        launchSweeps(group);
+        launchSolveGate(stream, d);
+        d.stopMask = 3u;
+        launchPiece(1, [&]() { tailLaunches(); });
+        d.stopMask = 0u;
+        cellsClean = false;
+        listsValid = true;
+        readStatus();
+        if (last.stop & 1u)
+        {
+            // nothing after the scans ran: back to the order before the search, lists unwritten
+            swapFields(false);
+            listsValid = false;
+            return 1;
+        }
+        return (last.stop & 2u) ? 2 : 0;
+    }
+
+    void stepOnce()
+    {
+        bool anyFluid = false;
+        uint32_t maxSweeps = 1;
+        const uint32_t group = firstSweepGroup(anyFluid, maxSweeps);
+        if (anyFluid && stepGraphAllowed())
+        {
+            const int r = stepGraph(group);
+            if (r != 1) { for (SceneHost& h : scenes) { h.flags |= TSP_FLAG_STEP_GRAPH; } }
+            if (r == 0)
+            {
+                stepBookkeeping();
+                return;
+            }
+            if (r == 2)
+            {
+                launchClearStop(stream, d);
+                solveLoop(group, 1, maxSweeps);
+                tailLaunches();
+                stepBookkeeping();
+                return;
+            }
+        }
+        neighborSearch(false);
+        predictLaunches();
+        if (anyFluid) { solveLoop(0, group, maxSweeps); }
+        else { readStatus(); }
+        tailLaunches();
+        stepBookkeeping();
     }
 
     void steps(uint32_t n)
@@ -1901,6 +2078,14 @@ int tsp_set_params(tsp_ctx* ctx, const tsp_params* p)
     });
 }
 
+int tsp_set_step_graph(tsp_ctx* ctx, int enable)
+{
+    return guarded(ctx, [&]() {
+        ctx->stepGraphOn = enable != 0;
+        return TSP_OK;
+    });
+}
+
 int tsp_set_cell_limit(tsp_ctx* ctx, uint64_t max_cells)
 {
     return guarded(ctx, [&]() {
@@ -2103,7 +2288,7 @@ int tsp_get_neighbors(tsp_ctx* ctx, int rel, uint32_t* counts, uint32_t* idx, ui
                 uint32_t j;
                 if (m.format == kFmtDelta16) { j = (slots ? 0u : m.base) + pool[m.offset + k * kSlice + lane]; }
                 else { std::memcpy(&j, &pool[m.offset + 2ull * (k * kSlice + lane)], 4); }
-                if (slots) { j = haloIdx[static_cast<size_t>(blk) * kHaloMax + j]; }
+                if (slots) { j = haloIdx[static_cast<size_t>(blk) * kHaloMax + (j >> kSlotShift)]; }
                 idx[w++] = j - sub;
             }
         }

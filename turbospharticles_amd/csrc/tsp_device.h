@@ -25,6 +25,8 @@
 //   TSP_TUNE_PREFETCH      the next four list entries load while the current four are summed
 //   TSP_TUNE_GATHER4       the sweeps read the LDS halo records of four entries before summing them
 //   TSP_TUNE_SWEEP_AHEAD   the first sweep group of a step: the previous step's sweeps plus this
+//   TSP_TUNE_STEP_GRAPH    a step runs as one captured graph launch with one status read (1), or
+//                          launch by launch with the host reading the status between phases (0)
 //   TSP_TUNE_JACOBI_REC24  kJacobi's halo records take 24 bytes instead of 32 (six blocks per CU)
 //   TSP_TUNE_KG_STORE      kSourceDiag stores each FF pair's gradient factor; the sweeps read it
 //   TSP_TUNE_HALO_MAX      FF halo records per 256-particle block (LDS per sweep block)
@@ -43,6 +45,9 @@
 // the settings chosen by turbospharticles_amd/tune.py, when present, come first
 #if __has_include("tsp_tuned.h")
 #include "tsp_tuned.h"
+#endif
+#ifndef TSP_TUNE_STEP_GRAPH
+#define TSP_TUNE_STEP_GRAPH 1
 #endif
 #ifndef TSP_TUNE_SWEEP_AHEAD
 #define TSP_TUNE_SWEEP_AHEAD -1
@@ -151,7 +156,8 @@ namespace tsp
         uint32_t bUnsorted;      // boundary ids are not non-decreasing under the frames
         uint32_t pairTiny;       // some FF or FB pair has 0 < d2 < 2^-96: the kernels take the general sqrt
         uint32_t scanFail;       // sticky: a one-pass scan gave up waiting (kResetSearch keeps it)
-        uint32_t pad0;
+        uint32_t stop;           // a captured step stops early: 1 = its search needs the host, 2 = its
+                                 // solve needs more sweeps (kernels launched with stopMask skip)
         uint64_t poolUnits[3];   // u16 units used by FF, FB, FR
         uint64_t entries[3];     // entries (without padding) of FF, FB, FR
         uint64_t rfTotal, rbTotal, rrTotal;
@@ -303,6 +309,8 @@ namespace tsp
         Consts *consts;
         Solve *solve;
         uint32_t key64;              // the keys are 64-bit (some frame needs more than 32 bits of key space)
+        uint32_t stopMask;           // nonzero in a captured step: a kernel returns at once when status->stop
+                                     // has one of these bits (0 everywhere else)
         uint64_t *sceneKey0;         // nScenes + 1 ascending key bases (the last one ends the key space)
         uint32_t *blockScene;        // scene of each fluid thread block
         uint16_t *sceneB;            // scene of each boundary particle
@@ -365,6 +373,12 @@ namespace tsp
     };
 
     __device__ __forceinline__ float rbv(const Dev& d, int f, uint32_t b) { return d.rb[f * d.nRB + b]; }
+
+    // a kernel of a captured step whose part of the step was called off (uniform over the grid)
+    __device__ __forceinline__ bool stopped(const Dev& d)
+    {
+        return d.stopMask != 0u && (d.status->stop & d.stopMask) != 0u;
+    }
 
     // key i of a key array (u32, or u64 when the context's keys are 64-bit)
     __device__ __forceinline__ uint64_t keyAt(const Dev& d, const uint32_t* keys, uint32_t i)
@@ -589,6 +603,20 @@ namespace tsp
         const uint32_t per = nb >> 3, rem = nb & 7u;
         return xcd < rem ? xcd * (per + 1u) + k : rem * (per + 1u) + (xcd - rem) * per + k;
     }
+    // FF entries of a block with a halo hold the byte offset of the neighbor's 16-byte record in
+    // the block's LDS halo (slot << kSlotShift), so a sweep addresses the record without a shift
+    constexpr uint32_t kSlotShift = 4;
+    static_assert((TSP_TUNE_HALO_MAX << 4) <= 0x10000u, "halo byte offsets fit the u16 entries");
+
+    // the record of a halo entry in an LDS array: 16-byte records at the entry's byte offset,
+    // 8-byte records at half of it
+    template <typename T>
+    __device__ __forceinline__ const T& recAt(const T* base, uint32_t off)
+    {
+        static_assert(sizeof(T) == 16 || sizeof(T) == 8, "16- or 8-byte records");
+        return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (sizeof(T) == 16 ? off : off >> 1));
+    }
+
     // --- SELL-32 list access (shared by the SPH, statistics and export kernels) ---
 
     struct List

@@ -1,5 +1,8 @@
 // tsp_tuned.h -- build-time variant chosen by turbospharticles_amd/tune.py on MI355X
-// (1M rigid cube drop: 3.79 ms per step; every candidate checked bit-identical)
+// (1M rigid cube drop: 3.763 ms per step; every candidate checked bit-identical)
+#ifndef TSP_TUNE_GATHER4
+#define TSP_TUNE_GATHER4 0
+#endif
 #ifndef TSP_TUNE_GRAPHS
 #define TSP_TUNE_GRAPHS 1
 #endif
@@ -35,6 +38,12 @@
 #endif
 #ifndef TSP_TUNE_SLICE
 #define TSP_TUNE_SLICE 64
+#endif
+#ifndef TSP_TUNE_STEP_GRAPH
+#define TSP_TUNE_STEP_GRAPH 0
+#endif
+#ifndef TSP_TUNE_SWEEP_AHEAD
+#define TSP_TUNE_SWEEP_AHEAD -1
 #endif
 #ifndef TSP_TUNE_SWEEP_THREADS
 #define TSP_TUNE_SWEEP_THREADS 256
