@@ -970,6 +970,147 @@ namespace tsp
         return h;
     }
 
+#if TSP_TUNE_HALO_PAGES
+    // The block's FF neighbors by pages of 4096 indices: one LDS bitmap per page (at most
+    // kHaloPages pages, taken in order of arrival), the pages ranked by their number, so the set
+    // bits in page order are the halo in ascending order and an entry's slot is the count of set
+    // bits before it. No hash table and no sort; a block whose neighbors span more pages keeps
+    // particle indices.
+    constexpr uint32_t kPageBits = 12;
+    constexpr uint32_t kPageWords = (1u << kPageBits) / 32;
+    constexpr uint32_t kHaloPages = 32;
+    static_assert(kHaloPages <= kBlock && (kHaloPages * kPageWords) % kBlock == 0, "page table sizes");
+
+    // the slot of page pg in the table (pages arrive in any order; slots fill from 0), or
+    // kHaloPages when the table is full
+    __device__ __forceinline__ uint32_t haloPageSlot(uint32_t* pageId, uint32_t pg)
+    {
+        for (uint32_t s = 0; s < kHaloPages; s++)
+        {
+            uint32_t v = *static_cast<volatile uint32_t*>(&pageId[s]);
+            if (v == kEmpty32) { v = atomicCAS(&pageId[s], kEmpty32, pg); if (v == kEmpty32) { return s; } }
+            if (v == pg) { return s; }
+        }
+        return kHaloPages;
+    }
+
+    __global__ void __launch_bounds__(kBlock) kHaloBuild(Dev d)
+    {
+        if (stopped(d)) { return; }
+        __shared__ uint32_t bits[kHaloPages * kPageWords];
+        __shared__ uint16_t wrank[kHaloPages * kPageWords];
+        __shared__ uint32_t pageId[kHaloPages], slotOfRank[kHaloPages];
+        __shared__ uint32_t full;
+        __shared__ uint32_t waveSum[kBlock / 64];
+        const uint32_t blk = blockIdx.x;
+        const uint32_t t = threadIdx.x;
+        const uint32_t i = blk * kBlock + t;
+        for (uint32_t q = t; q < kHaloPages * kPageWords; q += kBlock) { bits[q] = 0u; }
+        if (t < kHaloPages) { pageId[t] = kEmpty32; }
+        if (t == 0) { full = 0u; }
+        __syncthreads();
+        uint32_t n = 0;
+        SliceMeta m{};
+        if (i < d.nF)
+        {
+            n = d.cnt[REL_FF][i];
+            m = d.meta[REL_FF][i / kSlice];
+        }
+        const uint32_t lane = i & (kSlice - 1);
+        // the entries into their pages' bitmaps (consecutive entries mostly share a page)
+        uint32_t lastPage = kEmpty32, lastSlot = 0;
+        for (uint32_t k = 0; k < n; k++)
+        {
+            const uint32_t j = sellRead(d.pool[REL_FF], m, lane, k);
+            const uint32_t pg = j >> kPageBits;
+            if (pg != lastPage)
+            {
+                lastSlot = haloPageSlot(pageId, pg);
+                if (lastSlot == kHaloPages) { atomicOr(&full, 1u); break; }
+                lastPage = pg;
+            }
+            atomicOr(&bits[lastSlot * kPageWords + ((j >> 5) & (kPageWords - 1u))], 1u << (j & 31u));
+        }
+        __syncthreads();
+        if (full)
+        {
+            if (t == 0) { d.haloCount[blk] = kNoHalo; }
+            return;
+        }
+        // the pages in ascending order: slotOfRank[r] is the slot of the r-th smallest page
+        if (t < kHaloPages && pageId[t] != kEmpty32)
+        {
+            const uint32_t pg = pageId[t];
+            uint32_t r = 0;
+            for (uint32_t s = 0; s < kHaloPages; s++) { r += (pageId[s] != kEmpty32 && pageId[s] < pg) ? 1u : 0u; }
+            slotOfRank[r] = t;
+        }
+        __syncthreads();
+        uint32_t np = 0;
+        for (uint32_t s = 0; s < kHaloPages; s++) { np += pageId[s] != kEmpty32 ? 1u : 0u; }
+        // set bits before each word in page order: thread t takes kPer consecutive words of that order
+        constexpr uint32_t kPer = kHaloPages * kPageWords / kBlock;
+        uint32_t pc[kPer], sum = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < kPer; u++)
+        {
+            const uint32_t v = kPer * t + u, r = v / kPageWords;
+            pc[u] = r < np ? static_cast<uint32_t>(__popc(bits[slotOfRank[r] * kPageWords + v % kPageWords])) : 0u;
+            sum += pc[u];
+        }
+        uint32_t incl = sum;
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            const uint32_t v = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), o, 64));
+            if ((t & 63u) >= static_cast<uint32_t>(o)) { incl += v; }
+        }
+        if ((t & 63u) == 63u) { waveSum[t >> 6] = incl; }
+        __syncthreads();
+        uint32_t count = 0, before = 0;
+        for (uint32_t w = 0; w < kBlock / 64; w++) { before += w < (t >> 6) ? waveSum[w] : 0u; count += waveSum[w]; }
+        if (count > kHaloMax)
+        {
+            if (t == 0) { d.haloCount[blk] = kNoHalo; }
+            return;
+        }
+        uint32_t* out = d.haloIdx + static_cast<size_t>(blk) * kHaloMax;
+        uint32_t r = before + incl - sum;
+#pragma unroll
+        for (uint32_t u = 0; u < kPer; u++)
+        {
+            const uint32_t v = kPer * t + u, pr = v / kPageWords;
+            if (pr >= np) { break; }
+            const uint32_t slot = slotOfRank[pr], w = v % kPageWords;
+            wrank[slot * kPageWords + w] = static_cast<uint16_t>(r);
+            uint32_t b = bits[slot * kPageWords + w];
+            const uint32_t base = (pageId[slot] << kPageBits) + 32u * w;
+            while (b)
+            {
+                out[r++] = base + static_cast<uint32_t>(__ffs(static_cast<int>(b))) - 1u;
+                b &= b - 1u;
+            }
+        }
+        if (t == 0) { d.haloCount[blk] = count; }
+        __syncthreads();
+        // entries become the byte offsets of their slots, in their storage format
+        lastPage = kEmpty32;
+        for (uint32_t k = 0; k < n; k++)
+        {
+            const uint32_t j = sellRead(d.pool[REL_FF], m, lane, k);
+            const uint32_t pg = j >> kPageBits;
+            if (pg != lastPage)
+            {
+                lastSlot = 0;
+                while (pageId[lastSlot] != pg) { lastSlot++; }
+                lastPage = pg;
+            }
+            const uint32_t q = lastSlot * kPageWords + ((j >> 5) & (kPageWords - 1u));
+            const uint32_t slot = (wrank[q] + static_cast<uint32_t>(__popc(bits[q] & ((1u << (j & 31u)) - 1u)))) << kSlotShift;
+            if (m.format == kFmtDelta16) { d.pool[REL_FF][m.offset + k * kSlice + lane] = static_cast<uint16_t>(slot); }
+            else { reinterpret_cast<uint32_t*>(d.pool[REL_FF] + m.offset)[k * kSlice + lane] = slot; }
+        }
+    }
+#else
     __global__ void __launch_bounds__(kBlock) kHaloBuild(Dev d)
     {
         if (stopped(d)) { return; }
@@ -1135,6 +1276,7 @@ namespace tsp
             else { reinterpret_cast<uint32_t*>(d.pool[REL_FF] + m.offset)[k * kSlice + lane] = slot; }
         }
     }
+#endif
 
     // ----------------------------------------------------------------------------------------
     // rigid-source lists, one wave per rigid particle: mode 0 counts RF, RB and RR; mode 1 fills
@@ -1434,6 +1576,9 @@ namespace tsp
         (void)hipMemcpyAsync(idx, iIn, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s);
         return false;
     }
+
+    // a sort of n keys runs as one block and uses neither the digit counts nor the scan state
+    bool radixSortSingleBlock(uint32_t n) { return n <= kSmallSort; }
 
     // sorts (key, idx) stably by the low `bits` bits; tmp is a scan state for 256 x ceil(n / 2048)
     // values, fail its give-up word. The result ends in (key, idx), or, without copyBack, in

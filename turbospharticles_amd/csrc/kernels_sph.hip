@@ -732,17 +732,21 @@ namespace tsp
     }
 
     // ReduceRigidParticleForcesToBody, cpu_backend.cpp:2089-2172, with FP64 sums: one block per body
-    __global__ void __launch_bounds__(kBlock) kRigidReduce(Dev d)
+    // one block of kReduceThreads per body: a body's particles are few blocks' worth, so the
+    // block is wide to keep the chains of dependent loads and sums short
+    constexpr uint32_t kReduceThreads = 1024;
+
+    __global__ void __launch_bounds__(kReduceThreads) kRigidReduce(Dev d)
     {
         if (stopped(d)) { return; }
-        __shared__ double red[6][kBlock];
+        __shared__ double red[6][kReduceThreads];
         const uint32_t b = blockIdx.x;
         const uint32_t nb = d.nRB;
         const float a[9] = { rbv(d, RB_A00 + 0, b), rbv(d, RB_A00 + 1, b), rbv(d, RB_A00 + 2, b),
                              rbv(d, RB_A00 + 3, b), rbv(d, RB_A00 + 4, b), rbv(d, RB_A00 + 5, b),
                              rbv(d, RB_A00 + 6, b), rbv(d, RB_A00 + 7, b), rbv(d, RB_A00 + 8, b) };
         double acc[6] = { 0, 0, 0, 0, 0, 0 };
-        for (uint32_t q = d.byBodyOff[b] + threadIdx.x; q < d.byBodyOff[b + 1]; q += kBlock)
+        for (uint32_t q = d.byBodyOff[b] + threadIdx.x; q < d.byBodyOff[b + 1]; q += kReduceThreads)
         {
             const uint32_t i = d.byBodyIdx[q];
             const float lx = d.rxrel[i], ly = d.ryrel[i], lz = d.rzrel[i];
@@ -757,7 +761,7 @@ namespace tsp
         }
         for (int f = 0; f < 6; f++) { red[f][threadIdx.x] = acc[f]; }
         __syncthreads();
-        for (uint32_t o = kBlock / 2; o > 0; o >>= 1)
+        for (uint32_t o = kReduceThreads / 2; o > 0; o >>= 1)
         {
             if (threadIdx.x < o) { for (int f = 0; f < 6; f++) { red[f][threadIdx.x] += red[f][threadIdx.x + o]; } }
             __syncthreads();
@@ -892,7 +896,7 @@ namespace tsp
     {
         if (d.nR) { kRigidForces<<<blocksOf(64ull * d.nR), kBlock, 0, s>>>(d, pressure); }
     }
-    void launchRigidReduce(hipStream_t s, const Dev& d) { if (d.nRB) { kRigidReduce<<<d.nRB, kBlock, 0, s>>>(d); } }
+    void launchRigidReduce(hipStream_t s, const Dev& d) { if (d.nRB) { kRigidReduce<<<d.nRB, kReduceThreads, 0, s>>>(d); } }
     void launchRigidPredict(hipStream_t s, const Dev& d) { if (d.nRB) { kRigidPredict<<<blocksOf(d.nRB), kBlock, 0, s>>>(d); } }
     void launchRigidIntegrate(hipStream_t s, const Dev& d) { if (d.nRB) { kRigidIntegrate<<<blocksOf(d.nRB), kBlock, 0, s>>>(d); } }
 }

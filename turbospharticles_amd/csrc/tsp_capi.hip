@@ -33,6 +33,7 @@ namespace tsp
     void launchKeysSorted(hipStream_t, const Dev&, const float4*, uint32_t, uint32_t*, uint32_t*);
     bool launchRadixSort(hipStream_t, bool, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*,
                          uint32_t*, bool);
+    bool radixSortSingleBlock(uint32_t n);
     void launchPermute(hipStream_t, const Gather&, const uint32_t*);
     void launchClearCells(hipStream_t, const Dev&, const uint32_t*, uint32_t, uint2*);
     void launchClearAll(hipStream_t, uint2*, uint64_t);
@@ -279,6 +280,10 @@ struct tsp_ctx
 {
     int device = 0;
     hipStream_t stream = nullptr;
+    // the rigid particles' keys and single-block sort run beside the fluid sort on a second stream
+    // of the same priority, joined before the rigid cell table is built
+    hipStream_t side = nullptr;
+    hipEvent_t forkEv = nullptr, joinEv = nullptr;
     std::string err;
     tsp_params defaults{};
     bool paramsSet = false;
@@ -349,6 +354,9 @@ struct tsp_ctx
         releaseList(work);
         releaseList(arena);
         if (hostBlock) { (void)hipHostFree(hostBlock); }
+        if (forkEv) { (void)hipEventDestroy(forkEv); }
+        if (joinEv) { (void)hipEventDestroy(joinEv); }
+        if (side) { (void)hipStreamDestroy(side); }
         if (stream) { (void)hipStreamDestroy(stream); }
     }
 
@@ -1169,16 +1177,26 @@ struct tsp_ctx
                 launchClearCells(stream, d, d.keyR, d.nR, d.cellR);
             }
         }
+        // a single-block rigid sort shares no scratch with the fluid sort: it runs beside it
+        const bool fork = d.nR && radixSortSingleBlock(d.nR) && !prof.on;
         if (d.nR)
         {
             Scope sc(prof, stream, "keys_sort_rigid");
-            launchKeys(stream, d, kPopR, d.rpos, d.nR, true, d.keyR, d.idxR);
-            if (launchRadixSort(stream, d.key64, d.keyR, d.idxR, d.keyR2, d.idxR2, d.nR, keyBits, radixHist, d.scanTmp,
+            hipStream_t rs = stream;
+            if (fork)
+            {
+                hipCheck(hipEventRecord(forkEv, stream), "hipEventRecord");
+                hipCheck(hipStreamWaitEvent(side, forkEv, 0), "hipStreamWaitEvent");
+                rs = side;
+            }
+            launchKeys(rs, d, kPopR, d.rpos, d.nR, true, d.keyR, d.idxR);
+            if (launchRadixSort(rs, d.key64, d.keyR, d.idxR, d.keyR2, d.idxR2, d.nR, keyBits, radixHist, d.scanTmp,
                                 &d.status->scanFail, false))
             {
                 std::swap(d.keyR, d.keyR2);
                 std::swap(d.idxR, d.idxR2);
             }
+            if (fork) { hipCheck(hipEventRecord(joinEv, side), "hipEventRecord"); }
         }
         {
             Scope sc(prof, stream, "keys_sort");
@@ -1210,6 +1228,7 @@ struct tsp_ctx
             launchPermute(stream, g, d.idxF);
         }
         swapFields(allFields);
+        if (fork) { hipCheck(hipStreamWaitEvent(stream, joinEv, 0), "hipStreamWaitEvent"); }
         {
             Scope sc(prof, stream, "cells");
             launchBuildCells(stream, d, d.keyF, d.nF, d.cellF);
@@ -2031,6 +2050,9 @@ int tsp_create(int device, tsp_ctx** out)
     // statsScratchInit: the device address of this device's statistics scratch, once per device
     if (hipSetDevice(device) != hipSuccess || statsScratchInit(device) != hipSuccess
         || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess
+        || hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess
+        || hipEventCreateWithFlags(&c->forkEv, hipEventDisableTiming) != hipSuccess
+        || hipEventCreateWithFlags(&c->joinEv, hipEventDisableTiming) != hipSuccess
         || hipHostMalloc(reinterpret_cast<void**>(&c->hostBlock), tsp_ctx::kSolveOff) != hipSuccess)
     {
         delete c;

@@ -25,6 +25,8 @@
 //   TSP_TUNE_PREFETCH      the next four list entries load while the current four are summed
 //   TSP_TUNE_GATHER4       the sweeps read the LDS halo records of four entries before summing them
 //   TSP_TUNE_SWEEP_AHEAD   the first sweep group of a step: the previous step's sweeps plus this
+//   TSP_TUNE_HALO_PAGES    the FF halo of a block from bitmaps over pages of 4096 indices (1), or
+//                          from one 64 Ki-index window bitmap, else a sorted hash set (0)
 //   TSP_TUNE_STEP_GRAPH    a step runs as one captured graph launch with one status read (1), or
 //                          launch by launch with the host reading the status between phases (0)
 //   TSP_TUNE_JACOBI_REC24  kJacobi's halo records take 24 bytes instead of 32 (six blocks per CU)
@@ -45,6 +47,9 @@
 // the settings chosen by turbospharticles_amd/tune.py, when present, come first
 #if __has_include("tsp_tuned.h")
 #include "tsp_tuned.h"
+#endif
+#ifndef TSP_TUNE_HALO_PAGES
+#define TSP_TUNE_HALO_PAGES 0
 #endif
 #ifndef TSP_TUNE_STEP_GRAPH
 #define TSP_TUNE_STEP_GRAPH 1

@@ -5,7 +5,7 @@ height (32 or 64 particles per slice), the radix sort's digit width (8, 6 or 4 b
 over 28-bit keys), the threads of a solver sweep tile (256, or 128 with two particles each), the
 sweeps' LDS reads of four halo records ahead of their sums, the size of a step's first sweep
 group against the previous step's sweep count, the whole step as one graph launch or launch by
-launch, prefetched list entries, 24- or 32-byte Jacobi halo records, stored or
+launch, the FF halo built from page bitmaps or a window bitmap and hash set, prefetched list entries, 24- or 32-byte Jacobi halo records, stored or
 recomputed gradient factors, the halo cap (LDS per sweep block), register budgets of the sweep
 kernels, sweep groups as HIP graphs, the neighbor query's walk of the stencil (cell by cell, or
 rows of three cells) and its candidate loads in flight. The
@@ -44,7 +44,8 @@ SPACE = {
     "TSP_TUNE_PREFETCH": [1, 0],
     "TSP_TUNE_GATHER4": [0, 1],
     "TSP_TUNE_SWEEP_AHEAD": [-1, 0, 1, 2],
-    "TSP_TUNE_STEP_GRAPH": [1, 0],
+    "TSP_TUNE_STEP_GRAPH": [0, 1],
+    "TSP_TUNE_HALO_PAGES": [0, 1],
     "TSP_TUNE_JACOBI_REC24": [1, 0],
     "TSP_TUNE_KG_STORE": [0, 1],
     "TSP_TUNE_HALO_MAX": [1000, 960, 1024],
