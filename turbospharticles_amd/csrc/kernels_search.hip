@@ -692,7 +692,7 @@ namespace tsp
     }
 #endif
 
-    __global__ void __launch_bounds__(kBlock) kQueryCount(Dev d, uint32_t nSlices)
+    __global__ void __launch_bounds__(kBlock) TSP_WAVES_ATTR(TSP_TUNE_QUERY_WAVES) kQueryCount(Dev d, uint32_t nSlices)
     {
         const uint32_t blk = xcdBlock();
         const uint32_t i = blk * kBlock + threadIdx.x;
@@ -828,7 +828,7 @@ namespace tsp
 
     constexpr uint32_t kStage = 40; // FF entries per thread staged in LDS by the fill (u16 slices)
 
-    __global__ void __launch_bounds__(kBlock) kQueryFill(Dev d)
+    __global__ void __launch_bounds__(kBlock) TSP_WAVES_ATTR(TSP_TUNE_QUERY_WAVES) kQueryFill(Dev d)
     {
         if (stopped(d)) { return; }
         __shared__ uint16_t stage[kStage][kBlock];
@@ -994,7 +994,7 @@ namespace tsp
         return kHaloPages;
     }
 
-    __global__ void __launch_bounds__(kBlock) kHaloBuild(Dev d)
+    __global__ void __launch_bounds__(kBlock) TSP_WAVES_ATTR(TSP_TUNE_QUERY_WAVES) kHaloBuild(Dev d)
     {
         if (stopped(d)) { return; }
         __shared__ uint32_t bits[kHaloPages * kPageWords];
@@ -1111,7 +1111,7 @@ namespace tsp
         }
     }
 #else
-    __global__ void __launch_bounds__(kBlock) kHaloBuild(Dev d)
+    __global__ void __launch_bounds__(kBlock) TSP_WAVES_ATTR(TSP_TUNE_QUERY_WAVES) kHaloBuild(Dev d)
     {
         if (stopped(d)) { return; }
         __shared__ uint32_t table[kHashSlots];

@@ -25,6 +25,8 @@
 //   TSP_TUNE_PREFETCH      the next four list entries load while the current four are summed
 //   TSP_TUNE_GATHER4       the sweeps read the LDS halo records of four entries before summing them
 //   TSP_TUNE_SWEEP_AHEAD   the first sweep group of a step: the previous step's sweeps plus this
+//   TSP_TUNE_QUERY_WAVES   register budget of the neighbor query and halo kernels: waves per SIMD
+//                          (0: the compiler's choice)
 //   TSP_TUNE_HALO_PAGES    the FF halo of a block from bitmaps over pages of 4096 indices (1), or
 //                          from one 64 Ki-index window bitmap, else a sorted hash set (0)
 //   TSP_TUNE_STEP_GRAPH    a step runs as one captured graph launch with one status read (1), or
@@ -47,6 +49,9 @@
 // the settings chosen by turbospharticles_amd/tune.py, when present, come first
 #if __has_include("tsp_tuned.h")
 #include "tsp_tuned.h"
+#endif
+#ifndef TSP_TUNE_QUERY_WAVES
+#define TSP_TUNE_QUERY_WAVES 0
 #endif
 #ifndef TSP_TUNE_HALO_PAGES
 #define TSP_TUNE_HALO_PAGES 0
