@@ -505,6 +505,56 @@ namespace tsp
         return true;
     }
 
+    // The fluid cell ranges of a block's stencil box -- the bounding box of the stencils of its
+    // particles, a few hundred cells for a Morton-ordered block -- staged in LDS, so the rows of
+    // the walk read them there instead of from the cell table. F is null when the box is off for
+    // the block (larger than kBoxCells, or TSP_TUNE_BOX_CELLS = 0).
+    constexpr uint32_t kBoxCells = TSP_TUNE_BOX_CELLS;
+    struct CellBox
+    {
+        const uint2* F;
+        int x0, y0, z0;
+        uint32_t w, wh;
+
+        __device__ __forceinline__ uint32_t at(int x, int y, int z) const
+        {
+            return static_cast<uint32_t>(x - x0) + w * static_cast<uint32_t>(y - y0) + wh * static_cast<uint32_t>(z - z0);
+        }
+    };
+
+    // block-wide (every thread calls it): the box of the active stencils, staged into lds
+    // (kBoxCells entries) when it fits
+    __device__ __forceinline__ CellBox stageCellBox(const Dev& d, const Frame& f, const Stencil& s, bool active,
+                                                    uint2* lds, int* red)
+    {
+        CellBox b{};
+        if (kBoxCells == 0) { return b; }
+        if (threadIdx.x < 3) { red[threadIdx.x] = 0x7fffffff; red[threadIdx.x + 3] = -1; }
+        __syncthreads();
+        if (active)
+        {
+            for (int a = 0; a < 3; a++) { atomicMin(&red[a], s.lo[a]); atomicMax(&red[a + 3], s.hi[a]); }
+        }
+        __syncthreads();
+        b.x0 = red[0]; b.y0 = red[1]; b.z0 = red[2];
+        const int ex = red[3] - red[0] + 1, ey = red[4] - red[1] + 1, ez = red[5] - red[2] + 1;
+        __syncthreads();
+        if (ex <= 0) { return b; }   // no active particle
+        const uint64_t vol = static_cast<uint64_t>(ex) * static_cast<uint64_t>(ey) * static_cast<uint64_t>(ez);
+        if (vol > kBoxCells) { return b; }
+        b.w = static_cast<uint32_t>(ex);
+        b.wh = static_cast<uint32_t>(ex) * static_cast<uint32_t>(ey);
+        for (uint32_t q = threadIdx.x; q < vol; q += blockDim.x)
+        {
+            const int x = b.x0 + static_cast<int>(q % b.w), y = b.y0 + static_cast<int>((q / b.w) % static_cast<uint32_t>(ey));
+            const int z = b.z0 + static_cast<int>(q / b.wh);
+            lds[q] = d.cellF[linearCell(f, x, y, z)];
+        }
+        __syncthreads();
+        b.F = lds;
+        return b;
+    }
+
     __device__ __forceinline__ bool within(float xi, float yi, float zi, const float4& pj, float r2)
     {
         const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
@@ -607,8 +657,9 @@ namespace tsp
     // preF(row, rf) may take a row's fluid candidates itself (returns true) instead of the stream;
     // postF(row) follows the row's fluid part. row counts the rows of three cells in stencil order.
     template <typename OnF, typename OnB, typename OnR, typename NeedF, typename PreF, typename PostF>
-    __device__ __forceinline__ void forStencil(const Dev& d, const Frame& f, const Stencil& s, bool withF, bool withB,
-                                               bool withR, OnF onF, OnB onB, OnR onR, NeedF needF, PreF preF, PostF postF)
+    __device__ __forceinline__ void forStencil(const Dev& d, const Frame& f, const Stencil& s, const CellBox& box, bool withF,
+                                               bool withB, bool withR, OnF onF, OnB onB, OnR onR, NeedF needF, PreF preF,
+                                               PostF postF)
     {
         const auto same = [](uint32_t e) { return e; };
         uint32_t row = 0;
@@ -622,14 +673,31 @@ namespace tsp
                     const uint32_t lin = s.cell(cx, cy, cz);
                     const bool rowF = withF && needF(row);
                     uint2 rf[3], rb[3], rr[3];
+#if TSP_TUNE_RANGE_LOADS
+                    // every lane loads the row's nine ranges (cells past the row's end load its
+                    // last cell) and then selects: no predicated loads into zeroed registers, so
+                    // nothing waits on loads still in flight from the previous row
 #pragma unroll
                     for (int c = 0; c < 3; c++)
                     {
                         const bool in = c < nx;
-                        rf[c] = (in && rowF) ? d.cellF[lin + c] : make_uint2(0u, 0u);
+                        const uint32_t cc = lin + static_cast<uint32_t>(min(c, nx - 1));
+                        const uint2 vf = box.F ? box.F[box.at(cx, cy, cz) + static_cast<uint32_t>(min(c, nx - 1))] : d.cellF[cc];
+                        const uint2 vb = d.cellB[cc], vr = d.cellR[cc];
+                        rf[c] = (in && rowF) ? vf : make_uint2(0u, 0u);
+                        rb[c] = (in && withB) ? vb : make_uint2(0u, 0u);
+                        rr[c] = (in && withR && d.nR) ? vr : make_uint2(0u, 0u);
+                    }
+#else
+#pragma unroll
+                    for (int c = 0; c < 3; c++)
+                    {
+                        const bool in = c < nx;
+                        rf[c] = (in && rowF) ? (box.F ? box.F[box.at(cx, cy, cz) + c] : d.cellF[lin + c]) : make_uint2(0u, 0u);
                         rb[c] = (in && withB) ? d.cellB[lin + c] : make_uint2(0u, 0u);
                         rr[c] = (in && withR && d.nR) ? d.cellR[lin + c] : make_uint2(0u, 0u);
                     }
+#endif
                     if (rowF)
                     {
                         if (!preF(row, rf))
@@ -657,8 +725,8 @@ namespace tsp
     // Cell form: one cell at a time, its candidates one at a time (no hit masks: every row is
     // tested, row numbers are not passed)
     template <typename OnF, typename OnB, typename OnR, typename NeedF, typename PreF, typename PostF>
-    __device__ __forceinline__ void forStencil(const Dev& d, const Frame& f, const Stencil& s, bool withF, bool withB,
-                                               bool withR, OnF onF, OnB onB, OnR onR, NeedF, PreF, PostF)
+    __device__ __forceinline__ void forStencil(const Dev& d, const Frame& f, const Stencil& s, const CellBox&, bool withF,
+                                               bool withB, bool withR, OnF onF, OnB onB, OnR onR, NeedF, PreF, PostF)
     {
         for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
         {
@@ -711,7 +779,7 @@ namespace tsp
             const float4 pi = d.pos[i];
             const bool masks = TSP_TUNE_SEARCH_ROW == 1 && f.reach == 1;
             if (masks) { uncovered = 0; }
-            forStencil(d, f, s, true, true, true,
+            forStencil(d, f, s, CellBox{}, true, true, true,
                 [&](uint32_t j, const float4& pj, uint32_t row, uint32_t q)
                 {
                     if (within(pi.x, pi.y, pi.z, pj, r2))
@@ -841,6 +909,9 @@ namespace tsp
         const uint32_t lane = i & (kSlice - 1), slice = i / kSlice;
         Stencil s;
         const bool active = i < d.nF && stencilOf(f, keyAt(d, d.keyF, i), s);
+        __shared__ uint2 boxF[kBoxCells > 0 ? kBoxCells : 1];
+        __shared__ int boxRed[6];
+        const CellBox box = stageCellBox(d, f, s, active, boxF, boxRed);
         const uint32_t cF = i < d.nF ? d.cnt[REL_FF][i] : 0u;
         const bool wide = i < d.nF && d.ffWideMin[i] != kEmpty32;
         SliceMeta mF{}, mB{}, mR{};
@@ -860,7 +931,7 @@ namespace tsp
                 else { sellWrite(d.pool[REL_FF], mF, lane, kF, j); }
                 kF++;
             };
-            forStencil(d, f, s, !wide, d.cnt[REL_FB][i] != 0, d.cnt[REL_FR][i] != 0,
+            forStencil(d, f, s, box, !wide, d.cnt[REL_FB][i] != 0, d.cnt[REL_FR][i] != 0,
                 [&](uint32_t j, const float4& pj, uint32_t, uint32_t)
                 {
                     if (within(pi.x, pi.y, pi.z, pj, r2)) { ffHit(j); }
@@ -876,7 +947,12 @@ namespace tsp
                 [&](uint32_t row)
                 {
                     const uint32_t cur = nextMask;
+#if TSP_TUNE_RANGE_LOADS
+                    // unconditional (the last row's word again past the rows): the load stays in flight
+                    nextMask = d.ffMask[static_cast<size_t>(min(row + 1u, kMaskRows - 1u)) * d.nF + i];
+#else
                     if (row + 1u < kMaskRows) { nextMask = d.ffMask[static_cast<size_t>(row + 1u) * d.nF + i]; }
+#endif
                     if (row >= kMaskRows || ((uncovered >> row) & 1u)) { return true; }
                     rowMask = cur;
                     return rowMask != 0u;

@@ -25,6 +25,10 @@
 //   TSP_TUNE_PREFETCH      the next four list entries load while the current four are summed
 //   TSP_TUNE_GATHER4       the sweeps read the LDS halo records of four entries before summing them
 //   TSP_TUNE_SWEEP_AHEAD   the first sweep group of a step: the previous step's sweeps plus this
+//   TSP_TUNE_BOX_CELLS     the fill stages the fluid cell ranges of its block's stencil box in LDS
+//                          when the box has at most this many cells (0: off)
+//   TSP_TUNE_RANGE_LOADS   the stencil walk loads a row's nine cell ranges in every lane and selects
+//                          (1), or loads only the ranges a lane walks (0)
 //   TSP_TUNE_QUERY_WAVES   register budget of the neighbor query and halo kernels: waves per SIMD
 //                          (0: the compiler's choice)
 //   TSP_TUNE_HALO_PAGES    the FF halo of a block from bitmaps over pages of 4096 indices (1), or
@@ -49,6 +53,12 @@
 // the settings chosen by turbospharticles_amd/tune.py, when present, come first
 #if __has_include("tsp_tuned.h")
 #include "tsp_tuned.h"
+#endif
+#ifndef TSP_TUNE_BOX_CELLS
+#define TSP_TUNE_BOX_CELLS 0
+#endif
+#ifndef TSP_TUNE_RANGE_LOADS
+#define TSP_TUNE_RANGE_LOADS 0
 #endif
 #ifndef TSP_TUNE_QUERY_WAVES
 #define TSP_TUNE_QUERY_WAVES 0
