@@ -1,5 +1,8 @@
 // tsp_tuned.h -- build-time variant chosen by turbospharticles_amd/tune.py on MI355X
-// (1M rigid cube drop: 3.763 ms per step; every candidate checked bit-identical)
+// (1M rigid cube drop: 3.662 ms per step; every candidate checked bit-identical)
+#ifndef TSP_TUNE_BOX_CELLS
+#define TSP_TUNE_BOX_CELLS 0
+#endif
 #ifndef TSP_TUNE_GATHER4
 #define TSP_TUNE_GATHER4 0
 #endif
@@ -8,6 +11,9 @@
 #endif
 #ifndef TSP_TUNE_HALO_MAX
 #define TSP_TUNE_HALO_MAX 1000
+#endif
+#ifndef TSP_TUNE_HALO_PAGES
+#define TSP_TUNE_HALO_PAGES 0
 #endif
 #ifndef TSP_TUNE_JACOBI_REC24
 #define TSP_TUNE_JACOBI_REC24 1
@@ -27,8 +33,14 @@
 #ifndef TSP_TUNE_PREFETCH
 #define TSP_TUNE_PREFETCH 1
 #endif
+#ifndef TSP_TUNE_QUERY_WAVES
+#define TSP_TUNE_QUERY_WAVES 0
+#endif
 #ifndef TSP_TUNE_RADIX_BITS
 #define TSP_TUNE_RADIX_BITS 8
+#endif
+#ifndef TSP_TUNE_RANGE_LOADS
+#define TSP_TUNE_RANGE_LOADS 0
 #endif
 #ifndef TSP_TUNE_SEARCH_BATCH
 #define TSP_TUNE_SEARCH_BATCH 4
@@ -43,7 +55,7 @@
 #define TSP_TUNE_STEP_GRAPH 0
 #endif
 #ifndef TSP_TUNE_SWEEP_AHEAD
-#define TSP_TUNE_SWEEP_AHEAD -1
+#define TSP_TUNE_SWEEP_AHEAD 2
 #endif
 #ifndef TSP_TUNE_SWEEP_THREADS
 #define TSP_TUNE_SWEEP_THREADS 256
