@@ -390,9 +390,31 @@ namespace tsp
         return a;
     }
 
+    // blockSum of N values at once, each in blockSum's order (bit-identical), with two barriers
+    // instead of 2N
+    template <int N>
+    __device__ void blockSumN(double (&v)[N], double (*red)[kSolveThreads / 64])
+    {
+        for (int n = 0; n < N; n++)
+        {
+            for (int o = 32; o > 0; o >>= 1) { v[n] += __shfl_xor(v[n], o, 64); }
+        }
+        const uint32_t w = threadIdx.x >> 6;
+        __syncthreads();
+        if ((threadIdx.x & 63u) == 0) { for (int n = 0; n < N; n++) { red[n][w] = v[n]; } }
+        __syncthreads();
+        for (int n = 0; n < N; n++)
+        {
+            double a = red[n][0];
+            for (uint32_t q = 1; q < kSolveThreads / 64; q++) { a += red[n][q]; }
+            v[n] = a;
+        }
+    }
+
     __global__ void __launch_bounds__(kSolveThreads) kContactSolve(Dev d)
     {
         __shared__ double red[kSolveThreads / 64];
+        __shared__ double redN[6][kSolveThreads / 64];
         __shared__ uint32_t bad;
         const uint32_t sc = blockIdx.x;
         Solve& so = d.solve[sc];
@@ -492,10 +514,10 @@ namespace tsp
                     acc[4] += static_cast<double>(rz) * fx - static_cast<double>(rx) * fz;
                     acc[5] += static_cast<double>(rx) * fy - static_cast<double>(ry) * fx;
                 }
-                for (int f = 0; f < 6; f++)
+                blockSumN(acc, redN);
+                if (t == 0)
                 {
-                    const double v = blockSum(acc[f], red);
-                    if (t == 0) { d.cLoad[f * d.nRB + b] = 0.0f + static_cast<float>(v); }
+                    for (int f = 0; f < 6; f++) { d.cLoad[f * d.nRB + b] = 0.0f + static_cast<float>(acc[f]); }
                 }
             }
             __syncthreads();
@@ -514,7 +536,9 @@ namespace tsp
                 if (e != e) { flags |= kContactBadResidual; } else { eB += e; }
             }
             if (flags) { atomicOr(&bad, flags); }
-            const double sum = blockSum(eR, red) + blockSum(eB, red);
+            double e2[2] = { eR, eB };
+            blockSumN(e2, redN);
+            const double sum = e2[0] + e2[1];
             avg = contacts > 0ull ? static_cast<float>(sum / static_cast<double>(contacts)) : 0.0f;
             converged = avg <= k.contactDevTarget;
             __syncthreads();
