@@ -202,6 +202,23 @@ def served_bench(args):
         same += int(all(np.array_equal(a.result.fluid[f], c.result.fluid[f]) for f in a.result.fluid)
                     and np.array_equal(a.result.fluid_id, c.result.fluid_id))
     lb, lp = serving.latency_summary(rb), serving.latency_summary(rp)
+    # per size bucket: its share of the work, the GPU time the plain form spent on its scenes
+    # (admission to completion, one scene at a time) and when the served form finished its last
+    # scene; a bucket whose scenes fill the GPU alone runs no faster batched than plain
+    edges = sorted(serving.DEFAULT_BUCKETS)
+    by_bucket = []
+    for b in range(len(edges) + 1):
+        sel = [q for q in range(n) if sum(rb[q].n[0] >= e for e in edges) == b]
+        if not sel:
+            continue
+        lo = 0 if b == 0 else edges[b - 1]
+        by_bucket.append({
+            "fluid": f">={lo}" + (f",<{edges[b]}" if b < len(edges) else ""),
+            "requests": len(sel),
+            "work_share": round(sum(rb[q].particle_steps for q in sel) / work, 3),
+            "plain_service_s": round(sum(rp[q].done_s - rp[q].admitted_s for q in sel), 2),
+            "served_first_admit_s": round(min(rb[q].admitted_s for q in sel), 2),
+            "served_last_done_s": round(max(rb[q].done_s for q in sel), 2)})
     return {"requests": n,
             "workload": f"{n} requests, Poisson arrivals at {args.serve_rate}/s (seed {args.serve_seed}), "
                         f"dam break / cube drop 50/50, fluid {min(s ** 3 for s in sides)}..{max(s ** 3 for s in sides)}, "
@@ -211,6 +228,8 @@ def served_bench(args):
             "p50_latency_s": lb["p50_s"], "p99_latency_s": lb["p99_s"], "latency_samples": n,
             "plain_value": round(work / wall_p, 1), "plain_p50_latency_s": lp["p50_s"],
             "plain_p99_latency_s": lp["p99_s"], "speedup_vs_plain": round(wall_p / wall_b, 2),
+            "last_arrival_s": round(float(arrivals[-1]), 2), "served_wall_s": round(wall_b, 2),
+            "plain_wall_s": round(wall_p, 2), "by_bucket": by_bucket,
             "results_identical_to_plain": f"{same}/{n}"}
 
 
