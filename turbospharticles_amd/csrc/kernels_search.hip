@@ -816,12 +816,41 @@ namespace tsp
             d.cnt[REL_FF][i] = static_cast<uint16_t>(cnt[0]);
             d.cnt[REL_FB][i] = static_cast<uint16_t>(cnt[1]);
             d.cnt[REL_FR][i] = static_cast<uint16_t>(cnt[2]);
-            d.ffWideMin[i] = wide ? mn[0] : kEmpty32;
+            d.ffMin[i] = (cnt[0] > 0 ? mn[0] : 0u) | (wide ? kWideBit : 0u);
             // u16 counts, as the reference's (neighbor_lists.h:43)
             if (cnt[0] > 0xffffu || cnt[1] > 0xffffu || cnt[2] > 0xffffu) { atomicOr(&d.status->overflow, 2u); }
             if (tiny) { atomicOr(&d.status->pairTiny, 1u); }
         }
-        // per slice: pitch, base (min index), format, entries
+#if TSP_TUNE_LANE_SORT
+        // the block's list positions, longest lists first (ties in particle order): a thread's
+        // rank is the number of the block's sort keys above its own, the keys made distinct by
+        // the thread index in their low byte. The slice statistics below then run over positions.
+        {
+            __shared__ uint32_t sortKey[kBlock];
+            __shared__ uint32_t byPos[9][kBlock];
+            const uint32_t t = threadIdx.x;
+            const uint32_t nFF = min(cnt[0], 255u), nFB = min(cnt[1], 255u);
+            const uint32_t key = TSP_TUNE_LANE_SORT == 2 ? (nFB << 8) | nFF : nFF;
+            const uint32_t own = (key << 8) | (kBlock - 1u - t);
+            sortKey[t] = own;
+            __syncthreads();
+            uint32_t rank = 0;
+            for (uint32_t u = 0; u < kBlock; u += 4)
+            {
+                const uint4 v = *reinterpret_cast<const uint4*>(&sortKey[u]);
+                rank += (v.x > own ? 1u : 0u) + (v.y > own ? 1u : 0u) + (v.z > own ? 1u : 0u) + (v.w > own ? 1u : 0u);
+            }
+            if (i < d.nF)
+            {
+                d.laneRank[i] = static_cast<uint8_t>(rank);
+                d.rankWho[blk * kBlock + rank] = static_cast<uint8_t>(t);
+            }
+            for (int r = 0; r < 3; r++) { byPos[r][rank] = cnt[r]; byPos[3 + r][rank] = mn[r]; byPos[6 + r][rank] = mx[r]; }
+            __syncthreads();
+            for (int r = 0; r < 3; r++) { cnt[r] = byPos[r][t]; mn[r] = byPos[3 + r][t]; mx[r] = byPos[6 + r][t]; }
+        }
+#endif
+        // per slice (list positions i of the block): pitch, base (min index), format, entries
         const uint32_t slice = i / kSlice;
         for (int r = 0; r < 3; r++)
         {
@@ -906,17 +935,23 @@ namespace tsp
         const uint32_t sc = d.blockScene[blk];
         const Frame& f = d.frames[sc];
         const float r2 = d.consts[sc].r2;
-        const uint32_t lane = i & (kSlice - 1), slice = i / kSlice;
+        // this particle's list position (its lane and slice), from the count pass
+        const uint32_t pos = i < d.nF ? sellPos(d, i) : i;
+        const uint32_t lane = pos & (kSlice - 1), slice = pos / kSlice;
         Stencil s;
         const bool active = i < d.nF && stencilOf(f, keyAt(d, d.keyF, i), s);
         __shared__ uint2 boxF[kBoxCells > 0 ? kBoxCells : 1];
         __shared__ int boxRed[6];
         const CellBox box = stageCellBox(d, f, s, active, boxF, boxRed);
         const uint32_t cF = i < d.nF ? d.cnt[REL_FF][i] : 0u;
-        const bool wide = i < d.nF && d.ffWideMin[i] != kEmpty32;
+        const uint32_t fMin = i < d.nF ? d.ffMin[i] : 0u;
+        const bool wide = (fMin & kWideBit) != 0u;
         SliceMeta mF{}, mB{}, mR{};
         if (i < d.nF) { mF = d.meta[REL_FF][slice]; mB = d.meta[REL_FB][slice]; mR = d.meta[REL_FR][slice]; }
-        const bool staged = !wide && mF.format == kFmtDelta16;
+        // the FF entries stage in LDS as u16 offsets: from the slice base in a u16 slice, from the
+        // particle's smallest neighbor in a u32 slice (the list spans at most 0xffff unless wide)
+        const bool staged = !wide;
+        const uint32_t stageBase = mF.format == kFmtDelta16 ? mF.base : fMin;
         if (active)
         {
             const float4 pi = d.pos[i];
@@ -927,7 +962,7 @@ namespace tsp
             uint32_t rowMask = 0, nextMask = d.ffMask[i];
             const auto ffHit = [&](uint32_t j)
             {
-                if (staged && kF < kStage) { stage[kF][t] = static_cast<uint16_t>(j - mF.base); }
+                if (staged && kF < kStage) { stage[kF][pos - blk * kBlock] = static_cast<uint16_t>(j - stageBase); }
                 else { sellWrite(d.pool[REL_FF], mF, lane, kF, j); }
                 kF++;
             };
@@ -978,7 +1013,7 @@ namespace tsp
                 // stored from the back when it fits 0xffff itself, else the candidate order stays.
                 // A list of more than kDictMax entries is no dict: candidate order (pass 1 only).
                 const uint32_t n = cF;
-                const uint32_t minRef = d.ffWideMin[i];
+                const uint32_t minRef = fMin & ~kWideBit;
                 uint32_t base2 = kEmpty32, maxRef = 0;
                 for (int pass = n > kDictMax ? 1 : 0; pass < 3; pass++)
                 {
@@ -1017,14 +1052,41 @@ namespace tsp
                 }
             }
         }
-        // the staged FF entries leave as whole slice rows: column k of the 32 lanes per store
+        // the staged FF entries leave as whole slice rows: column k of the slice's lanes per store
+#if TSP_TUNE_LANE_SORT
+        // (thread t writes list position t of the block)
+        __shared__ uint8_t ownAt[kBlock];
+        __shared__ uint32_t baseAt[kBlock];
+        ownAt[pos - blk * kBlock] = static_cast<uint8_t>((active && staged) ? min(cF, kStage) : 0u);
+        baseAt[pos - blk * kBlock] = stageBase;
         __syncthreads();
-        const uint32_t own = (active && staged) ? min(cF, kStage) : 0u;
+        const uint32_t own = ownAt[t], base = baseAt[t];
+        const uint32_t p = blk * kBlock + t;
+        SliceMeta mw{};
+        if (own) { mw = d.meta[REL_FF][p / kSlice]; }
+        const uint32_t lw = p & (kSlice - 1);
+#else
+        __syncthreads();
+        const uint32_t own = (active && staged) ? min(cF, kStage) : 0u, base = stageBase;
+        const SliceMeta mw = mF;
+        const uint32_t lw = lane;
+#endif
         const uint32_t rows = sliceMax(own);
-        uint16_t* row = d.pool[REL_FF] + mF.offset + lane;
-        for (uint32_t k = 0; k < rows; k++)
+        if (mw.format == kFmtDelta16)
         {
-            if (k < own) { row[k * kSlice] = stage[k][t]; }
+            uint16_t* row = d.pool[REL_FF] + mw.offset + lw;
+            for (uint32_t k = 0; k < rows; k++)
+            {
+                if (k < own) { row[k * kSlice] = stage[k][t]; }
+            }
+        }
+        else
+        {
+            uint32_t* row = reinterpret_cast<uint32_t*>(d.pool[REL_FF] + mw.offset) + lw;
+            for (uint32_t k = 0; k < rows; k++)
+            {
+                if (k < own) { row[k * kSlice] = base + stage[k][t]; }
+            }
         }
     }
 
@@ -1033,8 +1095,25 @@ namespace tsp
     // ascending, and the block's FF entries rewritten as slots of that list. One block per
     // particle block; a halo larger than kHaloMax leaves the block's entries as particle indices.
 
-    constexpr uint32_t kHashSlots = 2048;   // > 2 (kHaloMax + kBlock): probes always end
-    constexpr uint32_t kSortKeys = 1024;    // power of two >= kHaloMax
+    constexpr uint32_t kHashSlots = 2048;   // > kHaloMax + kBlock: probes always end
+
+    // With TSP_TUNE_SLOT16 the halo slots of a u32 slice are stored as u16 in the first half of
+    // its storage (entry k of lane l at u16 position k * kSlice + l, which the lanes reach only
+    // after reading the u32 entries there), and the slice turns u16 once its entries are written.
+    __device__ __forceinline__ void slotsToU16(const Dev& d, uint32_t i, const SliceMeta& m)
+    {
+#if TSP_TUNE_SLOT16
+        if (i < d.nF && (i & (kSlice - 1)) == 0 && m.format != kFmtDelta16)
+        {
+            SliceMeta& w = d.meta[REL_FF][i / kSlice];
+            w.format = kFmtDelta16;
+            w.base = 0;
+        }
+#else
+        (void)d; (void)i; (void)m;
+#endif
+    }
+    constexpr uint32_t kSortKeys = kHaloMax <= 1024 ? 1024 : 2048;    // power of two >= kHaloMax
 
     __device__ __forceinline__ uint32_t haloHash(uint32_t j) { return (j * 2654435761u) >> (32 - 11); }
 
@@ -1089,7 +1168,8 @@ namespace tsp
         SliceMeta m{};
         if (i < d.nF)
         {
-            n = d.cnt[REL_FF][i];
+            // thread t walks the lists at list position i
+            n = d.cnt[REL_FF][particleAt(d, i)];
             m = d.meta[REL_FF][i / kSlice];
         }
         const uint32_t lane = i & (kSlice - 1);
@@ -1182,9 +1262,10 @@ namespace tsp
             }
             const uint32_t q = lastSlot * kPageWords + ((j >> 5) & (kPageWords - 1u));
             const uint32_t slot = (wrank[q] + static_cast<uint32_t>(__popc(bits[q] & ((1u << (j & 31u)) - 1u)))) << kSlotShift;
-            if (m.format == kFmtDelta16) { d.pool[REL_FF][m.offset + k * kSlice + lane] = static_cast<uint16_t>(slot); }
+            if (m.format == kFmtDelta16 || TSP_TUNE_SLOT16) { d.pool[REL_FF][m.offset + k * kSlice + lane] = static_cast<uint16_t>(slot); }
             else { reinterpret_cast<uint32_t*>(d.pool[REL_FF] + m.offset)[k * kSlice + lane] = slot; }
         }
+        slotsToU16(d, i, m);
     }
 #else
     __global__ void __launch_bounds__(kBlock) TSP_WAVES_ATTR(TSP_TUNE_QUERY_WAVES) kHaloBuild(Dev d)
@@ -1204,7 +1285,8 @@ namespace tsp
         SliceMeta m{};
         if (i < d.nF)
         {
-            n = d.cnt[REL_FF][i];
+            // thread t walks the lists at list position i
+            n = d.cnt[REL_FF][particleAt(d, i)];
             m = d.meta[REL_FF][i / kSlice];
         }
         const uint32_t lane = i & (kSlice - 1);
@@ -1310,8 +1392,9 @@ namespace tsp
             if (t == 0) { d.haloCount[blk] = kNoHalo; }
             return;
         }
-        // compact, pad, sort ascending (bitonic over kSortKeys keys)
-        for (uint32_t q = t; q < kSortKeys; q += kBlock) { keys[q] = kEmpty32; }
+        // compact, pad, sort ascending (bitonic over the power of two at or above the count)
+        const uint32_t nk = 1u << (32 - __clz(static_cast<int>(max(count, 2u) - 1u)));
+        for (uint32_t q = t; q < nk; q += kBlock) { keys[q] = kEmpty32; }
         __syncthreads();
         for (uint32_t q = t; q < kHashSlots; q += kBlock)
         {
@@ -1319,11 +1402,11 @@ namespace tsp
             if (v != kEmpty32) { keys[atomicAdd(&filled, 1u)] = v; }
         }
         __syncthreads();
-        for (uint32_t size = 2; size <= kSortKeys; size <<= 1)
+        for (uint32_t size = 2; size <= nk; size <<= 1)
         {
             for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
             {
-                for (uint32_t q = t; q < kSortKeys; q += kBlock)
+                for (uint32_t q = t; q < nk; q += kBlock)
                 {
                     const uint32_t r = q ^ stride;
                     if (r > q)
@@ -1348,9 +1431,10 @@ namespace tsp
         {
             const uint32_t j = sellRead(d.pool[REL_FF], m, lane, k);
             const uint32_t slot = rank[haloFind(table, j)] << kSlotShift;
-            if (m.format == kFmtDelta16) { d.pool[REL_FF][m.offset + k * kSlice + lane] = static_cast<uint16_t>(slot); }
+            if (m.format == kFmtDelta16 || TSP_TUNE_SLOT16) { d.pool[REL_FF][m.offset + k * kSlice + lane] = static_cast<uint16_t>(slot); }
             else { reinterpret_cast<uint32_t*>(d.pool[REL_FF] + m.offset)[k * kSlice + lane] = slot; }
         }
+        slotsToU16(d, i, m);
     }
 #endif
 

@@ -39,7 +39,8 @@ namespace tsp
         if (stopped(d)) { return; }
         __shared__ float4 hp[kHaloMax];
         const uint32_t blk = xcdBlock();
-        const uint32_t i = blk * kBlock + threadIdx.x;
+        const uint32_t p = blk * kBlock + threadIdx.x;   // list position; i its particle
+        const uint32_t i = particleAt(d, p);
         const uint32_t sc = d.blockScene[blk];
         const Consts& k = d.consts[sc];
         const uint32_t hc = d.haloCount[blk];
@@ -47,7 +48,7 @@ namespace tsp
         if (i - d.scenes[sc].offF >= d.scenes[sc].nF) { return; }
         const float4 pi = d.pos[i];
         const float xi = pi.x, yi = pi.y, zi = pi.z;
-        const List ff = listOf(d, REL_FF, i);
+        const List ff = listAt(d, REL_FF, p, i);
         float dF;
         if (ff.n <= kDictMax)
         {
@@ -66,13 +67,13 @@ namespace tsp
             dF = s;
         }
         float dB = 0.0f, dR = 0.0f;
-        const List fb = listOf(d, REL_FB, i);
+        const List fb = listAt(d, REL_FB, p, i);
         visit(fb, [&](uint32_t j, uint32_t)
         {
             const float4 bj = d.bpos[j];
             dB = __builtin_fmaf(bj.w, kwNormal<kFast>(xi - bj.x, yi - bj.y, zi - bj.z, k), dB);
         });
-        const List fr = listOf(d, REL_FR, i);
+        const List fr = listAt(d, REL_FR, p, i);
         visit(fr, [&](uint32_t j, uint32_t)
         {
             const float4 rj = d.rpos[j];
@@ -87,7 +88,8 @@ namespace tsp
         if (stopped(d)) { return; }
         __shared__ float4 hp[kHaloMax], hv[kHaloMax];
         const uint32_t blk = xcdBlock();
-        const uint32_t i = blk * kBlock + threadIdx.x;
+        const uint32_t p = blk * kBlock + threadIdx.x;   // list position; i its particle
+        const uint32_t i = particleAt(d, p);
         const uint32_t sc = d.blockScene[blk];
         const Consts& k = d.consts[sc];
         const uint32_t hc = d.haloCount[blk];
@@ -96,7 +98,7 @@ namespace tsp
         const float4 pi = d.pos[i], vi = d.vel[i];
         const float xi = pi.x, yi = pi.y, zi = pi.z, vxi = vi.x, vyi = vi.y, vzi = vi.z;
         float aFx = 0.0f, aFy = 0.0f, aFz = 0.0f;
-        const List ff = listOf(d, REL_FF, i);
+        const List ff = listAt(d, REL_FF, p, i);
         // dict chunks scale by the mass after the sum, normal chunks per neighbor (sph_solver.cpp:974-1049)
         const bool dict = ff.n <= kDictMax;
         const auto ffTerm = [&](const float4& pj, const float4& vj)
@@ -116,7 +118,7 @@ namespace tsp
                 [&](uint32_t j, uint32_t) { ffTerm(d.pos[j], d.vel[j]); });
         if (dict) { aFx *= k.mass; aFy *= k.mass; aFz *= k.mass; }
         float aBx = 0.0f, aBy = 0.0f, aBz = 0.0f;
-        const List fb = listOf(d, REL_FB, i);
+        const List fb = listAt(d, REL_FB, p, i);
         visit(fb, [&](uint32_t j, uint32_t e)
         {
             const float4 bj = d.bpos[j];
@@ -132,7 +134,7 @@ namespace tsp
             aBz = __builtin_fmaf(coeff, gz, aBz);
         });
         float aRx = 0.0f, aRy = 0.0f, aRz = 0.0f;
-        const List fr = listOf(d, REL_FR, i);
+        const List fr = listAt(d, REL_FR, p, i);
         visit(fr, [&](uint32_t j, uint32_t e)
         {
             const float4 rj = d.rpos[j], uj = d.rvel[j];
@@ -159,7 +161,8 @@ namespace tsp
         if (stopped(d)) { return; }
         __shared__ float4 hp[kHaloMax], hv[kHaloMax];
         const uint32_t blk = xcdBlock();
-        const uint32_t i = blk * kBlock + threadIdx.x;
+        const uint32_t p = blk * kBlock + threadIdx.x;   // list position; i its particle
+        const uint32_t i = particleAt(d, p);
         const uint32_t sc = d.blockScene[blk];
         const Consts& k = d.consts[sc];
         const uint32_t hc = d.haloCount[blk];
@@ -168,7 +171,7 @@ namespace tsp
         const float4 pi = d.pos[i], vi = d.vp[i];
         const float xi = pi.x, yi = pi.y, zi = pi.z, vxi = vi.x, vyi = vi.y, vzi = vi.z;
         float sF = 0.0f, cFx = 0.0f, cFy = 0.0f, cFz = 0.0f, g2 = 0.0f;
-        const List ff = listOf(d, REL_FF, i);
+        const List ff = listAt(d, REL_FF, p, i);
         const auto ffTerm = [&](const float4& pj, const float4& vj, uint32_t e)
         {
             const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
@@ -191,7 +194,7 @@ namespace tsp
         visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t e) { ffTerm(recAt(hp, q), recAt(hv, q), e); },
                 [&](uint32_t j, uint32_t e) { ffTerm(d.pos[j], d.vp[j], e); });
         float sB = 0.0f, cBx = 0.0f, cBy = 0.0f, cBz = 0.0f;
-        const List fb = listOf(d, REL_FB, i);
+        const List fb = listAt(d, REL_FB, p, i);
         visit(fb, [&](uint32_t j, uint32_t)
         {
             const float4 bj = d.bpos[j];
@@ -207,7 +210,7 @@ namespace tsp
             cBz = __builtin_fmaf(psi, gz, cBz);
         });
         float sR = 0.0f, cRx = 0.0f, cRy = 0.0f, cRz = 0.0f;
-        const List fr = listOf(d, REL_FR, i);
+        const List fr = listAt(d, REL_FR, p, i);
         visit(fr, [&](uint32_t j, uint32_t)
         {
             const float4 rj = d.rpos[j], uj = d.rvp[j];
@@ -423,12 +426,13 @@ namespace tsp
         stageHalo(d, blk, hc, [&](uint32_t q, uint32_t j) { hp[q] = d.pos[j]; });
         for (uint32_t r = 0; r < kBlock / kSweepThreads; r++)
         {
-        const uint32_t i = blk * kBlock + r * kSweepThreads + threadIdx.x;
+        const uint32_t p = blk * kBlock + r * kSweepThreads + threadIdx.x;   // list position; i its particle
+        const uint32_t i = particleAt(d, p);
         if (i - d.scenes[sc].offF >= d.scenes[sc].nF) { continue; }
         const float4 p4 = d.pos[i];
         const float xi = p4.x, yi = p4.y, zi = p4.z, pi = p4.w;
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-        const List ff = listOf(d, REL_FF, i);
+        const List ff = listAt(d, REL_FF, p, i);
         // dict chunks sum (p_i + p_j) kg and scale once; normal chunks scale each neighbor's
         // pressure term (sph_solver.cpp:1573-1640)
         const bool dict = ff.n <= kDictMax;
@@ -450,7 +454,7 @@ namespace tsp
         visitFFSplit(ff, hc <= kHaloMax, [&](uint32_t q) { return recAt(hp, q); }, ffTerm,
                      [&](uint32_t j, uint32_t e) { ffTerm(d.pos[j], e); });
         float ax = dict ? sx * k.presOuterFF : sx, ay = dict ? sy * k.presOuterFF : sy, az = dict ? sz * k.presOuterFF : sz;
-        const List fb = listOf(d, REL_FB, i);
+        const List fb = listAt(d, REL_FB, p, i);
         visit(fb, [&](uint32_t j, uint32_t)
         {
             const float4 bj = d.bpos[j];
@@ -461,7 +465,7 @@ namespace tsp
             ay = __builtin_fmaf(coeff, f * dy, ay);
             az = __builtin_fmaf(coeff, f * dz, az);
         });
-        const List fr = listOf(d, REL_FR, i);
+        const List fr = listAt(d, REL_FR, p, i);
         visit(fr, [&](uint32_t j, uint32_t)
         {
             const float4 rj = d.rpos[j];
@@ -523,13 +527,14 @@ namespace tsp
         double eRaw = 0.0, ePos = 0.0, eAbs = 0.0;
         for (uint32_t r = 0; r < kBlock / kSweepThreads; r++)
         {
-        const uint32_t i = blk * kBlock + r * kSweepThreads + threadIdx.x;
+        const uint32_t p = blk * kBlock + r * kSweepThreads + threadIdx.x;   // list position; i its particle
+        const uint32_t i = particleAt(d, p);
         if (i - S.offF < S.nF)
         {
             const float4 pi = d.pos[i], ai = d.ap[i];
             const float xi = pi.x, yi = pi.y, zi = pi.z, axi = ai.x, ayi = ai.y, azi = ai.z;
             float sf = 0.0f;
-            const List ff = listOf(d, REL_FF, i);
+            const List ff = listAt(d, REL_FF, p, i);
             const auto ffTerm = [&](float xj, float yj, float zj, float axj, float ayj, float azj, uint32_t e)
             {
                 const float dx = xi - xj, dy = yi - yj, dz = zi - zj;
@@ -570,7 +575,7 @@ namespace tsp
                          });
             sf *= k.mass;
             float sb = 0.0f;
-            const List fb = listOf(d, REL_FB, i);
+            const List fb = listAt(d, REL_FB, p, i);
             visit(fb, [&](uint32_t j, uint32_t)
             {
                 const float4 bj = d.bpos[j];
@@ -579,7 +584,7 @@ namespace tsp
                 const float sv = __builtin_fmaf(axi, f * dx, __builtin_fmaf(ayi, f * dy, azi * (f * dz)));
                 sb += bj.w * sv;
             });
-            const List fr = listOf(d, REL_FR, i);
+            const List fr = listAt(d, REL_FR, p, i);
             visit(fr, [&](uint32_t j, uint32_t)
             {
                 const float4 rj = d.rpos[j];

@@ -857,7 +857,9 @@ struct tsp_ctx
             d.scanTmp = alloc<uint32_t>(scanTmpWords(m), work);
             zero(d.scanTmp, sizeof(uint32_t) * scanTmpWords(m));
         }
-        d.ffWideMin = alloc<uint32_t>(nF, work, MEM_LF);
+        d.ffMin = alloc<uint32_t>(nF, work, MEM_LF);
+        d.laneRank = alloc<uint8_t>(nF, work, MEM_LF);
+        d.rankWho = alloc<uint8_t>(nF, work, MEM_LF);
         d.ffMask = alloc<uint32_t>(static_cast<size_t>(kMaskWords) * nF, work, MEM_SF);
         {
             const uint64_t nBlocks = nF / kBlock;
@@ -920,10 +922,10 @@ struct tsp_ctx
             solveCap = S;
         }
         if (S) { std::memset(hostSolve, 0, sizeof(Solve) * S); }
-        // pools: kept when large enough (about 48 FF, 16 FB and 8 FR entries per particle to
+        // pools: kept when large enough (about 56 FF, 16 FB and 8 FR entries per particle to
         // start, grown on demand)
         {
-            uint64_t cap[3] = { 48ull * nF + 1024, 16ull * nF + 1024, 8ull * nF + 1024 };
+            uint64_t cap[3] = { 56ull * nF + 1024, 16ull * nF + 1024, 8ull * nF + 1024 };
             uint64_t rfCap = 64ull * nR + 1024;
             const bool have = !poolAllocs.empty();
             if (!have || cap[0] > d.poolCap[0] || cap[1] > d.poolCap[1] || cap[2] > d.poolCap[2] || rfCap > d.rfCap)
@@ -2267,15 +2269,23 @@ int tsp_get_neighbors(tsp_ctx* ctx, int rel, uint32_t* counts, uint32_t* idx, ui
             }
             return TSP_OK;
         }
-        // the scene's slices (its fluid segment starts at a multiple of the slice height)
+        // the scene's slices (its fluid segment starts at a multiple of the block size and is
+        // padded to one)
         const uint32_t n = h.nF;
-        const uint32_t s0 = h.offF / kSlice, nSlices = (n + kSlice - 1) / kSlice;
+        const uint32_t s0 = h.offF / kSlice, nSlices = roundUp(n, kBlock) / kSlice;
         std::vector<uint16_t> cnt(n);
         std::vector<SliceMeta> meta(nSlices);
+        // particle i's lists sit at list position (i - i % 256) + laneRank[i] of the scene
+        std::vector<uint8_t> laneRank(n);
         if (n)
         {
             hipCheck(hipMemcpy(cnt.data(), d.cnt[rel] + h.offF, sizeof(uint16_t) * n, hipMemcpyDeviceToHost), "D2H");
             hipCheck(hipMemcpy(meta.data(), d.meta[rel] + s0, sizeof(SliceMeta) * nSlices, hipMemcpyDeviceToHost), "D2H");
+#if TSP_TUNE_LANE_SORT
+            hipCheck(hipMemcpy(laneRank.data(), d.laneRank + h.offF, n, hipMemcpyDeviceToHost), "D2H");
+#else
+            for (uint32_t i = 0; i < n; i++) { laneRank[i] = static_cast<uint8_t>(i % kBlock); }
+#endif
         }
         const uint32_t mask = 0xffffu;
         uint64_t tot = 0;
@@ -2301,8 +2311,9 @@ int tsp_get_neighbors(tsp_ctx* ctx, int rel, uint32_t* counts, uint32_t* idx, ui
         uint64_t w = 0;
         for (uint32_t i = 0; i < n; i++)
         {
-            const SliceMeta& m = meta[i / kSlice];
-            const uint32_t lane = i % kSlice;
+            const uint32_t p = i - i % kBlock + laneRank[i];
+            const SliceMeta& m = meta[p / kSlice];
+            const uint32_t lane = p % kSlice;
             const uint32_t blk = i / kBlock;
             const bool slots = rel == REL_FF && haloCount[blk] <= kHaloMax;
             for (uint32_t k = 0; k < (cnt[i] & mask); k++)
@@ -2618,11 +2629,12 @@ int tsp_batch_step(tsp_ctx* ctx, uint32_t n_steps)
 uint64_t tsp_batch_footprint(uint32_t n_fluid, uint32_t n_boundary, uint32_t n_rigid, uint32_t n_bodies)
 {
     // device bytes a scene holds while resident: per fluid slot (segment rounded to 256) about
-    // 190 B of records and second buffers, 16 B of sort keys, 150 B of SELL-32 lists at the
-    // initial pool sizes, 20 B of halo and counts, and up to 24 B of cell tables; per boundary
-    // particle 64 B; per rigid particle about 400 B (records, keys, the RF list); per body 256 B
+    // 190 B of records and second buffers, 16 B of sort keys, 160 B of SELL-32 lists at the
+    // initial pool sizes, 22 B of halo, counts and list positions, and up to 24 B of cell tables;
+    // per boundary particle 64 B; per rigid particle about 400 B (records, keys, the RF list);
+    // per body 256 B
     const uint64_t f = static_cast<uint64_t>(roundUp(n_fluid, kBlock));
-    return f * 400ull + static_cast<uint64_t>(n_boundary) * 64ull + static_cast<uint64_t>(n_rigid) * 400ull
+    return f * 412ull + static_cast<uint64_t>(n_boundary) * 64ull + static_cast<uint64_t>(n_rigid) * 400ull
          + static_cast<uint64_t>(n_bodies) * 256ull + (64ull << 10);
 }
 

@@ -11,10 +11,13 @@
 //              body-frame offsets, body (u32), id (u64), reaction force (f32); per-body state
 //              field-major (f32) with FP64 force/torque sums
 //   cell grid  dense over the cell frame, one uint2 {begin, end} per cell and particle class
-//   SELL-32    per slice of 32 consecutive fluid particles and relation (FF, FB, FR): one
+//   SELL-32    per slice of 32 fluid list positions and relation (FF, FB, FR): one
 //              SliceMeta {offset in u16 units, base, pitch, format}; entry k of lane l at
 //              offset + k*32 + l, as u16 (j - base) or u32 j. FF entries of a block with a halo
-//              hold the slot of j in the block's halo instead (same u16/u32 storage, no base)
+//              hold the slot of j in the block's halo instead (same u16/u32 storage, no base).
+//              List position p holds the lists of one particle of p's 256-particle block: p
+//              itself, or, with lane sorting, the block's particles by list length (laneRank,
+//              rankWho)
 //   halo       per 256-particle block: up to 1000 distinct FF neighbor indices, ascending
 #pragma once
 
@@ -47,6 +50,11 @@
 //   TSP_TUNE_RADIX_BITS    digit width of the LSD radix sorts (8: 256 buckets, 6, 4)
 //   TSP_TUNE_SWEEP_THREADS threads per 256-particle tile of the solver sweeps (256, or 128: two
 //                          particles per thread)
+//   TSP_TUNE_SLOT16        the halo build stores the halo slots of u32 slices as u16 (1), or keeps
+//                          their u32 storage (0)
+//   TSP_TUNE_LANE_SORT     the lists of each 256-particle block take its list positions longest
+//                          first, so a wave's lanes walk lists of similar length (1: by FF count,
+//                          2: by FB count, then FF count; 0: positions in particle order)
 //   TSP_TUNE_SEARCH_ROW    the neighbor query walks the stencil by rows of three cells, loading
 //                          the rows' ranges together and TSP_TUNE_SEARCH_BATCH candidates at a time
 //                          (1: the row's candidates as one stream, 2: cell by cell; 0: the cell walk)
@@ -101,6 +109,12 @@
 #endif
 #ifndef TSP_TUNE_PERSISTENT
 #define TSP_TUNE_PERSISTENT 0
+#endif
+#ifndef TSP_TUNE_SLOT16
+#define TSP_TUNE_SLOT16 1
+#endif
+#ifndef TSP_TUNE_LANE_SORT
+#define TSP_TUNE_LANE_SORT 1
 #endif
 #ifndef TSP_TUNE_SEARCH_BATCH
 #define TSP_TUNE_SEARCH_BATCH 4
@@ -158,6 +172,8 @@ namespace tsp
     constexpr uint32_t kNoHalo = 0xffffffffu;   // halo count of a block that keeps particle indices
 
     enum Rel { REL_FF = 0, REL_FB = 1, REL_FR = 2 };
+
+    constexpr uint32_t kWideBit = 0x80000000u;   // in ffMin: the particle's FF list spans more than 0xffff
 
     struct SliceMeta
     {
@@ -362,7 +378,10 @@ namespace tsp
         uint16_t *pool[3];
         uint64_t poolCap[3];     // u16 units
         float *kgFF;             // per FF entry (by its u16 position): gradient factor of the step (build variant)
-        uint32_t *ffWideMin;     // per particle: min FF index of a list spanning more than 0xffff, else kEmpty32
+        uint8_t *laneRank;       // per particle: its list position within its block (lane sorting)
+        uint8_t *rankWho;        // per list position: the particle's index within the block (lane sorting)
+        uint32_t *ffMin;         // per particle: smallest FF neighbor index (0 without FF neighbors), with
+                                 // kWideBit set when the list spans more than 0xffff
         uint32_t *ffMask;        // kMaskWords x nF: per stencil row, bit q = the row's q-th fluid candidate
                                  // is an FF neighbor; then the rows the masks do not cover
         uint32_t *haloIdx;       // per block: kHaloMax particle indices, ascending
@@ -647,15 +666,39 @@ namespace tsp
         __device__ __forceinline__ uint32_t operator[](uint32_t k) const { return sellRead(pool, m, lane, k); }
     };
 
-    __device__ __forceinline__ List listOf(const Dev& d, int rel, uint32_t i)
+    // the list position of particle i, and the particle at list position p (both in its block)
+    __device__ __forceinline__ uint32_t sellPos(const Dev& d, uint32_t i)
+    {
+#if TSP_TUNE_LANE_SORT
+        return (i & ~(kBlock - 1u)) + d.laneRank[i];
+#else
+        (void)d;
+        return i;
+#endif
+    }
+
+    __device__ __forceinline__ uint32_t particleAt(const Dev& d, uint32_t p)
+    {
+#if TSP_TUNE_LANE_SORT
+        return (p & ~(kBlock - 1u)) + d.rankWho[p];
+#else
+        (void)d;
+        return p;
+#endif
+    }
+
+    // the list of relation rel of particle i, stored at list position p
+    __device__ __forceinline__ List listAt(const Dev& d, int rel, uint32_t p, uint32_t i)
     {
         List l;
         l.pool = d.pool[rel];
-        l.m = d.meta[rel][i / kSlice];
-        l.lane = i & (kSlice - 1);
+        l.m = d.meta[rel][p / kSlice];
+        l.lane = p & (kSlice - 1);
         l.n = d.cnt[rel][i];
         return l;
     }
+
+    __device__ __forceinline__ List listOf(const Dev& d, int rel, uint32_t i) { return listAt(d, rel, sellPos(d, i), i); }
 
     // visits the entries of one list in storage order, as use(load(j), e): j the neighbor (or its
     // halo slot when kSlots: the stored value without the slice base) and e the entry's position

@@ -51,11 +51,13 @@ SPACE = {
     "TSP_TUNE_BOX_CELLS": [0, 512, 768, 1024],
     "TSP_TUNE_JACOBI_REC24": [1, 0],
     "TSP_TUNE_KG_STORE": [0, 1],
-    "TSP_TUNE_HALO_MAX": [1000, 960, 1024],
+    "TSP_TUNE_HALO_MAX": [1000, 960, 1024, 1064],
     "TSP_TUNE_GRAPHS": [1, 0],
     "TSP_TUNE_PA_WAVES": [0, 6, 8],
     "TSP_TUNE_JACOBI_WAVES": [0, 6],
     "TSP_TUNE_SEARCH_ROW": [1, 0, 2],
+    "TSP_TUNE_LANE_SORT": [1, 0, 2],
+    "TSP_TUNE_SLOT16": [1, 0],
     "TSP_TUNE_SEARCH_BATCH": [4, 8],
     "TSP_TUNE_PERSISTENT": [0, 1],
 }
