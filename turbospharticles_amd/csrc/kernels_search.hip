@@ -935,8 +935,10 @@ namespace tsp
         const uint32_t sc = d.blockScene[blk];
         const Frame& f = d.frames[sc];
         const float r2 = d.consts[sc].r2;
-        // this particle's list position (its lane and slice), from the count pass
+        // this particle's list position (its lane and slice), from the count pass; col is its
+        // column of the staging array
         const uint32_t pos = i < d.nF ? sellPos(d, i) : i;
+        const uint32_t col = pos & (kBlock - 1u);
         const uint32_t lane = pos & (kSlice - 1), slice = pos / kSlice;
         Stencil s;
         const bool active = i < d.nF && stencilOf(f, keyAt(d, d.keyF, i), s);
@@ -948,10 +950,11 @@ namespace tsp
         const bool wide = (fMin & kWideBit) != 0u;
         SliceMeta mF{}, mB{}, mR{};
         if (i < d.nF) { mF = d.meta[REL_FF][slice]; mB = d.meta[REL_FB][slice]; mR = d.meta[REL_FR][slice]; }
-        // the FF entries stage in LDS as u16 offsets: from the slice base in a u16 slice, from the
-        // particle's smallest neighbor in a u32 slice (the list spans at most 0xffff unless wide)
+        // the FF entries stage in LDS as u16 offsets from mF.base: the slice base in a u16 slice,
+        // the particle's smallest neighbor in a u32 slice, whose stores ignore the base (the list
+        // spans at most 0xffff unless wide)
+        if (mF.format != kFmtDelta16) { mF.base = fMin; }
         const bool staged = !wide;
-        const uint32_t stageBase = mF.format == kFmtDelta16 ? mF.base : fMin;
         if (active)
         {
             const float4 pi = d.pos[i];
@@ -962,7 +965,7 @@ namespace tsp
             uint32_t rowMask = 0, nextMask = d.ffMask[i];
             const auto ffHit = [&](uint32_t j)
             {
-                if (staged && kF < kStage) { stage[kF][pos - blk * kBlock] = static_cast<uint16_t>(j - stageBase); }
+                if (staged && kF < kStage) { stage[kF][col] = static_cast<uint16_t>(j - mF.base); }
                 else { sellWrite(d.pool[REL_FF], mF, lane, kF, j); }
                 kF++;
             };
@@ -1013,7 +1016,7 @@ namespace tsp
                 // stored from the back when it fits 0xffff itself, else the candidate order stays.
                 // A list of more than kDictMax entries is no dict: candidate order (pass 1 only).
                 const uint32_t n = cF;
-                const uint32_t minRef = fMin & ~kWideBit;
+                const uint32_t minRef = d.ffMin[i] & ~kWideBit;
                 uint32_t base2 = kEmpty32, maxRef = 0;
                 for (int pass = n > kDictMax ? 1 : 0; pass < 3; pass++)
                 {
@@ -1054,20 +1057,20 @@ namespace tsp
         }
         // the staged FF entries leave as whole slice rows: column k of the slice's lanes per store
 #if TSP_TUNE_LANE_SORT
-        // (thread t writes list position t of the block)
-        __shared__ uint8_t ownAt[kBlock];
-        __shared__ uint32_t baseAt[kBlock];
-        ownAt[pos - blk * kBlock] = static_cast<uint8_t>((active && staged) ? min(cF, kStage) : 0u);
-        baseAt[pos - blk * kBlock] = stageBase;
+        // (thread t writes list position t of the block: the staged entries of the particle there,
+        // whose count and base it reads back; a particle outside the frame has no FF entries)
         __syncthreads();
-        const uint32_t own = ownAt[t], base = baseAt[t];
         const uint32_t p = blk * kBlock + t;
+        const uint32_t w = particleAt(d, p);
+        const uint32_t wMin = d.ffMin[w];
+        const uint32_t own = (wMin & kWideBit) ? 0u : min(static_cast<uint32_t>(d.cnt[REL_FF][w]), kStage);
         SliceMeta mw{};
         if (own) { mw = d.meta[REL_FF][p / kSlice]; }
+        const uint32_t base = mw.format == kFmtDelta16 ? mw.base : wMin;
         const uint32_t lw = p & (kSlice - 1);
 #else
         __syncthreads();
-        const uint32_t own = (active && staged) ? min(cF, kStage) : 0u, base = stageBase;
+        const uint32_t own = (active && staged) ? min(cF, kStage) : 0u, base = mF.base;
         const SliceMeta mw = mF;
         const uint32_t lw = lane;
 #endif
