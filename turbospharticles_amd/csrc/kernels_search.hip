@@ -597,7 +597,7 @@ namespace tsp
     // Row form: the ranges of up to three x-adjacent cells load together, and each relation's
     // candidates along the row form one stream whose loads issue TSP_TUNE_SEARCH_BATCH at a time
     // before their tests.
-    template <typename Index, typename Load, typename Fn>
+    template <bool kFlat = false, typename Index, typename Load, typename Fn>
     __device__ __forceinline__ void rowStream(const uint2 (&r)[3], Index index, Load load, Fn fn)
     {
         constexpr uint32_t kW = TSP_TUNE_SEARCH_BATCH;
@@ -639,11 +639,23 @@ namespace tsp
 #pragma unroll
             for (uint32_t u = 0; u < kW; u++)
             {
-                const uint32_t q = t + u;
-                const uint32_t e = q + (q < n0 ? o0 : (q < n01 ? o1 : o2));
-                j[u] = 0u;
-                p[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (q < total) { j[u] = index(e); p[u] = load(j[u]); }
+                if (kFlat)
+                {
+                    // past the stream's end the last candidate loads again (fn does not see it):
+                    // the loads need no branch
+                    const uint32_t q = min(t + u, total - 1u);
+                    const uint32_t e = q + (q < n0 ? o0 : (q < n01 ? o1 : o2));
+                    j[u] = index(e);
+                    p[u] = load(j[u]);
+                }
+                else
+                {
+                    const uint32_t q = t + u;
+                    const uint32_t e = q + (q < n0 ? o0 : (q < n01 ? o1 : o2));
+                    j[u] = 0u;
+                    p[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if (q < total) { j[u] = index(e); p[u] = load(j[u]); }
+                }
             }
 #pragma unroll
             for (uint32_t u = 0; u < kW; u++)
@@ -656,7 +668,7 @@ namespace tsp
 
     // preF(row, rf) may take a row's fluid candidates itself (returns true) instead of the stream;
     // postF(row) follows the row's fluid part. row counts the rows of three cells in stencil order.
-    template <typename OnF, typename OnB, typename OnR, typename NeedF, typename PreF, typename PostF>
+    template <bool kFlat = false, typename OnF, typename OnB, typename OnR, typename NeedF, typename PreF, typename PostF>
     __device__ __forceinline__ void forStencil(const Dev& d, const Frame& f, const Stencil& s, const CellBox& box, bool withF,
                                                bool withB, bool withR, OnF onF, OnB onB, OnR onR, NeedF needF, PreF preF,
                                                PostF postF)
@@ -702,20 +714,20 @@ namespace tsp
                     {
                         if (!preF(row, rf))
                         {
-                            rowStream(rf, same, [&](uint32_t j) { return d.pos[j]; },
-                                      [&](uint32_t j, const float4& p, uint32_t q) { onF(j, p, row, q); });
+                            rowStream<kFlat>(rf, same, [&](uint32_t j) { return d.pos[j]; },
+                                             [&](uint32_t j, const float4& p, uint32_t q) { onF(j, p, row, q); });
                         }
                         postF(row);
                     }
                     if (withB)
                     {
-                        rowStream(rb, same, [&](uint32_t j) { return d.bpos[j]; },
-                                  [&](uint32_t j, const float4& p, uint32_t) { onB(j, p); });
+                        rowStream<kFlat>(rb, same, [&](uint32_t j) { return d.bpos[j]; },
+                                         [&](uint32_t j, const float4& p, uint32_t) { onB(j, p); });
                     }
                     if (withR)
                     {
-                        rowStream(rr, [&](uint32_t e) { return d.idxR[e]; }, [&](uint32_t j) { return d.rpos[j]; },
-                                  [&](uint32_t j, const float4& p, uint32_t) { onR(j, p); });
+                        rowStream<kFlat>(rr, [&](uint32_t e) { return d.idxR[e]; }, [&](uint32_t j) { return d.rpos[j]; },
+                                         [&](uint32_t j, const float4& p, uint32_t) { onR(j, p); });
                     }
                 }
             }
@@ -724,7 +736,7 @@ namespace tsp
 #else
     // Cell form: one cell at a time, its candidates one at a time (no hit masks: every row is
     // tested, row numbers are not passed)
-    template <typename OnF, typename OnB, typename OnR, typename NeedF, typename PreF, typename PostF>
+    template <bool kFlat = false, typename OnF, typename OnB, typename OnR, typename NeedF, typename PreF, typename PostF>
     __device__ __forceinline__ void forStencil(const Dev& d, const Frame& f, const Stencil& s, const CellBox&, bool withF,
                                                bool withB, bool withR, OnF onF, OnB onB, OnR onR, NeedF, PreF, PostF)
     {
@@ -779,6 +791,45 @@ namespace tsp
             const float4 pi = d.pos[i];
             const bool masks = TSP_TUNE_SEARCH_ROW == 1 && f.reach == 1;
             if (masks) { uncovered = 0; }
+#if TSP_TUNE_SEARCH_FLAT
+            // a hit updates the counts, spans, tiny flag and masks through selects, without a branch
+            // (nearly every candidate step of a wave has a hit in some lane)
+            const auto hitOf = [&](const float4& pj, float& d2)
+            {
+                d2 = dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
+                return d2 <= r2;
+            };
+            forStencil<true>(d, f, s, CellBox{}, true, true, true,
+                [&](uint32_t j, const float4& pj, uint32_t row, uint32_t q)
+                {
+                    float d2;
+                    const bool hit = hitOf(pj, d2);
+                    cnt[0] += hit ? 1u : 0u;
+                    mn[0] = min(mn[0], hit ? j : kEmpty32);
+                    mx[0] = max(mx[0], hit ? j : 0u);
+                    tiny = tiny || (hit && tinyDist2(d2));
+                    const uint32_t bit = hit ? 1u : 0u;
+                    rowMask |= q < 32u ? bit << q : 0u;
+                    uncovered |= (q >= 32u && row < kMaskRows) ? bit << row : 0u;
+                },
+                [&](uint32_t j, const float4& pj)
+                {
+                    float d2;
+                    const bool hit = hitOf(pj, d2);
+                    cnt[1] += hit ? 1u : 0u;
+                    mn[1] = min(mn[1], hit ? j : kEmpty32);
+                    mx[1] = max(mx[1], hit ? j : 0u);
+                    tiny = tiny || (hit && tinyDist2(d2));
+                },
+                [&](uint32_t j, const float4& pj)
+                {
+                    float d2;
+                    const bool hit = hitOf(pj, d2);
+                    cnt[2] += hit ? 1u : 0u;
+                    mn[2] = min(mn[2], hit ? j : kEmpty32);
+                    mx[2] = max(mx[2], hit ? j : 0u);
+                },
+#else
             forStencil(d, f, s, CellBox{}, true, true, true,
                 [&](uint32_t j, const float4& pj, uint32_t row, uint32_t q)
                 {
@@ -801,6 +852,7 @@ namespace tsp
                 {
                     if (within(pi.x, pi.y, pi.z, pj, r2)) { cnt[2]++; mn[2] = min(mn[2], j); mx[2] = max(mx[2], j); }
                 },
+#endif
                 [&](uint32_t) { return true; },
                 [&](uint32_t, const uint2 (&)[3]) { return false; },
                 [&](uint32_t row)

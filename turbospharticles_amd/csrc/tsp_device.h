@@ -55,6 +55,9 @@
 //   TSP_TUNE_LANE_SORT     the lists of each 256-particle block take its list positions longest
 //                          first, so a wave's lanes walk lists of similar length (1: by FF count,
 //                          2: by FB count, then FF count; 0: positions in particle order)
+//   TSP_TUNE_SEARCH_FLAT   the count pass's row streams load past their end instead of branching
+//                          around the loads, and it folds its hits in through selects (1), or
+//                          branches on each candidate (0)
 //   TSP_TUNE_SEARCH_ROW    the neighbor query walks the stencil by rows of three cells, loading
 //                          the rows' ranges together and TSP_TUNE_SEARCH_BATCH candidates at a time
 //                          (1: the row's candidates as one stream, 2: cell by cell; 0: the cell walk)
@@ -109,6 +112,9 @@
 #endif
 #ifndef TSP_TUNE_PERSISTENT
 #define TSP_TUNE_PERSISTENT 0
+#endif
+#ifndef TSP_TUNE_SEARCH_FLAT
+#define TSP_TUNE_SEARCH_FLAT 0
 #endif
 #ifndef TSP_TUNE_SLOT16
 #define TSP_TUNE_SLOT16 1

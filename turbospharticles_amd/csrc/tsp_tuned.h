@@ -68,3 +68,6 @@
 #ifndef TSP_TUNE_SLOT16
 #define TSP_TUNE_SLOT16 1
 #endif
+#ifndef TSP_TUNE_SEARCH_FLAT
+#define TSP_TUNE_SEARCH_FLAT 0
+#endif
