@@ -65,3 +65,25 @@ def merge_bodies(a, b):
         st.body[k][:] = np.concatenate([a.body[k], b.body[k]])
     st.domain = a.domain
     return st
+
+
+def state_digest(st):
+    """SHA-256 of a state's fluid (ids, then every field in name order) and body records."""
+    import hashlib
+    h = hashlib.sha256(np.ascontiguousarray(st.fluid_id).tobytes())
+    for f in sorted(st.fluid):
+        h.update(np.ascontiguousarray(st.fluid[f]).tobytes())
+    for f in sorted(st.body):
+        h.update(np.ascontiguousarray(st.body[f]).tobytes())
+    return h.hexdigest()
+
+
+def drop_record(st, params):
+    """Per-step figures of the 1M cube drop record: mean density error (rho / rho0 - 1 averaged
+    over the fluid, cpu_backend.cpp:1509-1514), kinetic energy 1/2 m sum |v|^2 and the body's
+    centre of mass, all in float64 from the float32 state."""
+    rho = st.fluid["density"].astype(np.float64)
+    v2 = sum(st.fluid[c].astype(np.float64) ** 2 for c in ("vx", "vy", "vz"))
+    return {"mean_density_error": float(np.mean(rho / np.float64(params.rest_density) - 1.0)),
+            "kinetic_energy": float(0.5 * np.float64(params.mass) * np.sum(v2)),
+            "xcm": [float(st.body[c][0]) for c in ("xcm", "ycm", "zcm")]}

@@ -873,6 +873,21 @@ namespace tsp
             if (cnt[0] > 0xffffu || cnt[1] > 0xffffu || cnt[2] > 0xffffu) { atomicOr(&d.status->overflow, 2u); }
             if (tiny) { atomicOr(&d.status->pairTiny, 1u); }
         }
+        // the particles whose FF list spans more than 0xffff join the list the wide fill walks
+        // (one counter update per wave)
+        {
+            const bool wideHere = i < d.nF && cnt[0] > 0 && mx[0] - mn[0] > 0xffffu;
+            const uint64_t ballot = __ballot(wideHere);
+            if (ballot != 0ull)
+            {
+                const uint32_t lane64 = threadIdx.x & 63u;
+                const int first = __ffsll(static_cast<unsigned long long>(ballot)) - 1;
+                uint32_t base = 0;
+                if (static_cast<int>(lane64) == first) { base = atomicAdd(&d.status->wideCount, static_cast<uint32_t>(__popcll(ballot))); }
+                base = static_cast<uint32_t>(__shfl(static_cast<int>(base), first, 64));
+                if (wideHere) { d.wideList[base + static_cast<uint32_t>(__popcll(ballot & ((1ull << lane64) - 1ull)))] = i; }
+            }
+        }
 #if TSP_TUNE_LANE_SORT
         // the block's list positions, longest lists first (ties in particle order): a thread's
         // rank is the number of the block's sort keys above its own, the keys made distinct by
@@ -1061,51 +1076,7 @@ namespace tsp
                     return true;
                 },
                 [&](uint32_t) {});
-            if (wide)
-            {
-                // DictDelta16x2 split of a per-particle dict (neighbor_lists.cpp:838-858): the neighbors
-                // within 0xffff of the smallest form the low group, the rest the high group, which is
-                // stored from the back when it fits 0xffff itself, else the candidate order stays.
-                // A list of more than kDictMax entries is no dict: candidate order (pass 1 only).
-                const uint32_t n = cF;
-                const uint32_t minRef = d.ffMin[i] & ~kWideBit;
-                uint32_t base2 = kEmpty32, maxRef = 0;
-                for (int pass = n > kDictMax ? 1 : 0; pass < 3; pass++)
-                {
-                    uint32_t kLow = 0, kHigh = n;
-                    for (int cz = s.lo[2]; cz <= s.hi[2]; cz++)
-                    {
-                        for (int cy = s.lo[1]; cy <= s.hi[1]; cy++)
-                        {
-                            for (int cx = s.lo[0]; cx <= s.hi[0]; cx++)
-                            {
-                                const uint2 rf = d.cellF[s.cell(cx, cy, cz)];
-                                for (uint32_t j = rf.x; j < rf.y; j++)
-                                {
-                                    if (!within(pi.x, pi.y, pi.z, d.pos[j], r2)) { continue; }
-                                    const bool low = j - minRef <= 0xffffu;
-                                    if (pass == 0)
-                                    {
-                                        maxRef = max(maxRef, j);
-                                        if (!low) { base2 = min(base2, j); }
-                                    }
-                                    else if (pass == 1)
-                                    {
-                                        sellWrite(d.pool[REL_FF], mF, lane, kLow++, j);
-                                    }
-                                    else
-                                    {
-                                        if (low) { sellWrite(d.pool[REL_FF], mF, lane, kLow++, j); }
-                                        else { sellWrite(d.pool[REL_FF], mF, lane, --kHigh, j); }
-                                    }
-                                }
-                            }
-                        }
-                    }
-                    if (pass == 0 && maxRef - base2 <= 0xffffu) { pass = 1; } // skip the candidate-order pass
-                    else if (pass == 1) { break; }
-                }
-            }
+            // (the FF lists of wide particles: kQueryFillWide)
         }
         // the staged FF entries leave as whole slice rows: column k of the slice's lanes per store
 #if TSP_TUNE_LANE_SORT
@@ -1143,6 +1114,82 @@ namespace tsp
                 if (k < own) { row[k * kSlice] = base + stage[k][t]; }
             }
         }
+    }
+
+    // The FF lists of the particles whose list spans more than 0xffff (the count pass lists them):
+    // the DictDelta16x2 split of a per-particle dict (neighbor_lists.cpp:838-858) -- the neighbors
+    // within 0xffff of the smallest form the low group, the rest the high group, which is stored
+    // from the back when it fits 0xffff itself, else the candidate order stays; a list of more than
+    // kDictMax entries is no dict (candidate order). One thread per listed particle walks its
+    // stencil with the count pass's hit masks, twice for a dict: the high group's smallest and the
+    // largest neighbor, then the entries. Such a particle's slice stores u32 entries.
+    __global__ void __launch_bounds__(kBlock) TSP_WAVES_ATTR(TSP_TUNE_QUERY_WAVES) kQueryFillWide(Dev d)
+    {
+        if (stopped(d)) { return; }
+        const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+        if (q >= d.status->wideCount) { return; }
+        const uint32_t i = d.wideList[q];
+        const uint32_t sc = d.blockScene[i / kBlock];
+        const Frame& f = d.frames[sc];
+        const float r2 = d.consts[sc].r2;
+        Stencil s;
+        if (!stencilOf(f, keyAt(d, d.keyF, i), s)) { return; }
+        const uint32_t pos = sellPos(d, i);
+        const uint32_t lane = pos & (kSlice - 1);
+        const SliceMeta mF = d.meta[REL_FF][pos / kSlice];
+        const float4 pi = d.pos[i];
+        const uint32_t n = d.cnt[REL_FF][i];
+        const uint32_t minRef = d.ffMin[i] & ~kWideBit;
+        const uint32_t uncovered = d.ffMask[static_cast<size_t>(kMaskRows) * d.nF + i];
+        const auto walk = [&](auto&& onHit)
+        {
+            uint32_t rowMask = 0, nextMask = d.ffMask[i];
+            forStencil(d, f, s, CellBox{}, true, false, false,
+                [&](uint32_t j, const float4& pj, uint32_t, uint32_t)
+                {
+                    if (within(pi.x, pi.y, pi.z, pj, r2)) { onHit(j); }
+                },
+                [&](uint32_t, const float4&) {},
+                [&](uint32_t, const float4&) {},
+                [&](uint32_t row)
+                {
+                    const uint32_t cur = nextMask;
+                    if (row + 1u < kMaskRows) { nextMask = d.ffMask[static_cast<size_t>(row + 1u) * d.nF + i]; }
+                    if (row >= kMaskRows || ((uncovered >> row) & 1u)) { return true; }
+                    rowMask = cur;
+                    return rowMask != 0u;
+                },
+                [&](uint32_t row, const uint2 (&rf)[3])
+                {
+                    if (row >= kMaskRows || ((uncovered >> row) & 1u)) { return false; }
+                    uint32_t m = rowMask;
+                    while (m)
+                    {
+                        const uint32_t b = static_cast<uint32_t>(__ffs(static_cast<int>(m))) - 1u;
+                        m &= m - 1u;
+                        onHit(rowCandidate(rf, b));
+                    }
+                    return true;
+                },
+                [&](uint32_t) {});
+        };
+        const bool dict = n <= kDictMax;
+        uint32_t base2 = kEmpty32, maxRef = 0;
+        if (dict)
+        {
+            walk([&](uint32_t j)
+            {
+                maxRef = max(maxRef, j);
+                if (j - minRef > 0xffffu) { base2 = min(base2, j); }
+            });
+        }
+        const bool split = dict && maxRef - base2 <= 0xffffu;
+        uint32_t kLow = 0, kHigh = n;
+        walk([&](uint32_t j)
+        {
+            if (!split || j - minRef <= 0xffffu) { sellWrite(d.pool[REL_FF], mF, lane, kLow++, j); }
+            else { sellWrite(d.pool[REL_FF], mF, lane, --kHigh, j); }
+        });
     }
 
     // ----------------------------------------------------------------------------------------
@@ -1848,7 +1895,11 @@ namespace tsp
 
     void launchQueryFill(hipStream_t s, const Dev& d)
     {
-        if (d.nF) { kQueryFill<<<blocksFor(d.nF), kBlock, 0, s>>>(d); }
+        if (d.nF)
+        {
+            kQueryFill<<<blocksFor(d.nF), kBlock, 0, s>>>(d);
+            kQueryFillWide<<<blocksFor(d.nF), kBlock, 0, s>>>(d);
+        }
     }
 
     void launchHaloBuild(hipStream_t s, const Dev& d)

@@ -858,6 +858,7 @@ struct tsp_ctx
             zero(d.scanTmp, sizeof(uint32_t) * scanTmpWords(m));
         }
         d.ffMin = alloc<uint32_t>(nF, work, MEM_LF);
+        d.wideList = alloc<uint32_t>(nF, work, MEM_LF);
         d.laneRank = alloc<uint8_t>(nF, work, MEM_LF);
         d.rankWho = alloc<uint8_t>(nF, work, MEM_LF);
         d.ffMask = alloc<uint32_t>(static_cast<size_t>(kMaskWords) * nF, work, MEM_SF);
@@ -2630,11 +2631,11 @@ uint64_t tsp_batch_footprint(uint32_t n_fluid, uint32_t n_boundary, uint32_t n_r
 {
     // device bytes a scene holds while resident: per fluid slot (segment rounded to 256) about
     // 190 B of records and second buffers, 16 B of sort keys, 160 B of SELL-32 lists at the
-    // initial pool sizes, 22 B of halo, counts and list positions, and up to 24 B of cell tables;
-    // per boundary particle 64 B; per rigid particle about 400 B (records, keys, the RF list);
-    // per body 256 B
+    // initial pool sizes, 26 B of halo, counts, list positions and the wide-list index, and up to
+    // 24 B of cell tables; per boundary particle 64 B; per rigid particle about 400 B (records,
+    // keys, the RF list); per body 256 B
     const uint64_t f = static_cast<uint64_t>(roundUp(n_fluid, kBlock));
-    return f * 412ull + static_cast<uint64_t>(n_boundary) * 64ull + static_cast<uint64_t>(n_rigid) * 400ull
+    return f * 416ull + static_cast<uint64_t>(n_boundary) * 64ull + static_cast<uint64_t>(n_rigid) * 400ull
          + static_cast<uint64_t>(n_bodies) * 256ull + (64ull << 10);
 }
 

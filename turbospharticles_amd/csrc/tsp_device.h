@@ -198,6 +198,7 @@ namespace tsp
         uint32_t bUnsorted;      // boundary ids are not non-decreasing under the frames
         uint32_t pairTiny;       // some FF or FB pair has 0 < d2 < 2^-96: the kernels take the general sqrt
         uint32_t scanFail;       // sticky: a one-pass scan gave up waiting (kResetSearch keeps it)
+        uint32_t wideCount;      // particles in wideList (FF lists spanning more than 0xffff)
         uint32_t stop;           // a captured step stops early: 1 = its search needs the host, 2 = its
                                  // solve needs more sweeps (kernels launched with stopMask skip)
         uint64_t poolUnits[3];   // u16 units used by FF, FB, FR
@@ -388,6 +389,7 @@ namespace tsp
         uint8_t *rankWho;        // per list position: the particle's index within the block (lane sorting)
         uint32_t *ffMin;         // per particle: smallest FF neighbor index (0 without FF neighbors), with
                                  // kWideBit set when the list spans more than 0xffff
+        uint32_t *wideList;      // the particles with kWideBit, in no particular order (status->wideCount)
         uint32_t *ffMask;        // kMaskWords x nF: per stencil row, bit q = the row's q-th fluid candidate
                                  // is an FF neighbor; then the rows the masks do not cover
         uint32_t *haloIdx;       // per block: kHaloMax particle indices, ascending
