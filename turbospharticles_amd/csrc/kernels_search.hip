@@ -1228,6 +1228,7 @@ namespace tsp
     }
 
 #if TSP_TUNE_HALO_PAGES
+#error "TSP_TUNE_HALO_PAGES=1 stopped responding in a timing run after the lists were ordered by length within each block; it stays disabled until the cause is found"
     // The block's FF neighbors by pages of 4096 indices: one LDS bitmap per page (at most
     // kHaloPages pages, taken in order of arrival), the pages ranked by their number, so the set
     // bits in page order are the halo in ascending order and an entry's slot is the count of set

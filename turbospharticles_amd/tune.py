@@ -45,7 +45,6 @@ SPACE = {
     "TSP_TUNE_GATHER4": [0, 1],
     "TSP_TUNE_SWEEP_AHEAD": [-1, 0, 1, 2],
     "TSP_TUNE_STEP_GRAPH": [0, 1],
-    "TSP_TUNE_HALO_PAGES": [0, 1],
     "TSP_TUNE_QUERY_WAVES": [0, 6, 8],
     "TSP_TUNE_RANGE_LOADS": [0, 1],
     "TSP_TUNE_BOX_CELLS": [0, 512, 768, 1024],
