@@ -24,6 +24,7 @@ per-launch events). Results go to lib/tune/results.json; `apply` picks the faste
 candidate.
 """
 import argparse
+import hashlib
 import json
 import os
 import subprocess
@@ -75,7 +76,10 @@ def current():
 
 
 def name_of(cfg):
-    return "_".join(f"{k.replace('TSP_TUNE_', '').lower()}{v}" for k, v in sorted(cfg.items()))
+    """The variant's directory name: the settings spelled out, or their hash when that is longer
+    than a file name may be (the directory's config.json holds the settings either way)."""
+    full = "_".join(f"{k.replace('TSP_TUNE_', '').lower()}{v}" for k, v in sorted(cfg.items()))
+    return full if len(full) <= 200 else "cfg_" + hashlib.sha1(full.encode()).hexdigest()[:16]
 
 
 def candidates():
