@@ -18,7 +18,8 @@
 //              List position p holds the lists of one particle of p's 256-particle block: p
 //              itself, or, with lane sorting, the block's particles by list length (laneRank,
 //              rankWho)
-//   halo       per 256-particle block: up to 1000 distinct FF neighbor indices, ascending
+//   halo       per 256-particle block: up to kHaloMax (1064 tuned) distinct FF neighbor indices,
+//              ascending
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -174,7 +175,7 @@ namespace tsp
     constexpr uint32_t kMaskWords = kMaskRows + 1;
     // FF neighbor halo of a 256-particle block: the distinct FF neighbors of its particles,
     // ascending; the FF entries of the block then hold halo slots instead of particle indices
-    constexpr uint32_t kHaloMax = TSP_TUNE_HALO_MAX; // 24 B x 1000 records in kJacobi: six blocks per CU
+    constexpr uint32_t kHaloMax = TSP_TUNE_HALO_MAX; // 24 B x 1064 records in kJacobi: six blocks per CU
     constexpr uint32_t kNoHalo = 0xffffffffu;   // halo count of a block that keeps particle indices
 
     enum Rel { REL_FF = 0, REL_FB = 1, REL_FR = 2 };
