@@ -101,21 +101,27 @@ namespace tsp
         const List ff = listAt(d, REL_FF, p, i);
         // dict chunks scale by the mass after the sum, normal chunks per neighbor (sph_solver.cpp:974-1049)
         const bool dict = ff.n <= kDictMax;
-        const auto ffTerm = [&](const float4& pj, const float4& vj)
+        // the two forms as two loops (a wave runs the normal one only for a list longer than kDictMax)
+        const auto sumFF = [&](auto dictFlag)
         {
-            const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
-            const float dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
-            const float d2 = dist2(dx, dy, dz);
-            const float kg = kgFactor<kFast>(d2, k);
-            const float dot = __builtin_fmaf(dvx, dx, __builtin_fmaf(dvy, dy, dvz * dz));
-            const float rj = vmax(vj.w, 1e-7f);
-            const float coeff = (dict ? dot : k.mass * dot) / (rj * (d2 + k.hsqEps)) * kg;
-            aFx = __builtin_fmaf(coeff, dx, aFx);
-            aFy = __builtin_fmaf(coeff, dy, aFy);
-            aFz = __builtin_fmaf(coeff, dz, aFz);
+            constexpr bool kDict = decltype(dictFlag)::value;
+            const auto ffTerm = [&](const float4& pj, const float4& vj)
+            {
+                const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
+                const float dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
+                const float d2 = dist2(dx, dy, dz);
+                const float kg = kgFactor<kFast>(d2, k);
+                const float dot = __builtin_fmaf(dvx, dx, __builtin_fmaf(dvy, dy, dvz * dz));
+                const float rj = vmax(vj.w, 1e-7f);
+                const float coeff = (kDict ? dot : k.mass * dot) / (rj * (d2 + k.hsqEps)) * kg;
+                aFx = __builtin_fmaf(coeff, dx, aFx);
+                aFy = __builtin_fmaf(coeff, dy, aFy);
+                aFz = __builtin_fmaf(coeff, dz, aFz);
+            };
+            visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(recAt(hp, q), recAt(hv, q)); },
+                    [&](uint32_t j, uint32_t) { ffTerm(d.pos[j], d.vel[j]); });
         };
-        visitFF(ff, hc <= kHaloMax, [&](uint32_t q, uint32_t) { ffTerm(recAt(hp, q), recAt(hv, q)); },
-                [&](uint32_t j, uint32_t) { ffTerm(d.pos[j], d.vel[j]); });
+        if (dict) { sumFF(Flag<true>{}); } else { sumFF(Flag<false>{}); }
         if (dict) { aFx *= k.mass; aFy *= k.mass; aFz *= k.mass; }
         float aBx = 0.0f, aBy = 0.0f, aBz = 0.0f;
         const List fb = listAt(d, REL_FB, p, i);
@@ -437,22 +443,28 @@ namespace tsp
         // pressure term (sph_solver.cpp:1573-1640)
         const bool dict = ff.n <= kDictMax;
         const float outerF = k.mass * pi * k.presDis;
-        const auto ffTerm = [&](const float4& pj, uint32_t e)
+        // the two forms as two loops (a wave runs the normal one only for a list longer than kDictMax)
+        const auto sumFF = [&](auto dictFlag)
         {
-            const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
+            constexpr bool kDict = decltype(dictFlag)::value;
+            const auto ffTerm = [&](const float4& pj, uint32_t e)
+            {
+                const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
 #if TSP_TUNE_KG_STORE
-            const float kg = d.kgFF[e];
+                const float kg = d.kgFF[e];
 #else
-            (void)e;
-            const float kg = kgFactor<kFast>(dist2(dx, dy, dz), k);
+                (void)e;
+                const float kg = kgFactor<kFast>(dist2(dx, dy, dz), k);
 #endif
-            const float coeff = (dict ? pi + pj.w : outerF + k.mass * pj.w * k.presDis) * kg;
-            sx = __builtin_fmaf(coeff, dx, sx);
-            sy = __builtin_fmaf(coeff, dy, sy);
-            sz = __builtin_fmaf(coeff, dz, sz);
+                const float coeff = (kDict ? pi + pj.w : outerF + k.mass * pj.w * k.presDis) * kg;
+                sx = __builtin_fmaf(coeff, dx, sx);
+                sy = __builtin_fmaf(coeff, dy, sy);
+                sz = __builtin_fmaf(coeff, dz, sz);
+            };
+            visitFFSplit(ff, hc <= kHaloMax, [&](uint32_t q) { return recAt(hp, q); }, ffTerm,
+                         [&](uint32_t j, uint32_t e) { ffTerm(d.pos[j], e); });
         };
-        visitFFSplit(ff, hc <= kHaloMax, [&](uint32_t q) { return recAt(hp, q); }, ffTerm,
-                     [&](uint32_t j, uint32_t e) { ffTerm(d.pos[j], e); });
+        if (dict) { sumFF(Flag<true>{}); } else { sumFF(Flag<false>{}); }
         float ax = dict ? sx * k.presOuterFF : sx, ay = dict ? sy * k.presOuterFF : sy, az = dict ? sz * k.presOuterFF : sz;
         const List fb = listAt(d, REL_FB, p, i);
         visit(fb, [&](uint32_t j, uint32_t)

@@ -182,6 +182,13 @@ namespace tsp
 
     constexpr uint32_t kWideBit = 0x80000000u;   // in ffMin: the particle's FF list spans more than 0xffff
 
+    // a compile-time flag passed to a generic lambda (one loop body per value)
+    template <bool B>
+    struct Flag
+    {
+        static constexpr bool value = B;
+    };
+
     struct SliceMeta
     {
         uint32_t offset;  // in u16 units from the start of the pool (storage units before the scan)
