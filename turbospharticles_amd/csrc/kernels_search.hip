@@ -799,7 +799,7 @@ namespace tsp
                 d2 = dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
                 return d2 <= r2;
             };
-            forStencil<true>(d, f, s, CellBox{}, true, true, true,
+            forStencil<TSP_TUNE_SEARCH_FLAT == 1>(d, f, s, CellBox{}, true, true, true,
                 [&](uint32_t j, const float4& pj, uint32_t row, uint32_t q)
                 {
                     float d2;

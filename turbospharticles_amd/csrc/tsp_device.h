@@ -56,9 +56,9 @@
 //   TSP_TUNE_LANE_SORT     the lists of each 256-particle block take its list positions longest
 //                          first, so a wave's lanes walk lists of similar length (1: by FF count,
 //                          2: by FB count, then FF count; 0: positions in particle order)
-//   TSP_TUNE_SEARCH_FLAT   the count pass's row streams load past their end instead of branching
-//                          around the loads, and it folds its hits in through selects (1), or
-//                          branches on each candidate (0)
+//   TSP_TUNE_SEARCH_FLAT   the count pass folds its hits in through selects, its row streams loading
+//                          past their end (1) or branching around those loads (2); or it branches
+//                          on each candidate (0)
 //   TSP_TUNE_SEARCH_ROW    the neighbor query walks the stencil by rows of three cells, loading
 //                          the rows' ranges together and TSP_TUNE_SEARCH_BATCH candidates at a time
 //                          (1: the row's candidates as one stream, 2: cell by cell; 0: the cell walk)

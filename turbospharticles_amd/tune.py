@@ -58,7 +58,7 @@ SPACE = {
     "TSP_TUNE_SEARCH_ROW": [1, 0, 2],
     "TSP_TUNE_LANE_SORT": [1, 0, 2],
     "TSP_TUNE_SLOT16": [1, 0],
-    "TSP_TUNE_SEARCH_FLAT": [0, 1],
+    "TSP_TUNE_SEARCH_FLAT": [0, 1, 2],
     "TSP_TUNE_SEARCH_BATCH": [4, 8],
     "TSP_TUNE_PERSISTENT": [0, 1],
 }
