@@ -1467,17 +1467,17 @@ namespace tsp
         // otherwise a hash set of the block's entries, sorted
         for (uint32_t q = t; q < kHashSlots; q += kBlock) { table[q] = kEmpty32; }
         __syncthreads();
-        // insert: open addressing, a plain read before the compare-and-swap (most entries are
-        // already present); a thread stops once the halo is known to be too large, so at most
+        // insert: open addressing, a relaxed LDS read before the compare-and-swap (most entries
+        // are already present); a thread stops once the halo is known to be too large, so at most
         // kHaloMax + kBlock values enter the kHashSlots slots and a probe always ends
         for (uint32_t k = 0; k < n; k++)
         {
-            if (*static_cast<volatile uint32_t*>(&unique) > kHaloMax) { break; }
+            if (__hip_atomic_load(&unique, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > kHaloMax) { break; }
             const uint32_t j = sellRead(d.pool[REL_FF], m, lane, k);
             uint32_t h = haloHash(j);
             for (uint32_t probe = 0; probe < kHashSlots; probe++)
             {
-                uint32_t v = *static_cast<volatile uint32_t*>(&table[h]);
+                uint32_t v = __hip_atomic_load(&table[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 if (v == kEmpty32)
                 {
                     v = atomicCAS(&table[h], kEmpty32, j);
